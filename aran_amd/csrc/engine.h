@@ -1,0 +1,208 @@
+// engine.h -- internal interface of the MI355X FMM engine (not installed).
+//
+// Device data layout (HBM):
+//   particles   : SoA, sorted by 63-bit Morton key (x, y, z, q as FP64)
+//   nodes       : SoA, breadth-first, Morton-ordered inside each level
+//   multipoles  : [node][p(p+1)/2]     complex FP64 (re, im interleaved)
+//   locals      : [node][(p+1)(p+2)/2] complex FP64
+//   lists       : CSR tables; far (M2L) list = (source node, offset code)
+//                 per node, near (P2P) list = particle ranges per leaf
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "tree_rules.h"
+
+namespace aran {
+
+constexpr int kMaxDepth = 21;   // 3 x 21 key bits
+constexpr int kMaxOrder = 15;   // local degrees 0..15
+constexpr int kNumCodes = 343;  // translation offsets in [-3,3]^3
+
+inline __host__ __device__ int code_of(int dx, int dy, int dz) {
+  return (dx + 3) * 49 + (dy + 3) * 7 + (dz + 3);
+}
+inline __host__ __device__ int tri(int n) { return n * (n + 1) / 2; }
+// Per-theta rotation tables: four (l+1)x(l+1) real matrices per degree l.
+inline __host__ __device__ int rot_block(int L) {
+  return (L + 1) * (L + 2) * (2 * L + 3) / 6;
+}
+inline __host__ __device__ int rot_ofs(int l) {
+  return l * (l + 1) * (2 * l + 1) / 6;
+}
+
+// Translation vector v (source centre -> target centre, integer units)
+// after the z flip of the reference (rotate.c:190-194).
+struct DirEntry {
+  int theta;    // index of the theta rotation table
+  double c;     // +1, or -1 when v.z < 0 (v is then negated)
+  double norm;  // |v|
+};
+
+struct OpTables {
+  const double *rot;     // [ntheta][4][rot_block(p)]   E+ E- F+ F-
+  const double2 *eiphi;  // [343][p+1]   e^{i m phi}
+  const DirEntry *dir;   // [343]
+  const double *tm2l;    // [p+1][p+1][p]   T(l,m,n)
+  const double *tm2m;    // [p][p][p]
+  const double *tl2l;    // [p+1][p+1][p+1]
+  const double *ynorm;   // [tri(p+2)]  N_l^m
+  int p, rb;
+};
+
+struct Stats {
+  long zero = 0, p2p = 0, p2m = 0, m2m = 0, m2l = 0, l2l = 0, l2p = 0;
+  int depth = 0;
+  long nnodes = 0, nleaves = 0;
+  // phase times in ms (only with Problem::timings)
+  float t_tree = 0, t_lists = 0, t_up = 0, t_m2l = 0, t_down = 0, t_near = 0,
+        t_total = 0;
+};
+
+// Grow-only device buffer.
+struct DevBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  hipError_t reserve(size_t nbytes);
+  hipError_t grow_keep(size_t nbytes, size_t keep, hipStream_t s);
+  void release();
+  template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+struct Tables {
+  int order = -1;
+  int ntheta = 0;
+  DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, dscr, thetas;
+  OpTables view() const;
+};
+
+struct Workspace {
+  DevBuf keys_a, keys_b, idx_a, idx_b, xs, ys, zs, qs, pot_s, grad_s;
+  DevBuf scan_tmp, hist, part;
+  DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
+      n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
+  DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
+  DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs;
+  DevBuf mult, loc, small;
+  long node_cap = 0;
+  void *host_small = nullptr;  // pinned, 4 KiB
+  Tables tab;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  hipEvent_t ev[8] = {};
+  bool events_ready = false;
+  int device = -1;
+  NodeArrays nodes() const;
+  ~Workspace();
+};
+
+struct Problem {
+  long n = 0;
+  int order = 8;
+  int leafcap = 20;
+  double lb[3] = {0, 0, 0}, ub[3] = {1, 1, 1};
+  const double *x = nullptr, *y = nullptr, *z = nullptr, *q = nullptr;
+  double *pot = nullptr;   // device, user order, may be null
+  double *grad = nullptr;  // device, user order [n][3], may be null
+  bool timings = false;
+};
+
+struct TreeInfo {
+  double lo[3] = {0, 0, 0};
+  double width = 1;
+  int depth = 0;
+  long nnodes = 0, nleaves = 0, nfar = 0, nnear = 0;
+  std::vector<int> level_start, level_count;
+};
+
+hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti);
+hipError_t ensure_tables(Workspace &ws, int order);
+hipError_t workspace_init(Workspace &ws, hipStream_t stream);
+const char *last_error();
+void set_error(const std::string &msg);
+
+// Root cube: centre (lb+ub)/2, half-width max((ub-lb)/2) (1 when zero),
+// doubled about the centre until [pmin, pmax] is inside.
+void root_box(const double *lb, const double *ub, const double *pmin,
+              const double *pmax, double *lo, double *width);
+
+// ---- tree.hip
+hipError_t launch_minmax(const double *x, const double *y, const double *z,
+                         long n, double *out6, double *part, hipStream_t s);
+hipError_t launch_keys(const double *x, const double *y, const double *z,
+                       long n, double lo0, double lo1, double lo2,
+                       double scale, uint64_t *keys, int *idx, hipStream_t s);
+hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
+                          DevBuf &tmp, hipStream_t s);
+hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
+                      int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
+                      int *total, hipStream_t s);
+hipError_t launch_gather(const int *idx, long n, const double *x,
+                         const double *y, const double *z, const double *q,
+                         double *xs, double *ys, double *zs, double *qs,
+                         hipStream_t s);
+hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
+                          const double *grad_s, double *pot, double *grad,
+                          hipStream_t s);
+hipError_t launch_root(const NodeArrays &nd, long n, const double *lo,
+                       double width, hipStream_t s);
+hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
+                              int lvl_count, const uint64_t *keys, int leafcap,
+                              int *bounds, int *cnt, hipStream_t s);
+hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
+                              int lvl_count, int next_begin, const int *bounds,
+                              const int *cnt, const int *ofs, const double *lo,
+                              double width, hipStream_t s);
+hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
+                             hipStream_t s);
+hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
+                               const int *pos, int *leaves, hipStream_t s);
+hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *lstart,
+                        const int *lcount, int pass, int *cnt, const int *ofs,
+                        int *src, short *code, hipStream_t s);
+hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
+                        const int *lstart, const int *lcount, int pass,
+                        int *cnt, const int *ofs, int *ubeg, int *uend,
+                        long long *pairs, hipStream_t s);
+hipError_t launch_sum_ll(const long long *v, int n, long long *out,
+                         hipStream_t s);
+
+// ---- operators.hip
+hipError_t launch_wigner_tables(const double *thetas, int ntheta, int L,
+                                double *dscr, double *rot, hipStream_t s);
+hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
+                      const int *leaves, int nleaves, const double *xs,
+                      const double *ys, const double *zs, const double *qs,
+                      double2 *M, hipStream_t s);
+hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
+                      int lvl_count, double width, double2 *M, hipStream_t s);
+hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
+                      const int *vofs, const int *vsrc, const short *vcode,
+                      double width, const double2 *M, double2 *L,
+                      hipStream_t s);
+hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
+                      int lvl_count, double width, double2 *L, hipStream_t s);
+hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
+                      const int *leaves, int nleaves, const double *xs,
+                      const double *ys, const double *zs, const double2 *L,
+                      double *pot, double *grad, hipStream_t s);
+
+hipError_t launch_translate_one(int kind, const OpTables &tb, double r,
+                                const double2 *src, double2 *dst,
+                                hipStream_t s);
+
+// ---- p2p.hip
+hipError_t launch_p2p(const int *leaves, int nleaves, const int *lpb,
+                      const int *lpe, const int *uofs, const int *ubeg,
+                      const int *uend, const double *xs, const double *ys,
+                      const double *zs, const double *qs, double *pot,
+                      double *grad, hipStream_t s);
+hipError_t launch_direct(const double *x, const double *y, const double *z,
+                         const double *q, long n, const long *targets, long nt,
+                         double *pot, double *grad, hipStream_t s);
+
+}  // namespace aran

@@ -1,0 +1,542 @@
+// engine.hip -- host orchestration of one FMM solve on one GPU.
+//
+// Order of work follows aran_solver3d_solve (aran/aransolver3d.c:1135-1202):
+// tree (replaces VsgPRTree3d insertion), upward pass (P2M at leaves, M2M
+// child -> father, level by level from the deepest), far field (M2L gathered
+// per target node), downward pass (L2L father -> child, level by level) and
+// near field + L2P at leaves.  Every accumulation is a gather with a fixed
+// order, so a solve is deterministic.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <complex>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+
+namespace aran {
+
+static thread_local std::string g_err;
+const char *last_error() { return g_err.c_str(); }
+void set_error(const std::string &msg) { g_err = msg; }
+
+#define CK(expr)                                                        \
+  do {                                                                  \
+    hipError_t _e = (expr);                                             \
+    if (_e != hipSuccess) {                                             \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));     \
+      return _e;                                                        \
+    }                                                                   \
+  } while (0)
+
+hipError_t DevBuf::reserve(size_t nbytes) {
+  if (nbytes <= bytes && ptr) return hipSuccess;
+  if (ptr) {
+    hipError_t e = hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    if (e != hipSuccess) return e;
+  }
+  size_t want = std::max<size_t>(nbytes + nbytes / 4, 256);
+  hipError_t e = hipMalloc(&ptr, want);
+  if (e != hipSuccess) {
+    ptr = nullptr;
+    return e;
+  }
+  bytes = want;
+  return hipSuccess;
+}
+
+hipError_t DevBuf::grow_keep(size_t nbytes, size_t keep, hipStream_t s) {
+  if (nbytes <= bytes && ptr) return hipSuccess;
+  void *np = nullptr;
+  size_t want = std::max<size_t>(nbytes + nbytes / 2, 256);
+  hipError_t e = hipMalloc(&np, want);
+  if (e != hipSuccess) return e;
+  if (ptr && keep) {
+    e = hipMemcpyAsync(np, ptr, std::min(keep, bytes), hipMemcpyDeviceToDevice,
+                       s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+  }
+  if (ptr) (void)hipFree(ptr);
+  ptr = np;
+  bytes = want;
+  return hipSuccess;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+OpTables Tables::view() const {
+  OpTables t;
+  t.rot = rot.as<double>();
+  t.eiphi = eiphi.as<double2>();
+  t.dir = dir.as<DirEntry>();
+  t.tm2l = tm2l.as<double>();
+  t.tm2m = tm2m.as<double>();
+  t.tl2l = tl2l.as<double>();
+  t.ynorm = ynorm.as<double>();
+  t.p = order;
+  t.rb = rot_block(order);
+  return t;
+}
+
+NodeArrays Workspace::nodes() const {
+  NodeArrays a;
+  a.depth = n_depth.as<int>();
+  a.ix = n_ix.as<int>();
+  a.iy = n_iy.as<int>();
+  a.iz = n_iz.as<int>();
+  a.pbegin = n_pbegin.as<int>();
+  a.pend = n_pend.as<int>();
+  a.parent = n_parent.as<int>();
+  a.child = n_child.as<int>();
+  a.nchild = n_nchild.as<int>();
+  a.leaf = n_leaf.as<int>();
+  a.key = n_key.as<uint64_t>();
+  a.cx = n_cx.as<double>();
+  a.cy = n_cy.as<double>();
+  a.cz = n_cz.as<double>();
+  return a;
+}
+
+Workspace::~Workspace() {
+  DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &xs, &ys, &zs, &qs,
+                   &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
+                   &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
+                   &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz,
+                   &split_bounds, &split_cnt, &split_ofs, &flags, &leaf_list,
+                   &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
+                   &u_ofs, &u_beg, &u_end, &u_pairs, &mult, &loc, &small,
+                   &tab.rot, &tab.eiphi, &tab.dir, &tab.tm2l, &tab.tm2m,
+                   &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas};
+  for (DevBuf *b : all) b->release();
+  if (host_small) (void)hipHostFree(host_small);
+  if (events_ready)
+    for (auto &e : ev) (void)hipEventDestroy(e);
+  if (own_stream && stream) (void)hipStreamDestroy(stream);
+}
+
+hipError_t workspace_init(Workspace &ws, hipStream_t stream) {
+  if (ws.device < 0) CK(hipGetDevice(&ws.device));
+  if (!ws.host_small) CK(hipHostMalloc(&ws.host_small, 4096));
+  if (stream) {
+    ws.stream = stream;
+  } else if (!ws.stream) {
+    CK(hipStreamCreate(&ws.stream));
+    ws.own_stream = true;
+  }
+  if (!ws.events_ready) {
+    for (auto &e : ws.ev) CK(hipEventCreate(&e));
+    ws.events_ready = true;
+  }
+  CK(ws.small.reserve(4096));
+  return hipSuccess;
+}
+
+void root_box(const double *lb, const double *ub, const double *pmin,
+              const double *pmax, double *lo, double *width) {
+  double c[3], h = 0.;
+  for (int k = 0; k < 3; k++) {
+    c[k] = 0.5 * (lb[k] + ub[k]);
+    if (0.5 * (ub[k] - lb[k]) > h) h = 0.5 * (ub[k] - lb[k]);
+  }
+  if (!(h > 0.)) h = 1.;
+  for (;;) {
+    bool inside = true;
+    for (int k = 0; k < 3; k++)
+      if (pmin[k] < c[k] - h || pmax[k] > c[k] + h) inside = false;
+    if (inside) break;
+    h *= 2.;
+  }
+  for (int k = 0; k < 3; k++) lo[k] = c[k] - h;
+  *width = 2. * h;
+}
+
+// ------------------------------------------------------------ tables
+
+// alpha_l^m = 1/sqrt((l-m)!(l+m)!) by the recurrences of _alpha_generator
+// (aran/aransphericalseriesd.c:45-59).
+static std::vector<double> alpha_table(int deg) {
+  std::vector<double> a(tri(deg + 1));
+  for (int l = 0; l <= deg; l++)
+    for (int m = 0; m <= l; m++) {
+      double v;
+      if (l == 0)
+        v = 1.;
+      else if (m == 0)
+        v = a[tri(l - 1)] / l;
+      else if (l == m)
+        v = a[tri(l - 1) + l - 1] / sqrt((l + l - 1.) * (l + l));
+      else
+        v = sqrt((l - m + 1.) / (l + m)) * a[tri(l) + m - 1];
+      a[tri(l) + m] = v;
+    }
+  return a;
+}
+
+static inline double beta_l(int l) { return sqrt((4. * M_PI) / (l + l + 1.)); }
+
+hipError_t ensure_tables(Workspace &ws, int p) {
+  if (ws.tab.order == p) return hipSuccess;
+  hipStream_t s = ws.stream;
+  Tables &T = ws.tab;
+  // directions and theta values
+  std::map<std::pair<int, int>, int> theta_idx;
+  std::vector<double> thetas;
+  std::vector<DirEntry> dirs(kNumCodes);
+  std::vector<std::complex<double>> eiphi((size_t)kNumCodes * (p + 1));
+  for (int dx = -3; dx <= 3; dx++)
+    for (int dy = -3; dy <= 3; dy++)
+      for (int dz = -3; dz <= 3; dz++) {
+        int code = code_of(dx, dy, dz);
+        int vx = dx, vy = dy, vz = dz;
+        double c = 1.;
+        if (vz < 0) {
+          c = -1.;
+          vx = -vx;
+          vy = -vy;
+          vz = -vz;
+        }
+        int rho2 = vx * vx + vy * vy;
+        DirEntry de;
+        de.c = c;
+        de.norm = sqrt((double)(rho2 + vz * vz));
+        if (rho2 + vz * vz == 0) {
+          de.theta = 0;
+          de.norm = 0.;
+        } else {
+          auto key = std::make_pair(vz, rho2);
+          auto it = theta_idx.find(key);
+          if (it == theta_idx.end()) {
+            int id = (int)thetas.size();
+            theta_idx[key] = id;
+            thetas.push_back(atan2(sqrt((double)rho2), (double)vz));
+            de.theta = id;
+          } else {
+            de.theta = it->second;
+          }
+        }
+        dirs[code] = de;
+        std::complex<double> e1(1., 0.), e = 1.;
+        if (rho2 > 0) e1 = std::complex<double>(vx, vy) / sqrt((double)rho2);
+        for (int m = 0; m <= p; m++) {
+          eiphi[(size_t)code * (p + 1) + m] = e;
+          e *= e1;
+        }
+      }
+  if (thetas.empty()) thetas.push_back(0.);
+  int nth = (int)thetas.size();
+  // translation factor tables
+  std::vector<double> al = alpha_table(2 * p + 2);
+  auto A = [&](int l, int m) { return al[tri(l) + m]; };
+  std::vector<double> tm2l((size_t)(p + 1) * (p + 1) * p, 0.);
+  for (int l = 0; l <= p; l++)
+    for (int m = 0; m <= l; m++)
+      for (int n = m; n < p; n++)
+        tm2l[((size_t)l * (p + 1) + m) * p + n] =
+            beta_l(l) / beta_l(n) * A(l, m) * A(n, m) / A(l + n, 0);
+  std::vector<double> tm2m((size_t)p * p * p, 0.);
+  for (int l = 0; l < p; l++)
+    for (int m = 0; m <= l; m++)
+      for (int n = m; n <= l; n++)
+        tm2m[((size_t)l * p + m) * p + n] =
+            A(l - n, 0) * A(n, m) / A(l, m) * (beta_l(l) / beta_l(n));
+  std::vector<double> tl2l((size_t)(p + 1) * (p + 1) * (p + 1), 0.);
+  for (int l = 0; l <= p; l++)
+    for (int m = 0; m <= l; m++)
+      for (int n = l; n <= p; n++)
+        tl2l[((size_t)l * (p + 1) + m) * (p + 1) + n] =
+            A(n - l, 0) * A(l, m) / A(n, m) * (beta_l(l) / beta_l(n));
+  // N_l^m (aran/aransphericalharmonic.c:31-38)
+  std::vector<double> yn(tri(p + 2));
+  for (int l = 0; l <= p + 1; l++)
+    for (int m = 0; m <= l; m++)
+      yn[tri(l) + m] = (m == 0) ? sqrt((l + l + 1.) / (4. * M_PI))
+                                : yn[tri(l) + m - 1] / sqrt((l - m + 1.) * (l + m));
+
+  int rb = rot_block(p);
+  int w = 2 * p + 1;
+  CK(T.thetas.reserve(sizeof(double) * nth));
+  CK(T.dscr.reserve(sizeof(double) * (size_t)nth * (p + 1) * w * w));
+  CK(T.rot.reserve(sizeof(double) * (size_t)nth * 4 * rb));
+  CK(T.eiphi.reserve(sizeof(double) * 2 * eiphi.size()));
+  CK(T.dir.reserve(sizeof(DirEntry) * dirs.size()));
+  CK(T.tm2l.reserve(sizeof(double) * std::max<size_t>(1, tm2l.size())));
+  CK(T.tm2m.reserve(sizeof(double) * std::max<size_t>(1, tm2m.size())));
+  CK(T.tl2l.reserve(sizeof(double) * tl2l.size()));
+  CK(T.ynorm.reserve(sizeof(double) * yn.size()));
+  CK(hipMemcpyAsync(T.thetas.ptr, thetas.data(), sizeof(double) * nth,
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemsetAsync(T.dscr.ptr, 0, sizeof(double) * (size_t)nth * (p + 1) * w * w,
+                    s));
+  CK(hipMemcpyAsync(T.eiphi.ptr, eiphi.data(), sizeof(double) * 2 * eiphi.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.dir.ptr, dirs.data(), sizeof(DirEntry) * dirs.size(),
+                    hipMemcpyHostToDevice, s));
+  if (!tm2l.empty())
+    CK(hipMemcpyAsync(T.tm2l.ptr, tm2l.data(), sizeof(double) * tm2l.size(),
+                      hipMemcpyHostToDevice, s));
+  if (!tm2m.empty())
+    CK(hipMemcpyAsync(T.tm2m.ptr, tm2m.data(), sizeof(double) * tm2m.size(),
+                      hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.tl2l.ptr, tl2l.data(), sizeof(double) * tl2l.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.ynorm.ptr, yn.data(), sizeof(double) * yn.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(launch_wigner_tables(T.thetas.as<double>(), nth, p, T.dscr.as<double>(),
+                          T.rot.as<double>(), s));
+  CK(hipStreamSynchronize(s));
+  T.ntheta = nth;
+  T.order = p;
+  return hipSuccess;
+}
+
+// ------------------------------------------------------------ solve
+
+static hipError_t read_int(Workspace &ws, const int *dev, int *out) {
+  int *h = static_cast<int *>(ws.host_small);
+  CK(hipMemcpyAsync(h, dev, sizeof(int), hipMemcpyDeviceToHost, ws.stream));
+  CK(hipStreamSynchronize(ws.stream));
+  *out = *h;
+  return hipSuccess;
+}
+
+static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
+  hipStream_t s = ws.stream;
+  DevBuf *ints[] = {&ws.n_depth, &ws.n_ix, &ws.n_iy, &ws.n_iz, &ws.n_pbegin,
+                    &ws.n_pend, &ws.n_parent, &ws.n_child, &ws.n_nchild,
+                    &ws.n_leaf};
+  for (DevBuf *b : ints) CK(b->grow_keep(sizeof(int) * cap, sizeof(int) * keep, s));
+  CK(ws.n_key.grow_keep(sizeof(uint64_t) * cap, sizeof(uint64_t) * keep, s));
+  DevBuf *dbl[] = {&ws.n_cx, &ws.n_cy, &ws.n_cz};
+  for (DevBuf *b : dbl)
+    CK(b->grow_keep(sizeof(double) * cap, sizeof(double) * keep, s));
+  ws.node_cap = std::min(ws.n_depth.bytes / sizeof(int),
+                         ws.n_cx.bytes / sizeof(double));
+  ws.node_cap = std::min<long>(ws.node_cap, ws.n_key.bytes / sizeof(uint64_t));
+  return hipSuccess;
+}
+
+hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
+  const long n = pb.n;
+  const int p = pb.order;
+  if (p < 1 || p > kMaxOrder) {
+    set_error("order must be in [1, 15]");
+    return hipErrorInvalidValue;
+  }
+  if (pb.leafcap < 1 || n < 0 || n > (1L << 30)) {
+    set_error("leaf capacity must be >= 1 and 0 <= n <= 2^30");
+    return hipErrorInvalidValue;
+  }
+  CK(workspace_init(ws, ws.stream));
+  CK(ensure_tables(ws, p));
+  hipStream_t s = ws.stream;
+  Stats local_st;
+  Stats &S = st ? *st : local_st;
+  S = Stats();
+  if (n == 0) return hipSuccess;
+  if (pb.timings) CK(hipEventRecord(ws.ev[0], s));
+
+  // ---- root box from the particle extent
+  CK(ws.part.reserve(sizeof(double) * 6 * 256));
+  double *d6 = ws.small.as<double>();
+  CK(launch_minmax(pb.x, pb.y, pb.z, n, d6, ws.part.as<double>(), s));
+  double *h6 = static_cast<double *>(ws.host_small);
+  CK(hipMemcpyAsync(h6, d6, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  if (!(std::isfinite(h6[0]) && std::isfinite(h6[1]) && std::isfinite(h6[2]) &&
+        std::isfinite(h6[3]) && std::isfinite(h6[4]) && std::isfinite(h6[5]))) {
+    set_error("non-finite particle coordinates");
+    return hipErrorInvalidValue;
+  }
+  double pmin[3] = {h6[0], h6[1], h6[2]}, pmax[3] = {h6[3], h6[4], h6[5]};
+  double lo[3], width;
+  root_box(pb.lb, pb.ub, pmin, pmax, lo, &width);
+  const double scale = 2097152.0 / width;
+
+  // ---- keys and sort
+  CK(ws.keys_a.reserve(sizeof(uint64_t) * n));
+  CK(ws.keys_b.reserve(sizeof(uint64_t) * n));
+  CK(ws.idx_a.reserve(sizeof(int) * n));
+  CK(ws.idx_b.reserve(sizeof(int) * n));
+  DevBuf *pbufs[] = {&ws.xs, &ws.ys, &ws.zs, &ws.qs, &ws.pot_s};
+  for (DevBuf *b : pbufs) CK(b->reserve(sizeof(double) * n));
+  CK(ws.grad_s.reserve(sizeof(double) * 3 * n));
+  int *d_total = ws.small.as<int>() + 64;
+  CK(launch_keys(pb.x, pb.y, pb.z, n, lo[0], lo[1], lo[2], scale,
+                 ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(), s));
+  CK(radix_sort(ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(),
+                ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), n, ws.hist,
+                ws.scan_tmp, d_total, s));
+  CK(launch_gather(ws.idx_a.as<int>(), n, pb.x, pb.y, pb.z, pb.q,
+                   ws.xs.as<double>(), ws.ys.as<double>(), ws.zs.as<double>(),
+                   ws.qs.as<double>(), s));
+  const uint64_t *keys = ws.keys_a.as<uint64_t>();
+
+  // ---- levels
+  long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64);
+  if (ws.node_cap < cap0) CK(reserve_nodes(ws, cap0, 0));
+  NodeArrays nd = ws.nodes();
+  CK(launch_root(nd, n, lo, width, s));
+  std::vector<int> lstart = {0}, lcount = {1};
+  long nnodes = 1;
+  for (int d = 0;; d++) {
+    int lc = lcount[d], lb = lstart[d];
+    CK(ws.split_bounds.reserve(sizeof(int) * 9 * (size_t)lc));
+    CK(ws.split_cnt.reserve(sizeof(int) * (size_t)lc));
+    CK(ws.split_ofs.reserve(sizeof(int) * ((size_t)lc + 1)));
+    CK(launch_split_count(nd, lb, lc, keys, pb.leafcap,
+                          ws.split_bounds.as<int>(), ws.split_cnt.as<int>(), s));
+    CK(scan_exclusive(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), lc,
+                      d_total, ws.scan_tmp, s));
+    int total = 0;
+    CK(read_int(ws, d_total, &total));
+    if (nnodes + total > ws.node_cap) {
+      CK(reserve_nodes(ws, 2 * (nnodes + total), nnodes));
+      nd = ws.nodes();
+    }
+    CK(launch_split_write(nd, lb, lc, (int)nnodes, ws.split_bounds.as<int>(),
+                          ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), lo,
+                          width, s));
+    if (total == 0) break;
+    lstart.push_back((int)nnodes);
+    lcount.push_back(total);
+    nnodes += total;
+  }
+  const int depth = (int)lstart.size() - 1;
+  const int nn = (int)nnodes;
+
+  // ---- leaves
+  CK(ws.flags.reserve(sizeof(int) * nn));
+  CK(ws.leaf_list.reserve(sizeof(int) * (nn + 1)));
+  CK(ws.split_ofs.reserve(sizeof(int) * (nn + 1)));
+  CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
+  CK(scan_exclusive(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
+                    ws.scan_tmp, s));
+  int nleaves = 0;
+  CK(read_int(ws, d_total, &nleaves));
+  CK(launch_leaf_compact(nd, nn, ws.split_ofs.as<int>(), ws.leaf_list.as<int>(),
+                         s));
+  const int *leaves = ws.leaf_list.as<int>();
+  CK(ws.lstart.reserve(sizeof(int) * (kMaxDepth + 2)));
+  CK(ws.lcount.reserve(sizeof(int) * (kMaxDepth + 2)));
+  CK(hipMemcpyAsync(ws.lstart.ptr, lstart.data(), sizeof(int) * lstart.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
+                    hipMemcpyHostToDevice, s));
+  if (pb.timings) CK(hipEventRecord(ws.ev[1], s));
+
+  // ---- far list
+  CK(ws.v_cnt.reserve(sizeof(int) * nn));
+  CK(ws.v_ofs.reserve(sizeof(int) * (nn + 1)));
+  CK(launch_vlist(nd, nn, ws.lstart.as<int>(), ws.lcount.as<int>(), 0,
+                  ws.v_cnt.as<int>(), nullptr, nullptr, nullptr, s));
+  CK(scan_exclusive(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
+                    ws.v_ofs.as<int>() + nn, ws.scan_tmp, s));
+  int nfar = 0;
+  CK(read_int(ws, ws.v_ofs.as<int>() + nn, &nfar));
+  CK(ws.v_src.reserve(sizeof(int) * std::max(1, nfar)));
+  CK(ws.v_code.reserve(sizeof(short) * std::max(1, nfar)));
+  CK(launch_vlist(nd, nn, ws.lstart.as<int>(), ws.lcount.as<int>(), 1,
+                  ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), ws.v_src.as<int>(),
+                  ws.v_code.as<short>(), s));
+  // ---- near list
+  CK(ws.u_cnt.reserve(sizeof(int) * nleaves));
+  CK(ws.u_ofs.reserve(sizeof(int) * (nleaves + 1)));
+  CK(ws.u_pairs.reserve(sizeof(long long) * (nleaves + 1)));
+  CK(launch_ulist(nd, nleaves, leaves, ws.lstart.as<int>(), ws.lcount.as<int>(),
+                  0, ws.u_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, s));
+  CK(scan_exclusive(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
+                    ws.u_ofs.as<int>() + nleaves, ws.scan_tmp, s));
+  int nnear = 0;
+  CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
+  CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
+  CK(ws.u_end.reserve(sizeof(int) * std::max(1, nnear)));
+  CK(launch_ulist(nd, nleaves, leaves, ws.lstart.as<int>(), ws.lcount.as<int>(),
+                  1, ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), ws.u_beg.as<int>(),
+                  ws.u_end.as<int>(), ws.u_pairs.as<long long>(), s));
+  long long *d_pairs = reinterpret_cast<long long *>(ws.small.as<char>() + 1024);
+  CK(launch_sum_ll(ws.u_pairs.as<long long>(), nleaves, d_pairs, s));
+  if (pb.timings) CK(hipEventRecord(ws.ev[2], s));
+
+  // ---- upward pass
+  const int mc = tri(p), lc = tri(p + 1);
+  CK(ws.mult.reserve(sizeof(double2) * (size_t)nn * mc));
+  CK(ws.loc.reserve(sizeof(double2) * (size_t)nn * lc));
+  double2 *M = ws.mult.as<double2>();
+  double2 *L = ws.loc.as<double2>();
+  OpTables tb = ws.tab.view();
+  CK(launch_p2m(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
+                ws.zs.as<double>(), ws.qs.as<double>(), M, s));
+  for (int d = depth - 1; d >= 0; d--)
+    CK(launch_m2m(tb, nd, lstart[d], lcount[d], width, M, s));
+  if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+  // ---- far field
+  CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
+                ws.v_code.as<short>(), width, M, L, s));
+  if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+  // ---- downward pass
+  for (int d = 1; d <= depth; d++)
+    CK(launch_l2l(tb, nd, lstart[d], lcount[d], width, L, s));
+  if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
+  // ---- near field and L2P
+  CK(launch_p2p(leaves, nleaves, nd.pbegin, nd.pend, ws.u_ofs.as<int>(),
+                ws.u_beg.as<int>(), ws.u_end.as<int>(), ws.xs.as<double>(),
+                ws.ys.as<double>(), ws.zs.as<double>(), ws.qs.as<double>(),
+                ws.pot_s.as<double>(), ws.grad_s.as<double>(), s));
+  CK(launch_l2p(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
+                ws.zs.as<double>(), L, ws.pot_s.as<double>(),
+                ws.grad_s.as<double>(), s));
+  CK(launch_scatter(ws.idx_a.as<int>(), n, ws.pot_s.as<double>(),
+                    ws.grad_s.as<double>(), pb.pot, pb.grad, s));
+  if (pb.timings) CK(hipEventRecord(ws.ev[6], s));
+
+  long long *hp = reinterpret_cast<long long *>(
+      static_cast<char *>(ws.host_small) + 1024);
+  CK(hipMemcpyAsync(hp, d_pairs, sizeof(long long), hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  S.zero = nn;
+  S.p2p = (long)(*hp / 2);
+  S.p2m = n;
+  S.m2m = nn - 1;
+  S.m2l = nfar;
+  S.l2l = nn - 1;
+  S.l2p = n;
+  S.depth = depth;
+  S.nnodes = nn;
+  S.nleaves = nleaves;
+  if (pb.timings) {
+    CK(hipEventElapsedTime(&S.t_tree, ws.ev[0], ws.ev[1]));
+    CK(hipEventElapsedTime(&S.t_lists, ws.ev[1], ws.ev[2]));
+    CK(hipEventElapsedTime(&S.t_up, ws.ev[2], ws.ev[3]));
+    CK(hipEventElapsedTime(&S.t_m2l, ws.ev[3], ws.ev[4]));
+    CK(hipEventElapsedTime(&S.t_down, ws.ev[4], ws.ev[5]));
+    CK(hipEventElapsedTime(&S.t_near, ws.ev[5], ws.ev[6]));
+    CK(hipEventElapsedTime(&S.t_total, ws.ev[0], ws.ev[6]));
+  }
+  if (ti) {
+    for (int k = 0; k < 3; k++) ti->lo[k] = lo[k];
+    ti->width = width;
+    ti->depth = depth;
+    ti->nnodes = nn;
+    ti->nleaves = nleaves;
+    ti->nfar = nfar;
+    ti->nnear = nnear;
+    ti->level_start = lstart;
+    ti->level_count = lcount;
+  }
+  return hipSuccess;
+}
+
+}  // namespace aran

@@ -1,0 +1,650 @@
+// operators.hip -- FP64 expansion operators of the 3D Newton FMM on gfx950.
+//
+//   k_wigner_tables : Wigner d^l_{a,b}(theta) by the three-term recurrence
+//                     in l for fixed (a, b), FP64, one block per theta;
+//                     folded into the real rotation matrices E+-, F+-
+//   k_p2m           : aran_development3d_p2m (arandevelopment3d.c:169-216)
+//   k_m2m           : aran_development3d_m2m_rotate (arandevelopment3d.c:
+//                     402-409, aransphericalseriesd-rotate.c:219-263)
+//   k_m2l           : aran_development3d_m2l_rotate (arandevelopment3d.c:
+//                     423-430, rotate.c:165-207, translate.c:609-676)
+//   k_l2l           : aran_development3d_l2l_rotate (arandevelopment3d.c:
+//                     444-451, translate.c:301-364)
+//   k_l2p           : local evaluate + gradient (aransphericalseriesd.c:
+//                     313-546)
+//
+// Rotation.  The reference applies D^l_{m,m'} = e^{-im alpha} d^l_{m,m'}
+// e^{-im' gamma} to src[l][m] with src[l][-m] = (-1)^m conj(src[l][m])
+// (rotate.c:34-80).  With the z-flipped translation angles (alpha, beta,
+// gamma) = (-phi, theta, 0) and the inverse (0, -theta, phi), the phases
+// factor out and each degree becomes two REAL (l+1)x(l+1) products:
+//   forward: u_m = e^{i m phi} s_m;  Re X = E+ Re u,  Im X = E- Im u,
+//            E+-[m'][m] = d_{m,m'}(theta) +- [m>0] (-1)^m d_{-m,m'}(theta)
+//   inverse: Re W = F+ Re s, Im W = F- Im s, dst_m' = e^{-i m' phi} W_m',
+//            F+-[m'][m] = d_{m',m}(theta) +- [m>0] (-1)^m d_{m',-m}(theta)
+// (d(-theta) is the transpose of d(theta)).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "engine.h"
+#include "tree_rules.h"
+
+namespace aran {
+
+__device__ inline void lm_of(int k, int &l, int &m) {
+  int t = (int)((sqrtf(8.f * k + 1.f) - 1.f) * 0.5f);
+  while ((t + 1) * (t + 2) / 2 <= k) t++;
+  while (t * (t + 1) / 2 > k) t--;
+  l = t;
+  m = k - t * (t + 1) / 2;
+}
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+__device__ inline double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+
+// ------------------------------------------------------------ Wigner tables
+
+__device__ double binom_exact(int n, int k) {
+  double c = 1.;
+  for (int i = 1; i <= k; i++) c = c * (double)(n - k + i) / (double)i;
+  return c;
+}
+
+__device__ double ipow(double x, int e) {
+  double r = 1.;
+  for (int i = 0; i < e; i++) r *= x;
+  return r;
+}
+
+// d^j_{a,b}(beta) for j = max(|a|,|b|) (closed forms of Wigner's sum with a
+// single term).
+__device__ double wigner_seed(int a, int b, double c, double s) {
+  int aa = abs(a), ab = abs(b);
+  if (aa >= ab) {
+    int j = aa;
+    double sq = sqrt(binom_exact(2 * j, j + b));
+    if (a == j) {
+      double v = sq * ipow(c, j + b) * ipow(s, j - b);
+      return ((j - b) & 1) ? -v : v;
+    }
+    return sq * ipow(c, j - b) * ipow(s, j + b);
+  }
+  int j = ab;
+  double sq = sqrt(binom_exact(2 * j, j + a));
+  if (b == j) return sq * ipow(c, j + a) * ipow(s, j - a);
+  double v = sq * ipow(c, j - a) * ipow(s, j + a);
+  return ((a + j) & 1) ? -v : v;
+}
+
+// One block per theta; (2L+1)^2 <= blockDim threads, one per (a, b).
+__global__ void k_wigner_tables(const double *__restrict__ thetas, int L,
+                                double *__restrict__ dscr,
+                                double *__restrict__ rot) {
+  const int w = 2 * L + 1;
+  const int sq = w * w;
+  const int th = blockIdx.x;
+  double *dth = dscr + (size_t)th * (L + 1) * sq;
+  const double beta = thetas[th];
+  const double cb = cos(beta), c = cos(0.5 * beta), s = sin(0.5 * beta);
+  int t = threadIdx.x;
+  if (t < sq) {
+    int a = t / w - L, b = t % w - L;
+    int j = max(abs(a), abs(b));
+    double dm2 = 0., dm1 = wigner_seed(a, b, c, s);
+    dth[(size_t)j * sq + t] = dm1;
+    for (int l = j + 1; l <= L; l++) {
+      double ll = (double)l * l;
+      double l1 = (double)(l - 1) * (l - 1);
+      double pre = l * (2. * l - 1.) / sqrt((ll - a * a) * (ll - b * b));
+      double mid = (l > 1) ? cb - (double)(a * b) / (l * (l - 1.)) : cb;
+      double back = (l > 1) ? sqrt((l1 - a * a) * (l1 - b * b)) /
+                                  ((l - 1.) * (2. * l - 1.))
+                            : 0.;
+      double dl = pre * (mid * dm1 - back * dm2);
+      dth[(size_t)l * sq + t] = dl;
+      dm2 = dm1;
+      dm1 = dl;
+    }
+  }
+  __syncthreads();
+  const int rb = rot_block(L);
+  double *R = rot + (size_t)th * 4 * rb;
+  for (int idx = threadIdx.x; idx < rb; idx += blockDim.x) {
+    int l = 0;
+    while (rot_ofs(l + 1) <= idx) l++;
+    int r = idx - rot_ofs(l);
+    int mp = r / (l + 1), m = r % (l + 1);
+    const double *dl = dth + (size_t)l * sq;
+#define DD(aa, bb) dl[((aa) + L) * w + ((bb) + L)]
+    double sgn = (m & 1) ? -1. : 1.;
+    double e0 = DD(m, mp), e1 = (m > 0) ? sgn * DD(-m, mp) : 0.;
+    double f0 = DD(mp, m), f1 = (m > 0) ? sgn * DD(mp, -m) : 0.;
+#undef DD
+    R[idx] = e0 + e1;
+    R[rb + idx] = e0 - e1;
+    R[2 * rb + idx] = f0 + f1;
+    R[3 * rb + idx] = f0 - f1;
+  }
+}
+
+hipError_t launch_wigner_tables(const double *thetas, int ntheta, int L,
+                                double *dscr, double *rot, hipStream_t s) {
+  int w = 2 * L + 1;
+  int nt = ((w * w + 63) / 64) * 64;
+  k_wigner_tables<<<ntheta, nt, 0, s>>>(thetas, L, dscr, rot);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ shared pieces
+
+constexpr int kMaxMc = kMaxOrder * (kMaxOrder + 1) / 2;        // 120
+constexpr int kMaxLc = (kMaxOrder + 1) * (kMaxOrder + 2) / 2;  // 136
+constexpr int kOpThreads = 64;
+
+
+
+// X[l][m'] = rotation (forward) of U over degrees 0..deg.
+__device__ inline void rotate_fwd(const double *__restrict__ R, int rb,
+                                  int deg, const double2 *U, double2 *X) {
+  int n = tri(deg + 1);
+  for (int k = threadIdx.x; k < n; k += kOpThreads) {
+    int l, mp;
+    lm_of(k, l, mp);
+    const double *ep = R + rot_ofs(l) + mp * (l + 1);
+    const double *em = ep + rb;
+    const double2 *u = U + tri(l);
+    double re = 0., im = 0.;
+    for (int m = 0; m <= l; m++) {
+      re += ep[m] * u[m].x;
+      im += em[m] * u[m].y;
+    }
+    X[k] = make_double2(re, im);
+  }
+}
+
+// out[k] (+)= e^{-i m' phi} * (inverse rotation of Z), thread-strided slots
+__device__ inline double2 rotate_inv_one(const double *__restrict__ R, int rb,
+                                         const double2 *__restrict__ ph,
+                                         const double2 *Z, int k) {
+  int l, mp;
+  lm_of(k, l, mp);
+  const double *fp = R + 2 * rb + rot_ofs(l) + mp * (l + 1);
+  const double *fm = fp + rb;
+  const double2 *z = Z + tri(l);
+  double re = 0., im = 0.;
+  for (int m = 0; m <= l; m++) {
+    re += fp[m] * z[m].x;
+    im += fm[m] * z[m].y;
+  }
+  return cmulc(ph[mp], make_double2(re, im));
+}
+
+// ------------------------------------------------------------ P2M
+
+// One block per leaf: M_l^m = sum_j q_j (4pi/(2l+1)) r^l N_l^m P_l^m e^{-im phi}
+__global__ __launch_bounds__(kOpThreads) void k_p2m(
+    OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
+    const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ zs, const double *__restrict__ qs,
+    double2 *__restrict__ M) {
+  __shared__ double sr[kOpThreads], sx[kOpThreads], ss[kOpThreads],
+      scp[kOpThreads], ssp[kOpThreads], sq[kOpThreads];
+  const int p = tb.p;
+  const int mc = tri(p);
+  const int A = leaves[blockIdx.x];
+  const int b = nd.pbegin[A], e = nd.pend[A];
+  const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
+  double2 acc[2] = {make_double2(0., 0.), make_double2(0., 0.)};
+  for (int c0 = b; c0 < e; c0 += kOpThreads) {
+    int j = c0 + threadIdx.x;
+    if (j < e) {
+      double vx = xs[j] - cx, vy = ys[j] - cy, vz = zs[j] - cz;
+      double rho2 = vx * vx + vy * vy;
+      double r = sqrt(rho2 + vz * vz), rho = sqrt(rho2);
+      sr[threadIdx.x] = r;
+      sx[threadIdx.x] = (r > 0.) ? vz / r : 1.;
+      ss[threadIdx.x] = (r > 0.) ? rho / r : 0.;
+      scp[threadIdx.x] = (rho > 0.) ? vx / rho : 1.;
+      ssp[threadIdx.x] = (rho > 0.) ? vy / rho : 0.;
+      sq[threadIdx.x] = qs[j];
+    }
+    __syncthreads();
+    int cnt = min(kOpThreads, e - c0);
+    for (int slot = 0; slot < 2; slot++) {
+      int k = threadIdx.x + slot * kOpThreads;
+      if (k >= mc) break;
+      int l, m;
+      lm_of(k, l, m);
+      double f = 4. * M_PI / (l + l + 1.) * tb.ynorm[k];
+      double2 a = acc[slot];
+      for (int jj = 0; jj < cnt; jj++) {
+        double x = sx[jj], s = ss[jj];
+        // P_m^m, then the three-term recurrence up to l
+        double pmm = 1.;
+        for (int i = 1; i <= m; i++) pmm = -(2. * i - 1.) * s * pmm;
+        double p2 = 0., p1 = pmm;
+        for (int i = m + 1; i <= l; i++) {
+          double pn = (i == m + 1)
+                          ? (2. * m + 1.) * x * pmm
+                          : ((2. * i - 1.) * x * p1 - (i + m - 1.) * p2) / (i - m);
+          p2 = p1;
+          p1 = pn;
+        }
+        double rl = 1.;
+        for (int i = 0; i < l; i++) rl *= sr[jj];
+        double2 e1 = make_double2(1., 0.);
+        double2 ep = make_double2(scp[jj], ssp[jj]);
+        for (int i = 0; i < m; i++) e1 = cmul(e1, ep);
+        double t = sq[jj] * f * rl * p1;
+        a.x += t * e1.x;
+        a.y -= t * e1.y;
+      }
+      acc[slot] = a;
+    }
+    __syncthreads();
+  }
+  for (int slot = 0; slot < 2; slot++) {
+    int k = threadIdx.x + slot * kOpThreads;
+    if (k < mc) M[(size_t)A * mc + k] = acc[slot];
+  }
+}
+
+// ------------------------------------------------------------ M2M
+
+// One block per parent node of one level: M(P) = sum over children.
+__global__ __launch_bounds__(kOpThreads) void k_m2m(
+    OpTables tb, NodeArrays nd, int lvl_begin, double width,
+    double2 *__restrict__ M) {
+  __shared__ double2 U[kMaxMc], X[kMaxMc], Z[kMaxMc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  const int P = lvl_begin + blockIdx.x;
+  if (nd.leaf[P]) return;
+  const int p = tb.p, mc = tri(p), rb = tb.rb;
+  const int d = nd.depth[P] + 1;
+  const double half = ldexp(width, -d) * 0.5;
+  double2 acc[2] = {make_double2(0., 0.), make_double2(0., 0.)};
+  const int cb = nd.child[P], ce = cb + nd.nchild[P];
+  for (int C = cb; C < ce; C++) {
+    int bx = nd.ix[C] & 1, by = nd.iy[C] & 1, bz = nd.iz[C] & 1;
+    int code = code_of(1 - 2 * bx, 1 - 2 * by, 1 - 2 * bz);
+    DirEntry de = tb.dir[code];
+    const double2 *ph = tb.eiphi + code * (p + 1);
+    const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+    for (int k = threadIdx.x; k < mc; k += kOpThreads) {
+      int l, m;
+      lm_of(k, l, m);
+      U[k] = cmul(ph[m], M[(size_t)C * mc + k]);
+    }
+    if (threadIdx.x < 2 * p + 2) {
+      double r = de.norm * half;
+      double v = 1.;
+      for (int i = 0; i < (int)threadIdx.x; i++) v *= -de.c * r;
+      rp[threadIdx.x] = v;  // ((-c) r)^k
+    }
+    __syncthreads();
+    rotate_fwd(R, rb, p - 1, U, X);
+    __syncthreads();
+    for (int k = threadIdx.x; k < mc; k += kOpThreads) {
+      int l, m;
+      lm_of(k, l, m);
+      const double *T = tb.tm2m + ((size_t)l * p + m) * p;
+      double re = 0., im = 0.;
+      for (int n = m; n <= l; n++) {
+        double f = T[n] * rp[l - n];
+        re += f * X[tri(n) + m].x;
+        im += f * X[tri(n) + m].y;
+      }
+      Z[k] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int slot = 0; slot < 2; slot++) {
+      int k = threadIdx.x + slot * kOpThreads;
+      if (k < mc) {
+        double2 v = rotate_inv_one(R, rb, ph, Z, k);
+        acc[slot].x += v.x;
+        acc[slot].y += v.y;
+      }
+    }
+    __syncthreads();
+  }
+  for (int slot = 0; slot < 2; slot++) {
+    int k = threadIdx.x + slot * kOpThreads;
+    if (k < mc) M[(size_t)P * mc + k] = acc[slot];
+  }
+}
+
+// ------------------------------------------------------------ M2L
+
+// One block per target node: L(A) = sum over its far list of
+// rotate -> vertical M2L -> rotate back.  Writes L(A) (zero when the list
+// is empty).
+__global__ __launch_bounds__(kOpThreads) void k_m2l(
+    OpTables tb, NodeArrays nd, const int *__restrict__ vofs,
+    const int *__restrict__ vsrc, const short *__restrict__ vcode,
+    double width, const double2 *__restrict__ M, double2 *__restrict__ L) {
+  __shared__ double2 U[kMaxMc], X[kMaxMc], Z[kMaxLc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  const int A = blockIdx.x;
+  const int p = tb.p, mc = tri(p), lc = tri(p + 1), rb = tb.rb;
+  const double w = ldexp(width, -nd.depth[A]);
+  double2 acc[3] = {make_double2(0., 0.), make_double2(0., 0.),
+                    make_double2(0., 0.)};
+  const int e0 = vofs[A], e1 = vofs[A + 1];
+  for (int e = e0; e < e1; e++) {
+    const int B = vsrc[e];
+    const int code = vcode[e];
+    DirEntry de = tb.dir[code];
+    const double2 *ph = tb.eiphi + code * (p + 1);
+    const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+    for (int k = threadIdx.x; k < mc; k += kOpThreads) {
+      int l, m;
+      lm_of(k, l, m);
+      U[k] = cmul(ph[m], M[(size_t)B * mc + k]);
+    }
+    if (threadIdx.x < 2 * p + 2) {
+      double rho = 1. / (de.norm * w);
+      int kk = threadIdx.x;
+      double v = 1.;
+      for (int i = 0; i < kk; i++) v *= rho;
+      if (kk > 0 && ((kk - 1) & 1)) v *= de.c;  // c^{l+n} with kk = l+n+1
+      rp[kk] = v;
+    }
+    __syncthreads();
+    rotate_fwd(R, rb, p - 1, U, X);
+    __syncthreads();
+    for (int k = threadIdx.x; k < lc; k += kOpThreads) {
+      int l, m;
+      lm_of(k, l, m);
+      const double *T = tb.tm2l + ((size_t)l * (p + 1) + m) * p;
+      double re = 0., im = 0.;
+      for (int n = m; n < p; n++) {
+        double f = T[n] * rp[l + n + 1];
+        re += f * X[tri(n) + m].x;
+        im += f * X[tri(n) + m].y;
+      }
+      if ((l + m) & 1) {
+        re = -re;
+        im = -im;
+      }
+      Z[k] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int slot = 0; slot < 3; slot++) {
+      int k = threadIdx.x + slot * kOpThreads;
+      if (k < lc) {
+        double2 v = rotate_inv_one(R, rb, ph, Z, k);
+        acc[slot].x += v.x;
+        acc[slot].y += v.y;
+      }
+    }
+    __syncthreads();
+  }
+  for (int slot = 0; slot < 3; slot++) {
+    int k = threadIdx.x + slot * kOpThreads;
+    if (k < lc) L[(size_t)A * lc + k] = acc[slot];
+  }
+}
+
+// ------------------------------------------------------------ L2L
+
+// One block per node of one level (depth >= 1): L(A) += T L(parent).
+__global__ __launch_bounds__(kOpThreads) void k_l2l(
+    OpTables tb, NodeArrays nd, int lvl_begin, double width,
+    double2 *__restrict__ L) {
+  __shared__ double2 U[kMaxLc], X[kMaxLc], Z[kMaxLc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  const int A = lvl_begin + blockIdx.x;
+  const int P = nd.parent[A];
+  const int p = tb.p, lc = tri(p + 1), rb = tb.rb;
+  const double half = ldexp(width, -nd.depth[A]) * 0.5;
+  int bx = nd.ix[A] & 1, by = nd.iy[A] & 1, bz = nd.iz[A] & 1;
+  int code = code_of(2 * bx - 1, 2 * by - 1, 2 * bz - 1);
+  DirEntry de = tb.dir[code];
+  const double2 *ph = tb.eiphi + code * (p + 1);
+  const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
+    int l, m;
+    lm_of(k, l, m);
+    U[k] = cmul(ph[m], L[(size_t)P * lc + k]);
+  }
+  if (threadIdx.x < 2 * p + 2) {
+    double r = de.norm * half;
+    double v = 1.;
+    for (int i = 0; i < (int)threadIdx.x; i++) v *= de.c * r;
+    rp[threadIdx.x] = v;  // (c r)^k
+  }
+  __syncthreads();
+  rotate_fwd(R, rb, p, U, X);
+  __syncthreads();
+  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
+    int l, m;
+    lm_of(k, l, m);
+    const double *T = tb.tl2l + ((size_t)l * (p + 1) + m) * (p + 1);
+    double re = 0., im = 0.;
+    for (int n = l; n <= p; n++) {
+      double f = T[n] * rp[n - l];
+      re += f * X[tri(n) + m].x;
+      im += f * X[tri(n) + m].y;
+    }
+    Z[k] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
+    double2 v = rotate_inv_one(R, rb, ph, Z, k);
+    double2 o = L[(size_t)A * lc + k];
+    L[(size_t)A * lc + k] = make_double2(o.x + v.x, o.y + v.y);
+  }
+}
+
+// ------------------------------------------------------------ L2P
+
+// One block per leaf, one thread per particle: potential and gradient of
+// the leaf's local expansion, accumulated into pot / grad (sorted order).
+__global__ __launch_bounds__(kOpThreads) void k_l2p(
+    OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
+    const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ zs, const double2 *__restrict__ L,
+    double *__restrict__ pot, double *__restrict__ grad) {
+  __shared__ double2 Ls[kMaxLc];
+  const int p = tb.p, lc = tri(p + 1);
+  const int A = leaves[blockIdx.x];
+  for (int k = threadIdx.x; k < lc; k += kOpThreads)
+    Ls[k] = L[(size_t)A * lc + k];
+  __syncthreads();
+  const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
+  const int b = nd.pbegin[A], e = nd.pend[A];
+  for (int i = b + threadIdx.x; i < e; i += kOpThreads) {
+    double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
+    double rho2 = vx * vx + vy * vy;
+    double r = sqrt(rho2 + vz * vz), rho = sqrt(rho2);
+    double x = (r > 0.) ? vz / r : 1., s = (r > 0.) ? rho / r : 0.;
+    double cp = (rho > 0.) ? vx / rho : 1., sp = (rho > 0.) ? vy / rho : 0.;
+    double phi = 0., dr = 0., dt = 0., dp = 0.;
+    double pmm = 1., smm = 0., rm = 1., rmm1 = 0.;
+    double2 eim = make_double2(1., 0.);
+    for (int m = 0; m <= p; m++) {
+      double wgt = (m == 0) ? 1. : 2.;
+      double p1 = 0., p2 = 0., s1 = 0., s2 = 0.;
+      double rl = rm, rlm1 = rmm1;
+      for (int l = m; l <= p; l++) {
+        double P, S;
+        if (l == m) {
+          P = pmm;
+          S = smm;
+        } else if (l == m + 1) {
+          P = (2. * m + 1.) * x * pmm;
+          S = (2. * m + 1.) * x * smm;
+        } else {
+          P = ((2. * l - 1.) * x * p1 - (l + m - 1.) * p2) / (l - m);
+          S = ((2. * l - 1.) * x * s1 - (l + m - 1.) * s2) / (l - m);
+        }
+        double2 z = cmul(Ls[tri(l) + m], eim);
+        double nlm = tb.ynorm[tri(l) + m];
+        double np = nlm * P;
+        phi += wgt * rl * np * z.x;
+        if (l >= 1) dr += wgt * l * rlm1 * np * z.x;
+        if (m >= 1) {
+          double dpdt = l * x * S - (l + m) * s1;
+          dt += wgt * rlm1 * nlm * dpdt * z.x;
+          dp -= wgt * m * rlm1 * nlm * S * z.y;
+        }
+        if (m == 1) {
+          // theta derivative of the m = 0 column: dP_l^0/dtheta = P_l^1
+          dt += rlm1 * tb.ynorm[tri(l)] * P * Ls[tri(l)].x;
+        }
+        p2 = p1;
+        p1 = P;
+        s2 = s1;
+        s1 = S;
+        rlm1 = rl;
+        rl *= r;
+      }
+      smm = -(2. * m + 1.) * pmm;
+      pmm = -(2. * m + 1.) * s * pmm;
+      rmm1 = rm;
+      rm *= r;
+      eim = cmul(eim, make_double2(cp, sp));
+    }
+    pot[i] += phi;
+    grad[3 * (size_t)i + 0] += s * cp * dr + x * cp * dt - sp * dp;
+    grad[3 * (size_t)i + 1] += s * sp * dr + x * sp * dt + cp * dp;
+    grad[3 * (size_t)i + 2] += x * dr - s * dt;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+
+hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
+                      const int *leaves, int nleaves, const double *xs,
+                      const double *ys, const double *zs, const double *qs,
+                      double2 *M, hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  k_p2m<<<nleaves, kOpThreads, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
+                      int lvl_count, double width, double2 *M, hipStream_t s) {
+  if (lvl_count == 0) return hipSuccess;
+  k_m2m<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, width, M);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
+                      const int *vofs, const int *vsrc, const short *vcode,
+                      double width, const double2 *M, double2 *L,
+                      hipStream_t s) {
+  if (nnodes == 0) return hipSuccess;
+  k_m2l<<<nnodes, kOpThreads, 0, s>>>(tb, nd, vofs, vsrc, vcode, width, M, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
+                      int lvl_count, double width, double2 *L, hipStream_t s) {
+  if (lvl_count == 0) return hipSuccess;
+  k_l2l<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, width, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
+                      const int *leaves, int nleaves, const double *xs,
+                      const double *ys, const double *zs, const double2 *L,
+                      double *pot, double *grad, hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  k_l2p<<<nleaves, kOpThreads, 0, s>>>(tb, nd, leaves, xs, ys, zs, L, pot,
+                                       grad);
+  return hipGetLastError();
+}
+
+}  // namespace aran
+
+namespace aran {
+
+// ------------------------------------------------------------ one operator
+
+// One translation (kind 0: M2M, 1: M2L, 2: L2L) along the single direction
+// stored at code 0 of tb, for the development API.  dst is overwritten.
+__global__ __launch_bounds__(kOpThreads) void k_translate_one(
+    int kind, OpTables tb, double r, const double2 *__restrict__ src,
+    double2 *__restrict__ dst) {
+  __shared__ double2 U[kMaxLc], X[kMaxLc], Z[kMaxLc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  const int p = tb.p, mc = tri(p), lc = tri(p + 1), rb = tb.rb;
+  const int nin = (kind == 2) ? lc : mc;
+  const int nout = (kind == 0) ? mc : lc;
+  const int din = (kind == 2) ? p : p - 1;
+  const int dout = (kind == 0) ? p - 1 : p;
+  DirEntry de = tb.dir[0];
+  const double2 *ph = tb.eiphi;
+  const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+  for (int k = threadIdx.x; k < nin; k += kOpThreads) {
+    int l, m;
+    lm_of(k, l, m);
+    U[k] = cmul(ph[m], src[k]);
+  }
+  if (threadIdx.x < 2 * p + 2) {
+    int kk = threadIdx.x;
+    double v = 1.;
+    if (kind == 1) {
+      for (int i = 0; i < kk; i++) v /= r;
+      if (kk > 0 && ((kk - 1) & 1)) v *= de.c;
+    } else {
+      double f = (kind == 0) ? -de.c * r : de.c * r;
+      for (int i = 0; i < kk; i++) v *= f;
+    }
+    rp[kk] = v;
+  }
+  __syncthreads();
+  rotate_fwd(R, rb, din, U, X);
+  __syncthreads();
+  for (int k = threadIdx.x; k < nout; k += kOpThreads) {
+    int l, m;
+    lm_of(k, l, m);
+    double re = 0., im = 0.;
+    if (kind == 0) {
+      const double *T = tb.tm2m + ((size_t)l * p + m) * p;
+      for (int n = m; n <= l; n++) {
+        double f = T[n] * rp[l - n];
+        re += f * X[tri(n) + m].x;
+        im += f * X[tri(n) + m].y;
+      }
+    } else if (kind == 1) {
+      const double *T = tb.tm2l + ((size_t)l * (p + 1) + m) * p;
+      for (int n = m; n < p; n++) {
+        double f = T[n] * rp[l + n + 1];
+        re += f * X[tri(n) + m].x;
+        im += f * X[tri(n) + m].y;
+      }
+      if ((l + m) & 1) {
+        re = -re;
+        im = -im;
+      }
+    } else {
+      const double *T = tb.tl2l + ((size_t)l * (p + 1) + m) * (p + 1);
+      for (int n = l; n <= p; n++) {
+        double f = T[n] * rp[n - l];
+        re += f * X[tri(n) + m].x;
+        im += f * X[tri(n) + m].y;
+      }
+    }
+    Z[k] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nout; k += kOpThreads)
+    dst[k] = rotate_inv_one(R, rb, ph, Z, k);
+  (void)dout;
+}
+
+hipError_t launch_translate_one(int kind, const OpTables &tb, double r,
+                                const double2 *src, double2 *dst,
+                                hipStream_t s) {
+  k_translate_one<<<1, kOpThreads, 0, s>>>(kind, tb, r, src, dst);
+  return hipGetLastError();
+}
+
+}  // namespace aran

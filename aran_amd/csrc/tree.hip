@@ -1,0 +1,655 @@
+// tree.hip -- Morton keys, LSD radix sort, level-synchronous octree build
+// and the device-side interaction tables (M2L and P2P lists).
+//
+// Replaces the particle filtering of LibVsg's VsgPRTree3d
+// (aran/aransolver3d.c:972-978) and its near/far traversal
+// (aran/aransolver3d.c:1189-1190) for the FMM hot path.  The tree rules are
+// spelled out in tree_rules.h and are shared with the FP64 oracle.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "engine.h"
+#include "tree_rules.h"
+
+namespace aran {
+
+// ------------------------------------------------------------ min / max
+
+__global__ void k_minmax_partial(const double *__restrict__ x,
+                                 const double *__restrict__ y,
+                                 const double *__restrict__ z, long n,
+                                 double *__restrict__ part) {
+  __shared__ double red[6][256];
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
+                 -INFINITY};
+  bool bad = false;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n;
+       i += (long)gridDim.x * 256L) {
+    double a = x[i], b = y[i], c = z[i];
+    bad |= !(isfinite(a) && isfinite(b) && isfinite(c));
+    v[0] = fmin(v[0], a);
+    v[1] = fmin(v[1], b);
+    v[2] = fmin(v[2], c);
+    v[3] = fmax(v[3], a);
+    v[4] = fmax(v[4], b);
+    v[5] = fmax(v[5], c);
+  }
+  if (bad) {
+    v[0] = NAN;
+  }
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      for (int k = 0; k < 3; k++) {
+        double a = red[k][threadIdx.x], b = red[k][threadIdx.x + s];
+        red[k][threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmin(a, b);
+      }
+      for (int k = 3; k < 6; k++)
+        red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void k_minmax_final(const double *__restrict__ part, int nb,
+                               double *__restrict__ out) {
+  if (threadIdx.x != 0) return;
+  double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
+                 -INFINITY};
+  bool bad = false;
+  for (int b = 0; b < nb; b++) {
+    for (int k = 0; k < 3; k++) {
+      double a = part[6 * b + k];
+      bad |= isnan(a);
+      v[k] = fmin(v[k], a);
+    }
+    for (int k = 3; k < 6; k++) v[k] = fmax(v[k], part[6 * b + k]);
+  }
+  for (int k = 0; k < 6; k++) out[k] = v[k];
+  if (bad) out[0] = NAN;
+}
+
+hipError_t launch_minmax(const double *x, const double *y, const double *z,
+                         long n, double *out6, double *part, hipStream_t s) {
+  int nb = 256;
+  k_minmax_partial<<<nb, 256, 0, s>>>(x, y, z, n, part);
+  k_minmax_final<<<1, 64, 0, s>>>(part, nb, out6);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ keys
+
+__global__ void k_keys(const double *__restrict__ x,
+                       const double *__restrict__ y,
+                       const double *__restrict__ z, long n, double lo0,
+                       double lo1, double lo2, double scale,
+                       uint64_t *__restrict__ keys, int *__restrict__ idx) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  keys[i] = morton_key(icoord(x[i], lo0, scale), icoord(y[i], lo1, scale),
+                       icoord(z[i], lo2, scale));
+  idx[i] = (int)i;
+}
+
+hipError_t launch_keys(const double *x, const double *y, const double *z,
+                       long n, double lo0, double lo1, double lo2,
+                       double scale, uint64_t *keys, int *idx, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_keys<<<(n + 255) / 256, 256, 0, s>>>(x, y, z, n, lo0, lo1, lo2, scale,
+                                         keys, idx);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ scan
+
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+// Block-wide exclusive scan of one tile; writes the tile total to sums.
+__global__ __launch_bounds__(kScanThreads) void k_scan_tile(
+    const int *__restrict__ in, int *__restrict__ out, long n,
+    int *__restrict__ sums) {
+  __shared__ int sh[kScanThreads];
+  long base = blockIdx.x * (long)kScanTile + threadIdx.x * kScanItems;
+  int v[kScanItems];
+  int tsum = 0;
+  for (int k = 0; k < kScanItems; k++) {
+    long i = base + k;
+    v[k] = (i < n) ? in[i] : 0;
+    tsum += v[k];
+  }
+  sh[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    int t = ((int)threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
+    __syncthreads();
+    sh[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int run = sh[threadIdx.x] - tsum;
+  for (int k = 0; k < kScanItems; k++) {
+    long i = base + k;
+    if (i < n) out[i] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_scan_add(int *__restrict__ out, long n,
+                           const int *__restrict__ offs) {
+  long i = blockIdx.x * (long)kScanTile + threadIdx.x;
+  int o = offs[blockIdx.x];
+  for (int k = 0; k < kScanItems; k++, i += kScanThreads)
+    if (i < n) out[i] += o;
+}
+
+__global__ void k_scan_total(const int *__restrict__ in,
+                             const int *__restrict__ out, long n,
+                             int *__restrict__ total) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *total = (n > 0) ? out[n - 1] + in[n - 1] : 0;
+}
+
+static size_t scan_tmp_ints(long n) {
+  size_t t = 0;
+  while (n > 1) {
+    n = (n + kScanTile - 1) / kScanTile;
+    t += n + 1;
+  }
+  return t + 2;
+}
+
+static hipError_t scan_rec(const int *in, int *out, long n, int *tmp,
+                           hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  long nb = (n + kScanTile - 1) / kScanTile;
+  int *sums = tmp;
+  int *rest = tmp + nb + 1;
+  k_scan_tile<<<nb, kScanThreads, 0, s>>>(in, out, n, sums);
+  if (nb > 1) {
+    hipError_t e = scan_rec(sums, sums, nb, rest, s);
+    if (e != hipSuccess) return e;
+    k_scan_add<<<nb, kScanThreads, 0, s>>>(out, n, sums);
+  }
+  return hipGetLastError();
+}
+
+// Exclusive scan; writes the grand total to *total (device).
+hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
+                          DevBuf &tmp, hipStream_t s) {
+  hipError_t e = tmp.reserve(scan_tmp_ints(n) * sizeof(int) * 2);
+  if (e != hipSuccess) return e;
+  if (n > 0) {
+    // in and out may alias: the total needs the last input first
+    int *last = tmp.as<int>() + scan_tmp_ints(n) * 2 - 1;
+    e = hipMemcpyAsync(last, in + n - 1, sizeof(int), hipMemcpyDeviceToDevice,
+                       s);
+    if (e != hipSuccess) return e;
+    e = scan_rec(in, out, n, tmp.as<int>(), s);
+    if (e != hipSuccess) return e;
+    k_scan_total<<<1, 64, 0, s>>>(last, out + n - 1, 1, total);
+  } else {
+    e = hipMemsetAsync(total, 0, sizeof(int), s);
+  }
+  return e == hipSuccess ? hipGetLastError() : e;
+}
+
+// ------------------------------------------------------------ radix sort
+
+constexpr int kRsThreads = 256;
+constexpr int kRsWaves = kRsThreads / 64;
+constexpr int kRsItems = 16;
+constexpr int kRsTile = kRsThreads * kRsItems;
+
+__global__ __launch_bounds__(kRsThreads) void k_rs_hist(
+    const uint64_t *__restrict__ keys, long n, int shift,
+    int *__restrict__ hist, int nblocks) {
+  __shared__ int cnt[256];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  long base = blockIdx.x * (long)kRsTile;
+  for (int k = 0; k < kRsItems; k++) {
+    long i = base + k * kRsThreads + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 255], 1);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: elements keep their input order inside each digit.
+__global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
+    const uint64_t *__restrict__ kin, const int *__restrict__ vin,
+    uint64_t *__restrict__ kout, int *__restrict__ vout, long n, int shift,
+    const int *__restrict__ offs, int nblocks) {
+  __shared__ int base[256];
+  __shared__ int wcnt[kRsWaves][256];
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  base[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  for (int w = 0; w < kRsWaves; w++) wcnt[w][threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  long tile = blockIdx.x * (long)kRsTile;
+  for (int k = 0; k < kRsItems; k++) {
+    long i = tile + k * kRsThreads + threadIdx.x;
+    bool valid = i < n;
+    uint64_t key = valid ? kin[i] : 0;
+    int val = valid ? vin[i] : 0;
+    int d = (int)((key >> shift) & 255);
+    uint64_t peers = __ballot(valid);
+    for (int b = 0; b < 8; b++) {
+      uint64_t m = __ballot(valid && ((d >> b) & 1));
+      peers &= ((d >> b) & 1) ? m : ~m;
+    }
+    int rank = __popcll(peers & lt);
+    int cnt = __popcll(peers);
+    if (valid && rank == 0) wcnt[wave][d] = cnt;
+    __syncthreads();
+    if (valid) {
+      int pos = base[d] + rank;
+      for (int w = 0; w < wave; w++) pos += wcnt[w][d];
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    int tot = 0;
+    for (int w = 0; w < kRsWaves; w++) {
+      tot += wcnt[w][threadIdx.x];
+      wcnt[w][threadIdx.x] = 0;
+    }
+    base[threadIdx.x] += tot;
+    __syncthreads();
+  }
+}
+
+// Sorts (keys, idx) by key; ties keep the input order.  The result ends in
+// keys / idx (8 passes, an even count).
+hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
+                      int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
+                      int *total, hipStream_t s) {
+  if (n <= 1) return hipSuccess;
+  int nb = (int)((n + kRsTile - 1) / kRsTile);
+  hipError_t e = hist.reserve(sizeof(int) * 256 * (size_t)nb * 2);
+  if (e != hipSuccess) return e;
+  int *h = hist.as<int>();
+  int *o = h + 256 * (size_t)nb;
+  uint64_t *ka = keys, *kb = keys_tmp;
+  int *va = idx, *vb = idx_tmp;
+  for (int pass = 0; pass < 8; pass++) {
+    int shift = 8 * pass;
+    k_rs_hist<<<nb, kRsThreads, 0, s>>>(ka, n, shift, h, nb);
+    e = scan_exclusive(h, o, 256L * nb, total, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    k_rs_scatter<<<nb, kRsThreads, 0, s>>>(ka, va, kb, vb, n, shift, o, nb);
+    uint64_t *tk = ka;
+    ka = kb;
+    kb = tk;
+    int *tv = va;
+    va = vb;
+    vb = tv;
+  }
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ gather / scatter
+
+__global__ void k_gather(const int *__restrict__ idx, long n,
+                         const double *__restrict__ x,
+                         const double *__restrict__ y,
+                         const double *__restrict__ z,
+                         const double *__restrict__ q, double *__restrict__ xs,
+                         double *__restrict__ ys, double *__restrict__ zs,
+                         double *__restrict__ qs) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int j = idx[i];
+  xs[i] = x[j];
+  ys[i] = y[j];
+  zs[i] = z[j];
+  qs[i] = q[j];
+}
+
+hipError_t launch_gather(const int *idx, long n, const double *x,
+                         const double *y, const double *z, const double *q,
+                         double *xs, double *ys, double *zs, double *qs,
+                         hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_gather<<<(n + 255) / 256, 256, 0, s>>>(idx, n, x, y, z, q, xs, ys, zs,
+                                           qs);
+  return hipGetLastError();
+}
+
+__global__ void k_scatter(const int *__restrict__ idx, long n,
+                          const double *__restrict__ pot_s,
+                          const double *__restrict__ grad_s,
+                          double *__restrict__ pot, double *__restrict__ grad) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int j = idx[i];
+  if (pot) pot[j] = pot_s[i];
+  if (grad) {
+    grad[3L * j + 0] = grad_s[3 * i + 0];
+    grad[3L * j + 1] = grad_s[3 * i + 1];
+    grad[3L * j + 2] = grad_s[3 * i + 2];
+  }
+}
+
+hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
+                          const double *grad_s, double *pot, double *grad,
+                          hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_scatter<<<(n + 255) / 256, 256, 0, s>>>(idx, n, pot_s, grad_s, pot, grad);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ level build
+
+__global__ void k_root(NodeArrays nd, long n, double lo0, double lo1,
+                       double lo2, double width) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  nd.depth[0] = 0;
+  nd.ix[0] = nd.iy[0] = nd.iz[0] = 0;
+  nd.key[0] = 0;
+  nd.pbegin[0] = 0;
+  nd.pend[0] = (int)n;
+  nd.parent[0] = -1;
+  nd.child[0] = -1;
+  nd.nchild[0] = 0;
+  nd.leaf[0] = 1;
+  double h = width;
+  nd.cx[0] = fma(0.5, h, lo0);
+  nd.cy[0] = fma(0.5, h, lo1);
+  nd.cz[0] = fma(0.5, h, lo2);
+}
+
+hipError_t launch_root(const NodeArrays &nd, long n, const double *lo,
+                       double width, hipStream_t s) {
+  k_root<<<1, 64, 0, s>>>(nd, n, lo[0], lo[1], lo[2], width);
+  return hipGetLastError();
+}
+
+// First index in [b, e) whose octant digit at 'shift' is >= o.
+__device__ inline int digit_lower_bound(const uint64_t *__restrict__ keys,
+                                        int b, int e, int shift, int o) {
+  while (b < e) {
+    int mid = b + ((e - b) >> 1);
+    if ((int)((keys[mid] >> shift) & 7) < o)
+      b = mid + 1;
+    else
+      e = mid;
+  }
+  return b;
+}
+
+__global__ void k_split_count(NodeArrays nd, int lvl_begin, int lvl_count,
+                              const uint64_t *__restrict__ keys, int leafcap,
+                              int *__restrict__ bounds, int *__restrict__ cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= lvl_count) return;
+  int node = lvl_begin + i;
+  int b = nd.pbegin[node], e = nd.pend[node], d = nd.depth[node];
+  int c = 0;
+  if (e - b > leafcap && d < kMaxDepth) {
+    int shift = 3 * (kMaxDepth - d - 1);
+    int prev = b;
+    bounds[9 * i] = b;
+    for (int o = 1; o <= 8; o++) {
+      int x = (o < 8) ? digit_lower_bound(keys, prev, e, shift, o) : e;
+      bounds[9 * i + o] = x;
+      c += (x > prev);
+      prev = x;
+    }
+  }
+  cnt[i] = c;
+}
+
+__global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
+                              int next_begin, const int *__restrict__ bounds,
+                              const int *__restrict__ cnt,
+                              const int *__restrict__ ofs, double lo0,
+                              double lo1, double lo2, double width) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= lvl_count) return;
+  int node = lvl_begin + i;
+  if (cnt[i] == 0) {
+    nd.leaf[node] = 1;
+    nd.child[node] = -1;
+    nd.nchild[node] = 0;
+    return;
+  }
+  int first = next_begin + ofs[i];
+  nd.leaf[node] = 0;
+  nd.child[node] = first;
+  nd.nchild[node] = cnt[i];
+  int d = nd.depth[node] + 1;
+  double h = ldexp(width, -d);
+  int k = 0;
+  for (int o = 0; o < 8; o++) {
+    int b = bounds[9 * i + o], e = bounds[9 * i + o + 1];
+    if (e <= b) continue;
+    int c = first + k++;
+    int ix = 2 * nd.ix[node] + (o & 1);
+    int iy = 2 * nd.iy[node] + ((o >> 1) & 1);
+    int iz = 2 * nd.iz[node] + ((o >> 2) & 1);
+    nd.depth[c] = d;
+    nd.ix[c] = ix;
+    nd.iy[c] = iy;
+    nd.iz[c] = iz;
+    nd.key[c] = (nd.key[node] << 3) | (uint64_t)o;
+    nd.pbegin[c] = b;
+    nd.pend[c] = e;
+    nd.parent[c] = node;
+    nd.child[c] = -1;
+    nd.nchild[c] = 0;
+    nd.leaf[c] = 1;
+    nd.cx[c] = fma(ix + 0.5, h, lo0);
+    nd.cy[c] = fma(iy + 0.5, h, lo1);
+    nd.cz[c] = fma(iz + 0.5, h, lo2);
+  }
+}
+
+hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
+                              int lvl_count, const uint64_t *keys, int leafcap,
+                              int *bounds, int *cnt, hipStream_t s) {
+  if (lvl_count == 0) return hipSuccess;
+  k_split_count<<<(lvl_count + 127) / 128, 128, 0, s>>>(
+      nd, lvl_begin, lvl_count, keys, leafcap, bounds, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
+                              int lvl_count, int next_begin, const int *bounds,
+                              const int *cnt, const int *ofs, const double *lo,
+                              double width, hipStream_t s) {
+  if (lvl_count == 0) return hipSuccess;
+  k_split_write<<<(lvl_count + 127) / 128, 128, 0, s>>>(
+      nd, lvl_begin, lvl_count, next_begin, bounds, cnt, ofs, lo[0], lo[1],
+      lo[2], width);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ leaves
+
+__global__ void k_leaf_flags(NodeArrays nd, int nnodes, int *__restrict__ flag) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnodes) flag[i] = nd.leaf[i];
+}
+
+__global__ void k_leaf_compact(NodeArrays nd, int nnodes,
+                               const int *__restrict__ pos,
+                               int *__restrict__ leaves) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nnodes && nd.leaf[i]) leaves[pos[i]] = i;
+}
+
+hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
+                             hipStream_t s) {
+  k_leaf_flags<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
+                               const int *pos, int *leaves, hipStream_t s) {
+  k_leaf_compact<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, pos, leaves);
+  return hipGetLastError();
+}
+
+// ------------------------------------------------------------ lists
+
+// Node of depth d at integer coordinates (ix, iy, iz), or -1.  Nodes of one
+// level are contiguous and sorted by their Morton key.
+__device__ inline int lookup(const NodeArrays &nd,
+                             const int *__restrict__ lstart,
+                             const int *__restrict__ lcount, int d, int ix,
+                             int iy, int iz) {
+  int lim = 1 << d;
+  if (ix < 0 || iy < 0 || iz < 0 || ix >= lim || iy >= lim || iz >= lim)
+    return -1;
+  uint64_t key = morton_key((uint64_t)ix, (uint64_t)iy, (uint64_t)iz);
+  int b = lstart[d], e = lstart[d] + lcount[d];
+  while (b < e) {
+    int mid = b + ((e - b) >> 1);
+    if (nd.key[mid] < key)
+      b = mid + 1;
+    else
+      e = mid;
+  }
+  return (b < lstart[d] + lcount[d] && nd.key[b] == key) ? b : -1;
+}
+
+// M2L (far) list of every node: same-depth nodes whose parents are
+// adjacent or equal to this node's parent and which are not adjacent to it.
+// pass 0 counts, pass 1 fills.
+__global__ void k_vlist(NodeArrays nd, int nnodes,
+                        const int *__restrict__ lstart,
+                        const int *__restrict__ lcount, int pass,
+                        int *__restrict__ cnt, const int *__restrict__ ofs,
+                        int *__restrict__ src, short *__restrict__ code) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nnodes) return;
+  int d = nd.depth[i];
+  int c = 0;
+  int pos = pass ? ofs[i] : 0;
+  if (d >= 1) {
+    int P = nd.parent[i];
+    int ax = nd.ix[i], ay = nd.iy[i], az = nd.iz[i];
+    int px = nd.ix[P], py = nd.iy[P], pz = nd.iz[P];
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          int j = lookup(nd, lstart, lcount, d - 1, px + dx, py + dy, pz + dz);
+          if (j < 0 || nd.leaf[j]) continue;
+          int cb = nd.child[j], ce = cb + nd.nchild[j];
+          for (int b = cb; b < ce; b++) {
+            int ox = ax - nd.ix[b], oy = ay - nd.iy[b], oz = az - nd.iz[b];
+            if (abs(ox) <= 1 && abs(oy) <= 1 && abs(oz) <= 1) continue;
+            if (pass) {
+              src[pos] = b;
+              code[pos] = (short)code_of(ox, oy, oz);
+              pos++;
+            } else {
+              c++;
+            }
+          }
+        }
+  }
+  if (!pass) cnt[i] = c;
+}
+
+// P2P (near) list of every leaf as particle ranges: same-depth neighbours
+// (whole subtrees), and shallower leaves adjacent to an ancestor.
+__global__ void k_ulist(NodeArrays nd, int nleaves,
+                        const int *__restrict__ leaves,
+                        const int *__restrict__ lstart,
+                        const int *__restrict__ lcount, int pass,
+                        int *__restrict__ cnt, const int *__restrict__ ofs,
+                        int *__restrict__ ubeg, int *__restrict__ uend,
+                        long long *__restrict__ pairs) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  int A = leaves[t];
+  int dA = nd.depth[A];
+  int anc[kMaxDepth + 1];
+  anc[dA] = A;
+  for (int d = dA - 1; d >= 0; d--) anc[d] = nd.parent[anc[d + 1]];
+  int c = 0;
+  int pos = pass ? ofs[t] : 0;
+  long long np = 0;
+  int nA = nd.pend[A] - nd.pbegin[A];
+  for (int d = 0; d <= dA; d++) {
+    int X = anc[d];
+    int xx = nd.ix[X], xy = nd.iy[X], xz = nd.iz[X];
+    for (int dx = -1; dx <= 1; dx++)
+      for (int dy = -1; dy <= 1; dy++)
+        for (int dz = -1; dz <= 1; dz++) {
+          bool self = (dx == 0 && dy == 0 && dz == 0);
+          if (d < dA && self) continue;
+          int j = lookup(nd, lstart, lcount, d, xx + dx, xy + dy, xz + dz);
+          if (j < 0) continue;
+          if (d < dA && !nd.leaf[j]) continue;
+          int b = nd.pbegin[j], e = nd.pend[j];
+          if (pass) {
+            ubeg[pos] = b;
+            uend[pos] = e;
+            pos++;
+            // reference pair counter: n(n+1)/2 for the leaf itself, half of
+            // the ordered count otherwise (each unordered pair is seen twice)
+            if (j == A)
+              np += 2LL * ((long long)nA * (nA + 1) / 2);
+            else
+              np += (long long)nA * (e - b);
+          } else {
+            c++;
+          }
+        }
+  }
+  if (!pass)
+    cnt[t] = c;
+  else
+    pairs[t] = np;
+}
+
+hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *lstart,
+                        const int *lcount, int pass, int *cnt, const int *ofs,
+                        int *src, short *code, hipStream_t s) {
+  if (nnodes == 0) return hipSuccess;
+  k_vlist<<<(nnodes + 127) / 128, 128, 0, s>>>(nd, nnodes, lstart, lcount,
+                                                pass, cnt, ofs, src, code);
+  return hipGetLastError();
+}
+
+hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
+                        const int *lstart, const int *lcount, int pass,
+                        int *cnt, const int *ofs, int *ubeg, int *uend,
+                        long long *pairs, hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  k_ulist<<<(nleaves + 127) / 128, 128, 0, s>>>(
+      nd, nleaves, leaves, lstart, lcount, pass, cnt, ofs, ubeg, uend, pairs);
+  return hipGetLastError();
+}
+
+// Sum of per-leaf pair counts (one block, deterministic).
+__global__ void k_sum_ll(const long long *__restrict__ v, int n,
+                         long long *__restrict__ out) {
+  __shared__ long long sh[256];
+  long long s = 0;
+  for (int i = threadIdx.x; i < n; i += 256) s += v[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int k = 128; k > 0; k >>= 1) {
+    if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = sh[0];
+}
+
+hipError_t launch_sum_ll(const long long *v, int n, long long *out,
+                         hipStream_t s) {
+  k_sum_ll<<<1, 256, 0, s>>>(v, n, out);
+  return hipGetLastError();
+}
+
+}  // namespace aran

@@ -1,0 +1,49 @@
+// tree_rules.h -- the octree rules shared by every tree kernel.
+//
+// LibVsg's VsgPRTree3d (the reference's tree, not in its source tree) is
+// replaced by these rules; the FP64 oracle implements the same ones:
+//   * integer coordinate i = floor((x - lo) * 2^21 / width), clamped to
+//     [0, 2^21 - 1]; (x - lo) * scale is a subtraction then a product, so no
+//     contraction can change it between host and device;
+//   * 63-bit Morton key, octant digit = bx | by << 1 | bz << 2, most
+//     significant digit first;
+//   * a node splits while it holds more than leafcap particles and its depth
+//     is below 21; empty octants create no node.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdint.h>
+
+namespace aran {
+
+struct NodeArrays {
+  int *depth, *ix, *iy, *iz, *pbegin, *pend, *parent, *child, *nchild, *leaf;
+  uint64_t *key;
+  double *cx, *cy, *cz;
+};
+
+__host__ __device__ inline uint64_t spread3(uint64_t v) {
+  v &= 0x1fffffull;
+  v = (v | (v << 32)) & 0x1f00000000ffffull;
+  v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+  v = (v | (v << 8)) & 0x100f00f00f00f00full;
+  v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+  v = (v | (v << 2)) & 0x1249249249249249ull;
+  return v;
+}
+
+__host__ __device__ inline uint64_t morton_key(uint64_t ix, uint64_t iy,
+                                               uint64_t iz) {
+  return spread3(ix) | (spread3(iy) << 1) | (spread3(iz) << 2);
+}
+
+__host__ __device__ inline uint64_t icoord(double x, double lo, double scale) {
+  double f = floor((x - lo) * scale);
+  const double top = 2097151.0;  // 2^21 - 1
+  f = f < 0. ? 0. : f;
+  f = f > top ? top : f;
+  return (uint64_t)f;
+}
+
+}  // namespace aran

@@ -1,0 +1,120 @@
+"""Python mirror of the device-resident entry points (include/aran/aranfmm3d.h).
+
+Inputs and outputs are torch float64 tensors on the GPU; the context
+launches on torch's current stream, so no extra synchronisation is needed
+around calls.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+STAT_NAMES = ("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p", "depth",
+              "nodes", "leaves", "far_entries", "near_entries")
+TIMER_NAMES = ("tree", "lists", "up", "m2l", "down", "near", "total")
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+class Fmm3d:
+    """One FMM context (device workspace kept between solves)."""
+
+    def __init__(self, stream=None):
+        import torch
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._ctx = lib().aran_fmm3d_new(C.c_void_p(stream))
+        if not self._ctx:
+            raise RuntimeError("aran_fmm3d_new failed")
+        self.n = 0
+
+    def close(self):
+        if self._ctx:
+            lib().aran_fmm3d_free(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_timings(self, on=True):
+        check(lib().aran_fmm3d_set_timings(self._ctx, int(on)), "set_timings")
+
+    def solve(self, x, y, z, q, lb, ub, order, leafcap, pot=None, grad=None,
+              want_pot=True, want_grad=True):
+        import torch
+        n = x.numel()
+        for t in (x, y, z, q):
+            assert t.dtype == torch.float64 and t.is_cuda and t.is_contiguous()
+            assert t.numel() == n
+        if want_pot and pot is None:
+            pot = torch.empty(n, dtype=torch.float64, device=x.device)
+        if want_grad and grad is None:
+            grad = torch.empty((n, 3), dtype=torch.float64, device=x.device)
+        lbv = (C.c_double * 3)(*[float(v) for v in lb])
+        ubv = (C.c_double * 3)(*[float(v) for v in ub])
+        rc = lib().aran_fmm3d_solve(self._ctx, n, _ptr(x), _ptr(y), _ptr(z),
+                                    _ptr(q), lbv, ubv, int(order),
+                                    int(leafcap), _ptr(pot), _ptr(grad))
+        check(rc, "aran_fmm3d_solve")
+        self.n = n
+        return pot, grad
+
+    def stats(self):
+        v = (C.c_long * 12)()
+        check(lib().aran_fmm3d_get_stats(self._ctx, v), "get_stats")
+        return dict(zip(STAT_NAMES, list(v)))
+
+    def timings(self):
+        v = (C.c_float * 7)()
+        check(lib().aran_fmm3d_get_timings(self._ctx, v), "get_timings")
+        return dict(zip(TIMER_NAMES, list(v)))
+
+    def export_tree(self):
+        st = self.stats()
+        nn = st["nodes"]
+        perm = np.zeros(self.n, dtype=np.int64)
+        nodes = np.zeros((nn, 10), dtype=np.int32)
+        centers = np.zeros((nn, 3))
+        lo = np.zeros(3)
+        width = np.zeros(1)
+        check(lib().aran_fmm3d_export_tree(
+            self._ctx, perm.ctypes.data, nodes.ctypes.data,
+            centers.ctypes.data, lo.ctypes.data, width.ctypes.data),
+            "export_tree")
+        return perm, nodes, centers, lo, float(width[0])
+
+    def export_lists(self):
+        st = self.stats()
+        nn, nl = st["nodes"], st["leaves"]
+        nf, nu = st["far_entries"], st["near_entries"]
+        far_ofs = np.zeros(nn + 1, dtype=np.int32)
+        far_src = np.zeros(max(nf, 1), dtype=np.int32)
+        far_code = np.zeros(max(nf, 1), dtype=np.int16)
+        leaves = np.zeros(max(nl, 1), dtype=np.int32)
+        near_ofs = np.zeros(nl + 1, dtype=np.int32)
+        nb = np.zeros(max(nu, 1), dtype=np.int32)
+        ne = np.zeros(max(nu, 1), dtype=np.int32)
+        check(lib().aran_fmm3d_export_lists(
+            self._ctx, far_ofs.ctypes.data, far_src.ctypes.data,
+            far_code.ctypes.data, leaves.ctypes.data, near_ofs.ctypes.data,
+            nb.ctypes.data, ne.ctypes.data), "export_lists")
+        return dict(far_ofs=far_ofs, far_src=far_src[:nf],
+                    far_code=far_code[:nf], leaves=leaves[:nl],
+                    near_ofs=near_ofs, near_begin=nb[:nu], near_end=ne[:nu])
+
+    def direct(self, x, y, z, q, targets):
+        import torch
+        t = targets.to(device=x.device, dtype=torch.int64).contiguous()
+        nt = t.numel()
+        pot = torch.empty(nt, dtype=torch.float64, device=x.device)
+        grad = torch.empty((nt, 3), dtype=torch.float64, device=x.device)
+        check(lib().aran_fmm3d_direct(self._ctx, x.numel(), _ptr(x), _ptr(y),
+                                      _ptr(z), _ptr(q), nt, _ptr(t),
+                                      _ptr(pot), _ptr(grad)), "direct")
+        return pot, grad

@@ -1,0 +1,150 @@
+"""Benchmark: 3D Newton FMM particles/s on one MI355X.
+
+Workload (one "step" = one full solve: tree build, interaction lists,
+upward pass, M2L, downward pass, P2P + L2P, results in input order):
+Plummer sphere (scale radius 0.1, truncated at radius 1, seeded), N = 2^20
+equal masses 1/N, expansion order p = 10, leaf capacity 64, potential and
+gradient.  Inputs are generated on the host, copied to HBM once, and stay
+resident; the timed region covers K solves.
+
+Also reported: the M2L kernel's roofline point (algorithmic flops per launch
+/ kernel duration from HIP events on the engine's stream), the error of the
+timed solve against an FP64 direct sum on 4096 sampled targets, and the
+FP64 CPU oracle timed on a bounded sample (one core) as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+MI355X_FP64_TFLOPS = 78.6
+MI355X_HBM_GBS = 8000.0
+
+
+def rot_macs(L):
+    return (L + 1) * (L + 2) * (4 * L + 3) // 6
+
+
+def m2l_flops(p):
+    """Algorithmic flops of one rotate / z-translate / rotate-back M2L with
+    multipole degrees < p and local degrees <= p (complex MAC = 8 flops,
+    complex x real = 4)."""
+    v = sum(p - m for l in range(p + 1) for m in range(min(l, p - 1) + 1))
+    return 8 * (rot_macs(p - 1) + rot_macs(p)) + 4 * v
+
+
+def cpu_baseline(n_sample, order, leafcap, seed):
+    import oracle as O
+    from aran_amd import workloads as W
+    x, y, z, m = W.plummer(n_sample, seed)
+    t0 = time.perf_counter()
+    O.fmm_solve(x, y, z, m, -np.ones(3), np.ones(3), order, leafcap)
+    dt = time.perf_counter() - t0
+    return {"value": n_sample / dt, "unit": "particles/s", "cores": 1,
+            "kind": "port",
+            "sample": "FP64 serial oracle FMM (restatement of the reference "
+                      "AranSolver3d path) on a %d-particle Plummer system, "
+                      "same a, truncation, p=%d, leaf %d; %.1f s"
+                      % (n_sample, order, leafcap, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--order", type=int, default=10)
+    ap.add_argument("--leafcap", type=int, default=64)
+    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this engine runs on one GPU; --gpus %d is not "
+              "supported (one-GPU measurements only)" % args.gpus)
+        sys.exit(2)
+
+    import torch
+
+    from aran_amd import workloads as W
+    from aran_amd.fmm import Fmm3d
+
+    torch.cuda.set_device(0)
+    seed = 0xA2A0 + 2
+    n, p, q = args.n, args.order, args.leafcap
+    x, y, z, m = W.plummer(n, seed)
+    dev = [torch.from_numpy(a).cuda() for a in (x, y, z, m)]
+    lb, ub = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
+    f = Fmm3d()
+    pot = torch.empty(n, dtype=torch.float64, device="cuda")
+    grad = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    for _ in range(args.warmup):
+        f.solve(*dev, lb, ub, p, q, pot, grad)
+    f.set_timings(True)
+    torch.cuda.synchronize()
+    t_m2l, t_tot = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        f.solve(*dev, lb, ub, p, q, pot, grad)
+        tm = f.timings()
+        t_m2l.append(tm["m2l"])
+        t_tot.append(tm["total"])
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    st = f.stats()
+    tm = f.timings()
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = n * args.steps / elapsed
+
+    m2l_ms = float(np.mean(t_m2l))
+    flops = st["m2l"] * m2l_flops(p)
+    achieved = flops / (m2l_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "kernel": "k_m2l (FP64)",
+            "achieved": round(achieved, 4), "peak": MI355X_FP64_TFLOPS,
+            "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP64_TFLOPS, 5),
+            "traffic": None,
+            "algorithmic": "%d M2L x %d flops" % (st["m2l"], m2l_flops(p)),
+            "kernel_ms": round(m2l_ms, 4)}
+
+    err = None
+    if not args.no_check:
+        tg = W.sample_targets(n, 4096, seed)
+        dpot, dgrad = f.direct(*dev, torch.from_numpy(tg))
+        dpot, dgrad = dpot.cpu().numpy(), dgrad.cpu().numpy()
+        pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
+        err = {"pot_l2": float(np.linalg.norm(pp - dpot) / np.linalg.norm(dpot)),
+               "pot_max": float(np.max(np.abs(pp - dpot) / np.abs(dpot))),
+               "grad_l2": float(np.linalg.norm(gg - dgrad) /
+                                np.linalg.norm(dgrad)),
+               "targets": 4096}
+
+    cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, p, q, seed)
+    line = {
+        "metric": "3D Newton FMM particles/s at N=2^20 (Plummer, p=10, "
+                  "leaf 64), rel err vs FP64 direct sum",
+        "value": value, "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded Plummer sphere, a=0.1, r<=1)",
+        "config": {"workload": "plummer_2p20_p10_q64_fp64", "n": n,
+                   "order": p, "leafcap": q, "form": "plain FP64",
+                   "outputs": "potential+gradient"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "error": err,
+        "phases_ms": {k: round(v, 4) for k, v in tm.items()},
+        "tree": {k: st[k] for k in ("depth", "nodes", "leaves", "m2l", "p2p")},
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,78 @@
+/*
+ * aranfmm3d.h -- device-resident entry points of the MI355X FMM engine.
+ *
+ * No counterpart in the reference: this is the engine under
+ * aran_solver3d_solve() (aran/aransolver3d.c:1135-1202) for callers whose
+ * particles already live in GPU memory (structure-of-arrays FP64 device
+ * pointers).  Every call is synchronous with respect to the host unless
+ * stated otherwise; all pointers marked "device" are HIP device pointers.
+ *
+ * Return codes: 0 on success, negative on error (aran_last_error() holds the
+ * message).
+ */
+#ifndef ARAN_FMM3D_H
+#define ARAN_FMM3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct _AranFmm3d AranFmm3d;
+
+#define ARAN_FMM3D_EINVAL (-1)
+#define ARAN_FMM3D_EHIP (-2)
+#define ARAN_FMM3D_ENOMEM (-3)
+
+/* Context bound to the current HIP device; stream NULL creates its own. */
+AranFmm3d *aran_fmm3d_new (void *hip_stream);
+void aran_fmm3d_free (AranFmm3d *ctx);
+
+/* One FMM solve: potential phi_i = sum_{j != i} q_j / |x_i - x_j| and its
+ * gradient, written (not accumulated) in the input order.  order p:
+ * multipole degrees 0..p-1, local degrees 0..p (1 <= p <= 15); leafcap: leaf
+ * capacity of the octree; lb/ub: host arrays of 3 doubles giving the root
+ * box before growth.  pot / grad (n x 3, row-major) may be NULL. */
+int aran_fmm3d_solve (AranFmm3d *ctx, long n, const double *x,
+                      const double *y, const double *z, const double *q,
+                      const double *lb, const double *ub, int order,
+                      int leafcap, double *pot, double *grad);
+
+/* Counters of the last solve: zero, p2p, p2m, m2m, m2l, l2l, l2p (the
+ * reference's semantics), then depth, nodes, leaves, far-list entries,
+ * near-list entries. */
+int aran_fmm3d_get_stats (const AranFmm3d *ctx, long *out12);
+
+/* Phase timers (HIP events) of the next solves when on != 0; read back as
+ * tree, lists, up, m2l, down, near, total in ms. */
+int aran_fmm3d_set_timings (AranFmm3d *ctx, int on);
+int aran_fmm3d_get_timings (const AranFmm3d *ctx, float *out7);
+
+/* Tree of the last solve, copied to host arrays sized from get_stats:
+ * perm[n] (sorted position -> input index), nodes[nnodes*10] with
+ * depth, ix, iy, iz, pbegin, pend, parent, isleaf, child_begin, nchild,
+ * centers[nnodes*3], lo[3] and width[1]. */
+int aran_fmm3d_export_tree (const AranFmm3d *ctx, long *perm, int *nodes,
+                            double *centers, double *lo, double *width);
+
+/* Interaction tables of the last solve (host arrays): far_ofs[nnodes+1],
+ * far_src[nfar], far_code[nfar] (offset code (dx+3)*49+(dy+3)*7+(dz+3) of
+ * target minus source), leaves[nleaves], near_ofs[nleaves+1],
+ * near_begin[nnear], near_end[nnear] (particle ranges in sorted order). */
+int aran_fmm3d_export_lists (const AranFmm3d *ctx, int *far_ofs,
+                             int *far_src, short *far_code, int *leaves,
+                             int *near_ofs, int *near_begin, int *near_end);
+
+/* FP64 direct sum on nt targets (device indices), compensated summation. */
+int aran_fmm3d_direct (AranFmm3d *ctx, long n, const double *x,
+                       const double *y, const double *z, const double *q,
+                       long nt, const long *targets, double *pot,
+                       double *grad);
+
+/* Stream the context launches on (hipStream_t). */
+void *aran_fmm3d_stream (const AranFmm3d *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ARAN_FMM3D_H */

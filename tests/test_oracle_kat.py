@@ -1,0 +1,367 @@
+"""Pins of the FP64 oracle (CPU only).
+
+Each reference known-answer test is restated with its own points, vectors,
+degrees and tolerances, and gates here (the reference's legendre, multipole3,
+taylor3, gradient3 programs print mismatches but always exit 0).  The golden
+fixtures of tests/golden/ (mpmath, 40 digits) pin the tables independently
+of the oracle's recurrences.
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+GOLD = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden.npz"))
+
+
+# ---------------------------------------------------------------- tables
+
+def _closed_legendre(l, m, x):
+    """tests/legendre.c:109-199 closed forms (Condon-Shortley phase)."""
+    s = math.sqrt(1 - x * x)
+    table = {
+        (0, 0): 1.0, (1, 0): x, (1, 1): -s,
+        (2, 0): 0.5 * (3 * x * x - 1), (2, 1): -3 * x * s, (2, 2): 3 * s * s,
+        (3, 0): 0.5 * (5 * x ** 3 - 3 * x), (3, 1): -1.5 * (5 * x * x - 1) * s,
+        (3, 2): 15 * x * s * s, (3, 3): -15 * s ** 3,
+        (4, 0): (35 * x ** 4 - 30 * x * x + 3) / 8,
+        (4, 1): -2.5 * (7 * x ** 3 - 3 * x) * s,
+        (4, 2): 7.5 * (7 * x * x - 1) * s * s, (4, 3): -105 * x * s ** 3,
+        (4, 4): 105 * s ** 4,
+        (5, 0): (63 * x ** 5 - 70 * x ** 3 + 15 * x) / 8,
+        (5, 1): -15 / 8 * (21 * x ** 4 - 14 * x * x + 1) * s,
+        (5, 2): 52.5 * (3 * x ** 3 - x) * s * s,
+        (5, 3): -52.5 * (9 * x * x - 1) * s ** 3, (5, 4): 945 * x * s ** 4,
+        (5, 5): -945 * s ** 5}
+    return table[(l, m)]
+
+
+def test_legendre_closed_forms():
+    for x in np.linspace(-1, 1, 10):
+        P = O.legendre(5, x)
+        for l in range(6):
+            for m in range(l + 1):
+                assert abs(P[O.tri(l) + m] - _closed_legendre(l, m, x)) <= 1e-10
+
+
+def test_legendre_golden():
+    for i, x in enumerate(GOLD["legendre_x"]):
+        P = O.legendre(12, x)
+        for l in range(13):
+            for m in range(l + 1):
+                ref = GOLD["legendre"][i, l, m]
+                assert abs(P[O.tri(l) + m] - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+def test_legendre_special_matches_division():
+    """P_l^m / sin(theta) for m >= 1 with the metric of
+    tests/special_legendre.c:32-70: x = 2i/100 - 1, l <= 40, relative error
+    when |ref| > 1e-4 else absolute, values >= 1e3 skipped, 1e-5."""
+    for i in range(1, 100):
+        x = i * 2.0 / 100 - 1.0
+        S = O.legendre_special(40, x)
+        P = O.legendre(40, x)
+        s = math.sqrt(1 - x * x)
+        for l in range(1, 41):
+            for m in range(1, l + 1):
+                ref = P[O.tri(l) + m] / s
+                err = ref - S[O.tri(l) + m]
+                if abs(ref) > 1e-4:
+                    err /= ref
+                if abs(ref) < 1e3:
+                    assert abs(err) <= 1e-5, (x, l, m)
+
+
+def test_harmonics_golden_and_closed_forms():
+    for i, (t, p) in enumerate(GOLD["harmonic_dirs"]):
+        Y = O.harmonics(10, t, p)
+        for l in range(11):
+            for m in range(l + 1):
+                assert abs(Y[O.tri(l) + m] - GOLD["harmonic"][i, l, m]) <= 1e-12
+    # MathWorld closed forms l <= 2 (tests/sphericalharmonic.c:120-212)
+    t, p = 0.7, 1.9
+    Y = O.harmonics(2, t, p)
+    c, s = math.cos(t), math.sin(t)
+    e = complex(math.cos(p), math.sin(p))
+    ref = {(0, 0): 0.5 / math.sqrt(math.pi),
+           (1, 0): 0.5 * math.sqrt(3 / math.pi) * c,
+           (1, 1): -0.5 * math.sqrt(1.5 / math.pi) * s * e,
+           (2, 0): 0.25 * math.sqrt(5 / math.pi) * (3 * c * c - 1),
+           (2, 1): -0.5 * math.sqrt(7.5 / math.pi) * s * c * e,
+           (2, 2): 0.25 * math.sqrt(7.5 / math.pi) * s * s * e * e}
+    for (l, m), v in ref.items():
+        assert abs(Y[O.tri(l) + m] - v) <= 1e-14
+
+
+def test_alpha_beta_closed_forms():
+    for l in range(30):
+        assert abs(O.lib().or_beta(l) - math.sqrt(4 * math.pi / (2 * l + 1))) \
+            <= 1e-15
+        for m in range(l + 1):
+            ref = 1.0 / math.sqrt(math.factorial(l - m) * math.factorial(l + m))
+            assert abs(O.lib().or_alpha(l, m) - ref) <= 1e-13 * ref
+
+
+# ---------------------------------------------------------------- Wigner
+
+def test_wigner_against_factorial_sum():
+    """tests/wigner.c:186-209: L=20, beta = i pi/30 (i < 30), pi/2 and
+    0.84106867056793033; relative 1e-6 (absolute below 1e-6)."""
+    L = 20
+    betas = [i * math.pi / 30 for i in range(30)] + [math.pi / 2,
+                                                      0.84106867056793033]
+    for b in betas:
+        if abs(math.cos(b / 2)) < 1e-3:
+            continue  # the recursion divides by cos(beta/2)
+        d = O.wigner_d(L, b)
+        for l in range(L + 1):
+            for mp in range(l + 1):
+                for m in range(-l, l + 1):
+                    res = d[O.wigner_index(l, mp, m)]
+                    ref = O.wigner_factorial(l, m, mp, b)
+                    err = abs(res - ref)
+                    if abs(ref) > 1e-6:
+                        err /= abs(ref)
+                    assert err < 1e-6, (b, l, mp, m, res, ref)
+
+
+def test_wigner_golden_and_unitarity():
+    for i, b in enumerate(GOLD["wigner_beta"]):
+        d = O.wigner_d(12, b)
+        for l in range(13):
+            for mp in range(l + 1):
+                for m in range(-l, l + 1):
+                    ref = GOLD["wigner"][i, l, m + 12, mp + 12]
+                    assert abs(d[O.wigner_index(l, mp, m)] - ref) <= 1e-12
+    # full square d^l(beta) from the m' >= 0 half by d_{m,-m'} =
+    # (-1)^{m+m'} d_{-m,m'}; orthogonality d d^T = I
+    b = 0.955316618124509
+    d = O.wigner_d(15, b)
+    for l in range(16):
+        D = np.zeros((2 * l + 1, 2 * l + 1))
+        for mp in range(-l, l + 1):
+            for m in range(-l, l + 1):
+                if mp >= 0:
+                    D[m + l, mp + l] = d[O.wigner_index(l, mp, m)]
+                else:
+                    D[m + l, mp + l] = (-1) ** (m + mp) * \
+                        d[O.wigner_index(l, -mp, -m)]
+        assert np.max(np.abs(D @ D.T - np.eye(2 * l + 1))) <= 1e-13
+
+
+def test_rotate_inverse_is_identity():
+    """tests/rotation.c:229-234 (rotate_inverse after rotate = id), L=29,
+    with nonzero alpha and gamma as well."""
+    rng = np.random.default_rng(0)
+    L = 29
+    src = rng.normal(size=O.tri(L + 1)) + 1j * rng.normal(size=O.tri(L + 1))
+    for l in range(L + 1):
+        src[O.tri(l)] = src[O.tri(l)].real
+    for (a, b, g) in [(0., 0.3, 0.), (0.4, 1.2, -0.7), (-2.0, 0.05, 1.0),
+                      (0., 1.5707963267948966, 0.)]:
+        rot = O.rotate(L, src, a, b, g)
+        back = O.rotate(L, rot, a, b, g, inverse=True)
+        assert np.max(np.abs(back - src)) <= 1e-10 * np.max(np.abs(src))
+
+
+# ---------------------------------------------------------------- operators
+
+def _grid(center, radius, n=10):
+    pts = []
+    for i in range(n):
+        p = 2 * math.pi * i / (n - 1)
+        for j in range(n):
+            t = math.pi * j / (n - 1)
+            for k in range(n):
+                r = radius * k / (n - 1)
+                pts.append(center + r * np.array([math.sin(t) * math.cos(p),
+                                                  math.sin(t) * math.sin(p),
+                                                  math.cos(t)]))
+    return pts
+
+
+def test_m2l_kat_seven_vectors():
+    """tests/m2l3.c:211-340: multipole (degrees < 26) of a unit charge at
+    (0.1,0.2,0.3); M2L (rotate) to 7 vectors; local (degree 25) vs 1/|x-p|
+    on the 10x10x10 grid r <= 1 at 1e-7."""
+    src = np.array([0.1, 0.2, 0.3])
+    deg = 25
+    M = O.p2m(deg + 1, src, 1.0, np.zeros(3))
+    for t in [(5, 5, 5), (5, 5, -5), (0, 0, -5), (0, 0, 5), (5, 0, 5),
+              (0, 5, 5), (-5, -5, 5)]:
+        t = np.array(t, dtype=float)
+        L = O.m2l(deg + 1, M, np.zeros(3), deg, t)
+        for x in _grid(t, 1.0):
+            ref = 1.0 / np.linalg.norm(x - src)
+            res = O.local_eval(deg, L, t, x)
+            assert abs(res - ref) <= 1e-7 * abs(ref), (t, x)
+
+
+def test_multipole_and_m2m_kat():
+    """tests/multipole3.c:270-329: multipole (order 30) of a unit charge at
+    (0.1,0.2,0.3); potential and gradient vs direct for r in [5,10]; M2M to
+    (1,1.1,1.2) likewise; 1e-7."""
+    src = np.array([0.1, 0.2, 0.3])
+    M = O.p2m(30, src, 1.0, np.zeros(3))
+    c2 = np.array([1.0, 1.1, 1.2])
+    M2 = O.m2m(30, M, np.zeros(3), 30, c2)
+    rng = np.random.default_rng(1)
+    for _ in range(200):
+        v = rng.normal(size=3)
+        v /= np.linalg.norm(v)
+        for cen, MM in ((np.zeros(3), M), (c2, M2)):
+            x = cen + v * rng.uniform(5, 10)
+            d = x - src
+            ref = 1 / np.linalg.norm(d)
+            gref = -d / np.linalg.norm(d) ** 3
+            assert abs(O.multipole_eval(30, MM, cen, x) - ref) <= 1e-7 * ref
+            g = O.multipole_grad(30, MM, cen, x)
+            assert np.linalg.norm(g - gref) <= 1e-7 * np.linalg.norm(gref)
+
+
+@pytest.mark.parametrize("charge", [(3., 0., 0.), (3., -1., -1.)])
+def test_local_l2l_and_gradient_kat(charge):
+    """tests/taylor3.c:155-259 and tests/gradient3.c:184-288: local
+    (degree 29) of a charge at (3,0,0) / (3,-1,-1) about (5,0,0), L2L to
+    (4.5,0,0); potential (both) and gradient (second charge, as
+    gradient3.c) on the 10x10x10 grids r <= 1 (original) and r <= 0.5
+    (translated) at 1e-9."""
+    p = np.array(charge)
+    c = np.array([5.0, 0.0, 0.0])
+    tr = np.array([4.5, 0.0, 0.0])
+    M = O.p2m(30, p, 1.0, p)
+    Lc = O.m2l(30, M, p, 29, c)
+    L2 = O.l2l(29, Lc, c, 29, tr)
+    for cen, LL, rad in ((c, Lc, 1.0), (tr, L2, 0.5)):
+        for x in _grid(cen, rad):
+            d = x - p
+            ref = 1 / np.linalg.norm(d)
+            assert abs(O.local_eval(29, LL, cen, x) - ref) <= 1e-9 * ref
+            if charge[1] != 0.0:  # gradient3.c checks the field
+                g = O.local_grad(29, LL, cen, x)
+                gref = -d / np.linalg.norm(d) ** 3
+                assert np.linalg.norm(g - gref) <= 1e-9 * np.linalg.norm(gref)
+
+
+# ---------------------------------------------------------------- tree
+
+def _pair_accounting(x, y, z, q):
+    lb, ub = np.zeros(3), np.ones(3)
+    perm, nodes = O.tree(x, y, z, lb, ub, q)
+    (fs, fd), (na, nb) = O.traverse(nodes)
+    n = len(x)
+    sorted_pos = np.empty(n, dtype=np.int64)
+    sorted_pos[perm] = np.arange(n)
+    cover = np.zeros((n, n), dtype=np.int64)
+
+    def members(a):
+        return np.arange(nodes["pbegin"][a], nodes["pend"][a])
+
+    for s, d in zip(fs, fd):
+        for i in members(d):
+            for j in members(s):
+                cover[i, j] += 1
+        # far pairs are same-depth, non-adjacent
+        assert nodes["depth"][s] == nodes["depth"][d]
+        assert max(abs(nodes["ix"][s] - nodes["ix"][d]),
+                   abs(nodes["iy"][s] - nodes["iy"][d]),
+                   abs(nodes["iz"][s] - nodes["iz"][d])) >= 2
+    for a, b in zip(na, nb):
+        assert nodes["isleaf"][a] and nodes["isleaf"][b]
+        for i in members(a):
+            for j in members(b):
+                cover[i, j] += 1
+                if a != b:
+                    cover[j, i] += 1
+    np.fill_diagonal(cover, 1)
+    return cover, nodes, perm
+
+
+@pytest.mark.parametrize("n,q,seed", [(60, 3, 0), (150, 8, 1), (200, 1, 2),
+                                      (120, 200, 3)])
+def test_every_pair_accounted_once(n, q, seed):
+    """Brute force: each ordered pair (i != j) is covered exactly once, by
+    one near leaf pair or one far (M2L) pair."""
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.random((3, n)) ** np.array([[1.0], [2.0], [3.0]])
+    cover, nodes, perm = _pair_accounting(x, y, z, q)
+    assert np.all(cover == 1)
+    # leaves partition the particles; capacity respected below depth 21
+    leaves = np.nonzero(nodes["isleaf"])[0]
+    sizes = nodes["pend"][leaves] - nodes["pbegin"][leaves]
+    assert sizes.sum() == n
+    assert np.all((sizes <= q) | (nodes["depth"][leaves] == 21))
+
+
+def test_keys_follow_split_planes():
+    """A particle's octant at each depth is the side of the node's split
+    plane it lies on (exact integer arithmetic on the scaled coordinate)."""
+    rng = np.random.default_rng(5)
+    x, y, z = rng.random((3, 500))
+    lo, width = O.root_box(np.zeros(3), np.ones(3), x, y, z)
+    keys = O.keys(x, y, z, lo, width)
+    for axis, coord in enumerate((x, y, z)):
+        ic = np.clip(np.floor((coord - lo[axis]) * (2 ** 21 / width)), 0,
+                     2 ** 21 - 1).astype(np.int64)
+        for d in range(1, 22):
+            bit = (keys >> np.uint64(3 * (21 - d) + axis)) & np.uint64(1)
+            assert np.array_equal(bit.astype(np.int64), (ic >> (21 - d)) & 1)
+
+
+# ---------------------------------------------------------------- solve
+
+def test_direct_sum_golden():
+    pos, q = GOLD["direct_pos"], GOLD["direct_q"]
+    pot, grad = O.direct(pos[:, 0], pos[:, 1], pos[:, 2], q,
+                         np.arange(len(q)))
+    assert np.max(np.abs(pot - GOLD["direct_pot"])) <= 1e-13 * \
+        np.max(np.abs(GOLD["direct_pot"]))
+    assert np.max(np.abs(grad - GOLD["direct_grad"])) <= 1e-13 * \
+        np.max(np.abs(GOLD["direct_grad"]))
+
+
+@pytest.mark.parametrize("n,q,p", [(240, 10, 20), (2400, 100, 20),
+                                   (2400, 100, 24)])
+def test_fmm_end_to_end_reference_gate(n, q, p):
+    """tests/newtonpot3.at:26-46 / newtonfield3.at:26-46: random particles in
+    [-0.999, 0.999]^3, leaf sizes 10 / 100, p in {20, 24}; pointwise 1e-3
+    against the direct sum (potential and field)."""
+    rng = np.random.default_rng(n + p)
+    x, y, z = rng.uniform(-0.999, 0.999, (3, n))
+    m = rng.uniform(0, 2.0 / n, n)
+    pot, grad, _ = O.fmm_solve(x, y, z, m, -np.ones(3), np.ones(3), p, q)
+    dp, dg = O.direct(x, y, z, m, np.arange(n))
+    assert np.max(np.abs(pot - dp) / np.abs(dp)) <= 1e-3
+    assert np.max(np.linalg.norm(grad - dg, axis=1) /
+                  np.linalg.norm(dg, axis=1)) <= 1e-3
+
+
+def test_fmm_error_decreases_with_order():
+    rng = np.random.default_rng(9)
+    n = 3000
+    x, y, z = rng.random((3, n))
+    m = 1 - rng.random(n)
+    tg = np.arange(0, n, 5)
+    dp, dg = O.direct(x, y, z, m, tg)
+    errs = []
+    for p in (4, 6, 8, 10, 12):
+        pot, grad, _ = O.fmm_solve(x, y, z, m, np.zeros(3), np.ones(3), p, 20)
+        errs.append(np.linalg.norm(pot[tg] - dp) / np.linalg.norm(dp))
+    assert all(b < a for a, b in zip(errs, errs[1:]))
+    assert errs[3] <= 1e-6
+
+
+def test_planted_mistake_is_caught():
+    """A flipped sign in one M2L output coefficient (planted on the oracle's
+    result) breaks the m2l3 check, so that check has teeth."""
+    src = np.array([0.1, 0.2, 0.3])
+    M = O.p2m(16, src, 1.0, np.zeros(3))
+    t = np.array([5.0, 5.0, 5.0])
+    L = O.m2l(16, M, np.zeros(3), 15, t)
+    L[O.tri(3) + 2] *= -1
+    x = t + np.array([0.3, -0.4, 0.5])
+    ref = 1 / np.linalg.norm(x - src)
+    assert abs(O.local_eval(15, L, t, x) - ref) > 1e-7 * ref
