@@ -25,6 +25,8 @@ def _declare(L):
                                    vp, vp]
     L.aran_fmm3d_get_stats.restype = ip
     L.aran_fmm3d_get_stats.argtypes = [vp, lp]
+    L.aran_fmm3d_set_precision.restype = ip
+    L.aran_fmm3d_set_precision.argtypes = [vp, ip]
     L.aran_fmm3d_set_timings.restype = ip
     L.aran_fmm3d_set_timings.argtypes = [vp, ip]
     L.aran_fmm3d_get_timings.restype = ip

@@ -14,7 +14,15 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libaran_amd.so")
-SOURCES = ["tree.hip", "operators.hip", "p2p.hip", "engine.hip", "capi.cpp"]
+# (source, object name, extra flags); the tiled M2L is compiled once per
+# range of expansion orders so its template instances build in parallel.
+M2L_RANGES = [(1, 6), (7, 9), (10, 11), (12, 13), (14, 15)]
+SOURCES = [("tree.hip", "tree", []), ("operators.hip", "operators", []),
+           ("m2l_prep.hip", "m2l_prep", []), ("p2p.hip", "p2p", []),
+           ("engine.hip", "engine", []), ("capi.cpp", "capi", [])] + [
+    ("m2l_tile.hip", "m2l_tile_%d" % a,
+     ["-DARAN_M2L_PMIN=%d" % a, "-DARAN_M2L_PMAX=%d" % b])
+    for a, b in M2L_RANGES]
 ARCH = "gfx950"
 
 COMMON = [
@@ -25,10 +33,12 @@ COMMON = [
 ]
 
 
-def _compile(src):
-    obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+def _compile(item):
+    src, name, extra = item
+    obj = os.path.join(BUILD, name + ".o")
     lang = ["-x", "hip"] if src.endswith(".cpp") else []
-    cmd = ["hipcc", "-c"] + lang + [os.path.join(CSRC, src), "-o", obj] + COMMON
+    cmd = (["hipcc", "-c"] + lang + [os.path.join(CSRC, src), "-o", obj] +
+           COMMON + extra)
     r = subprocess.run(cmd, capture_output=True, text=True)
     return src, obj, r
 
@@ -36,7 +46,7 @@ def _compile(src):
 def build(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     objs = []
-    with cf.ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(12, os.cpu_count() or 1)) as ex:
         for src, obj, r in ex.map(_compile, SOURCES):
             if r.returncode != 0:
                 sys.stderr.write(r.stdout + r.stderr)

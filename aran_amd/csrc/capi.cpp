@@ -138,6 +138,7 @@ struct _AranFmm3d {
   aran::Stats st;
   aran::TreeInfo ti;
   bool timings = false;
+  int precision = ARAN_FMM3D_PLAIN;
   long n_last = 0;
 };
 
@@ -183,6 +184,7 @@ extern "C" int aran_fmm3d_solve(AranFmm3d *ctx, long n, const double *x,
   pb.pot = pot;
   pb.grad = grad;
   pb.timings = ctx->timings;
+  pb.precision = ctx->precision;
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
   return code_of_err(e);
@@ -195,6 +197,15 @@ extern "C" int aran_fmm3d_get_stats(const AranFmm3d *ctx, long *o) {
                 s.m2l,           s.l2l,          s.l2p,         (long)s.depth,
                 s.nnodes,        s.nleaves,      ctx->ti.nfar,  ctx->ti.nnear};
   memcpy(o, v, sizeof v);
+  return 0;
+}
+
+extern "C" int aran_fmm3d_set_precision(AranFmm3d *ctx, int precision) {
+  if (!ctx || (precision != ARAN_FMM3D_PLAIN && precision != ARAN_FMM3D_MIXED)) {
+    aran::set_error("aran_fmm3d_set_precision: invalid precision");
+    return ARAN_FMM3D_EINVAL;
+  }
+  ctx->precision = precision;
   return 0;
 }
 

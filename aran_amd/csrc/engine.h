@@ -77,6 +77,9 @@ struct Tables {
   int order = -1;
   int ntheta = 0;
   DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, dscr, thetas;
+  // tiled far field (mixed form): FP32 copies and octant offset lists
+  DevBuf rot32, ephi32, vfac32, oct_codes, code_to_k;
+  std::vector<DirEntry> dir_host;
   OpTables view() const;
 };
 
@@ -88,6 +91,7 @@ struct Workspace {
   DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs;
   DevBuf mult, loc, small;
+  DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, mult32, loc32;
   long node_cap = 0;
   void *host_small = nullptr;  // pinned, 4 KiB
   Tables tab;
@@ -109,6 +113,7 @@ struct Problem {
   double *pot = nullptr;   // device, user order, may be null
   double *grad = nullptr;  // device, user order [n][3], may be null
   bool timings = false;
+  int precision = 0;  // 0: plain FP64, 1: mixed (FP32 far field and P2P)
 };
 
 struct TreeInfo {
@@ -196,8 +201,8 @@ hipError_t launch_translate_one(int kind, const OpTables &tb, double r,
                                 hipStream_t s);
 
 // ---- p2p.hip
-hipError_t launch_p2p(const int *leaves, int nleaves, const int *lpb,
-                      const int *lpe, const int *uofs, const int *ubeg,
+hipError_t launch_p2p(int precision, const int *leaves, int nleaves,
+                      const NodeArrays &nd, const int *uofs, const int *ubeg,
                       const int *uend, const double *xs, const double *ys,
                       const double *zs, const double *qs, double *pot,
                       double *grad, hipStream_t s);

@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "m2l_tile.h"
 
 namespace aran {
 
@@ -119,7 +120,10 @@ Workspace::~Workspace() {
                    &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &mult, &loc, &small,
                    &tab.rot, &tab.eiphi, &tab.dir, &tab.tm2l, &tab.tm2m,
-                   &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas};
+                   &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas, &tab.rot32,
+                   &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
+                   &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
+                   &mult32, &loc32};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
   if (events_ready)
@@ -186,6 +190,30 @@ static std::vector<double> alpha_table(int deg) {
 }
 
 static inline double beta_l(int l) { return sqrt((4. * M_PI) / (l + l + 1.)); }
+
+void octant_offsets(const std::vector<DirEntry> &dirs, std::vector<short> &codes,
+                    std::vector<short> &code_to_k) {
+  codes.assign(8 * kOctOffsets, 0);
+  code_to_k.assign(8 * kNumCodes, -1);
+  for (int oct = 0; oct < 8; oct++) {
+    int lo[3];
+    for (int a = 0; a < 3; a++) lo[a] = ((oct >> a) & 1) ? -2 : -3;
+    std::vector<int> list;
+    for (int dx = lo[0]; dx < lo[0] + 6; dx++)
+      for (int dy = lo[1]; dy < lo[1] + 6; dy++)
+        for (int dz = lo[2]; dz < lo[2] + 6; dz++) {
+          if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) continue;
+          list.push_back(code_of(dx, dy, dz));
+        }
+    std::stable_sort(list.begin(), list.end(), [&](int a, int b) {
+      return dirs[a].theta < dirs[b].theta;
+    });
+    for (int k = 0; k < (int)list.size(); k++) {
+      codes[oct * kOctOffsets + k] = (short)list[k];
+      code_to_k[oct * kNumCodes + list[k]] = (short)k;
+    }
+  }
+}
 
 hipError_t ensure_tables(Workspace &ws, int p) {
   if (ws.tab.order == p) return hipSuccess;
@@ -296,6 +324,24 @@ hipError_t ensure_tables(Workspace &ws, int p) {
                     hipMemcpyHostToDevice, s));
   CK(launch_wigner_tables(T.thetas.as<double>(), nth, p, T.dscr.as<double>(),
                           T.rot.as<double>(), s));
+  // tiled far field: octant offset lists and FP32 tables
+  std::vector<short> oct_codes, code_to_k;
+  octant_offsets(dirs, oct_codes, code_to_k);
+  CK(T.oct_codes.reserve(sizeof(short) * oct_codes.size()));
+  CK(T.code_to_k.reserve(sizeof(short) * code_to_k.size()));
+  CK(hipMemcpyAsync(T.oct_codes.ptr, oct_codes.data(),
+                    sizeof(short) * oct_codes.size(), hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.code_to_k.ptr, code_to_k.data(),
+                    sizeof(short) * code_to_k.size(), hipMemcpyHostToDevice, s));
+  const long nrot = (long)nth * 4 * rb;
+  CK(T.rot32.reserve(sizeof(float) * nrot));
+  CK(T.ephi32.reserve(sizeof(float) * 2 * eiphi.size()));
+  CK(T.vfac32.reserve(sizeof(float) * (size_t)kNumCodes * (p + 1) * (p + 1) *
+                      std::max(p, 1)));
+  CK(launch_m2l_tables(1, p, T.dir.as<DirEntry>(), T.tm2l.as<double>(),
+                       T.rot.as<double>(), nrot, T.eiphi.as<double2>(),
+                       T.vfac32.ptr, T.rot32.ptr, T.ephi32.ptr, s));
+  T.dir_host = dirs;
   CK(hipStreamSynchronize(s));
   T.ntheta = nth;
   T.order = p;
@@ -468,6 +514,46 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                   ws.u_end.as<int>(), ws.u_pairs.as<long long>(), s));
   long long *d_pairs = reinterpret_cast<long long *>(ws.small.as<char>() + 1024);
   CK(launch_sum_ll(ws.u_pairs.as<long long>(), nleaves, d_pairs, s));
+  // ---- tiles of the far field (mixed form)
+  int ntiles = 0;
+  if (pb.precision != 0 && depth >= 2) {
+    const int first = lstart[2], count = nn - lstart[2];
+    const int ngroups = (kMaxDepth + 1) * 8;
+    CK(ws.tile_cnt.reserve(sizeof(int) * 3 * ngroups));
+    int *cnt = ws.tile_cnt.as<int>();
+    CK(launch_fill_int(cnt, 2 * ngroups, 0, s));
+    CK(launch_tile_count(nd, first, count, cnt, s));
+    std::vector<int> hcnt(ngroups), base(ngroups);
+    CK(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * ngroups,
+                      hipMemcpyDeviceToHost, s));
+    CK(hipStreamSynchronize(s));
+    std::vector<int> toct;
+    for (int g = 0; g < ngroups; g++) {
+      base[g] = ntiles;
+      int t = (hcnt[g] + 63) / 64;
+      for (int k = 0; k < t; k++) toct.push_back(g % 8);
+      ntiles += t;
+    }
+    CK(ws.tile_oct.reserve(sizeof(int) * std::max(1, ntiles)));
+    CK(ws.tile_tgt.reserve(sizeof(int) * 64 * (size_t)std::max(1, ntiles)));
+    CK(ws.tile_src.reserve(sizeof(int) * 64 * (size_t)kOctOffsets *
+                           std::max(1, ntiles)));
+    CK(ws.node_slot.reserve(sizeof(int) * nn));
+    CK(hipMemcpyAsync(ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles,
+                      hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(cnt + 2 * ngroups, base.data(), sizeof(int) * ngroups,
+                      hipMemcpyHostToDevice, s));
+    CK(launch_fill_int(ws.tile_tgt.as<int>(), 64L * ntiles, -1, s));
+    CK(launch_fill_int(ws.tile_src.as<int>(), 64L * kOctOffsets * ntiles, -1,
+                       s));
+    CK(launch_tile_fill(nd, first, count, cnt + 2 * ngroups, cnt + ngroups,
+                        ws.tile_tgt.as<int>(), ws.node_slot.as<int>(), s));
+    CK(launch_tile_src(nd, first, count, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
+                       ws.v_code.as<short>(), ws.tab.code_to_k.as<short>(),
+                       ws.node_slot.as<int>(), ws.tile_src.as<int>(), s));
+    // the host vectors above must outlive the copies
+    CK(hipStreamSynchronize(s));
+  }
   if (pb.timings) CK(hipEventRecord(ws.ev[2], s));
 
   // ---- upward pass
@@ -481,17 +567,36 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                 ws.zs.as<double>(), ws.qs.as<double>(), M, s));
   for (int d = depth - 1; d >= 0; d--)
     CK(launch_m2m(tb, nd, lstart[d], lcount[d], width, M, s));
-  if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
-  // ---- far field
-  CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
-                ws.v_code.as<short>(), width, M, L, s));
-  if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+  // ---- far field (the M2L timer brackets the M2L kernel alone)
+  if (pb.precision == 0) {
+    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+    CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
+                  ws.v_code.as<short>(), width, M, L, s));
+    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+  } else {
+    CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
+    CK(ws.loc32.reserve(sizeof(float2) * (size_t)nn * lc));
+    CK(launch_scale_mult(1, nd, nn, p, width, M, ws.mult32.ptr, s));
+    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+    M2LTileArgs ta;
+    ta.tile_tgt = ws.tile_tgt.as<int>();
+    ta.tile_src = ws.tile_src.as<int>();
+    ta.tile_oct = ws.tile_oct.as<int>();
+    ta.oct_codes = ws.tab.oct_codes.as<short>();
+    ta.dir = ws.tab.dir.as<DirEntry>();
+    ta.ephi = ws.tab.ephi32.ptr;
+    ta.rot = ws.tab.rot32.ptr;
+    ta.vfac = ws.tab.vfac32.ptr;
+    CK(launch_m2l_tile(1, p, ntiles, ta, ws.mult32.ptr, ws.loc32.ptr, s));
+    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+    CK(launch_unscale_local(1, nd, nn, p, width, ws.loc32.ptr, L, s));
+  }
   // ---- downward pass
   for (int d = 1; d <= depth; d++)
     CK(launch_l2l(tb, nd, lstart[d], lcount[d], width, L, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
   // ---- near field and L2P
-  CK(launch_p2p(leaves, nleaves, nd.pbegin, nd.pend, ws.u_ofs.as<int>(),
+  CK(launch_p2p(pb.precision, leaves, nleaves, nd, ws.u_ofs.as<int>(),
                 ws.u_beg.as<int>(), ws.u_end.as<int>(), ws.xs.as<double>(),
                 ws.ys.as<double>(), ws.zs.as<double>(), ws.qs.as<double>(),
                 ws.pot_s.as<double>(), ws.grad_s.as<double>(), s));
@@ -519,6 +624,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (pb.timings) {
     CK(hipEventElapsedTime(&S.t_tree, ws.ev[0], ws.ev[1]));
     CK(hipEventElapsedTime(&S.t_lists, ws.ev[1], ws.ev[2]));
+    // up: P2M, M2M (and the FP32 scaling); down: L2L (and unscaling)
     CK(hipEventElapsedTime(&S.t_up, ws.ev[2], ws.ev[3]));
     CK(hipEventElapsedTime(&S.t_m2l, ws.ev[3], ws.ev[4]));
     CK(hipEventElapsedTime(&S.t_down, ws.ev[4], ws.ev[5]));

@@ -1,0 +1,218 @@
+// m2l_tile.hip -- far field (M2L) with one lane per target node.
+//
+// Same operator as aran_development3d_m2l_rotate (arandevelopment3d.c:
+// 423-430; rotate.c:165-207; translate.c:609-676), reorganised for a 64-wide
+// wavefront: the targets of one level that sit in the same octant of their
+// parent share the same 189 candidate source offsets, so a wave takes 64 such
+// targets and walks the offsets in a fixed order.  For each offset every
+// lane translates its own source, while the rotation matrices (by theta),
+// the phases (by phi) and the vertical factors (by |offset|) are the same
+// for the whole wave and are read from LDS as broadcasts.  Each lane
+// accumulates its target's local in registers, so the sum over sources has
+// a fixed order and needs no atomics.
+//
+// Expansions are scaled per level by the box width w: M^_n = M_n / w^n,
+// L^_l = L_l w^(l+1).  Then the vertical factor of offset o (box units) is
+// (-1)^(l+m) c^(l+n) T(l,m,n) / |o|^(l+n+1), the same at every level.
+#include <hip/hip_runtime.h>
+
+#include <type_traits>
+
+#include "engine.h"
+#include "m2l_tile.h"
+
+namespace aran {
+
+template <typename T> struct V2;
+template <> struct V2<float> {
+  typedef float2 type;
+};
+template <> struct V2<double> {
+  typedef double2 type;
+};
+
+template <int P> struct Dims {
+  static constexpr int MC = P * (P + 1) / 2;
+  static constexpr int LC = (P + 1) * (P + 2) / 2;
+  static constexpr int RB = (P + 1) * (P + 2) * (2 * P + 3) / 6;
+  static constexpr int NV = (P + 1) * (P + 1) * P;  // [l][m][n]
+};
+
+constexpr __host__ __device__ int tri_c(int n) { return n * (n + 1) / 2; }
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+constexpr __host__ __device__ int rofs_c(int l) {
+  return l * (l + 1) * (2 * l + 1) / 6;
+}
+
+template <int P, typename T>
+__global__ __launch_bounds__(64) void k_m2l_tile(
+    const int *__restrict__ tile_tgt, const int *__restrict__ tile_src,
+    const int *__restrict__ tile_oct, const short *__restrict__ oct_codes,
+    const DirEntry *__restrict__ dir, const T *__restrict__ ephi,
+    const T *__restrict__ rotT, const T *__restrict__ vfac,
+    const typename V2<T>::type *__restrict__ Ms,
+    typename V2<T>::type *__restrict__ Ls) {
+  typedef typename V2<T>::type T2;
+  constexpr int MC = Dims<P>::MC, LC = Dims<P>::LC, RB = Dims<P>::RB,
+                NV = Dims<P>::NV;
+  __shared__ T sR[4 * RB];
+  __shared__ T sV[NV];
+  __shared__ T sP[2 * (P + 1)];
+  __shared__ T2 sAcc[LC][64];  // the lane's target local, lane-minor
+  const int tile = blockIdx.x, lane = threadIdx.x;
+  const int A = tile_tgt[tile * 64 + lane];
+  const int oct = tile_oct[tile];
+  for (int k = 0; k < LC; k++) {
+    T2 z;
+    z.x = 0;
+    z.y = 0;
+    sAcc[k][lane] = z;
+  }
+  int cur_theta = -1;
+  for (int k = 0; k < kOctOffsets; k++) {
+    const int B = tile_src[((size_t)tile * kOctOffsets + k) * 64 + lane];
+    if (__ballot(B >= 0) == 0ull) continue;
+    const int code = oct_codes[oct * kOctOffsets + k];
+    const int th = dir[code].theta;
+    __syncthreads();
+    if (th != cur_theta) {
+      const T *src = rotT + (size_t)th * 4 * RB;
+      for (int i = lane; i < 4 * RB; i += 64) sR[i] = src[i];
+      cur_theta = th;
+    }
+    {
+      const T *src = vfac + (size_t)code * NV;
+      for (int i = lane; i < NV; i += 64) sV[i] = src[i];
+      if (lane < 2 * (P + 1)) sP[lane] = ephi[code * 2 * (P + 1) + lane];
+    }
+    __syncthreads();
+    if (B >= 0) {
+      T xr[MC], xi[MC];
+      const T2 *mb = Ms + (size_t)B * MC;
+      // phase: u_m = e^{i m phi} M_m
+      static_for<0, P>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        static_for<0, l + 1>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          const T2 v = mb[tri_c(l) + m];
+          const T c = sP[2 * m], s = sP[2 * m + 1];
+          xr[tri_c(l) + m] = c * v.x - s * v.y;
+          xi[tri_c(l) + m] = c * v.y + s * v.x;
+        });
+      });
+      // forward rotation, degree by degree
+      static_for<0, P>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        T tr[l + 1], ti[l + 1];
+        static_for<0, l + 1>([&](auto mpc) {
+          constexpr int mp = decltype(mpc)::value;
+          const T *ep = sR + rofs_c(l) + mp * (l + 1);
+          const T *em = ep + RB;
+          T a = 0, b = 0;
+          static_for<0, l + 1>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            a += ep[m] * xr[tri_c(l) + m];
+            b += em[m] * xi[tri_c(l) + m];
+          });
+          tr[mp] = a;
+          ti[mp] = b;
+        });
+        static_for<0, l + 1>([&](auto mpc) {
+          constexpr int mp = decltype(mpc)::value;
+          xr[tri_c(l) + mp] = tr[mp];
+          xi[tri_c(l) + mp] = ti[mp];
+        });
+      });
+      // vertical M2L and inverse rotation, degree by degree
+      static_for<0, P + 1>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        T zr[l + 1], zi[l + 1];
+        static_for<0, l + 1>([&](auto mc) {
+          constexpr int m = decltype(mc)::value;
+          const T *v = sV + (l * (P + 1) + m) * P;
+          T a = 0, b = 0;
+          static_for<m, P>([&](auto nc) {
+            constexpr int n = decltype(nc)::value;
+            a += v[n] * xr[tri_c(n) + m];
+            b += v[n] * xi[tri_c(n) + m];
+          });
+          zr[m] = a;
+          zi[m] = b;
+        });
+        static_for<0, l + 1>([&](auto mpc) {
+          constexpr int mp = decltype(mpc)::value;
+          const T *fp = sR + 2 * RB + rofs_c(l) + mp * (l + 1);
+          const T *fm = fp + RB;
+          T wr = 0, wi = 0;
+          static_for<0, l + 1>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            wr += fp[m] * zr[m];
+            wi += fm[m] * zi[m];
+          });
+          const T c = sP[2 * mp], s = sP[2 * mp + 1];
+          T2 acc = sAcc[tri_c(l) + mp][lane];
+          acc.x += c * wr + s * wi;
+          acc.y += c * wi - s * wr;
+          sAcc[tri_c(l) + mp][lane] = acc;
+        });
+      });
+    }
+  }
+  if (A >= 0) {
+    T2 *out = Ls + (size_t)A * LC;
+    for (int k = 0; k < LC; k++) out[k] = sAcc[k][lane];
+  }
+}
+
+template <int P, typename T>
+static void launch_one(int ntiles, const M2LTileArgs &a, const void *Ms,
+                       void *Ls, hipStream_t s) {
+  typedef typename V2<T>::type T2;
+  k_m2l_tile<P, T><<<ntiles, 64, 0, s>>>(
+      a.tile_tgt, a.tile_src, a.tile_oct, a.oct_codes, a.dir,
+      static_cast<const T *>(a.ephi), static_cast<const T *>(a.rot),
+      static_cast<const T *>(a.vfac), static_cast<const T2 *>(Ms),
+      static_cast<T2 *>(Ls));
+}
+
+// Orders instantiated in this object (the build compiles this file once per
+// range so that the instantiations compile in parallel).
+#ifndef ARAN_M2L_PMIN
+#define ARAN_M2L_PMIN 1
+#define ARAN_M2L_PMAX 15
+#endif
+
+template <int P>
+static hipError_t dispatch_range(int p, int ntiles, const M2LTileArgs &a,
+                                 const void *Ms, void *Ls, hipStream_t s) {
+  if constexpr (P > ARAN_M2L_PMAX) {
+    return hipErrorInvalidValue;
+  } else {
+    if (p == P) {
+      launch_one<P, float>(ntiles, a, Ms, Ls, s);
+      return hipGetLastError();
+    }
+    return dispatch_range<P + 1>(p, ntiles, a, Ms, Ls, s);
+  }
+}
+
+#define ARAN_CAT2(a, b) a##b
+#define ARAN_CAT(a, b) ARAN_CAT2(a, b)
+// launch_m2l_tile_<PMIN>: the FP32 tiled M2L for orders PMIN..PMAX
+hipError_t ARAN_CAT(launch_m2l_tile_, ARAN_M2L_PMIN)(int p, int ntiles,
+                                                     const M2LTileArgs &a,
+                                                     const void *Ms, void *Ls,
+                                                     hipStream_t s) {
+  if (p < ARAN_M2L_PMIN || p > ARAN_M2L_PMAX) return hipErrorInvalidValue;
+  return dispatch_range<ARAN_M2L_PMIN>(p, ntiles, a, Ms, Ls, s);
+}
+
+}  // namespace aran

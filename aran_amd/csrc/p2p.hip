@@ -2,82 +2,151 @@
 //
 // P2P replaces near_func_default / near_func_reflexive
 // (aran/aransolver3d.c:254-324) with the Newton callbacks of the reference
-// harness (tests/newtonpot3.c:61-78, tests/newtonfield3.c:91-119): each
+// harness (tests/newtonpot3.c:61-78, tests/newtonfield3.c:91-119).  Each
 // target leaf gathers from its list of source particle ranges, so every
-// result is written by exactly one thread (no atomics, fixed order).  Only
-// i == j is excluded.
+// result is written once, in a fixed order (no atomics).  Only i == j is
+// excluded.
+//
+// One 256-thread block per target leaf: the leaf's targets are spread over
+// the block as (target, slice) pairs, the near sources are staged in LDS in
+// passes of kStage particles (coordinates relative to the leaf centre), and
+// the slices of each target are summed in a fixed order at the end.  T is
+// the arithmetic type of one interaction (double: plain form; float: mixed
+// form, where each staged pass is summed in FP32 and passes in FP64).
 #include <hip/hip_runtime.h>
 
 #include "engine.h"
 
 namespace aran {
 
-constexpr int kP2PThreads = 64;
+constexpr int kNearThreads = 256;
+constexpr int kStage = 1024;
+constexpr int kListChunk = 256;
 
-__global__ __launch_bounds__(kP2PThreads) void k_p2p(
-    const int *__restrict__ leaves, const int *__restrict__ lpb,
-    const int *__restrict__ lpe, const int *__restrict__ uofs,
+template <typename T> __device__ inline T rsqrt_t(T x);
+template <> __device__ inline double rsqrt_t<double>(double x) {
+  return 1.0 / sqrt(x);
+}
+template <> __device__ inline float rsqrt_t<float>(float x) {
+  return rsqrtf(x);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kNearThreads) void k_near(
+    const int *__restrict__ leaves, NodeArrays nd, const int *__restrict__ uofs,
     const int *__restrict__ ubeg, const int *__restrict__ uend,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double *__restrict__ qs,
     double *__restrict__ pot, double *__restrict__ grad) {
-  __shared__ double sx[kP2PThreads], sy[kP2PThreads], sz[kP2PThreads],
-      sq[kP2PThreads];
+  __shared__ T sx[kStage], sy[kStage], sz[kStage], sq[kStage];
+  __shared__ int sid[kStage];
+  __shared__ int lbeg[kListChunk], lpre[kListChunk + 1];
+  __shared__ double red[4][kNearThreads];
+  const int tid = threadIdx.x;
   const int A = leaves[blockIdx.x];
-  const int tb = lpb[A], te = lpe[A];
+  const int tb = nd.pbegin[A], te = nd.pend[A];
+  const double ox = nd.cx[A], oy = nd.cy[A], oz = nd.cz[A];
   const int e0 = uofs[blockIdx.x], e1 = uofs[blockIdx.x + 1];
-  for (int t0 = tb; t0 < te; t0 += kP2PThreads) {
-    const int i = t0 + threadIdx.x;
-    const bool act = i < te;
-    const double xi = act ? xs[i] : 0., yi = act ? ys[i] : 0.,
-                 zi = act ? zs[i] : 0.;
-    double ph = 0., gx = 0., gy = 0., gz = 0.;
-    for (int e = e0; e < e1; e++) {
-      const int sb = ubeg[e], se = uend[e];
-      for (int s0 = sb; s0 < se; s0 += kP2PThreads) {
-        int j = s0 + threadIdx.x;
-        if (j < se) {
-          sx[threadIdx.x] = xs[j];
-          sy[threadIdx.x] = ys[j];
-          sz[threadIdx.x] = zs[j];
-          sq[threadIdx.x] = qs[j];
+  for (int t0 = tb; t0 < te; t0 += kNearThreads) {
+    const int ntc = min(kNearThreads, te - t0);
+    const int nsl = kNearThreads / ntc;
+    const int my_t = tid % ntc, my_s = tid / ntc;
+    const bool act = my_s < nsl;
+    const int i = t0 + my_t;
+    const T xi = (T)(xs[i] - ox), yi = (T)(ys[i] - oy), zi = (T)(zs[i] - oz);
+    double ap = 0., ax = 0., ay = 0., az = 0.;
+    for (int c0 = e0; c0 < e1; c0 += kListChunk) {
+      const int nc = min(kListChunk, e1 - c0);
+      __syncthreads();
+      if (tid < nc) lbeg[tid] = ubeg[c0 + tid];
+      if (tid == 0) {
+        int s = 0;
+        for (int k = 0; k < nc; k++) {
+          lpre[k] = s;
+          s += uend[c0 + k] - ubeg[c0 + k];
+        }
+        lpre[nc] = s;
+      }
+      __syncthreads();
+      const int total = lpre[nc];
+      for (int s0 = 0; s0 < total; s0 += kStage) {
+        const int ns = min(kStage, total - s0);
+        for (int f = tid; f < ns; f += kNearThreads) {
+          const int g = s0 + f;
+          int lo = 0, hi = nc - 1;
+          while (lo < hi) {  // last k with lpre[k] <= g
+            int mid = (lo + hi + 1) >> 1;
+            if (lpre[mid] <= g)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          const int j = lbeg[lo] + (g - lpre[lo]);
+          sx[f] = (T)(xs[j] - ox);
+          sy[f] = (T)(ys[j] - oy);
+          sz[f] = (T)(zs[j] - oz);
+          sq[f] = (T)qs[j];
+          sid[f] = j;
         }
         __syncthreads();
-        const int cnt = min(kP2PThreads, se - s0);
         if (act) {
-          for (int jj = 0; jj < cnt; jj++) {
-            if (s0 + jj == i) continue;
-            double dx = xi - sx[jj], dy = yi - sy[jj], dz = zi - sz[jj];
-            double r2 = dx * dx + dy * dy + dz * dz;
-            double ir = 1. / sqrt(r2);
-            double qr = sq[jj] * ir;
-            double qr3 = qr * ir * ir;
-            ph += qr;
+          T p = 0, gx = 0, gy = 0, gz = 0;
+          for (int f = my_s; f < ns; f += nsl) {
+            const T dx = xi - sx[f], dy = yi - sy[f], dz = zi - sz[f];
+            const T r2 = dx * dx + dy * dy + dz * dz;
+            const T ir = (sid[f] == i) ? (T)0 : rsqrt_t<T>(r2);
+            const T qr = sq[f] * ir;
+            const T qr3 = qr * ir * ir;
+            p += qr;
             gx -= qr3 * dx;
             gy -= qr3 * dy;
             gz -= qr3 * dz;
           }
+          ap += (double)p;
+          ax += (double)gx;
+          ay += (double)gy;
+          az += (double)gz;
         }
         __syncthreads();
       }
     }
-    if (act) {
-      pot[i] = ph;
-      grad[3 * (size_t)i + 0] = gx;
-      grad[3 * (size_t)i + 1] = gy;
-      grad[3 * (size_t)i + 2] = gz;
+    red[0][tid] = ap;
+    red[1][tid] = ax;
+    red[2][tid] = ay;
+    red[3][tid] = az;
+    __syncthreads();
+    if (tid < ntc) {
+      double s0 = 0., s1 = 0., s2 = 0., s3 = 0.;
+      for (int s = 0; s < nsl; s++) {
+        const int k = s * ntc + tid;
+        s0 += red[0][k];
+        s1 += red[1][k];
+        s2 += red[2][k];
+        s3 += red[3][k];
+      }
+      pot[i] = s0;
+      grad[3 * (size_t)i + 0] = s1;
+      grad[3 * (size_t)i + 1] = s2;
+      grad[3 * (size_t)i + 2] = s3;
     }
+    __syncthreads();
   }
 }
 
-hipError_t launch_p2p(const int *leaves, int nleaves, const int *lpb,
-                      const int *lpe, const int *uofs, const int *ubeg,
+hipError_t launch_p2p(int precision, const int *leaves, int nleaves,
+                      const NodeArrays &nd, const int *uofs, const int *ubeg,
                       const int *uend, const double *xs, const double *ys,
                       const double *zs, const double *qs, double *pot,
                       double *grad, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
-  k_p2p<<<nleaves, kP2PThreads, 0, s>>>(leaves, lpb, lpe, uofs, ubeg, uend, xs,
-                                        ys, zs, qs, pot, grad);
+  if (precision == 0)
+    k_near<double><<<nleaves, kNearThreads, 0, s>>>(leaves, nd, uofs, ubeg,
+                                                    uend, xs, ys, zs, qs, pot,
+                                                    grad);
+  else
+    k_near<float><<<nleaves, kNearThreads, 0, s>>>(leaves, nd, uofs, ubeg,
+                                                   uend, xs, ys, zs, qs, pot,
+                                                   grad);
   return hipGetLastError();
 }
 

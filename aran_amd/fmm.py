@@ -42,6 +42,14 @@ class Fmm3d:
         except Exception:
             pass
 
+    PLAIN, MIXED = 0, 1
+
+    def set_precision(self, precision):
+        """"plain" (FP64 throughout) or "mixed" (FP32 M2L and P2P)."""
+        code = {"plain": 0, "mixed": 1}.get(precision, precision)
+        check(lib().aran_fmm3d_set_precision(self._ctx, int(code)),
+              "set_precision")
+
     def set_timings(self, on=True):
         check(lib().aran_fmm3d_set_timings(self._ctx, int(on)), "set_timings")
 

@@ -24,6 +24,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 MI355X_FP64_TFLOPS = 78.6
+MI355X_FP32_TFLOPS = 157.3
 MI355X_HBM_GBS = 8000.0
 
 
@@ -54,6 +55,44 @@ def cpu_baseline(n_sample, order, leafcap, seed):
                       % (n_sample, order, leafcap, dt)}
 
 
+def run_form(f, dev, pot, grad, lb, ub, p, q, precision, steps, warmup):
+    import torch
+    n = dev[0].numel()
+    f.set_precision(precision)
+    f.set_timings(False)
+    for _ in range(warmup):
+        f.solve(*dev, lb, ub, p, q, pot, grad)
+    f.set_timings(True)
+    torch.cuda.synchronize()
+    t_m2l, phases = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.solve(*dev, lb, ub, p, q, pot, grad)
+        tm = f.timings()
+        t_m2l.append(tm["m2l"])
+        phases.append(tm)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    mean_phase = {k: round(float(np.mean([ph[k] for ph in phases])), 4)
+                  for k in phases[0]}
+    return {"value": n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "m2l_ms": float(np.mean(t_m2l)), "phases_ms": mean_phase,
+            "stats": f.stats()}
+
+
+def error_vs_direct(f, dev, pot, grad, n, seed):
+    import torch
+    from aran_amd import workloads as W
+    tg = W.sample_targets(n, 4096, seed)
+    dpot, dgrad = f.direct(*dev, torch.from_numpy(tg))
+    dpot, dgrad = dpot.cpu().numpy(), dgrad.cpu().numpy()
+    pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
+    return {"pot_l2": float(np.linalg.norm(pp - dpot) / np.linalg.norm(dpot)),
+            "pot_max": float(np.max(np.abs(pp - dpot) / np.abs(dpot))),
+            "grad_l2": float(np.linalg.norm(gg - dgrad) / np.linalg.norm(dgrad)),
+            "targets": 4096}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,9 +101,12 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--order", type=int, default=10)
     ap.add_argument("--leafcap", type=int, default=64)
-    ap.add_argument("--cpu-sample", type=int, default=32768)
+    ap.add_argument("--precision", choices=("mixed", "plain"), default="mixed")
+    ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 comparison run")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not "
@@ -85,62 +127,53 @@ def main():
     f = Fmm3d()
     pot = torch.empty(n, dtype=torch.float64, device="cuda")
     grad = torch.empty((n, 3), dtype=torch.float64, device="cuda")
-    for _ in range(args.warmup):
-        f.solve(*dev, lb, ub, p, q, pot, grad)
-    f.set_timings(True)
-    torch.cuda.synchronize()
-    t_m2l, t_tot = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        f.solve(*dev, lb, ub, p, q, pot, grad)
-        tm = f.timings()
-        t_m2l.append(tm["m2l"])
-        t_tot.append(tm["total"])
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    st = f.stats()
-    tm = f.timings()
-    ms_per_step = 1e3 * elapsed / args.steps
-    value = n * args.steps / elapsed
 
-    m2l_ms = float(np.mean(t_m2l))
+    head = run_form(f, dev, pot, grad, lb, ub, p, q, args.precision,
+                    args.steps, args.warmup)
+    err = None if args.no_check else error_vs_direct(f, dev, pot, grad, n,
+                                                     seed)
+    plain = None
+    if args.precision == "mixed" and not args.no_plain:
+        r = run_form(f, dev, pot, grad, lb, ub, p, q, "plain",
+                     max(2, args.steps // 2), 1)
+        plain = {"value": r["value"], "ms_per_step": r["ms_per_step"],
+                 "m2l_ms": round(r["m2l_ms"], 4), "phases_ms": r["phases_ms"]}
+        if not args.no_check:
+            plain["error"] = error_vs_direct(f, dev, pot, grad, n, seed)
+        plain["mixed_over_plain"] = head["value"] / r["value"]
+
+    st = head["stats"]
+    mixed = args.precision == "mixed"
+    peak = MI355X_FP32_TFLOPS if mixed else MI355X_FP64_TFLOPS
     flops = st["m2l"] * m2l_flops(p)
-    achieved = flops / (m2l_ms * 1e-3) / 1e12
-    roof = {"bound": "mfma", "kernel": "k_m2l (FP64)",
-            "achieved": round(achieved, 4), "peak": MI355X_FP64_TFLOPS,
-            "unit": "TFLOP/s", "frac": round(achieved / MI355X_FP64_TFLOPS, 5),
-            "traffic": None,
+    achieved = flops / (head["m2l_ms"] * 1e-3) / 1e12
+    roof = {"bound": "mfma",
+            "kernel": "k_m2l_tile (FP32)" if mixed else "k_m2l (FP64)",
+            "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
+            "frac": round(achieved / peak, 5), "traffic": None,
             "algorithmic": "%d M2L x %d flops" % (st["m2l"], m2l_flops(p)),
-            "kernel_ms": round(m2l_ms, 4)}
-
-    err = None
-    if not args.no_check:
-        tg = W.sample_targets(n, 4096, seed)
-        dpot, dgrad = f.direct(*dev, torch.from_numpy(tg))
-        dpot, dgrad = dpot.cpu().numpy(), dgrad.cpu().numpy()
-        pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
-        err = {"pot_l2": float(np.linalg.norm(pp - dpot) / np.linalg.norm(dpot)),
-               "pot_max": float(np.max(np.abs(pp - dpot) / np.abs(dpot))),
-               "grad_l2": float(np.linalg.norm(gg - dgrad) /
-                                np.linalg.norm(dgrad)),
-               "targets": 4096}
+            "kernel_ms": round(head["m2l_ms"], 4)}
 
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, p, q, seed)
     line = {
         "metric": "3D Newton FMM particles/s at N=2^20 (Plummer, p=10, "
                   "leaf 64), rel err vs FP64 direct sum",
-        "value": value, "unit": "particles/s", "n_gpus": 1,
+        "value": head["value"], "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if mixed else "f64",
         "data": "synthetic (seeded Plummer sphere, a=0.1, r<=1)",
-        "config": {"workload": "plummer_2p20_p10_q64_fp64", "n": n,
-                   "order": p, "leafcap": q, "form": "plain FP64",
-                   "outputs": "potential+gradient"},
+        "config": {"workload": "plummer_2p20_p10_q64_%s" % args.precision,
+                   "n": n, "order": p, "leafcap": q,
+                   "form": ("mixed: FP32 M2L and P2P on per-level scaled "
+                            "expansions, FP64 elsewhere") if mixed else
+                   "plain FP64", "outputs": "potential+gradient"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "error": err,
-        "phases_ms": {k: round(v, 4) for k, v in tm.items()},
+        "phases_ms": head["phases_ms"],
+        "plain_fp64": plain,
         "tree": {k: st[k] for k in ("depth", "nodes", "leaves", "m2l", "p2p")},
     }
     print(json.dumps(line))

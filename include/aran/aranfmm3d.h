@@ -37,6 +37,14 @@ int aran_fmm3d_solve (AranFmm3d *ctx, long n, const double *x,
                       const double *lb, const double *ub, int order,
                       int leafcap, double *pot, double *grad);
 
+/* Arithmetic form of the next solves: ARAN_FMM3D_PLAIN (FP64 throughout) or
+ * ARAN_FMM3D_MIXED (far field M2L and near field P2P in FP32 on expansions
+ * scaled per level by the box width, FP64 tables, tree, P2M, M2M, L2L, L2P
+ * and accumulation across staged source passes). */
+#define ARAN_FMM3D_PLAIN 0
+#define ARAN_FMM3D_MIXED 1
+int aran_fmm3d_set_precision (AranFmm3d *ctx, int precision);
+
 /* Counters of the last solve: zero, p2p, p2m, m2m, m2l, l2l, l2p (the
  * reference's semantics), then depth, nodes, leaves, far-list entries,
  * near-list entries. */

@@ -341,3 +341,30 @@ def test_solver_api_newtonpot3_style(torch_cuda):
     assert L.aran_solver3d_status(solver) != 0
     assert np.array_equal(before, np.array([pts[i].pot for i in range(n)]))
     L.aran_solver3d_free(solver)
+
+
+@pytest.mark.parametrize("kind,n,p,q", [("uniform", 10_000, 8, 20),
+                                        ("plummer", 20_000, 10, 64),
+                                        ("uniform", 3000, 14, 32),
+                                        ("plummer", 5000, 6, 16),
+                                        ("outside", 2000, 3, 8)])
+def test_mixed_form_close_to_plain(torch_cuda, kind, n, p, q):
+    """Mixed form (FP32 M2L on scaled expansions, FP32 P2P) against the plain
+    FP64 form on the same tree: potential L2 relative difference <= 5e-7 and
+    pointwise <= 1e-5, gradient L2 <= 5e-6; counters identical."""
+    torch = torch_cuda
+    (x, y, z, m), lb, ub = _tree_case(kind, n, 9)
+    f = _fmm(torch)
+    dev = _dev(torch, x, y, z, m)
+    f.set_precision("plain")
+    p0, g0 = [t.cpu().numpy() for t in f.solve(*dev, lb, ub, p, q)]
+    s0 = f.stats()
+    f.set_precision("mixed")
+    p1, g1 = [t.cpu().numpy() for t in f.solve(*dev, lb, ub, p, q)]
+    assert f.stats() == s0
+    assert np.linalg.norm(p1 - p0) / np.linalg.norm(p0) <= 5e-7
+    assert _relerr_pot(p1, p0) <= 1e-5
+    assert np.linalg.norm(g1 - g0) / np.linalg.norm(g0) <= 5e-6
+    # deterministic: a second mixed solve is bitwise identical
+    p2, g2 = [t.cpu().numpy() for t in f.solve(*dev, lb, ub, p, q)]
+    assert np.array_equal(p1, p2) and np.array_equal(g1, g2)
