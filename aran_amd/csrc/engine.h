@@ -89,7 +89,8 @@ struct Workspace {
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
   DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
-  DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs;
+  DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
+      colleagues;
   DevBuf mult, loc, small;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, mult32, loc32;
   long node_cap = 0;
@@ -166,13 +167,14 @@ hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
                              hipStream_t s);
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
                                const int *pos, int *leaves, hipStream_t s);
-hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *lstart,
-                        const int *lcount, int pass, int *cnt, const int *ofs,
-                        int *src, short *code, hipStream_t s);
+hipError_t launch_colleagues(const NodeArrays &nd, int first, int count,
+                             int *col, hipStream_t s);
+hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
+                        int pass, int *cnt, const int *ofs, int *src,
+                        short *code, hipStream_t s);
 hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
-                        const int *lstart, const int *lcount, int pass,
-                        int *cnt, const int *ofs, int *ubeg, int *uend,
-                        long long *pairs, hipStream_t s);
+                        const int *col, int pass, int *cnt, const int *ofs,
+                        int *ubeg, int *uend, long long *pairs, hipStream_t s);
 hipError_t launch_sum_ll(const long long *v, int n, long long *out,
                          hipStream_t s);
 

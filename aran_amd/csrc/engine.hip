@@ -8,6 +8,7 @@
 // order, so a solve is deterministic.
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -118,7 +119,8 @@ Workspace::~Workspace() {
                    &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz,
                    &split_bounds, &split_cnt, &split_ofs, &flags, &leaf_list,
                    &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
-                   &u_ofs, &u_beg, &u_end, &u_pairs, &mult, &loc, &small,
+                   &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &mult, &loc,
+                   &small,
                    &tab.rot, &tab.eiphi, &tab.dir, &tab.tm2l, &tab.tm2m,
                    &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas, &tab.rot32,
                    &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
@@ -213,6 +215,18 @@ void octant_offsets(const std::vector<DirEntry> &dirs, std::vector<short> &codes
       code_to_k[oct * kNumCodes + list[k]] = (short)k;
     }
   }
+}
+
+// Waves per far-field tile: the 189 offsets of a tile are split in this many
+// ranges so that a solve has several waves per SIMD (ARAN_M2L_SPLIT
+// overrides, 1..16).
+static int m2l_split() {
+  static int v = [] {
+    const char *e = getenv("ARAN_M2L_SPLIT");
+    int x = e ? atoi(e) : 8;
+    return std::min(16, std::max(1, x));
+  }();
+  return v;
 }
 
 hipError_t ensure_tables(Workspace &ws, int p) {
@@ -483,35 +497,39 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                     hipMemcpyHostToDevice, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[1], s));
 
+  // ---- colleagues, level by level
+  CK(ws.colleagues.reserve(sizeof(int) * 27 * (size_t)nn));
+  int *col = ws.colleagues.as<int>();
+  for (int d = 0; d <= depth; d++)
+    CK(launch_colleagues(nd, lstart[d], lcount[d], col, s));
   // ---- far list
   CK(ws.v_cnt.reserve(sizeof(int) * nn));
   CK(ws.v_ofs.reserve(sizeof(int) * (nn + 1)));
-  CK(launch_vlist(nd, nn, ws.lstart.as<int>(), ws.lcount.as<int>(), 0,
-                  ws.v_cnt.as<int>(), nullptr, nullptr, nullptr, s));
+  CK(launch_vlist(nd, nn, col, 0, ws.v_cnt.as<int>(), nullptr, nullptr,
+                  nullptr, s));
   CK(scan_exclusive(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
                     ws.v_ofs.as<int>() + nn, ws.scan_tmp, s));
   int nfar = 0;
   CK(read_int(ws, ws.v_ofs.as<int>() + nn, &nfar));
   CK(ws.v_src.reserve(sizeof(int) * std::max(1, nfar)));
   CK(ws.v_code.reserve(sizeof(short) * std::max(1, nfar)));
-  CK(launch_vlist(nd, nn, ws.lstart.as<int>(), ws.lcount.as<int>(), 1,
-                  ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), ws.v_src.as<int>(),
-                  ws.v_code.as<short>(), s));
+  CK(launch_vlist(nd, nn, col, 1, ws.v_cnt.as<int>(), ws.v_ofs.as<int>(),
+                  ws.v_src.as<int>(), ws.v_code.as<short>(), s));
   // ---- near list
   CK(ws.u_cnt.reserve(sizeof(int) * nleaves));
   CK(ws.u_ofs.reserve(sizeof(int) * (nleaves + 1)));
   CK(ws.u_pairs.reserve(sizeof(long long) * (nleaves + 1)));
-  CK(launch_ulist(nd, nleaves, leaves, ws.lstart.as<int>(), ws.lcount.as<int>(),
-                  0, ws.u_cnt.as<int>(), nullptr, nullptr, nullptr, nullptr, s));
+  CK(launch_ulist(nd, nleaves, leaves, col, 0, ws.u_cnt.as<int>(), nullptr,
+                  nullptr, nullptr, nullptr, s));
   CK(scan_exclusive(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
                     ws.u_ofs.as<int>() + nleaves, ws.scan_tmp, s));
   int nnear = 0;
   CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
   CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
   CK(ws.u_end.reserve(sizeof(int) * std::max(1, nnear)));
-  CK(launch_ulist(nd, nleaves, leaves, ws.lstart.as<int>(), ws.lcount.as<int>(),
-                  1, ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), ws.u_beg.as<int>(),
-                  ws.u_end.as<int>(), ws.u_pairs.as<long long>(), s));
+  CK(launch_ulist(nd, nleaves, leaves, col, 1, ws.u_cnt.as<int>(),
+                  ws.u_ofs.as<int>(), ws.u_beg.as<int>(), ws.u_end.as<int>(),
+                  ws.u_pairs.as<long long>(), s));
   long long *d_pairs = reinterpret_cast<long long *>(ws.small.as<char>() + 1024);
   CK(launch_sum_ll(ws.u_pairs.as<long long>(), nleaves, d_pairs, s));
   // ---- tiles of the far field (mixed form)
@@ -575,7 +593,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
   } else {
     CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
-    CK(ws.loc32.reserve(sizeof(float2) * (size_t)nn * lc));
+    const int nsplit = m2l_split();
+    const long split_stride = (long)nn * lc;
+    CK(ws.loc32.reserve(sizeof(float2) * (size_t)nsplit * split_stride));
     CK(launch_scale_mult(1, nd, nn, p, width, M, ws.mult32.ptr, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
     M2LTileArgs ta;
@@ -587,9 +607,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     ta.ephi = ws.tab.ephi32.ptr;
     ta.rot = ws.tab.rot32.ptr;
     ta.vfac = ws.tab.vfac32.ptr;
+    ta.nsplit = nsplit;
+    ta.split_stride = split_stride;
     CK(launch_m2l_tile(1, p, ntiles, ta, ws.mult32.ptr, ws.loc32.ptr, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
-    CK(launch_unscale_local(1, nd, nn, p, width, ws.loc32.ptr, L, s));
+    CK(launch_unscale_local(1, nd, nn, p, width, ws.loc32.ptr, nsplit,
+                            split_stride, L, s));
   }
   // ---- downward pass
   for (int d = 1; d <= depth; d++)

@@ -152,10 +152,12 @@ __global__ void k_scale_mult(NodeArrays nd, int nnodes, int p, double width,
   Ms[idx] = o;
 }
 
-// L = Ls / w^(l+1) for nodes of depth >= 2, zero above.
+// L = (sum of the nsplit partial slices of Ls, in slice order) / w^(l+1)
+// for nodes of depth >= 2, zero above.
 template <typename T>
 __global__ void k_unscale_local(NodeArrays nd, int nnodes, int p, double width,
                                 const typename V2<T>::type *__restrict__ Ls,
+                                int nsplit, long split_stride,
                                 double2 *__restrict__ L) {
   const int lc = tri_p(p + 1);
   long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -170,8 +172,13 @@ __global__ void k_unscale_local(NodeArrays nd, int nnodes, int p, double width,
   double w = ldexp(width, -nd.depth[node]);
   double f = 1. / w;
   for (int i = 0; i < l; i++) f /= w;
-  typename V2<T>::type v = Ls[idx];
-  L[idx] = make_double2((double)v.x * f, (double)v.y * f);
+  double sx = 0., sy = 0.;
+  for (int sl = 0; sl < nsplit; sl++) {
+    typename V2<T>::type v = Ls[sl * split_stride + idx];
+    sx += (double)v.x;
+    sy += (double)v.y;
+  }
+  L[idx] = make_double2(sx * f, sy * f);
 }
 
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
@@ -189,19 +196,20 @@ hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
 }
 
 hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int nnodes,
-                                int p, double width, const void *Ls, double2 *L,
-                                hipStream_t s) {
+                                int p, double width, const void *Ls, int nsplit,
+                                long split_stride, double2 *L, hipStream_t s) {
   long n = (long)nnodes * tri_p(p + 1);
   if (n == 0) return hipSuccess;
   if (precision == 0)
     k_unscale_local<double><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, width, static_cast<const double2 *>(Ls), L);
+        nd, nnodes, p, width, static_cast<const double2 *>(Ls), nsplit,
+        split_stride, L);
   else
     k_unscale_local<float><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, width, static_cast<const float2 *>(Ls), L);
+        nd, nnodes, p, width, static_cast<const float2 *>(Ls), nsplit,
+        split_stride, L);
   return hipGetLastError();
 }
-
 // ------------------------------------------------------------ tables
 
 // Per offset code: (-1)^(l+m) c^(l+n) T(l,m,n) / |o|^(l+n+1) in type T.

@@ -20,6 +20,8 @@ struct M2LTileArgs {
   const void *ephi;       // [343][2(p+1)]   e^{i m phi} as T
   const void *rot;        // [ntheta][4][rot_block(p)] as T
   const void *vfac;       // [343][(p+1)(p+1)p] scaled vertical factors
+  int nsplit;             // waves per tile (offset ranges), >= 1
+  long split_stride;      // elements between partial-local slices
 };
 
 // Octant offset lists (host): codes[8][189] sorted by theta table index,
@@ -44,8 +46,8 @@ hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
                              int p, double width, const double2 *M, void *Ms,
                              hipStream_t s);
 hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int nnodes,
-                                int p, double width, const void *Ls, double2 *L,
-                                hipStream_t s);
+                                int p, double width, const void *Ls, int nsplit,
+                                long split_stride, double2 *L, hipStream_t s);
 hipError_t launch_m2l_tables(int precision, int p, const DirEntry *dir,
                              const double *tm2l, const double *rot, long nrot,
                              const double2 *eiphi, void *vfac, void *rotT,

@@ -7,9 +7,11 @@
 // targets and walks the offsets in a fixed order.  For each offset every
 // lane translates its own source, while the rotation matrices (by theta),
 // the phases (by phi) and the vertical factors (by |offset|) are the same
-// for the whole wave and are read from LDS as broadcasts.  Each lane
-// accumulates its target's local in registers, so the sum over sources has
-// a fixed order and needs no atomics.
+// for the whole wave and come through the scalar cache as SGPR operands.
+// Each lane accumulates its target's local in registers, so the sum over
+// sources has a fixed order and needs no atomics.  The 189 offsets of a tile
+// are split over several waves (grid y); their partial locals are summed in
+// a fixed order by the unscaling kernel.
 //
 // Expansions are scaled per level by the box width w: M^_n = M_n / w^n,
 // L^_l = L_l w^(l+1).  Then the vertical factor of offset o (box units) is
@@ -53,47 +55,40 @@ constexpr __host__ __device__ int rofs_c(int l) {
 }
 
 template <int P, typename T>
-__global__ __launch_bounds__(64) void k_m2l_tile(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void k_m2l_tile(
     const int *__restrict__ tile_tgt, const int *__restrict__ tile_src,
     const int *__restrict__ tile_oct, const short *__restrict__ oct_codes,
     const DirEntry *__restrict__ dir, const T *__restrict__ ephi,
     const T *__restrict__ rotT, const T *__restrict__ vfac,
     const typename V2<T>::type *__restrict__ Ms,
-    typename V2<T>::type *__restrict__ Ls) {
+    typename V2<T>::type *__restrict__ Ls, int nsplit, long split_stride) {
   typedef typename V2<T>::type T2;
   constexpr int MC = Dims<P>::MC, LC = Dims<P>::LC, RB = Dims<P>::RB,
                 NV = Dims<P>::NV;
-  __shared__ T sR[4 * RB];
-  __shared__ T sV[NV];
-  __shared__ T sP[2 * (P + 1)];
-  __shared__ T2 sAcc[LC][64];  // the lane's target local, lane-minor
   const int tile = blockIdx.x, lane = threadIdx.x;
+  // this wave takes offsets [k0, k1) of the tile; its partial local goes to
+  // slice blockIdx.y of Ls (summed in slice order by the unscaling kernel)
+  const int chunk = (kOctOffsets + nsplit - 1) / nsplit;
+  const int k0 = blockIdx.y * chunk;
+  const int k1 = min(kOctOffsets, k0 + chunk);
   const int A = tile_tgt[tile * 64 + lane];
-  const int oct = tile_oct[tile];
-  for (int k = 0; k < LC; k++) {
-    T2 z;
-    z.x = 0;
-    z.y = 0;
-    sAcc[k][lane] = z;
-  }
-  int cur_theta = -1;
-  for (int k = 0; k < kOctOffsets; k++) {
+  const int oct = __builtin_amdgcn_readfirstlane(tile_oct[tile]);
+  T ar[LC], ai[LC];  // the lane's target local
+  static_for<0, LC>([&](auto kc) {
+    constexpr int k = decltype(kc)::value;
+    ar[k] = 0;
+    ai[k] = 0;
+  });
+  for (int k = k0; k < k1; k++) {
     const int B = tile_src[((size_t)tile * kOctOffsets + k) * 64 + lane];
     if (__ballot(B >= 0) == 0ull) continue;
-    const int code = oct_codes[oct * kOctOffsets + k];
-    const int th = dir[code].theta;
-    __syncthreads();
-    if (th != cur_theta) {
-      const T *src = rotT + (size_t)th * 4 * RB;
-      for (int i = lane; i < 4 * RB; i += 64) sR[i] = src[i];
-      cur_theta = th;
-    }
-    {
-      const T *src = vfac + (size_t)code * NV;
-      for (int i = lane; i < NV; i += 64) sV[i] = src[i];
-      if (lane < 2 * (P + 1)) sP[lane] = ephi[code * 2 * (P + 1) + lane];
-    }
-    __syncthreads();
+    // wave-uniform operands: rotation matrices of theta, vertical factors
+    // and phases of this offset, read through the scalar cache
+    const int code = __builtin_amdgcn_readfirstlane(oct_codes[oct * kOctOffsets + k]);
+    const int th = __builtin_amdgcn_readfirstlane(dir[code].theta);
+    const T *__restrict__ sR = rotT + (size_t)th * 4 * RB;
+    const T *__restrict__ sV = vfac + (size_t)code * NV;
+    const T *__restrict__ sP = ephi + code * 2 * (P + 1);
     if (B >= 0) {
       T xr[MC], xi[MC];
       const T2 *mb = Ms + (size_t)B * MC;
@@ -158,17 +153,21 @@ __global__ __launch_bounds__(64) void k_m2l_tile(
             wi += fm[m] * zi[m];
           });
           const T c = sP[2 * mp], s = sP[2 * mp + 1];
-          T2 acc = sAcc[tri_c(l) + mp][lane];
-          acc.x += c * wr + s * wi;
-          acc.y += c * wi - s * wr;
-          sAcc[tri_c(l) + mp][lane] = acc;
+          ar[tri_c(l) + mp] += c * wr + s * wi;
+          ai[tri_c(l) + mp] += c * wi - s * wr;
         });
       });
     }
   }
   if (A >= 0) {
-    T2 *out = Ls + (size_t)A * LC;
-    for (int k = 0; k < LC; k++) out[k] = sAcc[k][lane];
+    T2 *out = Ls + blockIdx.y * split_stride + (size_t)A * LC;
+    static_for<0, LC>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      T2 v;
+      v.x = ar[k];
+      v.y = ai[k];
+      out[k] = v;
+    });
   }
 }
 
@@ -176,11 +175,11 @@ template <int P, typename T>
 static void launch_one(int ntiles, const M2LTileArgs &a, const void *Ms,
                        void *Ls, hipStream_t s) {
   typedef typename V2<T>::type T2;
-  k_m2l_tile<P, T><<<ntiles, 64, 0, s>>>(
+  k_m2l_tile<P, T><<<dim3(ntiles, a.nsplit), 64, 0, s>>>(
       a.tile_tgt, a.tile_src, a.tile_oct, a.oct_codes, a.dir,
       static_cast<const T *>(a.ephi), static_cast<const T *>(a.rot),
       static_cast<const T *>(a.vfac), static_cast<const T2 *>(Ms),
-      static_cast<T2 *>(Ls));
+      static_cast<T2 *>(Ls), a.nsplit, a.split_stride);
 }
 
 // Orders instantiated in this object (the build compiles this file once per

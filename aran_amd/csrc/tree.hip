@@ -499,112 +499,113 @@ hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
 
 // ------------------------------------------------------------ lists
 
-// Node of depth d at integer coordinates (ix, iy, iz), or -1.  Nodes of one
-// level are contiguous and sorted by their Morton key.
-__device__ inline int lookup(const NodeArrays &nd,
-                             const int *__restrict__ lstart,
-                             const int *__restrict__ lcount, int d, int ix,
-                             int iy, int iz) {
-  int lim = 1 << d;
-  if (ix < 0 || iy < 0 || iz < 0 || ix >= lim || iy >= lim || iz >= lim)
-    return -1;
-  uint64_t key = morton_key((uint64_t)ix, (uint64_t)iy, (uint64_t)iz);
-  int b = lstart[d], e = lstart[d] + lcount[d];
-  while (b < e) {
-    int mid = b + ((e - b) >> 1);
-    if (nd.key[mid] < key)
-      b = mid + 1;
-    else
-      e = mid;
+// Colleagues: the 27 same-depth nodes adjacent to (or equal to) a node,
+// stored by offset slot (dx+1)*9 + (dy+1)*3 + (dz+1), -1 when absent.  The
+// colleagues of a node are children of its parent's colleagues, so one
+// launch per level, root first, fills them without any search.
+__global__ void k_colleagues(NodeArrays nd, int first, int count,
+                             int *__restrict__ col) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  int A = first + i;
+  int *ca = col + (size_t)A * 27;
+  for (int k = 0; k < 27; k++) ca[k] = -1;
+  if (nd.depth[A] == 0) {
+    ca[13] = A;
+    return;
   }
-  return (b < lstart[d] + lcount[d] && nd.key[b] == key) ? b : -1;
+  const int P = nd.parent[A];
+  const int ax = nd.ix[A], ay = nd.iy[A], az = nd.iz[A];
+  for (int s = 0; s < 27; s++) {
+    int C = col[(size_t)P * 27 + s];
+    if (C < 0 || nd.leaf[C]) continue;
+    for (int K = nd.child[C]; K < nd.child[C] + nd.nchild[C]; K++) {
+      int dx = nd.ix[K] - ax, dy = nd.iy[K] - ay, dz = nd.iz[K] - az;
+      if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1)
+        ca[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = K;
+    }
+  }
 }
 
-// M2L (far) list of every node: same-depth nodes whose parents are
-// adjacent or equal to this node's parent and which are not adjacent to it.
-// pass 0 counts, pass 1 fills.
-__global__ void k_vlist(NodeArrays nd, int nnodes,
-                        const int *__restrict__ lstart,
-                        const int *__restrict__ lcount, int pass,
-                        int *__restrict__ cnt, const int *__restrict__ ofs,
-                        int *__restrict__ src, short *__restrict__ code) {
+hipError_t launch_colleagues(const NodeArrays &nd, int first, int count,
+                             int *col, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  k_colleagues<<<(count + 127) / 128, 128, 0, s>>>(nd, first, count, col);
+  return hipGetLastError();
+}
+
+// M2L (far) list of every node: children of the parent's colleagues that
+// are not adjacent to the node.  pass 0 counts, pass 1 fills.
+__global__ void k_vlist(NodeArrays nd, int nnodes, const int *__restrict__ col,
+                        int pass, int *__restrict__ cnt,
+                        const int *__restrict__ ofs, int *__restrict__ src,
+                        short *__restrict__ code) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nnodes) return;
-  int d = nd.depth[i];
   int c = 0;
   int pos = pass ? ofs[i] : 0;
-  if (d >= 1) {
-    int P = nd.parent[i];
-    int ax = nd.ix[i], ay = nd.iy[i], az = nd.iz[i];
-    int px = nd.ix[P], py = nd.iy[P], pz = nd.iz[P];
-    for (int dx = -1; dx <= 1; dx++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dz = -1; dz <= 1; dz++) {
-          int j = lookup(nd, lstart, lcount, d - 1, px + dx, py + dy, pz + dz);
-          if (j < 0 || nd.leaf[j]) continue;
-          int cb = nd.child[j], ce = cb + nd.nchild[j];
-          for (int b = cb; b < ce; b++) {
-            int ox = ax - nd.ix[b], oy = ay - nd.iy[b], oz = az - nd.iz[b];
-            if (abs(ox) <= 1 && abs(oy) <= 1 && abs(oz) <= 1) continue;
-            if (pass) {
-              src[pos] = b;
-              code[pos] = (short)code_of(ox, oy, oz);
-              pos++;
-            } else {
-              c++;
-            }
-          }
+  if (nd.depth[i] >= 1) {
+    const int P = nd.parent[i];
+    const int ax = nd.ix[i], ay = nd.iy[i], az = nd.iz[i];
+    for (int s = 0; s < 27; s++) {
+      int C = col[(size_t)P * 27 + s];
+      if (C < 0 || nd.leaf[C]) continue;
+      for (int b = nd.child[C]; b < nd.child[C] + nd.nchild[C]; b++) {
+        int ox = ax - nd.ix[b], oy = ay - nd.iy[b], oz = az - nd.iz[b];
+        if (abs(ox) <= 1 && abs(oy) <= 1 && abs(oz) <= 1) continue;
+        if (pass) {
+          src[pos] = b;
+          code[pos] = (short)code_of(ox, oy, oz);
+          pos++;
+        } else {
+          c++;
         }
+      }
+    }
   }
   if (!pass) cnt[i] = c;
 }
 
-// P2P (near) list of every leaf as particle ranges: same-depth neighbours
-// (whole subtrees), and shallower leaves adjacent to an ancestor.
+// P2P (near) list of every leaf as particle ranges: its colleagues (whole
+// subtrees for non-leaf ones) and the leaf colleagues of its ancestors.
 __global__ void k_ulist(NodeArrays nd, int nleaves,
                         const int *__restrict__ leaves,
-                        const int *__restrict__ lstart,
-                        const int *__restrict__ lcount, int pass,
+                        const int *__restrict__ col, int pass,
                         int *__restrict__ cnt, const int *__restrict__ ofs,
                         int *__restrict__ ubeg, int *__restrict__ uend,
                         long long *__restrict__ pairs) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nleaves) return;
-  int A = leaves[t];
-  int dA = nd.depth[A];
+  const int A = leaves[t];
+  const int dA = nd.depth[A];
   int anc[kMaxDepth + 1];
   anc[dA] = A;
   for (int d = dA - 1; d >= 0; d--) anc[d] = nd.parent[anc[d + 1]];
   int c = 0;
   int pos = pass ? ofs[t] : 0;
   long long np = 0;
-  int nA = nd.pend[A] - nd.pbegin[A];
+  const int nA = nd.pend[A] - nd.pbegin[A];
   for (int d = 0; d <= dA; d++) {
-    int X = anc[d];
-    int xx = nd.ix[X], xy = nd.iy[X], xz = nd.iz[X];
-    for (int dx = -1; dx <= 1; dx++)
-      for (int dy = -1; dy <= 1; dy++)
-        for (int dz = -1; dz <= 1; dz++) {
-          bool self = (dx == 0 && dy == 0 && dz == 0);
-          if (d < dA && self) continue;
-          int j = lookup(nd, lstart, lcount, d, xx + dx, xy + dy, xz + dz);
-          if (j < 0) continue;
-          if (d < dA && !nd.leaf[j]) continue;
-          int b = nd.pbegin[j], e = nd.pend[j];
-          if (pass) {
-            ubeg[pos] = b;
-            uend[pos] = e;
-            pos++;
-            // reference pair counter: n(n+1)/2 for the leaf itself, half of
-            // the ordered count otherwise (each unordered pair is seen twice)
-            if (j == A)
-              np += 2LL * ((long long)nA * (nA + 1) / 2);
-            else
-              np += (long long)nA * (e - b);
-          } else {
-            c++;
-          }
-        }
+    const int *cx = col + (size_t)anc[d] * 27;
+    for (int s = 0; s < 27; s++) {
+      int j = cx[s];
+      if (j < 0) continue;
+      if (d < dA && (s == 13 || !nd.leaf[j])) continue;
+      int b = nd.pbegin[j], e = nd.pend[j];
+      if (pass) {
+        ubeg[pos] = b;
+        uend[pos] = e;
+        pos++;
+        // reference pair counter: n(n+1)/2 for the leaf itself, half of
+        // the ordered count otherwise (each unordered pair is seen twice)
+        if (j == A)
+          np += 2LL * ((long long)nA * (nA + 1) / 2);
+        else
+          np += (long long)nA * (e - b);
+      } else {
+        c++;
+      }
+    }
   }
   if (!pass)
     cnt[t] = c;
@@ -612,22 +613,21 @@ __global__ void k_ulist(NodeArrays nd, int nleaves,
     pairs[t] = np;
 }
 
-hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *lstart,
-                        const int *lcount, int pass, int *cnt, const int *ofs,
-                        int *src, short *code, hipStream_t s) {
+hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
+                        int pass, int *cnt, const int *ofs, int *src,
+                        short *code, hipStream_t s) {
   if (nnodes == 0) return hipSuccess;
-  k_vlist<<<(nnodes + 127) / 128, 128, 0, s>>>(nd, nnodes, lstart, lcount,
-                                                pass, cnt, ofs, src, code);
+  k_vlist<<<(nnodes + 127) / 128, 128, 0, s>>>(nd, nnodes, col, pass, cnt, ofs,
+                                                src, code);
   return hipGetLastError();
 }
 
 hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
-                        const int *lstart, const int *lcount, int pass,
-                        int *cnt, const int *ofs, int *ubeg, int *uend,
-                        long long *pairs, hipStream_t s) {
+                        const int *col, int pass, int *cnt, const int *ofs,
+                        int *ubeg, int *uend, long long *pairs, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
-  k_ulist<<<(nleaves + 127) / 128, 128, 0, s>>>(
-      nd, nleaves, leaves, lstart, lcount, pass, cnt, ofs, ubeg, uend, pairs);
+  k_ulist<<<(nleaves + 127) / 128, 128, 0, s>>>(nd, nleaves, leaves, col, pass,
+                                                 cnt, ofs, ubeg, uend, pairs);
   return hipGetLastError();
 }
 
