@@ -23,6 +23,9 @@ def _declare(L):
     L.aran_fmm3d_solve.restype = ip
     L.aran_fmm3d_solve.argtypes = [vp, C.c_long, vp, vp, vp, vp, dp, dp, ip, ip,
                                    vp, vp]
+    L.aran_fmm3d_solve_batch.restype = ip
+    L.aran_fmm3d_solve_batch.argtypes = [vp, ip, C.POINTER(C.c_long), vp, vp,
+                                         vp, vp, dp, dp, ip, ip, vp, vp]
     L.aran_fmm3d_get_stats.restype = ip
     L.aran_fmm3d_get_stats.argtypes = [vp, lp]
     L.aran_fmm3d_set_precision.restype = ip

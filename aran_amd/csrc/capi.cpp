@@ -190,6 +190,42 @@ extern "C" int aran_fmm3d_solve(AranFmm3d *ctx, long n, const double *x,
   return code_of_err(e);
 }
 
+extern "C" int aran_fmm3d_solve_batch(AranFmm3d *ctx, int nsys,
+                                      const long *offsets, const double *x,
+                                      const double *y, const double *z,
+                                      const double *q, const double *lbs,
+                                      const double *ubs, int order, int leafcap,
+                                      double *pot, double *grad) {
+  if (!ctx || nsys < 1 || !offsets || !lbs || !ubs) {
+    aran::set_error("aran_fmm3d_solve_batch: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  const long n = offsets[nsys];
+  if (n < 0 || (n > 0 && (!x || !y || !z || !q))) {
+    aran::set_error("aran_fmm3d_solve_batch: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  aran::Problem pb;
+  pb.n = n;
+  pb.order = order;
+  pb.leafcap = leafcap;
+  pb.nsys = nsys;
+  pb.soff = offsets;
+  pb.lbs = lbs;
+  pb.ubs = ubs;
+  pb.x = x;
+  pb.y = y;
+  pb.z = z;
+  pb.q = q;
+  pb.pot = pot;
+  pb.grad = grad;
+  pb.timings = ctx->timings;
+  pb.precision = ctx->precision;
+  hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
+  ctx->n_last = (e == hipSuccess) ? n : 0;
+  return code_of_err(e);
+}
+
 extern "C" int aran_fmm3d_get_stats(const AranFmm3d *ctx, long *o) {
   if (!ctx || !o) return ARAN_FMM3D_EINVAL;
   const aran::Stats &s = ctx->st;
@@ -484,6 +520,8 @@ int leaf_op_one(bool p2m, int p, const VsgVector3d &center,
   nd.leaf = &dh->ints[7];
   nd.child = &dh->ints[8];
   nd.nchild = &dh->ints[9];
+  nd.sys = nullptr;  // the leaf kernels use centres, not system boxes
+  nd.box = nullptr;
   nd.key = &dh->key;
   nd.cx = &dh->c[0];
   nd.cy = &dh->c[1];

@@ -93,6 +93,7 @@ struct Workspace {
       colleagues;
   DevBuf mult, loc, small;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, mult32, loc32;
+  DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos;
   long node_cap = 0;
   void *host_small = nullptr;  // pinned, 4 KiB
   Tables tab;
@@ -111,6 +112,12 @@ struct Problem {
   int leafcap = 20;
   double lb[3] = {0, 0, 0}, ub[3] = {1, 1, 1};
   const double *x = nullptr, *y = nullptr, *z = nullptr, *q = nullptr;
+  // independent systems: particles soff[s]..soff[s+1] of the inputs form
+  // system s with root bounds lbs/ubs[3s..3s+2] (host arrays); nsys == 0
+  // means one system with lb / ub
+  int nsys = 0;
+  const long *soff = nullptr;
+  const double *lbs = nullptr, *ubs = nullptr;
   double *pot = nullptr;   // device, user order, may be null
   double *grad = nullptr;  // device, user order [n][3], may be null
   bool timings = false;
@@ -118,8 +125,9 @@ struct Problem {
 };
 
 struct TreeInfo {
-  double lo[3] = {0, 0, 0};
+  double lo[3] = {0, 0, 0};  // root cube of system 0
   double width = 1;
+  int nsys = 1;
   int depth = 0;
   long nnodes = 0, nleaves = 0, nfar = 0, nnear = 0;
   std::vector<int> level_start, level_count;
@@ -137,16 +145,23 @@ void root_box(const double *lb, const double *ub, const double *pmin,
               const double *pmax, double *lo, double *width);
 
 // ---- tree.hip
-hipError_t launch_minmax(const double *x, const double *y, const double *z,
-                         long n, double *out6, double *part, hipStream_t s);
 hipError_t launch_keys(const double *x, const double *y, const double *z,
-                       long n, double lo0, double lo1, double lo2,
-                       double scale, uint64_t *keys, int *idx, hipStream_t s);
+                       long n, const long *soff, int nsys, const double *box,
+                       uint64_t *keys, int *idx, hipStream_t s);
+hipError_t launch_sys_keys(const int *idx, long n, const long *soff, int nsys,
+                           uint64_t *skey, int *pos, hipStream_t s);
+hipError_t launch_permute_keys(const int *pos, long n, const uint64_t *kin,
+                               const int *iin, uint64_t *kout, int *iout,
+                               hipStream_t s);
+int minmax_blocks(int nsys);
+hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
+                             const long *soff, int nsys, double *part,
+                             double *out, hipStream_t s);
 hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
                           DevBuf &tmp, hipStream_t s);
 hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
                       int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
-                      int *total, hipStream_t s);
+                      int *total, int passes, hipStream_t s);
 hipError_t launch_gather(const int *idx, long n, const double *x,
                          const double *y, const double *z, const double *q,
                          double *xs, double *ys, double *zs, double *qs,
@@ -154,15 +169,14 @@ hipError_t launch_gather(const int *idx, long n, const double *x,
 hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
                           const double *grad_s, double *pot, double *grad,
                           hipStream_t s);
-hipError_t launch_root(const NodeArrays &nd, long n, const double *lo,
-                       double width, hipStream_t s);
+hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
+                        hipStream_t s);
 hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
                               int lvl_count, const uint64_t *keys, int leafcap,
                               int *bounds, int *cnt, hipStream_t s);
 hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
                               int lvl_count, int next_begin, const int *bounds,
-                              const int *cnt, const int *ofs, const double *lo,
-                              double width, hipStream_t s);
+                              const int *cnt, const int *ofs, hipStream_t s);
 hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
                              hipStream_t s);
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
@@ -186,13 +200,12 @@ hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
                       const double *ys, const double *zs, const double *qs,
                       double2 *M, hipStream_t s);
 hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double width, double2 *M, hipStream_t s);
+                      int lvl_count, double2 *M, hipStream_t s);
 hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const int *vofs, const int *vsrc, const short *vcode,
-                      double width, const double2 *M, double2 *L,
-                      hipStream_t s);
+                      const double2 *M, double2 *L, hipStream_t s);
 hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double width, double2 *L, hipStream_t s);
+                      int lvl_count, double2 *L, hipStream_t s);
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *leaves, int nleaves, const double *xs,
                       const double *ys, const double *zs, const double2 *L,

@@ -105,6 +105,8 @@ NodeArrays Workspace::nodes() const {
   a.child = n_child.as<int>();
   a.nchild = n_nchild.as<int>();
   a.leaf = n_leaf.as<int>();
+  a.sys = n_sys.as<int>();
+  a.box = sys_box.as<double>();
   a.key = n_key.as<uint64_t>();
   a.cx = n_cx.as<double>();
   a.cy = n_cy.as<double>();
@@ -125,7 +127,8 @@ Workspace::~Workspace() {
                    &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas, &tab.rot32,
                    &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
                    &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
-                   &mult32, &loc32};
+                   &mult32, &loc32, &n_sys, &sys_box, &sys_off, &sys_mm,
+                   &skey, &spos};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
   if (events_ready)
@@ -376,7 +379,7 @@ static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
   hipStream_t s = ws.stream;
   DevBuf *ints[] = {&ws.n_depth, &ws.n_ix, &ws.n_iy, &ws.n_iz, &ws.n_pbegin,
                     &ws.n_pend, &ws.n_parent, &ws.n_child, &ws.n_nchild,
-                    &ws.n_leaf};
+                    &ws.n_leaf, &ws.n_sys};
   for (DevBuf *b : ints) CK(b->grow_keep(sizeof(int) * cap, sizeof(int) * keep, s));
   CK(ws.n_key.grow_keep(sizeof(uint64_t) * cap, sizeof(uint64_t) * keep, s));
   DevBuf *dbl[] = {&ws.n_cx, &ws.n_cy, &ws.n_cz};
@@ -408,24 +411,66 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (n == 0) return hipSuccess;
   if (pb.timings) CK(hipEventRecord(ws.ev[0], s));
 
-  // ---- root box from the particle extent
-  CK(ws.part.reserve(sizeof(double) * 6 * 256));
-  double *d6 = ws.small.as<double>();
-  CK(launch_minmax(pb.x, pb.y, pb.z, n, d6, ws.part.as<double>(), s));
-  double *h6 = static_cast<double *>(ws.host_small);
-  CK(hipMemcpyAsync(h6, d6, 6 * sizeof(double), hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
-  if (!(std::isfinite(h6[0]) && std::isfinite(h6[1]) && std::isfinite(h6[2]) &&
-        std::isfinite(h6[3]) && std::isfinite(h6[4]) && std::isfinite(h6[5]))) {
-    set_error("non-finite particle coordinates");
-    return hipErrorInvalidValue;
+  // ---- systems: particle ranges and root boxes from their extents
+  const int nsys = pb.nsys > 0 ? pb.nsys : 1;
+  std::vector<long> soff(nsys + 1);
+  std::vector<double> lbs(3 * nsys), ubs(3 * nsys);
+  if (pb.nsys > 0) {
+    for (int k = 0; k <= nsys; k++) soff[k] = pb.soff[k];
+    for (int k = 0; k < 3 * nsys; k++) {
+      lbs[k] = pb.lbs[k];
+      ubs[k] = pb.ubs[k];
+    }
+    for (int k = 0; k < nsys; k++)
+      if (soff[k + 1] < soff[k]) {
+        set_error("system offsets must be non-decreasing");
+        return hipErrorInvalidValue;
+      }
+    if (soff[0] != 0 || soff[nsys] != n) {
+      set_error("system offsets must start at 0 and end at n");
+      return hipErrorInvalidValue;
+    }
+  } else {
+    soff[0] = 0;
+    soff[1] = n;
+    for (int k = 0; k < 3; k++) {
+      lbs[k] = pb.lb[k];
+      ubs[k] = pb.ub[k];
+    }
   }
-  double pmin[3] = {h6[0], h6[1], h6[2]}, pmax[3] = {h6[3], h6[4], h6[5]};
-  double lo[3], width;
-  root_box(pb.lb, pb.ub, pmin, pmax, lo, &width);
-  const double scale = 2097152.0 / width;
+  CK(ws.sys_off.reserve(sizeof(long) * (nsys + 1)));
+  CK(ws.sys_mm.reserve(sizeof(double) * 6 * nsys));
+  CK(ws.sys_box.reserve(sizeof(double) * 4 * nsys));
+  CK(hipMemcpyAsync(ws.sys_off.ptr, soff.data(), sizeof(long) * (nsys + 1),
+                    hipMemcpyHostToDevice, s));
+  CK(ws.part.reserve(sizeof(double) * 6 * (size_t)nsys * minmax_blocks(nsys)));
+  CK(launch_sys_minmax(pb.x, pb.y, pb.z, ws.sys_off.as<long>(), nsys,
+                       ws.part.as<double>(), ws.sys_mm.as<double>(), s));
+  std::vector<double> mm(6 * nsys), box(4 * nsys);
+  CK(hipMemcpyAsync(mm.data(), ws.sys_mm.ptr, sizeof(double) * 6 * nsys,
+                    hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  for (int k = 0; k < nsys; k++) {
+    const double *v = &mm[6 * k];
+    if (soff[k + 1] == soff[k]) {
+      box[4 * k + 0] = box[4 * k + 1] = box[4 * k + 2] = 0.;
+      box[4 * k + 3] = 1.;
+      double pmin[3] = {lbs[3 * k], lbs[3 * k + 1], lbs[3 * k + 2]};
+      root_box(&lbs[3 * k], &ubs[3 * k], pmin, pmin, &box[4 * k],
+               &box[4 * k + 3]);
+      continue;
+    }
+    if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]) &&
+          std::isfinite(v[3]) && std::isfinite(v[4]) && std::isfinite(v[5]))) {
+      set_error("non-finite particle coordinates");
+      return hipErrorInvalidValue;
+    }
+    root_box(&lbs[3 * k], &ubs[3 * k], v, v + 3, &box[4 * k], &box[4 * k + 3]);
+  }
+  CK(hipMemcpyAsync(ws.sys_box.ptr, box.data(), sizeof(double) * 4 * nsys,
+                    hipMemcpyHostToDevice, s));
 
-  // ---- keys and sort
+  // ---- keys and sort (by system, then Morton key)
   CK(ws.keys_a.reserve(sizeof(uint64_t) * n));
   CK(ws.keys_b.reserve(sizeof(uint64_t) * n));
   CK(ws.idx_a.reserve(sizeof(int) * n));
@@ -434,23 +479,40 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   for (DevBuf *b : pbufs) CK(b->reserve(sizeof(double) * n));
   CK(ws.grad_s.reserve(sizeof(double) * 3 * n));
   int *d_total = ws.small.as<int>() + 64;
-  CK(launch_keys(pb.x, pb.y, pb.z, n, lo[0], lo[1], lo[2], scale,
-                 ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(), s));
+  CK(launch_keys(pb.x, pb.y, pb.z, n, ws.sys_off.as<long>(), nsys,
+                 ws.sys_box.as<double>(), ws.keys_a.as<uint64_t>(),
+                 ws.idx_a.as<int>(), s));
   CK(radix_sort(ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(),
                 ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), n, ws.hist,
-                ws.scan_tmp, d_total, s));
+                ws.scan_tmp, d_total, 8, s));
+  if (nsys > 1) {
+    // stable sort of the Morton order by system (2 or 4 byte passes)
+    const int passes = nsys <= 65536 ? 2 : 4;
+    CK(ws.skey.reserve(sizeof(uint64_t) * 2 * n));
+    CK(ws.spos.reserve(sizeof(int) * 2 * n));
+    uint64_t *sk = ws.skey.as<uint64_t>();
+    int *sp = ws.spos.as<int>();
+    CK(launch_sys_keys(ws.idx_a.as<int>(), n, ws.sys_off.as<long>(), nsys, sk,
+                       sp, s));
+    CK(radix_sort(sk, sp, sk + n, sp + n, n, ws.hist, ws.scan_tmp, d_total,
+                  passes, s));
+    CK(launch_permute_keys(sp, n, ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(),
+                           ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), s));
+    std::swap(ws.keys_a, ws.keys_b);
+    std::swap(ws.idx_a, ws.idx_b);
+  }
   CK(launch_gather(ws.idx_a.as<int>(), n, pb.x, pb.y, pb.z, pb.q,
                    ws.xs.as<double>(), ws.ys.as<double>(), ws.zs.as<double>(),
                    ws.qs.as<double>(), s));
   const uint64_t *keys = ws.keys_a.as<uint64_t>();
 
-  // ---- levels
-  long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64);
+  // ---- levels (level 0: one root per system)
+  long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64 + nsys);
   if (ws.node_cap < cap0) CK(reserve_nodes(ws, cap0, 0));
   NodeArrays nd = ws.nodes();
-  CK(launch_root(nd, n, lo, width, s));
-  std::vector<int> lstart = {0}, lcount = {1};
-  long nnodes = 1;
+  CK(launch_roots(nd, nsys, ws.sys_off.as<long>(), s));
+  std::vector<int> lstart = {0}, lcount = {nsys};
+  long nnodes = nsys;
   for (int d = 0;; d++) {
     int lc = lcount[d], lb = lstart[d];
     CK(ws.split_bounds.reserve(sizeof(int) * 9 * (size_t)lc));
@@ -467,8 +529,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
       nd = ws.nodes();
     }
     CK(launch_split_write(nd, lb, lc, (int)nnodes, ws.split_bounds.as<int>(),
-                          ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), lo,
-                          width, s));
+                          ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), s));
     if (total == 0) break;
     lstart.push_back((int)nnodes);
     lcount.push_back(total);
@@ -584,19 +645,19 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(launch_p2m(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
                 ws.zs.as<double>(), ws.qs.as<double>(), M, s));
   for (int d = depth - 1; d >= 0; d--)
-    CK(launch_m2m(tb, nd, lstart[d], lcount[d], width, M, s));
+    CK(launch_m2m(tb, nd, lstart[d], lcount[d], M, s));
   // ---- far field (the M2L timer brackets the M2L kernel alone)
   if (pb.precision == 0) {
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
     CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
-                  ws.v_code.as<short>(), width, M, L, s));
+                  ws.v_code.as<short>(), M, L, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
   } else {
     CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
     const int nsplit = m2l_split();
     const long split_stride = (long)nn * lc;
     CK(ws.loc32.reserve(sizeof(float2) * (size_t)nsplit * split_stride));
-    CK(launch_scale_mult(1, nd, nn, p, width, M, ws.mult32.ptr, s));
+    CK(launch_scale_mult(1, nd, nn, p, M, ws.mult32.ptr, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
     M2LTileArgs ta;
     ta.tile_tgt = ws.tile_tgt.as<int>();
@@ -611,12 +672,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     ta.split_stride = split_stride;
     CK(launch_m2l_tile(1, p, ntiles, ta, ws.mult32.ptr, ws.loc32.ptr, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
-    CK(launch_unscale_local(1, nd, nn, p, width, ws.loc32.ptr, nsplit,
+    CK(launch_unscale_local(1, nd, nn, p, ws.loc32.ptr, nsplit,
                             split_stride, L, s));
   }
   // ---- downward pass
   for (int d = 1; d <= depth; d++)
-    CK(launch_l2l(tb, nd, lstart[d], lcount[d], width, L, s));
+    CK(launch_l2l(tb, nd, lstart[d], lcount[d], L, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
   // ---- near field and L2P
   CK(launch_p2p(pb.precision, leaves, nleaves, nd, ws.u_ofs.as<int>(),
@@ -637,9 +698,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   S.zero = nn;
   S.p2p = (long)(*hp / 2);
   S.p2m = n;
-  S.m2m = nn - 1;
+  S.m2m = nn - nsys;
   S.m2l = nfar;
-  S.l2l = nn - 1;
+  S.l2l = nn - nsys;
   S.l2p = n;
   S.depth = depth;
   S.nnodes = nn;
@@ -655,8 +716,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(hipEventElapsedTime(&S.t_total, ws.ev[0], ws.ev[6]));
   }
   if (ti) {
-    for (int k = 0; k < 3; k++) ti->lo[k] = lo[k];
-    ti->width = width;
+    for (int k = 0; k < 3; k++) ti->lo[k] = box[k];
+    ti->width = box[3];
+    ti->nsys = nsys;
     ti->depth = depth;
     ti->nnodes = nn;
     ti->nleaves = nleaves;

@@ -133,7 +133,7 @@ hipError_t launch_fill_int(int *p, long n, int v, hipStream_t s) {
 
 // Ms = M / w^l (type T), for every node.
 template <typename T>
-__global__ void k_scale_mult(NodeArrays nd, int nnodes, int p, double width,
+__global__ void k_scale_mult(NodeArrays nd, int nnodes, int p,
                              const double2 *__restrict__ M,
                              typename V2<T>::type *__restrict__ Ms) {
   const int mc = tri_p(p);
@@ -142,7 +142,7 @@ __global__ void k_scale_mult(NodeArrays nd, int nnodes, int p, double width,
   int node = (int)(idx / mc), k = (int)(idx % mc);
   int l = 0;
   while (tri_p(l + 1) <= k) l++;
-  double w = ldexp(width, -nd.depth[node]);
+  double w = node_width(nd, node);
   double f = 1.;
   for (int i = 0; i < l; i++) f /= w;
   double2 v = M[idx];
@@ -155,7 +155,7 @@ __global__ void k_scale_mult(NodeArrays nd, int nnodes, int p, double width,
 // L = (sum of the nsplit partial slices of Ls, in slice order) / w^(l+1)
 // for nodes of depth >= 2, zero above.
 template <typename T>
-__global__ void k_unscale_local(NodeArrays nd, int nnodes, int p, double width,
+__global__ void k_unscale_local(NodeArrays nd, int nnodes, int p,
                                 const typename V2<T>::type *__restrict__ Ls,
                                 int nsplit, long split_stride,
                                 double2 *__restrict__ L) {
@@ -169,7 +169,7 @@ __global__ void k_unscale_local(NodeArrays nd, int nnodes, int p, double width,
   }
   int l = 0;
   while (tri_p(l + 1) <= k) l++;
-  double w = ldexp(width, -nd.depth[node]);
+  double w = node_width(nd, node);
   double f = 1. / w;
   for (int i = 0; i < l; i++) f /= w;
   double sx = 0., sy = 0.;
@@ -182,31 +182,30 @@ __global__ void k_unscale_local(NodeArrays nd, int nnodes, int p, double width,
 }
 
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
-                             int p, double width, const double2 *M, void *Ms,
-                             hipStream_t s) {
+                             int p, const double2 *M, void *Ms, hipStream_t s) {
   long n = (long)nnodes * tri_p(p);
   if (n == 0) return hipSuccess;
   if (precision == 0)
     k_scale_mult<double><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, width, M, static_cast<double2 *>(Ms));
+        nd, nnodes, p, M, static_cast<double2 *>(Ms));
   else
     k_scale_mult<float><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, width, M, static_cast<float2 *>(Ms));
+        nd, nnodes, p, M, static_cast<float2 *>(Ms));
   return hipGetLastError();
 }
 
 hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int nnodes,
-                                int p, double width, const void *Ls, int nsplit,
+                                int p, const void *Ls, int nsplit,
                                 long split_stride, double2 *L, hipStream_t s) {
   long n = (long)nnodes * tri_p(p + 1);
   if (n == 0) return hipSuccess;
   if (precision == 0)
     k_unscale_local<double><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, width, static_cast<const double2 *>(Ls), nsplit,
+        nd, nnodes, p, static_cast<const double2 *>(Ls), nsplit,
         split_stride, L);
   else
     k_unscale_local<float><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, width, static_cast<const float2 *>(Ls), nsplit,
+        nd, nnodes, p, static_cast<const float2 *>(Ls), nsplit,
         split_stride, L);
   return hipGetLastError();
 }

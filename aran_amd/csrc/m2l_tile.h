@@ -43,10 +43,9 @@ hipError_t launch_tile_src(const NodeArrays &nd, int first, int count,
                            int *tile_src, hipStream_t s);
 hipError_t launch_fill_int(int *p, long n, int v, hipStream_t s);
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
-                             int p, double width, const double2 *M, void *Ms,
-                             hipStream_t s);
+                             int p, const double2 *M, void *Ms, hipStream_t s);
 hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int nnodes,
-                                int p, double width, const void *Ls, int nsplit,
+                                int p, const void *Ls, int nsplit,
                                 long split_stride, double2 *L, hipStream_t s);
 hipError_t launch_m2l_tables(int precision, int p, const DirEntry *dir,
                              const double *tm2l, const double *rot, long nrot,

@@ -258,15 +258,13 @@ __global__ __launch_bounds__(kOpThreads) void k_p2m(
 
 // One block per parent node of one level: M(P) = sum over children.
 __global__ __launch_bounds__(kOpThreads) void k_m2m(
-    OpTables tb, NodeArrays nd, int lvl_begin, double width,
-    double2 *__restrict__ M) {
+    OpTables tb, NodeArrays nd, int lvl_begin, double2 *__restrict__ M) {
   __shared__ double2 U[kMaxMc], X[kMaxMc], Z[kMaxMc];
   __shared__ double rp[2 * kMaxOrder + 2];
   const int P = lvl_begin + blockIdx.x;
   if (nd.leaf[P]) return;
   const int p = tb.p, mc = tri(p), rb = tb.rb;
-  const int d = nd.depth[P] + 1;
-  const double half = ldexp(width, -d) * 0.5;
+  const double half = node_width(nd, P) * 0.25;  // half a child width
   double2 acc[2] = {make_double2(0., 0.), make_double2(0., 0.)};
   const int cb = nd.child[P], ce = cb + nd.nchild[P];
   for (int C = cb; C < ce; C++) {
@@ -326,12 +324,12 @@ __global__ __launch_bounds__(kOpThreads) void k_m2m(
 __global__ __launch_bounds__(kOpThreads) void k_m2l(
     OpTables tb, NodeArrays nd, const int *__restrict__ vofs,
     const int *__restrict__ vsrc, const short *__restrict__ vcode,
-    double width, const double2 *__restrict__ M, double2 *__restrict__ L) {
+    const double2 *__restrict__ M, double2 *__restrict__ L) {
   __shared__ double2 U[kMaxMc], X[kMaxMc], Z[kMaxLc];
   __shared__ double rp[2 * kMaxOrder + 2];
   const int A = blockIdx.x;
   const int p = tb.p, mc = tri(p), lc = tri(p + 1), rb = tb.rb;
-  const double w = ldexp(width, -nd.depth[A]);
+  const double w = node_width(nd, A);
   double2 acc[3] = {make_double2(0., 0.), make_double2(0., 0.),
                     make_double2(0., 0.)};
   const int e0 = vofs[A], e1 = vofs[A + 1];
@@ -394,14 +392,13 @@ __global__ __launch_bounds__(kOpThreads) void k_m2l(
 
 // One block per node of one level (depth >= 1): L(A) += T L(parent).
 __global__ __launch_bounds__(kOpThreads) void k_l2l(
-    OpTables tb, NodeArrays nd, int lvl_begin, double width,
-    double2 *__restrict__ L) {
+    OpTables tb, NodeArrays nd, int lvl_begin, double2 *__restrict__ L) {
   __shared__ double2 U[kMaxLc], X[kMaxLc], Z[kMaxLc];
   __shared__ double rp[2 * kMaxOrder + 2];
   const int A = lvl_begin + blockIdx.x;
   const int P = nd.parent[A];
   const int p = tb.p, lc = tri(p + 1), rb = tb.rb;
-  const double half = ldexp(width, -nd.depth[A]) * 0.5;
+  const double half = node_width(nd, A) * 0.5;
   int bx = nd.ix[A] & 1, by = nd.iy[A] & 1, bz = nd.iz[A] & 1;
   int code = code_of(2 * bx - 1, 2 * by - 1, 2 * bz - 1);
   DirEntry de = tb.dir[code];
@@ -529,25 +526,24 @@ hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
 }
 
 hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double width, double2 *M, hipStream_t s) {
+                      int lvl_count, double2 *M, hipStream_t s) {
   if (lvl_count == 0) return hipSuccess;
-  k_m2m<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, width, M);
+  k_m2m<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, M);
   return hipGetLastError();
 }
 
 hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const int *vofs, const int *vsrc, const short *vcode,
-                      double width, const double2 *M, double2 *L,
-                      hipStream_t s) {
+                      const double2 *M, double2 *L, hipStream_t s) {
   if (nnodes == 0) return hipSuccess;
-  k_m2l<<<nnodes, kOpThreads, 0, s>>>(tb, nd, vofs, vsrc, vcode, width, M, L);
+  k_m2l<<<nnodes, kOpThreads, 0, s>>>(tb, nd, vofs, vsrc, vcode, M, L);
   return hipGetLastError();
 }
 
 hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double width, double2 *L, hipStream_t s) {
+                      int lvl_count, double2 *L, hipStream_t s) {
   if (lvl_count == 0) return hipSuccess;
-  k_l2l<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, width, L);
+  k_l2l<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, L);
   return hipGetLastError();
 }
 

@@ -1,0 +1,362 @@
+// server.cpp -- continuous-batching request loop over the batched engine.
+//
+// No counterpart in the reference (LibAran solves one tree per call,
+// aran/aransolver3d.c:1135-1202).  Requests are copied in at submission and
+// queued.  Each worker owns an engine context on its own HIP stream; when a
+// worker is free it admits the oldest waiting requests that share the
+// oldest one's order and leaf capacity, while the estimated device
+// footprint of everything in flight stays under the budget, and solves them
+// together as independent systems.  With two workers one batch's copies
+// and kernels overlap another's.  A system's result does not depend on the
+// batch it ran in (the engine gives every system its own octree and
+// accumulates in a fixed order), so served results equal standalone ones.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+extern "C" {
+#include <aran/aranfmm3d.h>
+#include <aran/aranserver3d.h>
+}
+
+#include "engine.h"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct Request {
+  long id = 0;
+  long n = 0;
+  int order = 8, leafcap = 32, want_grad = 1;
+  double lb[3], ub[3];
+  std::vector<double> x, y, z, q, pot, grad;
+  double bytes = 0.;
+  Clock::time_point t_submit, t_start, t_done;
+  bool done = false;
+  int status = 0;
+};
+
+struct PinnedBuf {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  bool reserve(size_t n) {
+    if (n <= bytes && ptr) return true;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    size_t want = std::max<size_t>(n + n / 4, 4096);
+    if (hipHostMalloc(&ptr, want) != hipSuccess) return false;
+    bytes = want;
+    return true;
+  }
+  ~PinnedBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+};
+
+struct Worker {
+  AranFmm3d *ctx = nullptr;
+  aran::DevBuf din, dout;
+  PinnedBuf hin, hout;
+  std::thread th;
+};
+
+}  // namespace
+
+struct _AranServer3d {
+  double budget = 0., inflight = 0., peak_inflight = 0.;
+  int precision = ARAN_FMM3D_MIXED;
+  int max_batch = 64;
+  int device = 0;
+  std::mutex mu;
+  std::condition_variable cv_work, cv_done;
+  std::deque<Request *> queue;
+  std::map<long, std::unique_ptr<Request>> all;
+  long next_id = 0;
+  bool stop = false, paused = false;
+  long batches = 0, served = 0, max_batch_seen = 0;
+  std::vector<std::unique_ptr<Worker>> workers;
+};
+
+extern "C" double aran_server3d_footprint(long n, int order, int leafcap) {
+  if (n < 0 || order < 1 || leafcap < 1) return -1.;
+  const double mc = order * (order + 1) / 2., lc = (order + 1) * (order + 2) / 2.;
+  const double nodes = 64. + 4. * ((double)n / leafcap + 1.);
+  // particle arrays (inputs, outputs, sort keys, sorted copies) and, per
+  // node, tree arrays, colleagues, far list, tile table, FP64 and FP32
+  // expansions (8 partial FP32 locals)
+  const double per_particle = 200.;
+  const double per_node = 2600. + 24. * (mc + lc) + 64. * lc;
+  return 1.25 * (n * per_particle + nodes * per_node) + 1e6;
+}
+
+// Tables and fixed buffers of one worker context, outside the requests.
+static const double kWorkerBytes = 64e6;
+
+static void worker_loop(AranServer3d *S, Worker *W) {
+  (void)hipSetDevice(S->device);
+  for (;;) {
+    std::vector<Request *> batch;
+    double bytes = 0.;
+    {
+      std::unique_lock<std::mutex> lock(S->mu);
+      for (;;) {
+        if (S->stop && S->queue.empty()) return;
+        if (!S->queue.empty() && (!S->paused || S->stop)) {
+          Request *r0 = S->queue.front();
+          const double cap = S->budget / (double)S->workers.size();
+          if (S->inflight + r0->bytes <= S->budget || S->inflight == 0.) {
+            batch.push_back(r0);
+            bytes = r0->bytes;
+            for (auto it = S->queue.begin() + 1; it != S->queue.end(); ++it) {
+              Request *r = *it;
+              if ((int)batch.size() >= S->max_batch) break;
+              if (r->order != r0->order || r->leafcap != r0->leafcap) continue;
+              if (bytes + r->bytes > cap) continue;
+              if (S->inflight + bytes + r->bytes > S->budget) continue;
+              batch.push_back(r);
+              bytes += r->bytes;
+            }
+            for (Request *r : batch)
+              S->queue.erase(std::find(S->queue.begin(), S->queue.end(), r));
+            S->inflight += bytes;
+            S->peak_inflight = std::max(S->peak_inflight, S->inflight);
+            break;
+          }
+        }
+        S->cv_work.wait(lock);
+      }
+    }
+    const Clock::time_point t0 = Clock::now();
+    const int nsys = (int)batch.size();
+    std::vector<long> off(nsys + 1, 0);
+    std::vector<double> lbs(3 * nsys), ubs(3 * nsys);
+    for (int k = 0; k < nsys; k++) {
+      off[k + 1] = off[k] + batch[k]->n;
+      for (int a = 0; a < 3; a++) {
+        lbs[3 * k + a] = batch[k]->lb[a];
+        ubs[3 * k + a] = batch[k]->ub[a];
+      }
+    }
+    const long n = off[nsys];
+    int rc = 0;
+    const size_t nb = sizeof(double) * std::max<long>(n, 1);
+    hipStream_t st = static_cast<hipStream_t>(aran_fmm3d_stream(W->ctx));
+    if (!W->hin.reserve(4 * nb) || !W->hout.reserve(4 * nb) ||
+        W->din.reserve(4 * nb) != hipSuccess ||
+        W->dout.reserve(4 * nb) != hipSuccess)
+      rc = ARAN_FMM3D_ENOMEM;
+    double *hin = static_cast<double *>(W->hin.ptr);
+    double *din = W->din.as<double>();
+    double *hout = static_cast<double *>(W->hout.ptr);
+    double *dout = W->dout.as<double>();
+    if (rc == 0 && n > 0) {
+      for (int k = 0; k < nsys; k++) {
+        const Request *r = batch[k];
+        memcpy(hin + off[k], r->x.data(), sizeof(double) * r->n);
+        memcpy(hin + n + off[k], r->y.data(), sizeof(double) * r->n);
+        memcpy(hin + 2 * n + off[k], r->z.data(), sizeof(double) * r->n);
+        memcpy(hin + 3 * n + off[k], r->q.data(), sizeof(double) * r->n);
+      }
+      if (hipMemcpyAsync(din, hin, 4 * sizeof(double) * n,
+                         hipMemcpyHostToDevice, st) != hipSuccess)
+        rc = ARAN_FMM3D_EHIP;
+    }
+    if (rc == 0)
+      rc = aran_fmm3d_solve_batch(W->ctx, nsys, off.data(), din, din + n,
+                                  din + 2 * n, din + 3 * n, lbs.data(),
+                                  ubs.data(), batch[0]->order,
+                                  batch[0]->leafcap, dout, dout + n);
+    if (rc == 0 && n > 0) {
+      if (hipMemcpyAsync(hout, dout, 4 * sizeof(double) * n,
+                         hipMemcpyDeviceToHost, st) != hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        rc = ARAN_FMM3D_EHIP;
+    }
+    {
+      std::lock_guard<std::mutex> lock(S->mu);
+      const Clock::time_point t1 = Clock::now();
+      for (int k = 0; k < nsys; k++) {
+        Request *r = batch[k];
+        r->status = rc;
+        if (rc == 0) {
+          r->pot.assign(hout + off[k], hout + off[k] + r->n);
+          if (r->want_grad)
+            r->grad.assign(hout + n + 3 * off[k], hout + n + 3 * off[k + 1]);
+        }
+        r->x.clear();
+        r->x.shrink_to_fit();
+        r->y.clear();
+        r->y.shrink_to_fit();
+        r->z.clear();
+        r->z.shrink_to_fit();
+        r->q.clear();
+        r->q.shrink_to_fit();
+        r->t_start = t0;
+        r->t_done = t1;
+        r->done = true;
+      }
+      S->inflight -= bytes;
+      S->batches++;
+      S->served += nsys;
+      S->max_batch_seen = std::max<long>(S->max_batch_seen, nsys);
+    }
+    S->cv_done.notify_all();
+    S->cv_work.notify_all();
+  }
+}
+
+extern "C" AranServer3d *aran_server3d_new(double hbm_budget_bytes,
+                                           int nworkers, int precision,
+                                           int max_batch) {
+  if (!(hbm_budget_bytes > 0.) || nworkers < 1 || nworkers > 8 ||
+      max_batch < 1 ||
+      (precision != ARAN_FMM3D_PLAIN && precision != ARAN_FMM3D_MIXED)) {
+    aran::set_error("aran_server3d_new: invalid argument");
+    return nullptr;
+  }
+  if (hbm_budget_bytes <= kWorkerBytes * nworkers) {
+    aran::set_error("aran_server3d_new: budget below the workers' tables");
+    return nullptr;
+  }
+  AranServer3d *S = new AranServer3d;
+  S->budget = hbm_budget_bytes - kWorkerBytes * nworkers;
+  S->precision = precision;
+  S->max_batch = max_batch;
+  (void)hipGetDevice(&S->device);
+  for (int k = 0; k < nworkers; k++) {
+    std::unique_ptr<Worker> w(new Worker);
+    w->ctx = aran_fmm3d_new(nullptr);
+    if (!w->ctx) {
+      for (auto &o : S->workers) aran_fmm3d_free(o->ctx);
+      delete S;
+      return nullptr;
+    }
+    aran_fmm3d_set_precision(w->ctx, precision);
+    S->workers.push_back(std::move(w));
+  }
+  for (auto &w : S->workers) w->th = std::thread(worker_loop, S, w.get());
+  return S;
+}
+
+extern "C" void aran_server3d_free(AranServer3d *S) {
+  if (!S) return;
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    S->stop = true;
+  }
+  S->cv_work.notify_all();
+  for (auto &w : S->workers) {
+    if (w->th.joinable()) w->th.join();
+    aran_fmm3d_free(w->ctx);
+    w->din.release();
+    w->dout.release();
+  }
+  delete S;
+}
+
+extern "C" long aran_server3d_submit(AranServer3d *S, long n, const double *x,
+                                     const double *y, const double *z,
+                                     const double *q, const double *lb,
+                                     const double *ub, int order, int leafcap,
+                                     int want_grad) {
+  if (!S || n < 0 || (n > 0 && (!x || !y || !z || !q)) || !lb || !ub ||
+      order < 1 || order > aran::kMaxOrder || leafcap < 1) {
+    aran::set_error("aran_server3d_submit: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  const double bytes = aran_server3d_footprint(n, order, leafcap);
+  if (bytes > S->budget / (double)S->workers.size()) {
+    aran::set_error("aran_server3d_submit: request exceeds the HBM budget");
+    return ARAN_SERVER3D_EBUDGET;
+  }
+  std::unique_ptr<Request> r(new Request);
+  r->n = n;
+  r->order = order;
+  r->leafcap = leafcap;
+  r->want_grad = want_grad != 0;
+  for (int a = 0; a < 3; a++) {
+    r->lb[a] = lb[a];
+    r->ub[a] = ub[a];
+  }
+  r->x.assign(x, x + n);
+  r->y.assign(y, y + n);
+  r->z.assign(z, z + n);
+  r->q.assign(q, q + n);
+  r->bytes = bytes;
+  r->t_submit = Clock::now();
+  long id;
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    if (S->stop) {
+      aran::set_error("aran_server3d_submit: server is stopping");
+      return ARAN_FMM3D_EINVAL;
+    }
+    id = S->next_id++;
+    r->id = id;
+    S->queue.push_back(r.get());
+    S->all[id] = std::move(r);
+  }
+  S->cv_work.notify_one();
+  return id;
+}
+
+extern "C" int aran_server3d_wait(AranServer3d *S, long id, double *pot,
+                                  double *grad, double *times3) {
+  if (!S) return ARAN_FMM3D_EINVAL;
+  std::unique_ptr<Request> r;
+  {
+    std::unique_lock<std::mutex> lock(S->mu);
+    auto it = S->all.find(id);
+    if (it == S->all.end()) {
+      aran::set_error("aran_server3d_wait: unknown request");
+      return ARAN_FMM3D_EINVAL;
+    }
+    Request *rp = it->second.get();
+    S->cv_done.wait(lock, [&] { return rp->done; });
+    r = std::move(it->second);
+    S->all.erase(it);
+  }
+  if (r->status != 0) return r->status;
+  if (pot) memcpy(pot, r->pot.data(), sizeof(double) * r->n);
+  if (grad && r->want_grad)
+    memcpy(grad, r->grad.data(), sizeof(double) * 3 * r->n);
+  if (times3) {
+    using sec = std::chrono::duration<double>;
+    times3[0] = sec(r->t_done - r->t_submit).count();       // latency
+    times3[1] = sec(r->t_start - r->t_submit).count();      // queueing
+    times3[2] = sec(r->t_done - r->t_start).count();        // batch service
+  }
+  return 0;
+}
+
+extern "C" int aran_server3d_pause(AranServer3d *S, int paused) {
+  if (!S) return ARAN_FMM3D_EINVAL;
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    S->paused = paused != 0;
+  }
+  S->cv_work.notify_all();
+  return 0;
+}
+
+extern "C" int aran_server3d_stats(AranServer3d *S, double *out4) {
+  if (!S || !out4) return ARAN_FMM3D_EINVAL;
+  std::lock_guard<std::mutex> lock(S->mu);
+  out4[0] = (double)S->batches;
+  out4[1] = (double)S->served;
+  out4[2] = (double)S->max_batch_seen;
+  out4[3] = S->peak_inflight;
+  return 0;
+}

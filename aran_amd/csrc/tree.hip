@@ -8,23 +8,104 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "engine.h"
 #include "tree_rules.h"
 
 namespace aran {
 
-// ------------------------------------------------------------ min / max
+// ------------------------------------------------------------ keys
 
-__global__ void k_minmax_partial(const double *__restrict__ x,
-                                 const double *__restrict__ y,
-                                 const double *__restrict__ z, long n,
-                                 double *__restrict__ part) {
+// System of particle i (systems are contiguous ranges soff[s]..soff[s+1]).
+__device__ inline int system_of(const long *__restrict__ soff, int nsys,
+                                long i) {
+  int lo = 0, hi = nsys - 1;
+  while (lo < hi) {  // last s with soff[s] <= i
+    int mid = (lo + hi + 1) >> 1;
+    if (soff[mid] <= i)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void k_keys(const double *__restrict__ x,
+                       const double *__restrict__ y,
+                       const double *__restrict__ z, long n,
+                       const long *__restrict__ soff, int nsys,
+                       const double *__restrict__ box,
+                       uint64_t *__restrict__ keys, int *__restrict__ idx) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int sy = system_of(soff, nsys, i);
+  const double *b = box + 4 * sy;
+  const double scale = 2097152.0 / b[3];
+  keys[i] = morton_key(icoord(x[i], b[0], scale), icoord(y[i], b[1], scale),
+                       icoord(z[i], b[2], scale));
+  idx[i] = (int)i;
+}
+
+hipError_t launch_keys(const double *x, const double *y, const double *z,
+                       long n, const long *soff, int nsys, const double *box,
+                       uint64_t *keys, int *idx, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_keys<<<(n + 255) / 256, 256, 0, s>>>(x, y, z, n, soff, nsys, box, keys,
+                                         idx);
+  return hipGetLastError();
+}
+
+// Second sort key: the system of each (already Morton-sorted) particle.
+__global__ void k_sys_keys(const int *__restrict__ idx, long n,
+                           const long *__restrict__ soff, int nsys,
+                           uint64_t *__restrict__ skey, int *__restrict__ pos) {
+  long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  skey[j] = (uint64_t)system_of(soff, nsys, idx[j]);
+  pos[j] = (int)j;
+}
+
+__global__ void k_permute_keys(const int *__restrict__ pos, long n,
+                               const uint64_t *__restrict__ kin,
+                               const int *__restrict__ iin,
+                               uint64_t *__restrict__ kout,
+                               int *__restrict__ iout) {
+  long j = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  kout[j] = kin[pos[j]];
+  iout[j] = iin[pos[j]];
+}
+
+hipError_t launch_sys_keys(const int *idx, long n, const long *soff, int nsys,
+                           uint64_t *skey, int *pos, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_sys_keys<<<(n + 255) / 256, 256, 0, s>>>(idx, n, soff, nsys, skey, pos);
+  return hipGetLastError();
+}
+
+hipError_t launch_permute_keys(const int *pos, long n, const uint64_t *kin,
+                               const int *iin, uint64_t *kout, int *iout,
+                               hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_permute_keys<<<(n + 255) / 256, 256, 0, s>>>(pos, n, kin, iin, kout, iout);
+  return hipGetLastError();
+}
+
+// Per-system extent: grid (systems, blocks per system); partials are then
+// combined per system.
+__global__ void k_sys_minmax(const double *__restrict__ x,
+                             const double *__restrict__ y,
+                             const double *__restrict__ z,
+                             const long *__restrict__ soff,
+                             double *__restrict__ part) {
   __shared__ double red[6][256];
+  const int sy = blockIdx.x, nb = gridDim.y;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
                  -INFINITY};
   bool bad = false;
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < n;
-       i += (long)gridDim.x * 256L) {
+  for (long i = soff[sy] + blockIdx.y * 256L + threadIdx.x; i < soff[sy + 1];
+       i += 256L * nb) {
     double a = x[i], b = y[i], c = z[i];
     bad |= !(isfinite(a) && isfinite(b) && isfinite(c));
     v[0] = fmin(v[0], a);
@@ -34,71 +115,53 @@ __global__ void k_minmax_partial(const double *__restrict__ x,
     v[4] = fmax(v[4], b);
     v[5] = fmax(v[5], c);
   }
-  if (bad) {
-    v[0] = NAN;
-  }
+  if (bad) v[0] = NAN;
   for (int k = 0; k < 6; k++) red[k][threadIdx.x] = v[k];
   __syncthreads();
-  for (int s = 128; s > 0; s >>= 1) {
-    if ((int)threadIdx.x < s) {
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
       for (int k = 0; k < 3; k++) {
-        double a = red[k][threadIdx.x], b = red[k][threadIdx.x + s];
+        double a = red[k][threadIdx.x], b = red[k][threadIdx.x + st];
         red[k][threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmin(a, b);
       }
       for (int k = 3; k < 6; k++)
-        red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + s]);
+        red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + st]);
     }
     __syncthreads();
   }
-  if (threadIdx.x < 6) part[blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x < 6)
+    part[((size_t)sy * nb + blockIdx.y) * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_minmax_final(const double *__restrict__ part, int nb,
-                               double *__restrict__ out) {
-  if (threadIdx.x != 0) return;
+__global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
+                                   int nb, double *__restrict__ out) {
+  int sy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sy >= nsys) return;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
                  -INFINITY};
   bool bad = false;
   for (int b = 0; b < nb; b++) {
+    const double *p = part + ((size_t)sy * nb + b) * 6;
     for (int k = 0; k < 3; k++) {
-      double a = part[6 * b + k];
-      bad |= isnan(a);
-      v[k] = fmin(v[k], a);
+      bad |= isnan(p[k]);
+      v[k] = fmin(v[k], p[k]);
     }
-    for (int k = 3; k < 6; k++) v[k] = fmax(v[k], part[6 * b + k]);
+    for (int k = 3; k < 6; k++) v[k] = fmax(v[k], p[k]);
   }
-  for (int k = 0; k < 6; k++) out[k] = v[k];
-  if (bad) out[0] = NAN;
+  for (int k = 0; k < 6; k++) out[6 * sy + k] = v[k];
+  if (bad) out[6 * sy] = NAN;
 }
 
-hipError_t launch_minmax(const double *x, const double *y, const double *z,
-                         long n, double *out6, double *part, hipStream_t s) {
-  int nb = 256;
-  k_minmax_partial<<<nb, 256, 0, s>>>(x, y, z, n, part);
-  k_minmax_final<<<1, 64, 0, s>>>(part, nb, out6);
-  return hipGetLastError();
-}
+// part must hold 6 * nsys * minmax_blocks(nsys) doubles.
+int minmax_blocks(int nsys) { return std::max(1, std::min(128, 2048 / std::max(nsys, 1))); }
 
-// ------------------------------------------------------------ keys
-
-__global__ void k_keys(const double *__restrict__ x,
-                       const double *__restrict__ y,
-                       const double *__restrict__ z, long n, double lo0,
-                       double lo1, double lo2, double scale,
-                       uint64_t *__restrict__ keys, int *__restrict__ idx) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  keys[i] = morton_key(icoord(x[i], lo0, scale), icoord(y[i], lo1, scale),
-                       icoord(z[i], lo2, scale));
-  idx[i] = (int)i;
-}
-
-hipError_t launch_keys(const double *x, const double *y, const double *z,
-                       long n, double lo0, double lo1, double lo2,
-                       double scale, uint64_t *keys, int *idx, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_keys<<<(n + 255) / 256, 256, 0, s>>>(x, y, z, n, lo0, lo1, lo2, scale,
-                                         keys, idx);
+hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
+                             const long *soff, int nsys, double *part,
+                             double *out, hipStream_t s) {
+  if (nsys == 0) return hipSuccess;
+  const int nb = minmax_blocks(nsys);
+  k_sys_minmax<<<dim3(nsys, nb), 256, 0, s>>>(x, y, z, soff, part);
+  k_sys_minmax_final<<<(nsys + 63) / 64, 64, 0, s>>>(part, nsys, nb, out);
   return hipGetLastError();
 }
 
@@ -265,11 +328,11 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
   }
 }
 
-// Sorts (keys, idx) by key; ties keep the input order.  The result ends in
-// keys / idx (8 passes, an even count).
+// Sorts (keys, idx) by the low 8*passes bits of key; ties keep the input
+// order.  passes must be even: the result then ends in keys / idx.
 hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
                       int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
-                      int *total, hipStream_t s) {
+                      int *total, int passes, hipStream_t s) {
   if (n <= 1) return hipSuccess;
   int nb = (int)((n + kRsTile - 1) / kRsTile);
   hipError_t e = hist.reserve(sizeof(int) * 256 * (size_t)nb * 2);
@@ -278,7 +341,7 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
   int *o = h + 256 * (size_t)nb;
   uint64_t *ka = keys, *kb = keys_tmp;
   int *va = idx, *vb = idx_tmp;
-  for (int pass = 0; pass < 8; pass++) {
+  for (int pass = 0; pass < passes; pass++) {
     int shift = 8 * pass;
     k_rs_hist<<<nb, kRsThreads, 0, s>>>(ka, n, shift, h, nb);
     e = scan_exclusive(h, o, 256L * nb, total, scan_tmp, s);
@@ -347,27 +410,29 @@ hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
 
 // ------------------------------------------------------------ level build
 
-__global__ void k_root(NodeArrays nd, long n, double lo0, double lo1,
-                       double lo2, double width) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  nd.depth[0] = 0;
-  nd.ix[0] = nd.iy[0] = nd.iz[0] = 0;
-  nd.key[0] = 0;
-  nd.pbegin[0] = 0;
-  nd.pend[0] = (int)n;
-  nd.parent[0] = -1;
-  nd.child[0] = -1;
-  nd.nchild[0] = 0;
-  nd.leaf[0] = 1;
-  double h = width;
-  nd.cx[0] = fma(0.5, h, lo0);
-  nd.cy[0] = fma(0.5, h, lo1);
-  nd.cz[0] = fma(0.5, h, lo2);
+// One root per system, covering the system's particle range.
+__global__ void k_roots(NodeArrays nd, int nsys, const long *__restrict__ soff) {
+  int sy = blockIdx.x * blockDim.x + threadIdx.x;
+  if (sy >= nsys) return;
+  nd.depth[sy] = 0;
+  nd.ix[sy] = nd.iy[sy] = nd.iz[sy] = 0;
+  nd.key[sy] = 0;
+  nd.sys[sy] = sy;
+  nd.pbegin[sy] = (int)soff[sy];
+  nd.pend[sy] = (int)soff[sy + 1];
+  nd.parent[sy] = -1;
+  nd.child[sy] = -1;
+  nd.nchild[sy] = 0;
+  nd.leaf[sy] = 1;
+  const double *b = nd.box + 4 * sy;
+  nd.cx[sy] = fma(0.5, b[3], b[0]);
+  nd.cy[sy] = fma(0.5, b[3], b[1]);
+  nd.cz[sy] = fma(0.5, b[3], b[2]);
 }
 
-hipError_t launch_root(const NodeArrays &nd, long n, const double *lo,
-                       double width, hipStream_t s) {
-  k_root<<<1, 64, 0, s>>>(nd, n, lo[0], lo[1], lo[2], width);
+hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
+                        hipStream_t s) {
+  k_roots<<<(nsys + 63) / 64, 64, 0, s>>>(nd, nsys, soff);
   return hipGetLastError();
 }
 
@@ -409,8 +474,7 @@ __global__ void k_split_count(NodeArrays nd, int lvl_begin, int lvl_count,
 __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
                               int next_begin, const int *__restrict__ bounds,
                               const int *__restrict__ cnt,
-                              const int *__restrict__ ofs, double lo0,
-                              double lo1, double lo2, double width) {
+                              const int *__restrict__ ofs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= lvl_count) return;
   int node = lvl_begin + i;
@@ -424,12 +488,14 @@ __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
   nd.leaf[node] = 0;
   nd.child[node] = first;
   nd.nchild[node] = cnt[i];
-  int d = nd.depth[node] + 1;
-  double h = ldexp(width, -d);
+  const int d = nd.depth[node] + 1;
+  const int sy = nd.sys[node];
+  const double *b = nd.box + 4 * sy;
+  const double h = ldexp(b[3], -d), lo0 = b[0], lo1 = b[1], lo2 = b[2];
   int k = 0;
   for (int o = 0; o < 8; o++) {
-    int b = bounds[9 * i + o], e = bounds[9 * i + o + 1];
-    if (e <= b) continue;
+    int bb = bounds[9 * i + o], e = bounds[9 * i + o + 1];
+    if (e <= bb) continue;
     int c = first + k++;
     int ix = 2 * nd.ix[node] + (o & 1);
     int iy = 2 * nd.iy[node] + ((o >> 1) & 1);
@@ -439,8 +505,9 @@ __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
     nd.iy[c] = iy;
     nd.iz[c] = iz;
     nd.key[c] = (nd.key[node] << 3) | (uint64_t)o;
-    nd.pbegin[c] = b;
+    nd.pbegin[c] = bb;
     nd.pend[c] = e;
+    nd.sys[c] = sy;
     nd.parent[c] = node;
     nd.child[c] = -1;
     nd.nchild[c] = 0;
@@ -462,12 +529,10 @@ hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
 
 hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
                               int lvl_count, int next_begin, const int *bounds,
-                              const int *cnt, const int *ofs, const double *lo,
-                              double width, hipStream_t s) {
+                              const int *cnt, const int *ofs, hipStream_t s) {
   if (lvl_count == 0) return hipSuccess;
   k_split_write<<<(lvl_count + 127) / 128, 128, 0, s>>>(
-      nd, lvl_begin, lvl_count, next_begin, bounds, cnt, ofs, lo[0], lo[1],
-      lo[2], width);
+      nd, lvl_begin, lvl_count, next_begin, bounds, cnt, ofs);
   return hipGetLastError();
 }
 

@@ -19,9 +19,16 @@ namespace aran {
 
 struct NodeArrays {
   int *depth, *ix, *iy, *iz, *pbegin, *pend, *parent, *child, *nchild, *leaf;
+  int *sys;           // system (independent particle set) of the node
   uint64_t *key;
   double *cx, *cy, *cz;
+  const double *box;  // per system: lo x, lo y, lo z, width of the root cube
 };
+
+// Width of a node's box.
+__host__ __device__ inline double node_width(const NodeArrays &nd, int node) {
+  return ldexp(nd.box[4 * nd.sys[node] + 3], -nd.depth[node]);
+}
 
 __host__ __device__ inline uint64_t spread3(uint64_t v) {
   v &= 0x1fffffull;

@@ -73,6 +73,30 @@ class Fmm3d:
         self.n = n
         return pot, grad
 
+    def solve_batch(self, offsets, x, y, z, q, lbs, ubs, order, leafcap,
+                    pot=None, grad=None):
+        """Independent systems in one solve (offsets: nsys+1 host ints)."""
+        import torch
+        n = x.numel()
+        nsys = len(offsets) - 1
+        assert offsets[0] == 0 and offsets[-1] == n
+        if pot is None:
+            pot = torch.empty(n, dtype=torch.float64, device=x.device)
+        if grad is None:
+            grad = torch.empty((n, 3), dtype=torch.float64, device=x.device)
+        off = (C.c_long * (nsys + 1))(*[int(v) for v in offsets])
+        lbv = (C.c_double * (3 * nsys))(*[float(v) for v in
+                                          np.asarray(lbs).ravel()])
+        ubv = (C.c_double * (3 * nsys))(*[float(v) for v in
+                                          np.asarray(ubs).ravel()])
+        rc = lib().aran_fmm3d_solve_batch(self._ctx, nsys, off, _ptr(x),
+                                          _ptr(y), _ptr(z), _ptr(q), lbv, ubv,
+                                          int(order), int(leafcap), _ptr(pot),
+                                          _ptr(grad))
+        check(rc, "aran_fmm3d_solve_batch")
+        self.n = n
+        return pot, grad
+
     def stats(self):
         v = (C.c_long * 12)()
         check(lib().aran_fmm3d_get_stats(self._ctx, v), "get_stats")

@@ -93,6 +93,70 @@ def error_vs_direct(f, dev, pot, grad, n, seed):
             "targets": 4096}
 
 
+def serve_session(n_requests, rate, seed, precision="mixed"):
+    """Poisson arrivals at `rate` requests/s; per request N log-uniform in
+    [1e3, 2e6], p in {6, 8, 12}, leaf 64, half potential-only, half
+    potential + gradient, uniform cube or Plummer (first N points of seeded
+    2e6-point pools, so each request is an i.i.d. sample), masses in (0,1].
+    Admission under a 200 GB budget.  Returns latency percentiles and
+    throughput."""
+    import threading
+
+    from aran_amd import workloads as W
+    from aran_amd.server import Server
+    nmax = 2_000_000
+    pools = {"cube": W.uniform_cube(nmax, seed), "plummer": W.plummer(nmax, seed)}
+    mass = 1.0 - W.uniform(seed, 9, nmax)
+    u = W.uniform(seed, 10, 5 * n_requests).reshape(n_requests, 5)
+    reqs = []
+    for k in range(n_requests):
+        n = int(round(10 ** (3 + u[k, 0] * np.log10(2000))))
+        order = (6, 8, 12)[min(2, int(u[k, 1] * 3))]
+        kind = "cube" if u[k, 2] < 0.5 else "plummer"
+        want_grad = u[k, 3] >= 0.5
+        x, y, z, _ = pools[kind]
+        lb, ub = ((0, 0, 0), (1, 1, 1)) if kind == "cube" else \
+            ((-1, -1, -1), (1, 1, 1))
+        reqs.append((x[:n], y[:n], z[:n], mass[:n], lb, ub, order, 64,
+                     want_grad))
+    gaps = -np.log(1.0 - u[:, 4]) / rate
+    srv = Server(budget_bytes=200e9, nworkers=2, precision=precision,
+                 max_batch=32)
+    ids = [None] * n_requests
+    done = {}
+
+    def waiter():
+        for k in range(n_requests):
+            while ids[k] is None:
+                time.sleep(1e-4)
+            pot, grad, t = srv.wait(ids[k])
+            done[k] = t
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    t_next = t0
+    for k, r in enumerate(reqs):
+        t_next += gaps[k]
+        dt = t_next - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        ids[k] = srv.submit(*r[:4], r[4], r[5], r[6], r[7], want_grad=r[8])
+    th.join()
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([done[k]["latency"] for k in range(n_requests)])
+    parts = sum(len(r[0]) for r in reqs)
+    return {"offered_load_req_s": rate, "requests": n_requests,
+            "p50_ms": float(1e3 * np.percentile(lat, 50)),
+            "p99_ms": float(1e3 * np.percentile(lat, 99)),
+            "mean_ms": float(1e3 * lat.mean()),
+            "throughput_req_s": n_requests / wall,
+            "throughput_particles_s": parts / wall,
+            "batches": st["batches"], "max_batch": st["max_batch"],
+            "workers": 2, "budget_bytes": 200e9, "form": precision}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +171,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparison run")
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="offered load of the serving session, requests/s")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this engine runs on one GPU; --gpus %d is not "
@@ -154,6 +221,10 @@ def main():
             "algorithmic": "%d M2L x %d flops" % (st["m2l"], m2l_flops(p)),
             "kernel_ms": round(head["m2l_ms"], 4)}
 
+    serve = None
+    if args.serve_requests > 0:
+        serve = serve_session(args.serve_requests, args.serve_rate, seed + 3,
+                              args.precision)
     cpu = None if args.no_cpu else cpu_baseline(args.cpu_sample, p, q, seed)
     line = {
         "metric": "3D Newton FMM particles/s at N=2^20 (Plummer, p=10, "
@@ -174,6 +245,7 @@ def main():
         "error": err,
         "phases_ms": head["phases_ms"],
         "plain_fp64": plain,
+        "serving": serve,
         "tree": {k: st[k] for k in ("depth", "nodes", "leaves", "m2l", "p2p")},
     }
     print(json.dumps(line))

@@ -37,6 +37,18 @@ int aran_fmm3d_solve (AranFmm3d *ctx, long n, const double *x,
                       const double *lb, const double *ub, int order,
                       int leafcap, double *pot, double *grad);
 
+/* Independent systems in one solve: system s is particles
+ * offsets[s] .. offsets[s+1]-1 of the concatenated device arrays (offsets is
+ * a host array of nsys+1 values, offsets[0] = 0), with root bounds
+ * lbs[3s..3s+2], ubs[3s..3s+2] (host).  All systems share order and leaf
+ * capacity; each gets its own octree and its results are those of a solve
+ * of that system alone. */
+int aran_fmm3d_solve_batch (AranFmm3d *ctx, int nsys, const long *offsets,
+                            const double *x, const double *y, const double *z,
+                            const double *q, const double *lbs,
+                            const double *ubs, int order, int leafcap,
+                            double *pot, double *grad);
+
 /* Arithmetic form of the next solves: ARAN_FMM3D_PLAIN (FP64 throughout) or
  * ARAN_FMM3D_MIXED (far field M2L and near field P2P in FP32 on expansions
  * scaled per level by the box width, FP64 tables, tree, P2M, M2M, L2L, L2P

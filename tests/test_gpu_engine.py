@@ -368,3 +368,98 @@ def test_mixed_form_close_to_plain(torch_cuda, kind, n, p, q):
     # deterministic: a second mixed solve is bitwise identical
     p2, g2 = [t.cpu().numpy() for t in f.solve(*dev, lb, ub, p, q)]
     assert np.array_equal(p1, p2) and np.array_equal(g1, g2)
+
+
+@pytest.mark.parametrize("precision", ["plain", "mixed"])
+def test_batch_equals_individual_solves(torch_cuda, precision):
+    """Independent systems solved in one launch sequence give, for each
+    system, exactly (bitwise) the result of solving it alone; counters add
+    up.  Includes an empty system and systems outside their given bounds."""
+    torch = torch_cuda
+    rng = np.random.default_rng(3)
+    systems = []
+    for k in range(7):
+        n = [900, 0, 3000, 1, 2500, 40, 1800][k]
+        if k % 2 == 0:
+            x, y, z, m = W.uniform_cube(n, 100 + k)
+            lb, ub = (0, 0, 0), (1, 1, 1)
+        else:
+            x, y, z, m = W.plummer(n, 100 + k) if n else (np.zeros(0),) * 4
+            lb, ub = (-1, -1, -1), (1, 1, 1)
+        if k == 4:
+            x = x * 3 - 1  # grows the root cube
+        systems.append((x, y, z, m, lb, ub))
+    p, q = 8, 24
+    f = _fmm(torch)
+    f.set_precision(precision)
+    offs = np.cumsum([0] + [len(s[0]) for s in systems])
+    cat = [np.concatenate([s[i] for s in systems]) for i in range(4)]
+    dev = _dev(torch, *cat)
+    pb, gb = [t.cpu().numpy() for t in f.solve_batch(
+        offs, *dev, [s[4] for s in systems], [s[5] for s in systems], p, q)]
+    sb = f.stats()
+    tot = {k: 0 for k in ("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p")}
+    for k, (x, y, z, m, lb, ub) in enumerate(systems):
+        if len(x) == 0:
+            continue
+        d = _dev(torch, x, y, z, m)
+        p1, g1 = [t.cpu().numpy() for t in f.solve(*d, lb, ub, p, q)]
+        sl = slice(offs[k], offs[k + 1])
+        assert np.array_equal(pb[sl], p1), k
+        assert np.array_equal(gb[sl], g1), k
+        s1 = f.stats()
+        for key in tot:
+            tot[key] += s1[key]
+    tot["zero"] += 1  # the empty system's root
+    for key in tot:
+        assert sb[key] == tot[key], key
+
+
+def test_served_results_equal_standalone(torch_cuda):
+    """Requests of mixed sizes, orders and outputs submitted together are
+    served in batches; every served result is bitwise the standalone solve
+    of that request (mixed form); a request over the budget is refused."""
+    torch = torch_cuda
+    from aran_amd.server import EBUDGET, Server, footprint
+    reqs = []
+    for k in range(12):
+        n = [1200, 5000, 800, 20000, 3000, 1, 9000, 2500, 700, 15000, 4000,
+             6000][k]
+        order = [6, 8][k % 2]
+        if k % 3 == 0:
+            x, y, z, m = W.plummer(n, 500 + k)
+            lb, ub = (-1, -1, -1), (1, 1, 1)
+        else:
+            x, y, z, m = W.uniform_cube(n, 500 + k)
+            lb, ub = (0, 0, 0), (1, 1, 1)
+        reqs.append((x, y, z, m, lb, ub, order, 32, k % 2 == 0))
+    srv = Server(budget_bytes=footprint(40000, 8, 32) * 2 + 128e6, nworkers=2,
+                 precision="mixed", max_batch=8)
+    srv.pause()
+    ids = [srv.submit(*r[:4], r[4], r[5], r[6], r[7], want_grad=r[8])
+           for r in reqs]
+    srv.pause(False)
+    assert all(i >= 0 for i in ids)
+    assert srv.submit(*W.uniform_cube(10, 1), (0, 0, 0), (1, 1, 1), 8, 32) >= 0
+    big = srv.submit(*W.uniform_cube(10, 1), (0, 0, 0), (1, 1, 1), 8, 32)
+    srv.wait(big - 1)
+    srv.wait(big)
+    out = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    assert st["served"] == len(reqs) + 2 and st["batches"] < st["served"]
+    srv.close()
+    f = _fmm(torch)
+    f.set_precision("mixed")
+    for r, (pot, grad, times) in zip(reqs, out):
+        d = _dev(torch, *r[:4])
+        p1, g1 = [t.cpu().numpy() for t in f.solve(*d, r[4], r[5], r[6], r[7])]
+        assert np.array_equal(pot, p1)
+        if r[8]:
+            assert np.array_equal(grad, g1)
+        else:
+            assert grad is None
+        assert times["latency"] >= times["service"] > 0
+    small = Server(budget_bytes=footprint(1000, 8, 32) * 2 + 128e6, nworkers=2)
+    rid = small.submit(*W.uniform_cube(5000, 3), (0, 0, 0), (1, 1, 1), 8, 32)
+    assert rid == EBUDGET
+    small.close()
