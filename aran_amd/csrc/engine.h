@@ -58,6 +58,7 @@ struct Stats {
   long zero = 0, p2p = 0, p2m = 0, m2m = 0, m2l = 0, l2l = 0, l2p = 0;
   int depth = 0;
   long nnodes = 0, nleaves = 0;
+  int m2l_split = 0, m2l_tiles = 0;  // mixed form: tiled far-field launch
   // phase times in ms (only with Problem::timings)
   float t_tree = 0, t_lists = 0, t_up = 0, t_m2l = 0, t_down = 0, t_near = 0,
         t_total = 0;
@@ -92,7 +93,8 @@ struct Workspace {
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
       colleagues;
   DevBuf mult, loc, small;
-  DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, mult32, loc32;
+  DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
+      tile_meta, mult32, loc32;
   DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos;
   long node_cap = 0;
   void *host_small = nullptr;  // pinned, 4 KiB

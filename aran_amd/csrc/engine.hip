@@ -127,7 +127,7 @@ Workspace::~Workspace() {
                    &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas, &tab.rot32,
                    &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
                    &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
-                   &mult32, &loc32, &n_sys, &sys_box, &sys_off, &sys_mm,
+                   &tile_rank, &tile_meta, &mult32, &loc32, &n_sys, &sys_box, &sys_off, &sys_mm,
                    &skey, &spos};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
@@ -223,15 +223,6 @@ void octant_offsets(const std::vector<DirEntry> &dirs, std::vector<short> &codes
 // Waves per far-field tile: the 189 offsets of a tile are split in this many
 // ranges so that a solve has several waves per SIMD (ARAN_M2L_SPLIT
 // overrides, 1..16).
-static int m2l_split() {
-  static int v = [] {
-    const char *e = getenv("ARAN_M2L_SPLIT");
-    int x = e ? atoi(e) : 8;
-    return std::min(16, std::max(1, x));
-  }();
-  return v;
-}
-
 hipError_t ensure_tables(Workspace &ws, int p) {
   if (ws.tab.order == p) return hipSuccess;
   hipStream_t s = ws.stream;
@@ -353,8 +344,8 @@ hipError_t ensure_tables(Workspace &ws, int p) {
   const long nrot = (long)nth * 4 * rb;
   CK(T.rot32.reserve(sizeof(float) * nrot));
   CK(T.ephi32.reserve(sizeof(float) * 2 * eiphi.size()));
-  CK(T.vfac32.reserve(sizeof(float) * (size_t)kNumCodes * (p + 1) * (p + 1) *
-                      std::max(p, 1)));
+  CK(T.vfac32.reserve(sizeof(float) * (size_t)kNumCodes *
+                      std::max(1, vfac_size(p))));
   CK(launch_m2l_tables(1, p, T.dir.as<DirEntry>(), T.tm2l.as<double>(),
                        T.rot.as<double>(), nrot, T.eiphi.as<double2>(),
                        T.vfac32.ptr, T.rot32.ptr, T.ephi32.ptr, s));
@@ -596,23 +587,49 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // ---- tiles of the far field (mixed form)
   int ntiles = 0;
   if (pb.precision != 0 && depth >= 2) {
+    // Tiles: 64 targets of one (level, octant) group each, in Morton order
+    // inside the group; within a level the tiles take chunk j of every
+    // octant in turn, so that consecutive tiles cover the same region.
     const int first = lstart[2], count = nn - lstart[2];
     const int ngroups = (kMaxDepth + 1) * 8;
-    CK(ws.tile_cnt.reserve(sizeof(int) * 3 * ngroups));
+    CK(ws.tile_cnt.reserve(sizeof(int) * ngroups));
     int *cnt = ws.tile_cnt.as<int>();
-    CK(launch_fill_int(cnt, 2 * ngroups, 0, s));
+    CK(launch_fill_int(cnt, ngroups, 0, s));
     CK(launch_tile_count(nd, first, count, cnt, s));
-    std::vector<int> hcnt(ngroups), base(ngroups);
+    CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)count + 1)));
+    int *rank = ws.tile_rank.as<int>();
+    CK(launch_octant_flags(nd, first, count, rank, s));
+    CK(scan_exclusive(rank, rank, 8L * count, rank + 8L * count, ws.scan_tmp,
+                      s));
+    std::vector<int> hcnt(ngroups);
     CK(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * ngroups,
                       hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
-    std::vector<int> toct;
+    // host tables: lstart[kMaxDepth + 2], gofs[ngroups] (first chunk of each
+    // group in tile_of), tile_of[ntiles] (tile of each chunk)
+    std::vector<int> meta(kMaxDepth + 2 + ngroups, 0), toct;
+    for (int d = 0; d <= depth; d++) meta[d] = lstart[d];
+    int *gofs = meta.data() + kMaxDepth + 2;
+    int nchunks = 0;
     for (int g = 0; g < ngroups; g++) {
-      base[g] = ntiles;
-      int t = (hcnt[g] + 63) / 64;
-      for (int k = 0; k < t; k++) toct.push_back(g % 8);
-      ntiles += t;
+      gofs[g] = nchunks;
+      nchunks += (hcnt[g] + 63) / 64;
     }
+    meta.resize(meta.size() + nchunks);
+    gofs = meta.data() + kMaxDepth + 2;
+    int *tile_of = gofs + ngroups;
+    for (int d = 2; d <= depth; d++) {
+      int maxch = 0;
+      for (int o = 0; o < 8; o++)
+        maxch = std::max(maxch, (hcnt[d * 8 + o] + 63) / 64);
+      for (int j = 0; j < maxch; j++)
+        for (int o = 0; o < 8; o++)
+          if (j < (hcnt[d * 8 + o] + 63) / 64) {
+            tile_of[gofs[d * 8 + o] + j] = ntiles++;
+            toct.push_back(o);
+          }
+    }
+    CK(ws.tile_meta.reserve(sizeof(int) * meta.size()));
     CK(ws.tile_oct.reserve(sizeof(int) * std::max(1, ntiles)));
     CK(ws.tile_tgt.reserve(sizeof(int) * 64 * (size_t)std::max(1, ntiles)));
     CK(ws.tile_src.reserve(sizeof(int) * 64 * (size_t)kOctOffsets *
@@ -620,12 +637,14 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.node_slot.reserve(sizeof(int) * nn));
     CK(hipMemcpyAsync(ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles,
                       hipMemcpyHostToDevice, s));
-    CK(hipMemcpyAsync(cnt + 2 * ngroups, base.data(), sizeof(int) * ngroups,
+    CK(hipMemcpyAsync(ws.tile_meta.ptr, meta.data(), sizeof(int) * meta.size(),
                       hipMemcpyHostToDevice, s));
+    const int *d_meta = ws.tile_meta.as<int>();
     CK(launch_fill_int(ws.tile_tgt.as<int>(), 64L * ntiles, -1, s));
     CK(launch_fill_int(ws.tile_src.as<int>(), 64L * kOctOffsets * ntiles, -1,
                        s));
-    CK(launch_tile_fill(nd, first, count, cnt + 2 * ngroups, cnt + ngroups,
+    CK(launch_tile_fill(nd, first, count, rank, d_meta, d_meta + kMaxDepth + 2,
+                        d_meta + kMaxDepth + 2 + ngroups,
                         ws.tile_tgt.as<int>(), ws.node_slot.as<int>(), s));
     CK(launch_tile_src(nd, first, count, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
                        ws.v_code.as<short>(), ws.tab.code_to_k.as<short>(),
@@ -654,13 +673,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
   } else {
     CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
-    const int nsplit = m2l_split();
-    const long split_stride = (long)nn * lc;
-    CK(ws.loc32.reserve(sizeof(float2) * (size_t)nsplit * split_stride));
+    const int nsplit = m2l_choose_split(ntiles, p);
+    CK(ws.loc32.reserve(sizeof(float2) * (size_t)nsplit *
+                        std::max(1, ntiles) * 64 * lc));
     CK(launch_scale_mult(1, nd, nn, p, M, ws.mult32.ptr, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
     M2LTileArgs ta;
-    ta.tile_tgt = ws.tile_tgt.as<int>();
     ta.tile_src = ws.tile_src.as<int>();
     ta.tile_oct = ws.tile_oct.as<int>();
     ta.oct_codes = ws.tab.oct_codes.as<short>();
@@ -669,11 +687,15 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     ta.rot = ws.tab.rot32.ptr;
     ta.vfac = ws.tab.vfac32.ptr;
     ta.nsplit = nsplit;
-    ta.split_stride = split_stride;
-    CK(launch_m2l_tile(1, p, ntiles, ta, ws.mult32.ptr, ws.loc32.ptr, s));
+    ta.tile_major = m2l_tile_major();
+    if (ntiles > 0)
+      CK(launch_m2l_tile(1, p, ntiles, ta, ws.mult32.ptr, ws.loc32.ptr, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
-    CK(launch_unscale_local(1, nd, nn, p, ws.loc32.ptr, nsplit,
-                            split_stride, L, s));
+    const int first = depth >= 2 ? lstart[2] : nn;
+    CK(launch_unscale_local(1, nd, p, first, ntiles, ws.tile_tgt.as<int>(),
+                            ws.loc32.ptr, nsplit, L, s));
+    S.m2l_split = nsplit;
+    S.m2l_tiles = ntiles;
   }
   // ---- downward pass
   for (int d = 1; d <= depth; d++)

@@ -3,6 +3,9 @@
 // expansions and the FP32 tables.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdlib>
+
 #include "engine.h"
 #include "m2l_tile.h"
 
@@ -53,18 +56,37 @@ __global__ void k_tile_count(NodeArrays nd, int first, int count,
   atomicAdd(&cnt[nd.depth[node] * 8 + oct], 1);
 }
 
-// Slot of each node inside the tiles of its (level, octant) group.  The
-// slot only chooses the lane; a target's result does not depend on it.
+// flags[o * count + i] = 1 if node first + i is child o of its parent.
+__global__ void k_octant_flags(NodeArrays nd, int first, int count,
+                               int *__restrict__ flags) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= 8L * count) return;
+  const int o = (int)(idx / count), node = first + (int)(idx % count);
+  const int oct = (nd.ix[node] & 1) | ((nd.iy[node] & 1) << 1) |
+                  ((nd.iz[node] & 1) << 2);
+  flags[idx] = oct == o;
+}
+
+// Slot of each node: its rank r among the nodes of its (level, octant)
+// group (Morton order; from the exclusive scan of the octant flags) picks
+// chunk r / 64 of the group, whose tile is tile_of[gofs[group] + r / 64],
+// and lane r % 64.
 __global__ void k_tile_fill(NodeArrays nd, int first, int count,
-                            const int *__restrict__ group_base,
-                            int *__restrict__ fill, int *__restrict__ tile_tgt,
+                            const int *__restrict__ rank_scan,
+                            const int *__restrict__ lstart,
+                            const int *__restrict__ gofs,
+                            const int *__restrict__ tile_of,
+                            int *__restrict__ tile_tgt,
                             int *__restrict__ node_slot) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= count) return;
-  int node = first + i;
-  int oct = (nd.ix[node] & 1) | ((nd.iy[node] & 1) << 1) | ((nd.iz[node] & 1) << 2);
-  int g = nd.depth[node] * 8 + oct;
-  int slot = group_base[g] * 64 + atomicAdd(&fill[g], 1);
+  const int node = first + i;
+  const int oct = (nd.ix[node] & 1) | ((nd.iy[node] & 1) << 1) |
+                  ((nd.iz[node] & 1) << 2);
+  const int d = nd.depth[node];
+  const long row = (long)oct * count;
+  const int r = rank_scan[row + i] - rank_scan[row + (lstart[d] - first)];
+  const int slot = tile_of[gofs[d * 8 + oct] + r / 64] * 64 + r % 64;
   tile_tgt[slot] = node;
   node_slot[node] = slot;
 }
@@ -103,12 +125,21 @@ hipError_t launch_tile_count(const NodeArrays &nd, int first, int count,
   return hipGetLastError();
 }
 
+hipError_t launch_octant_flags(const NodeArrays &nd, int first, int count,
+                               int *flags, hipStream_t s) {
+  const long n = 8L * count;
+  if (n <= 0) return hipSuccess;
+  k_octant_flags<<<(n + 255) / 256, 256, 0, s>>>(nd, first, count, flags);
+  return hipGetLastError();
+}
+
 hipError_t launch_tile_fill(const NodeArrays &nd, int first, int count,
-                            const int *group_base, int *fill, int *tile_tgt,
+                            const int *rank_scan, const int *lstart,
+                            const int *gofs, const int *tile_of, int *tile_tgt,
                             int *node_slot, hipStream_t s) {
   if (count <= 0) return hipSuccess;
-  k_tile_fill<<<(count + 255) / 256, 256, 0, s>>>(nd, first, count, group_base,
-                                                  fill, tile_tgt, node_slot);
+  k_tile_fill<<<(count + 255) / 256, 256, 0, s>>>(
+      nd, first, count, rank_scan, lstart, gofs, tile_of, tile_tgt, node_slot);
   return hipGetLastError();
 }
 
@@ -152,33 +183,91 @@ __global__ void k_scale_mult(NodeArrays nd, int nnodes, int p,
   Ms[idx] = o;
 }
 
-// L = (sum of the nsplit partial slices of Ls, in slice order) / w^(l+1)
-// for nodes of depth >= 2, zero above.
+// One workgroup per tile: sum the nsplit partial slices [split][tile][k][lane]
+// in slice order (coalesced reads), transpose through LDS, and write the 64
+// target rows L[node][k] = sum / w^(l+1) (coalesced writes), kUnscaleRows
+// coefficients at a time.
+constexpr int kUnscaleRows = 32;
+
 template <typename T>
-__global__ void k_unscale_local(NodeArrays nd, int nnodes, int p,
-                                const typename V2<T>::type *__restrict__ Ls,
-                                int nsplit, long split_stride,
-                                double2 *__restrict__ L) {
+__global__ __launch_bounds__(256) void k_unscale_local(
+    NodeArrays nd, int p, int ntiles, const int *__restrict__ tile_tgt,
+    const typename V2<T>::type *__restrict__ Ls, int nsplit,
+    double2 *__restrict__ L) {
+  __shared__ double2 sl[64 * (kUnscaleRows + 1)];
+  __shared__ double sw[64];
+  __shared__ int snode[64];
   const int lc = tri_p(p + 1);
-  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (idx >= (long)nnodes * lc) return;
-  int node = (int)(idx / lc), k = (int)(idx % lc);
-  if (nd.depth[node] < 2) {
-    L[idx] = make_double2(0., 0.);
-    return;
+  const int row = kUnscaleRows + 1;  // odd row stride in LDS
+  const int tile = blockIdx.x;
+  const long slice = (long)ntiles * lc * 64;
+  const typename V2<T>::type *src = Ls + (long)tile * lc * 64;
+  if (threadIdx.x < 64) {
+    const int node = tile_tgt[tile * 64 + threadIdx.x];
+    snode[threadIdx.x] = node;
+    sw[threadIdx.x] = node >= 0 ? node_width(nd, node) : 1.;
   }
-  int l = 0;
-  while (tri_p(l + 1) <= k) l++;
-  double w = node_width(nd, node);
-  double f = 1. / w;
-  for (int i = 0; i < l; i++) f /= w;
-  double sx = 0., sy = 0.;
-  for (int sl = 0; sl < nsplit; sl++) {
-    typename V2<T>::type v = Ls[sl * split_stride + idx];
-    sx += (double)v.x;
-    sy += (double)v.y;
+  for (int k0 = 0; k0 < lc; k0 += kUnscaleRows) {
+    const int nk = min(kUnscaleRows, lc - k0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
+      const int kk = e / 64, lane = e % 64;
+      double sx = 0., sy = 0.;
+      for (int s = 0; s < nsplit; s++) {
+        typename V2<T>::type v = src[s * slice + (long)(k0 + kk) * 64 + lane];
+        sx += (double)v.x;
+        sy += (double)v.y;
+      }
+      sl[lane * row + kk] = make_double2(sx, sy);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
+      const int lane = e / nk, kk = e % nk, k = k0 + kk;
+      const int node = snode[lane];
+      if (node < 0) continue;
+      int l = 0;
+      while (tri_p(l + 1) <= k) l++;
+      const double w = sw[lane];
+      double f = 1. / w;
+      for (int i = 0; i < l; i++) f /= w;
+      const double2 v = sl[lane * row + kk];
+      L[(long)node * lc + k] = make_double2(v.x * f, v.y * f);
+    }
   }
-  L[idx] = make_double2(sx * f, sy * f);
+}
+
+__global__ void k_zero_double2(double2 *__restrict__ p, long n) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) p[i] = make_double2(0., 0.);
+}
+
+int m2l_tile_major() {
+  if (const char *e = getenv("ARAN_M2L_TILE_MAJOR")) return atoi(e) != 0;
+  return 1;
+}
+
+int m2l_choose_split(int ntiles, int p) {
+  if (const char *e = getenv("ARAN_M2L_SPLIT")) {
+    int v = atoi(e);
+    if (v >= 1 && v <= kOctOffsets) return v;
+  }
+  // A CU holds 4 * m2l_tile_waves(p) / kTileWaves workgroups; the duration
+  // is modelled as the number of workgroup rounds times the offsets per
+  // workgroup (plus a fixed cost of two offsets for the output rows and the
+  // slice reduction).
+  const int ngroups = (ntiles + kTileWaves - 1) / kTileWaves;
+  const long slots = 256L * std::max(1, 4 * m2l_tile_waves(p) / kTileWaves);
+  int best = 1;
+  long best_cost = -1;
+  for (int ns = 1; ns <= 24; ns++) {
+    long rounds = ((long)ngroups * ns + slots - 1) / slots;
+    long cost = rounds * ((kOctOffsets + ns - 1) / ns + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = ns;
+    }
+  }
+  return best;
 }
 
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
@@ -194,19 +283,20 @@ hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
   return hipGetLastError();
 }
 
-hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int nnodes,
-                                int p, const void *Ls, int nsplit,
-                                long split_stride, double2 *L, hipStream_t s) {
-  long n = (long)nnodes * tri_p(p + 1);
-  if (n == 0) return hipSuccess;
+hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int p,
+                                int first, int ntiles, const int *tile_tgt,
+                                const void *Ls, int nsplit, double2 *L,
+                                hipStream_t s) {
+  const int lc = tri_p(p + 1);
+  const long nz = (long)first * lc;
+  if (nz > 0) k_zero_double2<<<(nz + 255) / 256, 256, 0, s>>>(L, nz);
+  if (ntiles <= 0) return hipGetLastError();
   if (precision == 0)
-    k_unscale_local<double><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, static_cast<const double2 *>(Ls), nsplit,
-        split_stride, L);
+    k_unscale_local<double><<<ntiles, 256, 0, s>>>(
+        nd, p, ntiles, tile_tgt, static_cast<const double2 *>(Ls), nsplit, L);
   else
-    k_unscale_local<float><<<(n + 255) / 256, 256, 0, s>>>(
-        nd, nnodes, p, static_cast<const float2 *>(Ls), nsplit,
-        split_stride, L);
+    k_unscale_local<float><<<ntiles, 256, 0, s>>>(
+        nd, p, ntiles, tile_tgt, static_cast<const float2 *>(Ls), nsplit, L);
   return hipGetLastError();
 }
 // ------------------------------------------------------------ tables
@@ -216,20 +306,22 @@ template <typename T>
 __global__ void k_vfac(int p, const DirEntry *__restrict__ dir,
                        const double *__restrict__ tm2l, T *__restrict__ out) {
   const int code = blockIdx.x;
-  const int nv = (p + 1) * (p + 1) * p;
+  const int vs = vfac_size(p);
   const DirEntry de = dir[code];
-  for (int i = threadIdx.x; i < nv; i += blockDim.x) {
-    int l = i / ((p + 1) * p), m = (i / p) % (p + 1), n = i % p;
-    double v = 0.;
-    if (de.norm > 0. && m <= l && n >= m) {
-      double rho = 1. / de.norm, f = 1.;
-      for (int k = 0; k < l + n + 1; k++) f *= rho;
-      v = tm2l[i] * f;
-      if ((l + m) & 1) v = -v;
-      if ((l + n) & 1) v *= de.c;
-    }
-    out[(size_t)code * nv + i] = (T)v;
-  }
+  for (int l = 0; l <= p; l++)
+    for (int m = 0; m <= min(l, p - 1); m++)
+      for (int n = m + threadIdx.x; n < p; n += blockDim.x) {
+        const int i = (l * (p + 1) + m) * p + n;  // [l][m][n] of tm2l
+        double v = 0.;
+        if (de.norm > 0.) {
+          double rho = 1. / de.norm, f = 1.;
+          for (int k = 0; k < l + n + 1; k++) f *= rho;
+          v = tm2l[i] * f;
+          if ((l + m) & 1) v = -v;
+          if ((l + n) & 1) v *= de.c;
+        }
+        out[(size_t)code * vs + vfac_ofs(p, l, m) + (n - m)] = (T)v;
+      }
 }
 
 template <typename T>

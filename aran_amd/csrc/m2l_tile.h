@@ -11,17 +11,37 @@ namespace aran {
 // list, for one octant of its parent: 6^3 - 3^3 = 189.
 constexpr int kOctOffsets = 189;
 
+// Tiles (one wave each) per workgroup of the tiled far-field kernel.
+constexpr int kTileWaves = 4;
+
+// Waves per SIMD of the tiled far-field kernel of order p: two while the
+// expansions fit in half the register file, else one (the kernel then also
+// keeps values in the accumulation registers instead of scratch memory).
+constexpr __host__ __device__ int m2l_tile_waves(int p) { return p <= 10 ? 2 : 1; }
+
+// Packed vertical factors of one offset: for l = 0..p, m = 0..min(l, p-1),
+// n = m..p-1; vfac_ofs(p, l, m) is the position of (l, m, n = m).
+constexpr __host__ __device__ int vfac_ofs(int p, int l, int m) {
+  int o = 0;
+  for (int a = 0; a < l; a++)
+    for (int b = 0; b <= (a < p - 1 ? a : p - 1); b++) o += p - b;
+  for (int b = 0; b < m; b++) o += p - b;
+  return o;
+}
+constexpr __host__ __device__ int vfac_size(int p) {
+  return vfac_ofs(p, p + 1, 0);
+}
+
 struct M2LTileArgs {
-  const int *tile_tgt;    // [ntiles * 64] target node or -1
   const int *tile_src;    // [ntiles * 189 * 64] source node or -1
   const int *tile_oct;    // [ntiles] octant shared by the tile
   const short *oct_codes; // [8 * 189] offset code, grouped by theta
   const DirEntry *dir;    // [343]
   const void *ephi;       // [343][2(p+1)]   e^{i m phi} as T
   const void *rot;        // [ntheta][4][rot_block(p)] as T
-  const void *vfac;       // [343][(p+1)(p+1)p] scaled vertical factors
-  int nsplit;             // waves per tile (offset ranges), >= 1
-  long split_stride;      // elements between partial-local slices
+  const void *vfac;       // [343][vfac_size(p)] scaled vertical factors
+  int nsplit;             // offset ranges per tile (partial slices), >= 1
+  int tile_major;         // XCD order: all ranges of a tile group together
 };
 
 // Octant offset lists (host): codes[8][189] sorted by theta table index,
@@ -34,8 +54,11 @@ hipError_t launch_m2l_tile(int precision, int p, int ntiles,
                            hipStream_t s);
 hipError_t launch_tile_count(const NodeArrays &nd, int first, int count,
                              int *cnt, hipStream_t s);
+hipError_t launch_octant_flags(const NodeArrays &nd, int first, int count,
+                               int *flags, hipStream_t s);
 hipError_t launch_tile_fill(const NodeArrays &nd, int first, int count,
-                            const int *group_base, int *fill, int *tile_tgt,
+                            const int *rank_scan, const int *lstart,
+                            const int *gofs, const int *tile_of, int *tile_tgt,
                             int *node_slot, hipStream_t s);
 hipError_t launch_tile_src(const NodeArrays &nd, int first, int count,
                            const int *vofs, const int *vsrc, const short *vcode,
@@ -44,9 +67,16 @@ hipError_t launch_tile_src(const NodeArrays &nd, int first, int count,
 hipError_t launch_fill_int(int *p, long n, int v, hipStream_t s);
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
                              int p, const double2 *M, void *Ms, hipStream_t s);
-hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int nnodes,
-                                int p, const void *Ls, int nsplit,
-                                long split_stride, double2 *L, hipStream_t s);
+// L of the tiled targets = (sum of the nsplit slices [split][tile][coef][lane]
+// of Ls, in slice order) / w^(l+1); rows of nodes [0, first) are zeroed.
+hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int p,
+                                int first, int ntiles, const int *tile_tgt,
+                                const void *Ls, int nsplit, double2 *L,
+                                hipStream_t s);
+// Offset ranges per tile for a launch over ntiles tiles of order p.
+int m2l_choose_split(int ntiles, int p);
+// Workgroup order of that launch (M2LTileArgs::tile_major).
+int m2l_tile_major();
 hipError_t launch_m2l_tables(int precision, int p, const DirEntry *dir,
                              const double *tm2l, const double *rot, long nrot,
                              const double2 *eiphi, void *vfac, void *rotT,

@@ -171,6 +171,10 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparison run")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run one elementwise kernel that reads and "
+                         "writes 512 MiB (the reference point for HBM byte "
+                         "counters)")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="offered load of the serving session, requests/s")
@@ -221,6 +225,13 @@ def main():
             "algorithmic": "%d M2L x %d flops" % (st["m2l"], m2l_flops(p)),
             "kernel_ms": round(head["m2l_ms"], 4)}
 
+    if args.calibrate_copy:
+        # one elementwise kernel that reads 512 MiB and writes 512 MiB
+        a = torch.ones(1 << 26, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        torch.add(a, 1.0, out=b)
+        torch.cuda.synchronize()
+        del a, b
     serve = None
     if args.serve_requests > 0:
         serve = serve_session(args.serve_requests, args.serve_rate, seed + 3,
