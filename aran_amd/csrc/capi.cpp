@@ -313,11 +313,16 @@ extern "C" int aran_fmm3d_export_lists(const AranFmm3d *ctx, int *far_ofs,
                                        int *leaves, int *near_ofs,
                                        int *near_begin, int *near_end) {
   if (!ctx) return ARAN_FMM3D_EINVAL;
-  const aran::Workspace &ws = ctx->ws;
+  aran::Workspace &ws = const_cast<aran::Workspace &>(ctx->ws);
   hipStream_t s = ws.stream;
   long nn = ctx->ti.nnodes, nl = ctx->ti.nleaves, nf = ctx->ti.nfar,
        nu = ctx->ti.nnear;
   hipError_t e = hipSuccess;
+  if (nn > 0 && ws.far_csr_nodes != nn && (far_ofs || far_src || far_code)) {
+    int nfar = 0;  // the mixed form keeps no CSR far lists: build them now
+    e = aran::build_far_lists(ws, nn, &nfar);
+    if (e == hipSuccess && nfar != nf) e = hipErrorUnknown;
+  }
   if (far_ofs && nn) e = d2h(far_ofs, ws.v_ofs, nn + 1, s);
   if (e == hipSuccess && far_src) e = d2h(far_src, ws.v_src, nf, s);
   if (e == hipSuccess && far_code) e = d2h(far_code, ws.v_code, nf, s);

@@ -51,6 +51,8 @@ struct OpTables {
   const double *tm2m;    // [p][p][p]
   const double *tl2l;    // [p+1][p+1][p+1]
   const double *ynorm;   // [tri(p+2)]  N_l^m
+  const double *ogain;   // [tri(p+1)]  N_l^m (l+m)!
+  const double *pgain;   // [tri(p)]    4 pi / (2l+1) N_l^m (l+m)!
   int p, rb;
 };
 
@@ -77,7 +79,7 @@ struct DevBuf {
 struct Tables {
   int order = -1;
   int ntheta = 0;
-  DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, dscr, thetas;
+  DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, ogain, pgain, dscr, thetas;
   // tiled far field (mixed form): FP32 copies and octant offset lists
   DevBuf rot32, ephi32, vfac32, oct_codes, code_to_k;
   std::vector<DirEntry> dir_host;
@@ -92,11 +94,12 @@ struct Workspace {
   DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
       colleagues;
-  DevBuf mult, loc, small;
+  DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
       tile_meta, mult32, loc32;
   DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos;
   long node_cap = 0;
+  long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
   void *host_small = nullptr;  // pinned, 4 KiB
   Tables tab;
   hipStream_t stream = nullptr;
@@ -136,6 +139,8 @@ struct TreeInfo {
 };
 
 hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti);
+// CSR far lists (v_ofs, v_src, v_code) of the last tree, from its colleagues.
+hipError_t build_far_lists(Workspace &ws, long nn, int *nfar);
 hipError_t ensure_tables(Workspace &ws, int order);
 hipError_t workspace_init(Workspace &ws, hipStream_t stream);
 const char *last_error();
@@ -208,8 +213,12 @@ hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const double2 *M, double2 *L, hipStream_t s);
 hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
                       int lvl_count, double2 *L, hipStream_t s);
+// pleaf[j] = leaf node of sorted particle j (every particle of the leaves)
+hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
+                                int nleaves, int *pleaf, hipStream_t s);
+// L2P of sorted particles [0, n), adding into pot / grad
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
-                      const int *leaves, int nleaves, const double *xs,
+                      const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
                       double *pot, double *grad, hipStream_t s);
 

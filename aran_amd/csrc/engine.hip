@@ -88,6 +88,8 @@ OpTables Tables::view() const {
   t.tm2m = tm2m.as<double>();
   t.tl2l = tl2l.as<double>();
   t.ynorm = ynorm.as<double>();
+  t.ogain = ogain.as<double>();
+  t.pgain = pgain.as<double>();
   t.p = order;
   t.rb = rot_block(order);
   return t;
@@ -122,9 +124,9 @@ Workspace::~Workspace() {
                    &split_bounds, &split_cnt, &split_ofs, &flags, &leaf_list,
                    &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &mult, &loc,
-                   &small,
+                   &small, &pleaf,
                    &tab.rot, &tab.eiphi, &tab.dir, &tab.tm2l, &tab.tm2m,
-                   &tab.tl2l, &tab.ynorm, &tab.dscr, &tab.thetas, &tab.rot32,
+                   &tab.tl2l, &tab.ynorm, &tab.ogain, &tab.pgain, &tab.dscr, &tab.thetas, &tab.rot32,
                    &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
                    &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &n_sys, &sys_box, &sys_off, &sys_mm,
@@ -301,6 +303,17 @@ hipError_t ensure_tables(Workspace &ws, int p) {
       yn[tri(l) + m] = (m == 0) ? sqrt((l + l + 1.) / (4. * M_PI))
                                 : yn[tri(l) + m - 1] / sqrt((l - m + 1.) * (l + m));
 
+  // gains of the solid harmonics: g_l^m = N_l^m (l+m)!, and for P2M
+  // 4 pi / (2l+1) g_l^m
+  std::vector<double> og(tri(p + 1)), pg(std::max(1, tri(p)));
+  for (int l = 0; l <= p; l++)
+    for (int m = 0; m <= l; m++) {
+      double f = 1.;
+      for (int i = 2; i <= l + m; i++) f *= i;
+      og[tri(l) + m] = yn[tri(l) + m] * f;
+      if (l < p) pg[tri(l) + m] = 4. * M_PI / (l + l + 1.) * og[tri(l) + m];
+    }
+
   int rb = rot_block(p);
   int w = 2 * p + 1;
   CK(T.thetas.reserve(sizeof(double) * nth));
@@ -312,6 +325,8 @@ hipError_t ensure_tables(Workspace &ws, int p) {
   CK(T.tm2m.reserve(sizeof(double) * std::max<size_t>(1, tm2m.size())));
   CK(T.tl2l.reserve(sizeof(double) * tl2l.size()));
   CK(T.ynorm.reserve(sizeof(double) * yn.size()));
+  CK(T.ogain.reserve(sizeof(double) * og.size()));
+  CK(T.pgain.reserve(sizeof(double) * pg.size()));
   CK(hipMemcpyAsync(T.thetas.ptr, thetas.data(), sizeof(double) * nth,
                     hipMemcpyHostToDevice, s));
   CK(hipMemsetAsync(T.dscr.ptr, 0, sizeof(double) * (size_t)nth * (p + 1) * w * w,
@@ -329,6 +344,10 @@ hipError_t ensure_tables(Workspace &ws, int p) {
   CK(hipMemcpyAsync(T.tl2l.ptr, tl2l.data(), sizeof(double) * tl2l.size(),
                     hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(T.ynorm.ptr, yn.data(), sizeof(double) * yn.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.ogain.ptr, og.data(), sizeof(double) * og.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.pgain.ptr, pg.data(), sizeof(double) * pg.size(),
                     hipMemcpyHostToDevice, s));
   CK(launch_wigner_tables(T.thetas.as<double>(), nth, p, T.dscr.as<double>(),
                           T.rot.as<double>(), s));
@@ -379,6 +398,29 @@ static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
   ws.node_cap = std::min(ws.n_depth.bytes / sizeof(int),
                          ws.n_cx.bytes / sizeof(double));
   ws.node_cap = std::min<long>(ws.node_cap, ws.n_key.bytes / sizeof(uint64_t));
+  return hipSuccess;
+}
+
+// Far (M2L) lists in CSR form from the colleagues of the last tree: the
+// children of the parent's colleagues that are not adjacent to the node.
+hipError_t build_far_lists(Workspace &ws, long nn, int *nfar) {
+  hipStream_t s = ws.stream;
+  NodeArrays nd = ws.nodes();
+  const int *col = ws.colleagues.as<int>();
+  CK(ws.v_cnt.reserve(sizeof(int) * nn));
+  CK(ws.v_ofs.reserve(sizeof(int) * (nn + 1)));
+  CK(launch_vlist(nd, (int)nn, col, 0, ws.v_cnt.as<int>(), nullptr, nullptr,
+                  nullptr, s));
+  CK(scan_exclusive(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
+                    ws.v_ofs.as<int>() + nn, ws.scan_tmp, s));
+  int nf = 0;
+  CK(read_int(ws, ws.v_ofs.as<int>() + nn, &nf));
+  CK(ws.v_src.reserve(sizeof(int) * std::max(1, nf)));
+  CK(ws.v_code.reserve(sizeof(short) * std::max(1, nf)));
+  CK(launch_vlist(nd, (int)nn, col, 1, ws.v_cnt.as<int>(), ws.v_ofs.as<int>(),
+                  ws.v_src.as<int>(), ws.v_code.as<short>(), s));
+  ws.far_csr_nodes = nn;
+  *nfar = nf;
   return hipSuccess;
 }
 
@@ -554,19 +596,11 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   int *col = ws.colleagues.as<int>();
   for (int d = 0; d <= depth; d++)
     CK(launch_colleagues(nd, lstart[d], lcount[d], col, s));
-  // ---- far list
-  CK(ws.v_cnt.reserve(sizeof(int) * nn));
-  CK(ws.v_ofs.reserve(sizeof(int) * (nn + 1)));
-  CK(launch_vlist(nd, nn, col, 0, ws.v_cnt.as<int>(), nullptr, nullptr,
-                  nullptr, s));
-  CK(scan_exclusive(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
-                    ws.v_ofs.as<int>() + nn, ws.scan_tmp, s));
+  // ---- far list (CSR; the mixed form reads its far field straight from
+  // the colleagues and builds this only when the lists are exported)
   int nfar = 0;
-  CK(read_int(ws, ws.v_ofs.as<int>() + nn, &nfar));
-  CK(ws.v_src.reserve(sizeof(int) * std::max(1, nfar)));
-  CK(ws.v_code.reserve(sizeof(short) * std::max(1, nfar)));
-  CK(launch_vlist(nd, nn, col, 1, ws.v_cnt.as<int>(), ws.v_ofs.as<int>(),
-                  ws.v_src.as<int>(), ws.v_code.as<short>(), s));
+  ws.far_csr_nodes = -1;
+  if (pb.precision == 0) CK(build_far_lists(ws, nn, &nfar));
   // ---- near list
   CK(ws.u_cnt.reserve(sizeof(int) * nleaves));
   CK(ws.u_ofs.reserve(sizeof(int) * (nleaves + 1)));
@@ -583,6 +617,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                   ws.u_ofs.as<int>(), ws.u_beg.as<int>(), ws.u_end.as<int>(),
                   ws.u_pairs.as<long long>(), s));
   long long *d_pairs = reinterpret_cast<long long *>(ws.small.as<char>() + 1024);
+  // (d_pairs is followed by the 64 partial sums of launch_sum_ll)
+  int *d_far = reinterpret_cast<int *>(ws.small.as<char>() + 2048);
   CK(launch_sum_ll(ws.u_pairs.as<long long>(), nleaves, d_pairs, s));
   // ---- tiles of the far field (mixed form)
   int ntiles = 0;
@@ -592,23 +628,27 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     // octant in turn, so that consecutive tiles cover the same region.
     const int first = lstart[2], count = nn - lstart[2];
     const int ngroups = (kMaxDepth + 1) * 8;
+    // device tables: lstart[kMaxDepth + 2], gofs[ngroups] (first chunk of
+    // each group in tile_of), tile_of[chunks] (tile of each chunk)
+    const size_t nmeta = kMaxDepth + 2 + ngroups + count / 64 + ngroups;
+    CK(ws.tile_meta.reserve(sizeof(int) * nmeta));
+    std::vector<int> meta(kMaxDepth + 2 + ngroups, 0), toct;
+    for (int d = 0; d <= depth; d++) meta[d] = lstart[d];
+    int *d_meta = ws.tile_meta.as<int>();
+    CK(hipMemcpyAsync(d_meta, meta.data(), sizeof(int) * (kMaxDepth + 2),
+                      hipMemcpyHostToDevice, s));
     CK(ws.tile_cnt.reserve(sizeof(int) * ngroups));
     int *cnt = ws.tile_cnt.as<int>();
-    CK(launch_fill_int(cnt, ngroups, 0, s));
-    CK(launch_tile_count(nd, first, count, cnt, s));
     CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)count + 1)));
     int *rank = ws.tile_rank.as<int>();
     CK(launch_octant_flags(nd, first, count, rank, s));
     CK(scan_exclusive(rank, rank, 8L * count, rank + 8L * count, ws.scan_tmp,
                       s));
+    CK(launch_group_counts(first, count, depth, d_meta, rank, cnt, s));
     std::vector<int> hcnt(ngroups);
     CK(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * ngroups,
                       hipMemcpyDeviceToHost, s));
     CK(hipStreamSynchronize(s));
-    // host tables: lstart[kMaxDepth + 2], gofs[ngroups] (first chunk of each
-    // group in tile_of), tile_of[ntiles] (tile of each chunk)
-    std::vector<int> meta(kMaxDepth + 2 + ngroups, 0), toct;
-    for (int d = 0; d <= depth; d++) meta[d] = lstart[d];
     int *gofs = meta.data() + kMaxDepth + 2;
     int nchunks = 0;
     for (int g = 0; g < ngroups; g++) {
@@ -629,7 +669,6 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
             toct.push_back(o);
           }
     }
-    CK(ws.tile_meta.reserve(sizeof(int) * meta.size()));
     CK(ws.tile_oct.reserve(sizeof(int) * std::max(1, ntiles)));
     CK(ws.tile_tgt.reserve(sizeof(int) * 64 * (size_t)std::max(1, ntiles)));
     CK(ws.tile_src.reserve(sizeof(int) * 64 * (size_t)kOctOffsets *
@@ -637,18 +676,18 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.node_slot.reserve(sizeof(int) * nn));
     CK(hipMemcpyAsync(ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles,
                       hipMemcpyHostToDevice, s));
-    CK(hipMemcpyAsync(ws.tile_meta.ptr, meta.data(), sizeof(int) * meta.size(),
+    CK(hipMemcpyAsync(d_meta + kMaxDepth + 2, meta.data() + kMaxDepth + 2,
+                      sizeof(int) * (meta.size() - kMaxDepth - 2),
                       hipMemcpyHostToDevice, s));
-    const int *d_meta = ws.tile_meta.as<int>();
     CK(launch_fill_int(ws.tile_tgt.as<int>(), 64L * ntiles, -1, s));
-    CK(launch_fill_int(ws.tile_src.as<int>(), 64L * kOctOffsets * ntiles, -1,
-                       s));
     CK(launch_tile_fill(nd, first, count, rank, d_meta, d_meta + kMaxDepth + 2,
                         d_meta + kMaxDepth + 2 + ngroups,
                         ws.tile_tgt.as<int>(), ws.node_slot.as<int>(), s));
-    CK(launch_tile_src(nd, first, count, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
-                       ws.v_code.as<short>(), ws.tab.code_to_k.as<short>(),
-                       ws.node_slot.as<int>(), ws.tile_src.as<int>(), s));
+    CK(launch_fill_int(d_far, 1, 0, s));
+    CK(launch_tile_src(nd, ntiles, ws.tile_tgt.as<int>(),
+                       ws.tile_oct.as<int>(), col,
+                       ws.tab.oct_codes.as<short>(), ws.tile_src.as<int>(),
+                       d_far, s));
     // the host vectors above must outlive the copies
     CK(hipStreamSynchronize(s));
   }
@@ -706,7 +745,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                 ws.u_beg.as<int>(), ws.u_end.as<int>(), ws.xs.as<double>(),
                 ws.ys.as<double>(), ws.zs.as<double>(), ws.qs.as<double>(),
                 ws.pot_s.as<double>(), ws.grad_s.as<double>(), s));
-  CK(launch_l2p(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
+  CK(ws.pleaf.reserve(sizeof(int) * std::max<long>(1, n)));
+  CK(launch_particle_leaf(nd, leaves, nleaves, ws.pleaf.as<int>(), s));
+  CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
+                ws.ys.as<double>(),
                 ws.zs.as<double>(), L, ws.pot_s.as<double>(),
                 ws.grad_s.as<double>(), s));
   CK(launch_scatter(ws.idx_a.as<int>(), n, ws.pot_s.as<double>(),
@@ -715,8 +757,13 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
 
   long long *hp = reinterpret_cast<long long *>(
       static_cast<char *>(ws.host_small) + 1024);
+  int *hfar = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
+                                      2048);
   CK(hipMemcpyAsync(hp, d_pairs, sizeof(long long), hipMemcpyDeviceToHost, s));
+  if (pb.precision != 0 && depth >= 2)
+    CK(hipMemcpyAsync(hfar, d_far, sizeof(int), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
+  if (pb.precision != 0) nfar = depth >= 2 ? *hfar : 0;
   S.zero = nn;
   S.p2p = (long)(*hp / 2);
   S.p2m = n;

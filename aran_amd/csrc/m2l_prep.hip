@@ -46,14 +46,23 @@ hipError_t launch_m2l_tile(int precision, int p, int ntiles,
 
 // ------------------------------------------------------------ preparation
 
-// Per-level octant counts of the nodes of depth >= 2.
-__global__ void k_tile_count(NodeArrays nd, int first, int count,
-                             int *__restrict__ cnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  int node = first + i;
-  int oct = (nd.ix[node] & 1) | ((nd.iy[node] & 1) << 1) | ((nd.iz[node] & 1) << 2);
-  atomicAdd(&cnt[nd.depth[node] * 8 + oct], 1);
+// Node count of each (level, octant) group g = 8 d + o, d >= 2, from the
+// exclusive scan of the octant flags (rank_scan[8 * count] is the total).
+__global__ void k_group_counts(int first, int count, int depth,
+                               const int *__restrict__ lstart,
+                               const int *__restrict__ rank_scan,
+                               int *__restrict__ cnt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= (kMaxDepth + 1) * 8) return;
+  const int d = g / 8, o = g % 8;
+  int c = 0;
+  if (d >= 2 && d <= depth) {
+    const long row = (long)o * count;
+    const int a = lstart[d] - first;
+    const int b = d < depth ? lstart[d + 1] - first : count;
+    c = rank_scan[row + b] - rank_scan[row + a];
+  }
+  cnt[g] = c;
 }
 
 // flags[o * count + i] = 1 if node first + i is child o of its parent.
@@ -91,26 +100,69 @@ __global__ void k_tile_fill(NodeArrays nd, int first, int count,
   node_slot[node] = slot;
 }
 
-// Dense source table of the tiles from the far (M2L) lists.
-__global__ void k_tile_src(NodeArrays nd, int first, int count,
-                           const int *__restrict__ vofs,
-                           const int *__restrict__ vsrc,
-                           const short *__restrict__ vcode,
-                           const short *__restrict__ code_to_k,
-                           const int *__restrict__ node_slot,
-                           int *__restrict__ tile_src) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  int node = first + i;
-  int oct = (nd.ix[node] & 1) | ((nd.iy[node] & 1) << 1) | ((nd.iz[node] & 1) << 2);
-  int slot = node_slot[node];
-  int tile = slot / 64, lane = slot % 64;
-  for (int e = vofs[node]; e < vofs[node + 1]; e++) {
-    int k = code_to_k[oct * kNumCodes + vcode[e]];
-    if (k < 0) continue;  // not a far offset of this octant (cannot happen
-                          // for lists built by k_vlist)
-    tile_src[((size_t)tile * kOctOffsets + k) * 64 + lane] = vsrc[e];
+// Dense source table of the tiles, straight from the colleagues: for a
+// target A (octant b in its parent P) and far offset o (one of the 189 of
+// octant b, target minus source), the source sits at A - o, whose parent is
+// the colleague of P at floor((b - o) / 2) and whose octant there is
+// (b - o) mod 2; it is a source when that colleague is not a leaf and has
+// that child.  Entries with no source, and lanes with no target, are -1.
+// One workgroup per tile: the children of the 27 parent colleagues of each
+// lane's target are first summarised in LDS (first child, octant mask; the
+// children of a node are stored in octant order), then the 189 x 64 entries
+// are written with coalesced stores.  *nfar receives the number of sources
+// (the far-field operation count).
+__global__ __launch_bounds__(256) void k_tile_src(
+    NodeArrays nd, int ntiles, const int *__restrict__ tile_tgt,
+    const int *__restrict__ tile_oct, const int *__restrict__ col,
+    const short *__restrict__ oct_codes, int *__restrict__ tile_src,
+    int *__restrict__ nfar) {
+  __shared__ int sbase[27 * 64];
+  __shared__ unsigned char smask[27 * 64];
+  __shared__ int wcount[4];
+  const int tile = blockIdx.x;
+  for (int e = threadIdx.x; e < 27 * 64; e += blockDim.x) {
+    const int ps = e / 64, lane = e % 64;
+    const int A = tile_tgt[tile * 64 + lane];
+    int base = 0, mask = 0;
+    if (A >= 0) {
+      const int C = col[(size_t)nd.parent[A] * 27 + ps];
+      if (C >= 0 && !nd.leaf[C]) {
+        base = nd.child[C];
+        for (int K = base; K < base + nd.nchild[C]; K++)
+          mask |= 1 << ((nd.ix[K] & 1) | (nd.iy[K] & 1) << 1 |
+                        (nd.iz[K] & 1) << 2);
+      }
+    }
+    sbase[e] = base;
+    smask[e] = (unsigned char)mask;
   }
+  __syncthreads();
+  const int b = tile_oct[tile];
+  const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
+  int found = 0;
+  for (int e = threadIdx.x; e < kOctOffsets * 64; e += blockDim.x) {
+    const int k = e / 64, lane = e % 64;
+    const int code = oct_codes[b * kOctOffsets + k];
+    const int ox = code / 49 - 3, oy = (code / 7) % 7 - 3, oz = code % 7 - 3;
+    const int ux = bx - ox, uy = by - oy, uz = bz - oz;  // in [-2, 3]
+    int res = -1;
+    if (ux >= -2 && ux <= 3 && uy >= -2 && uy <= 3 && uz >= -2 && uz <= 3) {
+      const int ps =
+          ((ux >> 1) + 1) * 9 + ((uy >> 1) + 1) * 3 + ((uz >> 1) + 1);
+      const int co = (ux & 1) | (uy & 1) << 1 | (uz & 1) << 2;
+      const int mask = smask[ps * 64 + lane];
+      if (mask >> co & 1)
+        res = sbase[ps * 64 + lane] + __popc(mask & ((1 << co) - 1));
+    }
+    tile_src[(size_t)tile * kOctOffsets * 64 + e] = res;
+    found += res >= 0;
+  }
+  // per-wave then per-block count, one atomic per block (exact integers)
+  for (int off = 32; off > 0; off >>= 1) found += __shfl_down(found, off);
+  if (threadIdx.x % 64 == 0) wcount[threadIdx.x / 64] = found;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    atomicAdd(nfar, wcount[0] + wcount[1] + wcount[2] + wcount[3]);
 }
 
 __global__ void k_fill_int(int *__restrict__ p, long n, int v) {
@@ -118,10 +170,11 @@ __global__ void k_fill_int(int *__restrict__ p, long n, int v) {
   if (i < n) p[i] = v;
 }
 
-hipError_t launch_tile_count(const NodeArrays &nd, int first, int count,
-                             int *cnt, hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  k_tile_count<<<(count + 255) / 256, 256, 0, s>>>(nd, first, count, cnt);
+hipError_t launch_group_counts(int first, int count, int depth,
+                               const int *lstart, const int *rank_scan,
+                               int *cnt, hipStream_t s) {
+  k_group_counts<<<1, (kMaxDepth + 1) * 8, 0, s>>>(first, count, depth,
+                                                   lstart, rank_scan, cnt);
   return hipGetLastError();
 }
 
@@ -143,14 +196,13 @@ hipError_t launch_tile_fill(const NodeArrays &nd, int first, int count,
   return hipGetLastError();
 }
 
-hipError_t launch_tile_src(const NodeArrays &nd, int first, int count,
-                           const int *vofs, const int *vsrc, const short *vcode,
-                           const short *code_to_k, const int *node_slot,
-                           int *tile_src, hipStream_t s) {
-  if (count <= 0) return hipSuccess;
-  k_tile_src<<<(count + 255) / 256, 256, 0, s>>>(nd, first, count, vofs, vsrc,
-                                                 vcode, code_to_k, node_slot,
-                                                 tile_src);
+hipError_t launch_tile_src(const NodeArrays &nd, int ntiles,
+                           const int *tile_tgt, const int *tile_oct,
+                           const int *col, const short *oct_codes,
+                           int *tile_src, int *nfar, hipStream_t s) {
+  if (ntiles <= 0) return hipSuccess;
+  k_tile_src<<<ntiles, 256, 0, s>>>(nd, ntiles, tile_tgt, tile_oct, col,
+                                    oct_codes, tile_src, nfar);
   return hipGetLastError();
 }
 

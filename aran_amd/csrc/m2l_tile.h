@@ -12,7 +12,7 @@ namespace aran {
 constexpr int kOctOffsets = 189;
 
 // Tiles (one wave each) per workgroup of the tiled far-field kernel.
-constexpr int kTileWaves = 4;
+constexpr int kTileWaves = 8;
 
 // Waves per SIMD of the tiled far-field kernel of order p: two while the
 // expansions fit in half the register file, else one (the kernel then also
@@ -52,18 +52,21 @@ void octant_offsets(const std::vector<DirEntry> &dirs, std::vector<short> &codes
 hipError_t launch_m2l_tile(int precision, int p, int ntiles,
                            const M2LTileArgs &a, const void *Ms, void *Ls,
                            hipStream_t s);
-hipError_t launch_tile_count(const NodeArrays &nd, int first, int count,
-                             int *cnt, hipStream_t s);
+// Node counts of the (level, octant) groups from the octant-flag scan
+// (lstart: level starts on the device).
+hipError_t launch_group_counts(int first, int count, int depth,
+                               const int *lstart, const int *rank_scan,
+                               int *cnt, hipStream_t s);
 hipError_t launch_octant_flags(const NodeArrays &nd, int first, int count,
                                int *flags, hipStream_t s);
 hipError_t launch_tile_fill(const NodeArrays &nd, int first, int count,
                             const int *rank_scan, const int *lstart,
                             const int *gofs, const int *tile_of, int *tile_tgt,
                             int *node_slot, hipStream_t s);
-hipError_t launch_tile_src(const NodeArrays &nd, int first, int count,
-                           const int *vofs, const int *vsrc, const short *vcode,
-                           const short *code_to_k, const int *node_slot,
-                           int *tile_src, hipStream_t s);
+hipError_t launch_tile_src(const NodeArrays &nd, int ntiles,
+                           const int *tile_tgt, const int *tile_oct,
+                           const int *col, const short *oct_codes,
+                           int *tile_src, int *nfar, hipStream_t s);
 hipError_t launch_fill_int(int *p, long n, int v, hipStream_t s);
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
                              int p, const double2 *M, void *Ms, hipStream_t s);

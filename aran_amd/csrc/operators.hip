@@ -184,73 +184,100 @@ __device__ inline double2 rotate_inv_one(const double *__restrict__ R, int rb,
   return cmulc(ph[mp], make_double2(re, im));
 }
 
-// ------------------------------------------------------------ P2M
+// ------------------------------------------------------------ P2M / L2P
+//
+// Both go through the normalised regular solid harmonics
+//   O_l^m = r^l P_l^m(cos theta) e^{i m phi} / (l+m)!
+// (P with the Condon-Shortley phase), from
+//   O_0^0 = 1,  O_m^m = -(x + iy) / (2m) O_{m-1}^{m-1},
+//   O_l^m = ((2l-1) z O_{l-1}^m - r^2 O_{l-2}^m) / (l^2 - m^2)   (l > m),
+// which need no square root or angle and are exact at r = 0, and whose
+// derivatives are again harmonics of one degree less:
+//   d/dz O_l^m = O_{l-1}^m,  (d/dx + i d/dy) O_l^m = O_{l-1}^{m+1},
+//   (d/dx - i d/dy) O_l^m = -O_{l-1}^{m-1}.
+// With g_l^m = N_l^m (l+m)!, r^l Y_l^m = g_l^m O_l^m.
 
-// One block per leaf: M_l^m = sum_j q_j (4pi/(2l+1)) r^l N_l^m P_l^m e^{-im phi}
-__global__ __launch_bounds__(kOpThreads) void k_p2m(
+// 1/n for 0 < n < 256, each correctly rounded (compile-time constants).
+struct InvTable {
+  double v[256];
+};
+constexpr InvTable make_inv_table() {
+  InvTable t{};
+  for (int i = 1; i < 256; i++) t.v[i] = 1.0 / i;
+  return t;
+}
+__constant__ InvTable kInv = make_inv_table();
+
+constexpr int kP2mSlots = 1024;  // LDS staging, double2 (16 KiB)
+
+// One wave per leaf: M_l^m = (4pi/(2l+1)) g_l^m conj(sum_j q_j O_l^m(x_j))
+// (aran_development3d_p2m, arandevelopment3d.c:169-216).  Per chunk of
+// particles, lanes take (particle, m) columns and store q O_l^m in LDS (the
+// reciprocals are computed, so no memory access sits in the recurrence),
+// then lanes take coefficients and add the chunk's particles in index order.
+__global__ __launch_bounds__(64) void k_p2m(
     OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double *__restrict__ qs,
     double2 *__restrict__ M) {
-  __shared__ double sr[kOpThreads], sx[kOpThreads], ss[kOpThreads],
-      scp[kOpThreads], ssp[kOpThreads], sq[kOpThreads];
+  __shared__ double2 T[kP2mSlots];
   const int p = tb.p;
   const int mc = tri(p);
+  const int lane = threadIdx.x;
   const int A = leaves[blockIdx.x];
   const int b = nd.pbegin[A], e = nd.pend[A];
   const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
+  const int chunk = min(64, kP2mSlots / mc);
   double2 acc[2] = {make_double2(0., 0.), make_double2(0., 0.)};
-  for (int c0 = b; c0 < e; c0 += kOpThreads) {
-    int j = c0 + threadIdx.x;
-    if (j < e) {
-      double vx = xs[j] - cx, vy = ys[j] - cy, vz = zs[j] - cz;
-      double rho2 = vx * vx + vy * vy;
-      double r = sqrt(rho2 + vz * vz), rho = sqrt(rho2);
-      sr[threadIdx.x] = r;
-      sx[threadIdx.x] = (r > 0.) ? vz / r : 1.;
-      ss[threadIdx.x] = (r > 0.) ? rho / r : 0.;
-      scp[threadIdx.x] = (rho > 0.) ? vx / rho : 1.;
-      ssp[threadIdx.x] = (rho > 0.) ? vy / rho : 0.;
-      sq[threadIdx.x] = qs[j];
+  for (int c0 = b; c0 < e; c0 += chunk) {
+    const int cnt = min(chunk, e - c0);
+    __syncthreads();
+    for (int t = lane; t < cnt * p; t += 64) {
+      const int j = t / p, m = t - j * p;
+      const int i = c0 + j;
+      const double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
+      const double r2 = vx * vx + vy * vy + vz * vz;
+      double ox = qs[i], oy = 0.;  // q O_m^m
+      for (int k = 1; k <= m; k++) {
+        const double f = -0.5 / (double)k;
+        const double ax = f * vx, ay = f * vy;
+        const double nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
+        ox = nx;
+        oy = ny;
+      }
+      double2 *row = T + j * mc;
+      row[tri(m) + m] = make_double2(ox, oy);
+      double px = ox, py = oy, qx = 0., qy = 0.;  // degrees l-1, l-2
+      for (int l = m + 1; l < p; l++) {
+        const double a = (2. * l - 1.) * vz;
+        const double g = 1. / (double)(l * l - m * m);
+        const double nx = (a * px - r2 * qx) * g, ny = (a * py - r2 * qy) * g;
+        qx = px;
+        qy = py;
+        px = nx;
+        py = ny;
+        row[tri(l) + m] = make_double2(nx, ny);
+      }
     }
     __syncthreads();
-    int cnt = min(kOpThreads, e - c0);
     for (int slot = 0; slot < 2; slot++) {
-      int k = threadIdx.x + slot * kOpThreads;
+      const int k = lane + 64 * slot;
       if (k >= mc) break;
-      int l, m;
-      lm_of(k, l, m);
-      double f = 4. * M_PI / (l + l + 1.) * tb.ynorm[k];
       double2 a = acc[slot];
-      for (int jj = 0; jj < cnt; jj++) {
-        double x = sx[jj], s = ss[jj];
-        // P_m^m, then the three-term recurrence up to l
-        double pmm = 1.;
-        for (int i = 1; i <= m; i++) pmm = -(2. * i - 1.) * s * pmm;
-        double p2 = 0., p1 = pmm;
-        for (int i = m + 1; i <= l; i++) {
-          double pn = (i == m + 1)
-                          ? (2. * m + 1.) * x * pmm
-                          : ((2. * i - 1.) * x * p1 - (i + m - 1.) * p2) / (i - m);
-          p2 = p1;
-          p1 = pn;
-        }
-        double rl = 1.;
-        for (int i = 0; i < l; i++) rl *= sr[jj];
-        double2 e1 = make_double2(1., 0.);
-        double2 ep = make_double2(scp[jj], ssp[jj]);
-        for (int i = 0; i < m; i++) e1 = cmul(e1, ep);
-        double t = sq[jj] * f * rl * p1;
-        a.x += t * e1.x;
-        a.y -= t * e1.y;
+      for (int j = 0; j < cnt; j++) {
+        const double2 v = T[j * mc + k];
+        a.x += v.x;
+        a.y += v.y;
       }
       acc[slot] = a;
     }
-    __syncthreads();
   }
   for (int slot = 0; slot < 2; slot++) {
-    int k = threadIdx.x + slot * kOpThreads;
-    if (k < mc) M[(size_t)A * mc + k] = acc[slot];
+    const int k = lane + 64 * slot;
+    if (k < mc) {
+      const double g = tb.pgain[k];
+      M[(size_t)A * mc + k] = make_double2(g * acc[slot].x, -g * acc[slot].y);
+    }
   }
 }
 
@@ -440,78 +467,91 @@ __global__ __launch_bounds__(kOpThreads) void k_l2l(
 
 // ------------------------------------------------------------ L2P
 
-// One block per leaf, one thread per particle: potential and gradient of
-// the leaf's local expansion, accumulated into pot / grad (sorted order).
-__global__ __launch_bounds__(kOpThreads) void k_l2p(
-    OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
+// One lane per particle (aran_development3d_local_evaluate and
+// local_gradient_evaluate, arandevelopment3d.c:485-494,541-552): with
+// c_l^m = g_l^m L_l^m of the particle's leaf,
+//   phi = sum_l [c_l^0 O_l^0 + 2 Re sum_{m>0} c_l^m O_l^m],
+//   d/dz phi = same sum with O_{l-1}^m,
+//   (d/dx + i d/dy) phi = sum_l [c_l^0 O_{l-1}^1
+//       + sum_{m>0} (c_l^m O_{l-1}^{m+1} - conj(c_l^m O_{l-1}^{m-1}))],
+// accumulated per harmonic O_j^m as it is generated (column by column).
+__global__ __launch_bounds__(256) void k_l2p(
+    OpTables tb, NodeArrays nd, const int *__restrict__ pleaf, int n,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double2 *__restrict__ L,
     double *__restrict__ pot, double *__restrict__ grad) {
-  __shared__ double2 Ls[kMaxLc];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   const int p = tb.p, lc = tri(p + 1);
-  const int A = leaves[blockIdx.x];
-  for (int k = threadIdx.x; k < lc; k += kOpThreads)
-    Ls[k] = L[(size_t)A * lc + k];
-  __syncthreads();
-  const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
-  const int b = nd.pbegin[A], e = nd.pend[A];
-  for (int i = b + threadIdx.x; i < e; i += kOpThreads) {
-    double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
-    double rho2 = vx * vx + vy * vy;
-    double r = sqrt(rho2 + vz * vz), rho = sqrt(rho2);
-    double x = (r > 0.) ? vz / r : 1., s = (r > 0.) ? rho / r : 0.;
-    double cp = (rho > 0.) ? vx / rho : 1., sp = (rho > 0.) ? vy / rho : 0.;
-    double phi = 0., dr = 0., dt = 0., dp = 0.;
-    double pmm = 1., smm = 0., rm = 1., rmm1 = 0.;
-    double2 eim = make_double2(1., 0.);
-    for (int m = 0; m <= p; m++) {
-      double wgt = (m == 0) ? 1. : 2.;
-      double p1 = 0., p2 = 0., s1 = 0., s2 = 0.;
-      double rl = rm, rlm1 = rmm1;
-      for (int l = m; l <= p; l++) {
-        double P, S;
-        if (l == m) {
-          P = pmm;
-          S = smm;
-        } else if (l == m + 1) {
-          P = (2. * m + 1.) * x * pmm;
-          S = (2. * m + 1.) * x * smm;
-        } else {
-          P = ((2. * l - 1.) * x * p1 - (l + m - 1.) * p2) / (l - m);
-          S = ((2. * l - 1.) * x * s1 - (l + m - 1.) * s2) / (l - m);
-        }
-        double2 z = cmul(Ls[tri(l) + m], eim);
-        double nlm = tb.ynorm[tri(l) + m];
-        double np = nlm * P;
-        phi += wgt * rl * np * z.x;
-        if (l >= 1) dr += wgt * l * rlm1 * np * z.x;
-        if (m >= 1) {
-          double dpdt = l * x * S - (l + m) * s1;
-          dt += wgt * rlm1 * nlm * dpdt * z.x;
-          dp -= wgt * m * rlm1 * nlm * S * z.y;
-        }
-        if (m == 1) {
-          // theta derivative of the m = 0 column: dP_l^0/dtheta = P_l^1
-          dt += rlm1 * tb.ynorm[tri(l)] * P * Ls[tri(l)].x;
-        }
-        p2 = p1;
-        p1 = P;
-        s2 = s1;
-        s1 = S;
-        rlm1 = rl;
-        rl *= r;
-      }
-      smm = -(2. * m + 1.) * pmm;
-      pmm = -(2. * m + 1.) * s * pmm;
-      rmm1 = rm;
-      rm *= r;
-      eim = cmul(eim, make_double2(cp, sp));
+  const int A = pleaf[i];
+  const double2 *__restrict__ La = L + (size_t)A * lc;
+  const double *__restrict__ g = tb.ogain;
+  const double vx = xs[i] - nd.cx[A], vy = ys[i] - nd.cy[A],
+               vz = zs[i] - nd.cz[A];
+  const double r2 = vx * vx + vy * vy + vz * vz;
+  double phi = 0., dz = 0., gx = 0., gy = 0.;
+  double mx = 1., my = 0.;  // O_m^m
+  for (int m = 0; m <= p; m++) {
+    if (m > 0) {
+      const double f = -0.5 * kInv.v[m];
+      const double ax = f * vx, ay = f * vy;
+      const double nx = mx * ax - my * ay, ny = mx * ay + my * ax;
+      mx = nx;
+      my = ny;
     }
-    pot[i] += phi;
-    grad[3 * (size_t)i + 0] += s * cp * dr + x * cp * dt - sp * dp;
-    grad[3 * (size_t)i + 1] += s * sp * dr + x * sp * dt + cp * dp;
-    grad[3 * (size_t)i + 2] += x * dr - s * dt;
+    const double w = m ? 2. : 1.;
+    double ox = mx, oy = my, qx = 0., qy = 0.;  // O_j^m, O_{j-1}^m
+    double2 c = La[tri(m) + m];
+    double gc = g[tri(m) + m];
+    c.x *= gc;
+    c.y *= gc;
+    for (int j = m; j <= p; j++) {
+      if (j > m) {
+        const double a = (2. * j - 1.) * vz, s = kInv.v[j * j - m * m];
+        const double nx = (a * ox - r2 * qx) * s, ny = (a * oy - r2 * qy) * s;
+        qx = ox;
+        qy = oy;
+        ox = nx;
+        oy = ny;
+      }
+      phi += w * (c.x * ox - c.y * oy);
+      if (j < p) {
+        const int k1 = tri(j + 1) + m;
+        double2 c1 = La[k1];  // c_{j+1}^m
+        const double g1 = g[k1];
+        c1.x *= g1;
+        c1.y *= g1;
+        dz += w * (c1.x * ox - c1.y * oy);
+        if (m > 0) {  // c_{j+1}^{m-1} O_j^m
+          double2 c2 = La[k1 - 1];
+          const double g2 = g[k1 - 1];
+          gx += g2 * (c2.x * ox - c2.y * oy);
+          gy += g2 * (c2.x * oy + c2.y * ox);
+        }
+        {  // - conj(c_{j+1}^{m+1} O_j^m)
+          double2 c3 = La[k1 + 1];
+          const double g3 = g[k1 + 1];
+          gx -= g3 * (c3.x * ox - c3.y * oy);
+          gy += g3 * (c3.x * oy + c3.y * ox);
+        }
+        c = c1;
+      }
+    }
   }
+  pot[i] += phi;
+  grad[3 * (size_t)i + 0] += gx;
+  grad[3 * (size_t)i + 1] += gy;
+  grad[3 * (size_t)i + 2] += dz;
+}
+
+// pleaf[j] = leaf node of sorted particle j.
+__global__ void k_particle_leaf(NodeArrays nd, const int *__restrict__ leaves,
+                                int nleaves, int *__restrict__ pleaf) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int f = t / 8, sub = t % 8;  // 8 lanes per leaf
+  if (f >= nleaves) return;
+  const int A = leaves[f];
+  for (int j = nd.pbegin[A] + sub; j < nd.pend[A]; j += 8) pleaf[j] = A;
 }
 
 // ------------------------------------------------------------ launchers
@@ -521,7 +561,7 @@ hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
                       const double *ys, const double *zs, const double *qs,
                       double2 *M, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
-  k_p2m<<<nleaves, kOpThreads, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
+  k_p2m<<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
   return hipGetLastError();
 }
 
@@ -547,13 +587,21 @@ hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
   return hipGetLastError();
 }
 
+hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
+                                int nleaves, int *pleaf, hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  const long t = 8L * nleaves;
+  k_particle_leaf<<<(t + 255) / 256, 256, 0, s>>>(nd, leaves, nleaves, pleaf);
+  return hipGetLastError();
+}
+
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
-                      const int *leaves, int nleaves, const double *xs,
+                      const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
                       double *pot, double *grad, hipStream_t s) {
-  if (nleaves == 0) return hipSuccess;
-  k_l2p<<<nleaves, kOpThreads, 0, s>>>(tb, nd, leaves, xs, ys, zs, L, pot,
-                                       grad);
+  if (n == 0) return hipSuccess;
+  k_l2p<<<(n + 255) / 256, 256, 0, s>>>(tb, nd, pleaf, n, xs, ys, zs, L, pot,
+                                        grad);
   return hipGetLastError();
 }
 

@@ -31,6 +31,30 @@ template <> __device__ inline float rsqrt_t<float>(float x) {
   return rsqrtf(x);
 }
 
+template <typename T> struct Src4;
+template <> struct Src4<float> {
+  typedef float4 type;
+};
+template <> struct Src4<double> {
+  typedef double4 type;
+};
+
+// One source against the lane's target; f == fself is the target itself.
+template <typename T>
+__device__ __forceinline__ void near_one(const typename Src4<T>::type &s,
+                                         T xi, T yi, T zi, bool self, T &p,
+                                         T &gx, T &gy, T &gz) {
+  const T dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
+  const T r2 = dx * dx + dy * dy + dz * dz;
+  const T ir = self ? (T)0 : rsqrt_t<T>(r2);
+  const T qr = s.w * ir;
+  const T qr3 = qr * (ir * ir);
+  p += qr;
+  gx -= qr3 * dx;
+  gy -= qr3 * dy;
+  gz -= qr3 * dz;
+}
+
 template <typename T>
 __global__ __launch_bounds__(kNearThreads) void k_near(
     const int *__restrict__ leaves, NodeArrays nd, const int *__restrict__ uofs,
@@ -38,9 +62,10 @@ __global__ __launch_bounds__(kNearThreads) void k_near(
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double *__restrict__ qs,
     double *__restrict__ pot, double *__restrict__ grad) {
-  __shared__ T sx[kStage], sy[kStage], sz[kStage], sq[kStage];
-  __shared__ int sid[kStage];
+  typedef typename Src4<T>::type S4;
+  __shared__ S4 src[kStage];  // x, y, z relative to the leaf centre; q
   __shared__ int lbeg[kListChunk], lpre[kListChunk + 1];
+  __shared__ int own;  // staged position of the target leaf's first particle
   __shared__ double red[4][kNearThreads];
   const int tid = threadIdx.x;
   const int A = leaves[blockIdx.x];
@@ -60,12 +85,14 @@ __global__ __launch_bounds__(kNearThreads) void k_near(
       __syncthreads();
       if (tid < nc) lbeg[tid] = ubeg[c0 + tid];
       if (tid == 0) {
-        int s = 0;
+        int s = 0, o = -(1 << 30);
         for (int k = 0; k < nc; k++) {
           lpre[k] = s;
+          if (ubeg[c0 + k] == tb) o = s;
           s += uend[c0 + k] - ubeg[c0 + k];
         }
         lpre[nc] = s;
+        own = o;
       }
       __syncthreads();
       const int total = lpre[nc];
@@ -82,26 +109,30 @@ __global__ __launch_bounds__(kNearThreads) void k_near(
               hi = mid - 1;
           }
           const int j = lbeg[lo] + (g - lpre[lo]);
-          sx[f] = (T)(xs[j] - ox);
-          sy[f] = (T)(ys[j] - oy);
-          sz[f] = (T)(zs[j] - oz);
-          sq[f] = (T)qs[j];
-          sid[f] = j;
+          S4 v;
+          v.x = (T)(xs[j] - ox);
+          v.y = (T)(ys[j] - oy);
+          v.z = (T)(zs[j] - oz);
+          v.w = (T)qs[j];
+          src[f] = v;
         }
         __syncthreads();
         if (act) {
+          // the target's own staged position in this pass (never matches
+          // when the target leaf is not in it)
+          const int fself = own - s0 + (i - tb);
           T p = 0, gx = 0, gy = 0, gz = 0;
-          for (int f = my_s; f < ns; f += nsl) {
-            const T dx = xi - sx[f], dy = yi - sy[f], dz = zi - sz[f];
-            const T r2 = dx * dx + dy * dy + dz * dz;
-            const T ir = (sid[f] == i) ? (T)0 : rsqrt_t<T>(r2);
-            const T qr = sq[f] * ir;
-            const T qr3 = qr * ir * ir;
-            p += qr;
-            gx -= qr3 * dx;
-            gy -= qr3 * dy;
-            gz -= qr3 * dz;
+          int f = my_s;
+          for (; f + 3 * nsl < ns; f += 4 * nsl) {
+            const S4 a = src[f], b = src[f + nsl], c = src[f + 2 * nsl],
+                     d = src[f + 3 * nsl];
+            near_one<T>(a, xi, yi, zi, f == fself, p, gx, gy, gz);
+            near_one<T>(b, xi, yi, zi, f + nsl == fself, p, gx, gy, gz);
+            near_one<T>(c, xi, yi, zi, f + 2 * nsl == fself, p, gx, gy, gz);
+            near_one<T>(d, xi, yi, zi, f + 3 * nsl == fself, p, gx, gy, gz);
           }
+          for (; f < ns; f += nsl)
+            near_one<T>(src[f], xi, yi, zi, f == fself, p, gx, gy, gz);
           ap += (double)p;
           ax += (double)gx;
           ay += (double)gy;

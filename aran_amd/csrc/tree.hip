@@ -449,26 +449,40 @@ __device__ inline int digit_lower_bound(const uint64_t *__restrict__ keys,
   return b;
 }
 
+// Eight lanes per node of the level: lane o finds where octant digit o
+// starts inside the node's sorted key range (lane 0: the range start); the
+// node splits when it holds more than leafcap particles below the maximum
+// depth, and cnt counts its non-empty octants.
 __global__ void k_split_count(NodeArrays nd, int lvl_begin, int lvl_count,
                               const uint64_t *__restrict__ keys, int leafcap,
                               int *__restrict__ bounds, int *__restrict__ cnt) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= lvl_count) return;
-  int node = lvl_begin + i;
-  int b = nd.pbegin[node], e = nd.pend[node], d = nd.depth[node];
-  int c = 0;
-  if (e - b > leafcap && d < kMaxDepth) {
-    int shift = 3 * (kMaxDepth - d - 1);
-    int prev = b;
-    bounds[9 * i] = b;
-    for (int o = 1; o <= 8; o++) {
-      int x = (o < 8) ? digit_lower_bound(keys, prev, e, shift, o) : e;
-      bounds[9 * i + o] = x;
-      c += (x > prev);
-      prev = x;
-    }
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const int i = (int)(t / 8), o = (int)(t % 8);
+  const bool valid = i < lvl_count;
+  int b = 0, e = 0, d = kMaxDepth;
+  if (valid) {
+    const int node = lvl_begin + i;
+    b = nd.pbegin[node];
+    e = nd.pend[node];
+    d = nd.depth[node];
   }
-  cnt[i] = c;
+  const bool split = valid && e - b > leafcap && d < kMaxDepth;
+  int x = b;
+  if (split && o > 0)
+    x = digit_lower_bound(keys, b, e, 3 * (kMaxDepth - d - 1), o);
+  // next boundary: the lane above in the group of 8, the range end for o = 7
+  int nx = __shfl_down(x, 1, 8);
+  if (o == 7) nx = e;
+  int c = (split && nx > x) ? 1 : 0;
+  c += __shfl_xor(c, 1, 8);
+  c += __shfl_xor(c, 2, 8);
+  c += __shfl_xor(c, 4, 8);
+  if (!valid) return;
+  if (split) {
+    bounds[9 * i + o] = x;
+    if (o == 7) bounds[9 * i + 8] = e;
+  }
+  if (o == 0) cnt[i] = c;
 }
 
 __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
@@ -522,8 +536,9 @@ hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
                               int lvl_count, const uint64_t *keys, int leafcap,
                               int *bounds, int *cnt, hipStream_t s) {
   if (lvl_count == 0) return hipSuccess;
-  k_split_count<<<(lvl_count + 127) / 128, 128, 0, s>>>(
-      nd, lvl_begin, lvl_count, keys, leafcap, bounds, cnt);
+  const long t = 8L * lvl_count;
+  k_split_count<<<(t + 255) / 256, 256, 0, s>>>(nd, lvl_begin, lvl_count, keys,
+                                                leafcap, bounds, cnt);
   return hipGetLastError();
 }
 
@@ -566,36 +581,42 @@ hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
 
 // Colleagues: the 27 same-depth nodes adjacent to (or equal to) a node,
 // stored by offset slot (dx+1)*9 + (dy+1)*3 + (dz+1), -1 when absent.  The
-// colleagues of a node are children of its parent's colleagues, so one
-// launch per level, root first, fills them without any search.
+// neighbour of node A (octant b in its parent P) at offset d has parent
+// P + floor((b + d) / 2), a colleague of P, and octant (b + d) mod 2 there;
+// so one launch per level, root first, fills the slots with one thread per
+// (node, slot) and a scan of at most 8 children.
 __global__ void k_colleagues(NodeArrays nd, int first, int count,
                              int *__restrict__ col) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= count) return;
-  int A = first + i;
-  int *ca = col + (size_t)A * 27;
-  for (int k = 0; k < 27; k++) ca[k] = -1;
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 27L * count) return;
+  const int A = first + (int)(t / 27), slot = (int)(t % 27);
+  int res = -1;
   if (nd.depth[A] == 0) {
-    ca[13] = A;
-    return;
-  }
-  const int P = nd.parent[A];
-  const int ax = nd.ix[A], ay = nd.iy[A], az = nd.iz[A];
-  for (int s = 0; s < 27; s++) {
-    int C = col[(size_t)P * 27 + s];
-    if (C < 0 || nd.leaf[C]) continue;
-    for (int K = nd.child[C]; K < nd.child[C] + nd.nchild[C]; K++) {
-      int dx = nd.ix[K] - ax, dy = nd.iy[K] - ay, dz = nd.iz[K] - az;
-      if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1)
-        ca[(dx + 1) * 9 + (dy + 1) * 3 + (dz + 1)] = K;
+    res = slot == 13 ? A : -1;
+  } else {
+    const int dx = slot / 9 - 1, dy = (slot / 3) % 3 - 1, dz = slot % 3 - 1;
+    const int ux = (nd.ix[A] & 1) + dx, uy = (nd.iy[A] & 1) + dy,
+              uz = (nd.iz[A] & 1) + dz;  // in [-1, 2]
+    const int ps = ((ux >> 1) + 1) * 9 + ((uy >> 1) + 1) * 3 + ((uz >> 1) + 1);
+    const int C = col[(size_t)nd.parent[A] * 27 + ps];
+    if (C >= 0 && !nd.leaf[C]) {
+      const int ox = ux & 1, oy = uy & 1, oz = uz & 1;
+      for (int K = nd.child[C]; K < nd.child[C] + nd.nchild[C]; K++)
+        if ((nd.ix[K] & 1) == ox && (nd.iy[K] & 1) == oy &&
+            (nd.iz[K] & 1) == oz) {
+          res = K;
+          break;
+        }
     }
   }
+  col[(size_t)A * 27 + slot] = res;
 }
 
 hipError_t launch_colleagues(const NodeArrays &nd, int first, int count,
                              int *col, hipStream_t s) {
   if (count == 0) return hipSuccess;
-  k_colleagues<<<(count + 127) / 128, 128, 0, s>>>(nd, first, count, col);
+  const long t = 27L * count;
+  k_colleagues<<<(t + 255) / 256, 256, 0, s>>>(nd, first, count, col);
   return hipGetLastError();
 }
 
@@ -696,24 +717,40 @@ hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
   return hipGetLastError();
 }
 
-// Sum of per-leaf pair counts (one block, deterministic).
+// Sum of per-leaf pair counts: kSumBlocks blocks sum fixed chunks into
+// partials, then one block adds the partials (integers: exact).
+constexpr int kSumBlocks = 64;
+
 __global__ void k_sum_ll(const long long *__restrict__ v, int n,
-                         long long *__restrict__ out) {
+                         long long *__restrict__ part) {
   __shared__ long long sh[256];
+  const int chunk = (n + kSumBlocks - 1) / kSumBlocks;
+  const int b = blockIdx.x * chunk, e = min(n, b + chunk);
   long long s = 0;
-  for (int i = threadIdx.x; i < n; i += 256) s += v[i];
+  for (int i = b + threadIdx.x; i < e; i += 256) s += v[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int k = 128; k > 0; k >>= 1) {
     if ((int)threadIdx.x < k) sh[threadIdx.x] += sh[threadIdx.x + k];
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = sh[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+
+__global__ void k_sum_parts(const long long *__restrict__ part,
+                            long long *__restrict__ out) {
+  if (threadIdx.x == 0) {
+    long long s = 0;
+    for (int i = 0; i < kSumBlocks; i++) s += part[i];
+    *out = s;
+  }
 }
 
 hipError_t launch_sum_ll(const long long *v, int n, long long *out,
                          hipStream_t s) {
-  k_sum_ll<<<1, 256, 0, s>>>(v, n, out);
+  // partials live right after *out (the caller's buffer holds 1 + 64 values)
+  k_sum_ll<<<kSumBlocks, 256, 0, s>>>(v, n, out + 1);
+  k_sum_parts<<<1, 64, 0, s>>>(out + 1, out);
   return hipGetLastError();
 }
 
