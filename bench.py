@@ -40,6 +40,21 @@ def m2l_flops(p):
     return 8 * (rot_macs(p - 1) + rot_macs(p)) + 4 * v
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch of the dominant kernel for this workload, from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised under profiles/
+    (a TRAFFIC line), or None when none matches."""
+    import glob
+    for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith("TRAFFIC "):
+                    rec = json.loads(line[len("TRAFFIC "):])
+                    if rec.get("workload") == workload:
+                        return rec
+    return None
+
+
 def cpu_baseline(n_sample, order, leafcap, seed):
     import oracle as O
     from aran_amd import workloads as W
@@ -218,10 +233,17 @@ def main():
     peak = MI355X_FP32_TFLOPS if mixed else MI355X_FP64_TFLOPS
     flops = st["m2l"] * m2l_flops(p)
     achieved = flops / (head["m2l_ms"] * 1e-3) / 1e12
+    workload = "plummer_2p20_p10_q64_%s" % args.precision
+    tr = measured_traffic(workload) if (n, p, q) == (1 << 20, 10, 64) else None
     roof = {"bound": "mfma",
             "kernel": "k_m2l_tile (FP32)" if mixed else "k_m2l (FP64)",
             "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
-            "frac": round(achieved / peak, 5), "traffic": None,
+            "frac": round(achieved / peak, 5),
+            "traffic": tr["bytes_per_launch"] if tr else None,
+            "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE x %.1f + "
+                            "WRITE_SIZE x %.1f)" % (tr["fetch_factor"],
+                                                   tr["write_factor"])
+            if tr else None,
             "algorithmic": "%d M2L x %d flops" % (st["m2l"], m2l_flops(p)),
             "kernel_ms": round(head["m2l_ms"], 4)}
 
@@ -246,7 +268,7 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if mixed else "f64",
         "data": "synthetic (seeded Plummer sphere, a=0.1, r<=1)",
-        "config": {"workload": "plummer_2p20_p10_q64_%s" % args.precision,
+        "config": {"workload": workload,
                    "n": n, "order": p, "leafcap": q,
                    "form": ("mixed: FP32 M2L and P2P on per-level scaled "
                             "expansions, FP64 elsewhere") if mixed else

@@ -86,6 +86,15 @@ def lib():
         L.or_traverse.argtypes = [C.c_void_p, C.c_long, _lp, _lp, C.c_long,
                                   C.POINTER(C.c_long), _lp, _lp, C.c_long,
                                   C.POINTER(C.c_long)]
+        L.or_p2l.argtypes = [i, _dp, d, _dp, _dp]
+        _cl = C.POINTER(C.c_long)
+        L.or_traverse_full.argtypes = [C.c_void_p, C.c_long, C.c_long, _lp,
+                                       _lp, C.c_long, _cl, _lp, _lp,
+                                       C.c_long, _cl, _lp, _lp, C.c_long,
+                                       _cl]
+        L.or_fmm_solve_full.restype = i
+        L.or_fmm_solve_full.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp,
+                                        _dp, i, i, C.c_long, _dp, _dp, _lp]
         L.or_fmm_solve.restype = i
         L.or_fmm_solve.argtypes = [C.c_long, _dp, _dp, _dp, _dp, _dp, _dp, i,
                                    i, _dp, _dp, _lp]
@@ -193,6 +202,13 @@ def local_grad(pl, L, center, pos):
     return g
 
 
+def p2l(pl, pos, q, center, L=None):
+    if L is None:
+        L = np.zeros(tri(pl + 1), dtype=np.complex128)
+    lib().or_p2l(pl, _c(pos), float(q), _c(center), L.view(np.float64))
+    return L
+
+
 def multipole_eval(nm, M, center, pos):
     return lib().or_multipole_evaluate(nm, _c(M.view(np.float64)),
                                        _c(center), _c(pos))
@@ -248,16 +264,45 @@ def traverse(nodes, max_far=None, max_near=None):
     return (fs[:nf.value], fd[:nf.value]), (na[:nn_.value], nb[:nn_.value])
 
 
-def fmm_solve(x, y, z, q, lb, ub, order, leafcap):
+def traverse_semifar(nodes, thr, max_far=None, max_near=None, max_semi=None):
+    """Traversal with semifar pairs (leaf, deeper node) for a threshold
+    thr >= 0: returns (far_src, far_dst), (near_a, near_b),
+    (semifar_leaf, semifar_node)."""
+    nn = len(nodes)
+    max_far = max_far or 400 * nn + 16
+    max_near = max_near or 64 * nn + 16
+    max_semi = max_semi or 64 * nn + 16
+    bufs = [np.zeros(k, dtype=np.int64) for k in
+            (max_far, max_far, max_near, max_near, max_semi, max_semi)]
+    nf, nn_, ns = C.c_long(), C.c_long(), C.c_long()
+    nodes = np.ascontiguousarray(nodes)
+    lib().or_traverse_full(nodes.ctypes.data, nn, int(thr), bufs[0], bufs[1],
+                           max_far, C.byref(nf), bufs[2], bufs[3], max_near,
+                           C.byref(nn_), bufs[4], bufs[5], max_semi,
+                           C.byref(ns))
+    if nf.value > max_far or nn_.value > max_near or ns.value > max_semi:
+        return traverse_semifar(nodes, thr, 2 * nf.value + 16,
+                                2 * nn_.value + 16, 2 * ns.value + 16)
+    return ((bufs[0][:nf.value], bufs[1][:nf.value]),
+            (bufs[2][:nn_.value], bufs[3][:nn_.value]),
+            (bufs[4][:ns.value], bufs[5][:ns.value]))
+
+
+def fmm_solve(x, y, z, q, lb, ub, order, leafcap, semifar=None):
+    """Serial FP64 FMM.  semifar: None (no semifar pairs) or a particle-count
+    threshold >= 0.  Returns pot, grad (n x 3) and the counters zero, p2p,
+    p2m, m2m, m2l, l2l, l2p (and p2l, m2p when semifar is given)."""
     n = len(x)
     pot = np.zeros(n)
     grad = np.zeros(3 * n)
-    stats = np.zeros(7, dtype=np.int64)
-    rc = lib().or_fmm_solve(n, _c(x), _c(y), _c(z), _c(q), _c(lb), _c(ub),
-                            int(order), int(leafcap), pot, grad, stats)
+    stats = np.zeros(9, dtype=np.int64)
+    thr = -1 if semifar is None else int(semifar)
+    rc = lib().or_fmm_solve_full(n, _c(x), _c(y), _c(z), _c(q), _c(lb),
+                                 _c(ub), int(order), int(leafcap), thr, pot,
+                                 grad, stats)
     if rc != 0:
         raise RuntimeError("oracle solve failed rc=%d" % rc)
-    return pot, grad.reshape(n, 3), stats
+    return pot, grad.reshape(n, 3), (stats[:7] if semifar is None else stats)
 
 
 def direct(x, y, z, q, targets):

@@ -27,7 +27,12 @@
  *     more along some axis (M2L both ways); near leaves interact by P2P; a
  *     leaf near a deeper subtree interacts with every particle of it by P2P
  *     (LibVsg's near/far traversal with no semifar function,
- *     aran/aransolver3d.c:1189-1190).
+ *     aran/aransolver3d.c:1189-1190);
+ *   - with a semifar threshold t: when the recursion pairs a leaf with a
+ *     deeper node whose box does not touch the leaf's box and which holds
+ *     more than t particles, the pair is semifar (P2L of the leaf's
+ *     particles into the node's local, M2P of the node's multipole onto
+ *     them, aran/aransolver3d.c:357-396) and the recursion stops there.
  */
 #include "aran_oracle.h"
 
@@ -607,6 +612,33 @@ void or_p2m (int nm, const double *pos, double q, const double *center,
     }
 }
 
+/* aran_development3d_p2l (aran/arandevelopment3d.c:227-277):
+ * L_l^m += q (4 pi / (2l+1)) conj (Y_l^m) / r^(l+1) for l = 0..pl, about
+ * the local's centre. */
+void or_p2l (int pl, const double *pos, double q, const double *center,
+             double *Ld)
+{
+  cplx *L = (cplx *) Ld;
+  cplx Y[TRI (OR_MAXDEG + 1, 0)];
+  double v[3], r, cost, sint, cosp, sinp, invr, fact;
+  int l, m;
+  for (l = 0; l < 3; l++)
+    v[l] = pos[l] - center[l];
+  or_to_spherical (v, &r, &cost, &sint, &cosp, &sinp);
+  or_harmonics (pl, cost, sint, cosp, sinp, (double *) Y);
+  invr = 1. / r;
+  fact = q * invr;
+  for (l = 0; l <= pl; l++)
+    {
+      for (m = 0; m <= l; m++)
+        {
+          cplx term = fact * (4. * M_PI / (l + l + 1.)) * Y[TRI (l, m)];
+          L[TRI (l, m)] += conj (term);
+        }
+      fact *= invr;
+    }
+}
+
 /* local part of aran_spherical_seriesd_evaluate_internal
  * (aran/aransphericalseriesd.c:313-346) */
 double or_local_evaluate (int pl, const double *Ld, const double *center,
@@ -899,8 +931,27 @@ typedef struct
   const OrNode *nodes;
   void (*far) (void *, long, long);
   void (*near) (void *, long, long);
+  /* semifar (leaf, deeper node), or NULL; used when the deeper node does
+   * not touch the leaf and holds more than thr particles */
+  void (*semifar) (void *, long, long);
+  long thr;
   void *ud;
 } Trav;
+
+/* Boxes of a and of b (b at least as deep as a) touch or overlap. */
+static int touching (const OrNode *a, const OrNode *b)
+{
+  const int s = b->depth - a->depth;
+  const long w = 1L << s;
+  const long lo[3] = { (long) a->ix << s, (long) a->iy << s,
+                       (long) a->iz << s };
+  const long c[3] = { b->ix, b->iy, b->iz };
+  int k;
+  for (k = 0; k < 3; k++)
+    if (c[k] < lo[k] - 1 || c[k] > lo[k] + w)
+      return 0;
+  return 1;
+}
 
 static int adjacent (const OrNode *a, const OrNode *b)
 {
@@ -916,6 +967,17 @@ static void trav_pair (const Trav *t, long a, long b)
     {
       t->far (t->ud, a, b);
       return;
+    }
+  if (t->semifar && A->depth != B->depth)
+    {
+      /* the shallower side is a leaf (only non-leaves are descended) */
+      const long s = A->depth < B->depth ? a : b, d = s == a ? b : a;
+      const OrNode *S = &t->nodes[s], *D = &t->nodes[d];
+      if (!touching (S, D) && (long) (D->pend - D->pbegin) > t->thr)
+        {
+          t->semifar (t->ud, s, d);
+          return;
+        }
     }
   if (A->isleaf && B->isleaf)
     {
@@ -963,6 +1025,7 @@ typedef struct
 {
   long *fs, *fd, maxf, nf;
   long *na, *nb, maxn, nn;
+  long *sl, *sd, maxs, ns;
 } PairOut;
 
 static void po_far (void *ud, long a, long b)
@@ -989,16 +1052,45 @@ static void po_near (void *ud, long a, long b)
   o->nn++;
 }
 
-void or_traverse (const OrNode *nodes, long nnodes, long *far_src,
-                  long *far_dst, long max_far, long *nfar, long *near_a,
-                  long *near_b, long max_near, long *nnear)
+static void po_semifar (void *ud, long leaf, long node)
 {
-  PairOut o = { far_src, far_dst, max_far, 0, near_a, near_b, max_near, 0 };
-  Trav t = { nodes, po_far, po_near, &o };
+  PairOut *o = ud;
+  if (o->ns < o->maxs)
+    {
+      o->sl[o->ns] = leaf;
+      o->sd[o->ns] = node;
+    }
+  o->ns++;
+}
+
+void or_traverse_full (const OrNode *nodes, long nnodes, long semifar_thr,
+                       long *far_src, long *far_dst, long max_far, long *nfar,
+                       long *near_a, long *near_b, long max_near, long *nnear,
+                       long *semi_leaf, long *semi_node, long max_semi,
+                       long *nsemi)
+{
+  PairOut o = { far_src, far_dst, max_far, 0, near_a, near_b, max_near, 0,
+                semi_leaf, semi_node, max_semi, 0 };
+  Trav t = { nodes, po_far, po_near, NULL, 0, &o };
+  if (semifar_thr >= 0)
+    {
+      t.semifar = po_semifar;
+      t.thr = semifar_thr;
+    }
   if (nnodes > 0)
     trav_self (&t, 0);
   *nfar = o.nf;
   *nnear = o.nn;
+  *nsemi = o.ns;
+}
+
+void or_traverse (const OrNode *nodes, long nnodes, long *far_src,
+                  long *far_dst, long max_far, long *nfar, long *near_a,
+                  long *near_b, long max_near, long *nnear)
+{
+  long ns;
+  or_traverse_full (nodes, nnodes, -1, far_src, far_dst, max_far, nfar,
+                    near_a, near_b, max_near, nnear, NULL, NULL, 0, &ns);
 }
 
 /* ----------------------------------------------------------------- solve */
@@ -1071,6 +1163,31 @@ static void solve_far (void *ud, long a, long b)
   s->stats[4] += 2;
 }
 
+/* semifar_func (aran/aransolver3d.c:357-396) with the newtonfield3
+ * callbacks (tests/newtonfield3.c:139-153): every particle of the leaf adds
+ * to the deeper node's local (P2L) and takes the potential and gradient of
+ * its multipole (M2P). */
+static void solve_semifar (void *ud, long leaf, long node)
+{
+  Solve *s = ud;
+  const OrNode *A = &s->nodes[leaf], *D = &s->nodes[node];
+  long i;
+  for (i = A->pbegin; i < A->pend; i++)
+    {
+      double pos[3] = { s->x[i], s->y[i], s->z[i] }, g[3];
+      or_p2l (s->pl, pos, s->q[i], D->center, s->L + 2 * s->lc * node);
+      s->pot[i] += or_multipole_evaluate (s->nm, s->M + 2 * s->mc * node,
+                                          D->center, pos);
+      or_multipole_gradient (s->nm, s->M + 2 * s->mc * node, D->center, pos,
+                             g);
+      s->grad[3 * i + 0] += g[0];
+      s->grad[3 * i + 1] += g[1];
+      s->grad[3 * i + 2] += g[2];
+    }
+  s->stats[7] += A->pend - A->pbegin;
+  s->stats[8] += A->pend - A->pbegin;
+}
+
 /* up_func, post-order (aran/aransolver3d.c:412-450) */
 static void solve_up (Solve *s, long a)
 {
@@ -1133,6 +1250,19 @@ int or_fmm_solve (long n, const double *x, const double *y, const double *z,
                   int order, int leafcap, double *pot, double *grad,
                   long *stats)
 {
+  long st[9];
+  int rc = or_fmm_solve_full (n, x, y, z, q, lb, ub, order, leafcap, -1, pot,
+                              grad, st);
+  memcpy (stats, st, 7 * sizeof (long));
+  return rc;
+}
+
+int or_fmm_solve_full (long n, const double *x, const double *y,
+                       const double *z, const double *q, const double *lb,
+                       const double *ub, int order, int leafcap,
+                       long semifar_thr, double *pot, double *grad,
+                       long *stats)
+{
   long max_nodes = 64 + 8 * (n / (leafcap + 1) + 1) * (OR_MAXDEPTH + 1);
   OrNode *nodes;
   long *perm, nn, i;
@@ -1179,13 +1309,15 @@ int or_fmm_solve (long n, const double *x, const double *y, const double *z,
   s.pot = ps;
   s.grad = gs;
   s.stats = stats;
-  for (i = 0; i < 7; i++)
+  for (i = 0; i < 9; i++)
     stats[i] = 0;
   stats[0] = nn; /* clear_func on every node */
   solve_up (&s, 0);
   t.nodes = nodes;
   t.far = solve_far;
   t.near = solve_near;
+  t.semifar = semifar_thr >= 0 ? solve_semifar : NULL;
+  t.thr = semifar_thr;
   t.ud = &s;
   trav_self (&t, 0);
   solve_down (&s, 0);

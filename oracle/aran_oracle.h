@@ -62,6 +62,8 @@ void or_local_gradient (int pl, const double *L, const double *center,
                         const double *pos, double *grad);
 double or_multipole_evaluate (int nm, const double *M, const double *center,
                               const double *pos);
+void or_p2l (int pl, const double *pos, double q, const double *center,
+             double *L);
 void or_multipole_gradient (int nm, const double *M, const double *center,
                             const double *pos, double *grad);
 
@@ -89,11 +91,26 @@ void or_traverse (const OrNode *nodes, long nnodes,
                   long *far_src, long *far_dst, long max_far, long *nfar,
                   long *near_a, long *near_b, long max_near, long *nnear);
 
+/* the same with semifar pairs (leaf, deeper node) when semifar_thr >= 0 */
+void or_traverse_full (const OrNode *nodes, long nnodes, long semifar_thr,
+                       long *far_src, long *far_dst, long max_far, long *nfar,
+                       long *near_a, long *near_b, long max_near, long *nnear,
+                       long *semi_leaf, long *semi_node, long max_semi,
+                       long *nsemi);
+
 /* whole solve.  stats: zero, p2p, p2m, m2m, m2l, l2l, l2p (7 longs). */
 int or_fmm_solve (long n, const double *x, const double *y, const double *z,
                   const double *q, const double *lb, const double *ub,
                   int order, int leafcap, double *pot, double *grad,
                   long *stats);
+
+/* whole solve with semifar pairs when semifar_thr >= 0 (none when < 0).
+ * stats: the 7 above, then p2l, m2p (9 longs). */
+int or_fmm_solve_full (long n, const double *x, const double *y,
+                       const double *z, const double *q, const double *lb,
+                       const double *ub, int order, int leafcap,
+                       long semifar_thr, double *pot, double *grad,
+                       long *stats);
 
 /* FP64 direct sum with compensated accumulation, on chosen targets */
 void or_direct (long n, const double *x, const double *y, const double *z,
