@@ -139,6 +139,7 @@ struct _AranFmm3d {
   aran::TreeInfo ti;
   bool timings = false;
   int precision = ARAN_FMM3D_PLAIN;
+  long semifar = -1;
   long n_last = 0;
 };
 
@@ -185,6 +186,7 @@ extern "C" int aran_fmm3d_solve(AranFmm3d *ctx, long n, const double *x,
   pb.grad = grad;
   pb.timings = ctx->timings;
   pb.precision = ctx->precision;
+  pb.semifar = ctx->semifar;
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
   return code_of_err(e);
@@ -221,6 +223,7 @@ extern "C" int aran_fmm3d_solve_batch(AranFmm3d *ctx, int nsys,
   pb.grad = grad;
   pb.timings = ctx->timings;
   pb.precision = ctx->precision;
+  pb.semifar = ctx->semifar;
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
   return code_of_err(e);
@@ -229,12 +232,27 @@ extern "C" int aran_fmm3d_solve_batch(AranFmm3d *ctx, int nsys,
 extern "C" int aran_fmm3d_get_stats(const AranFmm3d *ctx, long *o) {
   if (!ctx || !o) return ARAN_FMM3D_EINVAL;
   const aran::Stats &s = ctx->st;
-  long v[12] = {s.zero,          s.p2p,          s.p2m,         s.m2m,
-                s.m2l,           s.l2l,          s.l2p,         (long)s.depth,
-                s.nnodes,        s.nleaves,      ctx->ti.nfar,  ctx->ti.nnear};
+  long v[15] = {s.zero,   s.p2p,     s.p2m,        s.m2m,         s.m2l,
+                s.l2l,    s.l2p,     (long)s.depth, s.nnodes,     s.nleaves,
+                ctx->ti.nfar, ctx->ti.nnear, s.p2l, s.m2p,        s.semifar};
   memcpy(o, v, sizeof v);
   return 0;
 }
+
+extern "C" int aran_fmm3d_set_semifar(AranFmm3d *ctx, long threshold) {
+  if (!ctx) return ARAN_FMM3D_EINVAL;
+  ctx->semifar = threshold < 0 ? -1 : threshold;
+  return 0;
+}
+
+extern "C" long aran_fmm3d_semifar_threshold_auto(int order) {
+  if (order < 1 || order > aran::kMaxOrder) return -1;
+  // (P2L + M2P per particle) / P2P per pair, in algorithmic flops:
+  // P2L 14 and M2P 30 per local coefficient, P2P 20 per pair
+  const long lc = (long)(order + 1) * (order + 2) / 2;
+  return (44 * lc + 19) / 20;
+}
+
 
 extern "C" int aran_fmm3d_set_precision(AranFmm3d *ctx, int precision) {
   if (!ctx || (precision != ARAN_FMM3D_PLAIN && precision != ARAN_FMM3D_MIXED)) {
@@ -332,6 +350,28 @@ extern "C" int aran_fmm3d_export_lists(const AranFmm3d *ctx, int *far_ofs,
   if (e == hipSuccess && near_end) e = d2h(near_end, ws.u_end, nu, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) aran::set_error(hipGetErrorString(e));
+  return code_of_err(e);
+}
+
+extern "C" int aran_fmm3d_export_semifar(const AranFmm3d *ctx, int *w_ofs,
+                                         int *w_node, int *x_ofs,
+                                         int *x_leaf) {
+  if (!ctx || !w_ofs || !w_node || !x_ofs || !x_leaf) {
+    aran::set_error("aran_fmm3d_export_semifar: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  const aran::Workspace &ws = ctx->ws;
+  if (ws.semifar_nodes < 0 || ws.semifar_nodes != ctx->ti.nnodes) {
+    aran::set_error("aran_fmm3d_export_semifar: the last solve had no "
+                    "semifar lists");
+    return ARAN_FMM3D_EINVAL;
+  }
+  const long nl = ctx->ti.nleaves, nn = ctx->ti.nnodes, ns = ctx->st.semifar;
+  hipError_t e = d2h(w_ofs, ws.w_ofs, nl + 1, ws.stream);
+  if (e == hipSuccess && ns > 0) e = d2h(w_node, ws.w_node, ns, ws.stream);
+  if (e == hipSuccess) e = d2h(x_ofs, ws.x_ofs, nn + 1, ws.stream);
+  if (e == hipSuccess && ns > 0) e = d2h(x_leaf, ws.x_leaf, ns, ws.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ws.stream);
   return code_of_err(e);
 }
 
@@ -468,10 +508,24 @@ int translate_one(int kind, int p, const double *src, const OneDir &d,
   return 0;
 }
 
-// P2M or L2P of one particle through the batched leaf kernels (one leaf,
-// one particle).  For P2M, M (host) receives the added multipole; for L2P,
-// pot / grad receive the values.
+// One particle through the batched leaf kernels (one node that is both the
+// expansion's node and the particle's leaf).  kOpP2M / kOpP2L: M / L (host)
+// receive the added expansion; kOpL2P / kOpM2P: pot / grad receive the
+// values of the expansion (L or M) at the particle.
+enum { kOpP2M = 0, kOpL2P = 1, kOpP2L = 2, kOpM2P = 3 };
+
+int leaf_op_one(int op, int p, const VsgVector3d &center,
+                const VsgVector3d &pos, double charge, double *M,
+                const double *L, double *pot, double *grad);
+
 int leaf_op_one(bool p2m, int p, const VsgVector3d &center,
+                const VsgVector3d &pos, double charge, double *M,
+                const double *L, double *pot, double *grad) {
+  return leaf_op_one(p2m ? kOpP2M : kOpL2P, p, center, pos, charge, M, L, pot,
+                     grad);
+}
+
+int leaf_op_one(int op, int p, const VsgVector3d &center,
                 const VsgVector3d &pos, double charge, double *M,
                 const double *L, double *pot, double *grad) {
   OpContext &oc = opctx();
@@ -486,6 +540,7 @@ int leaf_op_one(bool p2m, int p, const VsgVector3d &center,
   struct Host {
     int ints[10];
     int leaf0;
+    int lofs[2];  // one semifar list entry: node 0
     uint64_t key;
     double c[3];
     double part[4];
@@ -498,6 +553,8 @@ int leaf_op_one(bool p2m, int p, const VsgVector3d &center,
   h.ints[7] = 1;  // leaf
   h.ints[8] = -1;
   h.leaf0 = 0;
+  h.lofs[0] = 0;
+  h.lofs[1] = 1;
   h.c[0] = center.x;
   h.c[1] = center.y;
   h.c[2] = center.z;
@@ -532,24 +589,37 @@ int leaf_op_one(bool p2m, int p, const VsgVector3d &center,
   nd.cy = &dh->c[1];
   nd.cz = &dh->c[2];
   aran::OpTables tb = ws.tab.view();
-  if (p2m) {
-    if (e == hipSuccess)
+  if (op == kOpP2M || op == kOpP2L) {
+    const int nc = op == kOpP2M ? mc : lc;
+    if (e == hipSuccess && op == kOpP2M)
       e = aran::launch_p2m(tb, nd, &dh->leaf0, 1, &dh->part[0], &dh->part[1],
                            &dh->part[2], &dh->part[3], dexp, s);
-    std::vector<double> out(2 * mc);
+    if (e == hipSuccess && op == kOpP2L)
+      e = hipMemsetAsync(dexp, 0, sizeof(double2) * lc, s);
+    if (e == hipSuccess && op == kOpP2L)
+      e = aran::launch_p2l(tb, nd, 1, &dh->lofs[0], &dh->leaf0, &dh->part[0],
+                           &dh->part[1], &dh->part[2], &dh->part[3], dexp, s);
+    std::vector<double> out(2 * nc);
     if (e == hipSuccess)
       e = hipMemcpyAsync(out.data(), dexp, sizeof(double) * out.size(),
                          hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    double *dst = op == kOpP2M ? M : const_cast<double *>(L);
     if (e == hipSuccess)
-      for (size_t i = 0; i < out.size(); i++) M[i] += out[i];
+      for (size_t i = 0; i < out.size(); i++) dst[i] += out[i];
   } else {
+    const double *src = op == kOpL2P ? L : M;
+    const int nc = op == kOpL2P ? lc : mc;
     if (e == hipSuccess)
-      e = hipMemcpyAsync(dexp, L, sizeof(double) * 2 * lc,
+      e = hipMemcpyAsync(dexp, src, sizeof(double) * 2 * nc,
                          hipMemcpyHostToDevice, s);
-    if (e == hipSuccess)
+    if (e == hipSuccess && op == kOpL2P)
       e = aran::launch_l2p(tb, nd, &dh->leaf0, 1, &dh->part[0], &dh->part[1],
                            &dh->part[2], dexp, &dh->res[0], &dh->res[1], s);
+    if (e == hipSuccess && op == kOpM2P)
+      e = aran::launch_m2p(tb, nd, &dh->leaf0, 1, &dh->lofs[0], &dh->leaf0,
+                           &dh->part[0], &dh->part[1], &dh->part[2], dexp,
+                           &dh->res[0], &dh->res[1], s);
     if (e == hipSuccess)
       e = hipMemcpyAsync(h.res, dh->res, sizeof h.res, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -725,6 +795,59 @@ extern "C" void aran_development3d_local_gradient_evaluate(
     grad->y = g[1];
     grad->z = g[2];
   }
+}
+
+extern "C" void aran_development3d_p2l(const VsgVector3d *position,
+                                       const gdouble charge,
+                                       const VsgPRTree3dNodeInfo *dst_node,
+                                       AranDevelopment3d *dst) {
+  if (!position || !dst_node || !dev_ok(dst, "aran_development3d_p2l")) return;
+  if (leaf_op_one(kOpP2L, dev_local_degree(dst), dst_node->center, *position,
+                  charge, nullptr, dst->local, nullptr, nullptr) != 0)
+    critical("aran_development3d_p2l: %s", aran::last_error());
+}
+
+extern "C" gcomplex128 aran_development3d_multipole_evaluate(
+    const VsgPRTree3dNodeInfo *devel_node, AranDevelopment3d *devel,
+    const VsgVector3d *pos) {
+  double pot = 0.;
+  if (devel_node && pos &&
+      dev_ok(devel, "aran_development3d_multipole_evaluate"))
+    if (leaf_op_one(kOpM2P, devel->negdeg, devel_node->center, *pos, 0.,
+                    devel->multipole, nullptr, &pot, nullptr) != 0)
+      critical("aran_development3d_multipole_evaluate: %s",
+               aran::last_error());
+  return pot;
+}
+
+extern "C" void aran_development3d_multipole_gradient_evaluate(
+    const VsgPRTree3dNodeInfo *devel_node, AranDevelopment3d *devel,
+    const VsgVector3d *pos, VsgVector3d *grad) {
+  double g[3] = {0., 0., 0.};
+  if (devel_node && pos && grad &&
+      dev_ok(devel, "aran_development3d_multipole_gradient_evaluate"))
+    if (leaf_op_one(kOpM2P, devel->negdeg, devel_node->center, *pos, 0.,
+                    devel->multipole, nullptr, nullptr, g) != 0)
+      critical("aran_development3d_multipole_gradient_evaluate: %s",
+               aran::last_error());
+  if (grad) {
+    grad->x = g[0];
+    grad->y = g[1];
+    grad->z = g[2];
+  }
+}
+
+extern "C" gcomplex128 aran_development3d_m2p(const VsgPRTree3dNodeInfo *n,
+                                              AranDevelopment3d *d,
+                                              const VsgVector3d *pos) {
+  return aran_development3d_multipole_evaluate(n, d, pos);
+}
+
+extern "C" void aran_development3d_m2pv(const VsgPRTree3dNodeInfo *n,
+                                        AranDevelopment3d *d,
+                                        const VsgVector3d *pos,
+                                        VsgVector3d *grad) {
+  aran_development3d_multipole_gradient_evaluate(n, d, pos, grad);
 }
 
 extern "C" gcomplex128 aran_development3d_l2p(const VsgPRTree3dNodeInfo *n,
@@ -1078,6 +1201,34 @@ extern "C" void aran_newton3d_l2p_potfield(const VsgPRTree3dNodeInfo *src_node,
   aran_newton3d_l2p_field(src_node, src, particle);
 }
 
+extern "C" void aran_newton3d_p2l(VsgPoint3 particle,
+                                  const VsgPRTree3dNodeInfo *dst_node,
+                                  gpointer dst) {
+  aran_development3d_p2l(vec_of(particle), charge_of(particle), dst_node,
+                         static_cast<AranDevelopment3d *>(dst));
+}
+
+extern "C" void aran_newton3d_m2p_pot(const VsgPRTree3dNodeInfo *src_node,
+                                      gpointer src, VsgPoint3 particle) {
+  gcomplex128 v = aran_development3d_multipole_evaluate(
+      src_node, static_cast<AranDevelopment3d *>(src), vec_of(particle));
+  add_pot(particle, __real__ v);
+}
+
+extern "C" void aran_newton3d_m2p_field(const VsgPRTree3dNodeInfo *src_node,
+                                        gpointer src, VsgPoint3 particle) {
+  VsgVector3d g;
+  aran_development3d_multipole_gradient_evaluate(
+      src_node, static_cast<AranDevelopment3d *>(src), vec_of(particle), &g);
+  add_field(particle, g.x, g.y, g.z);
+}
+
+extern "C" void aran_newton3d_m2p_potfield(const VsgPRTree3dNodeInfo *src_node,
+                                           gpointer src, VsgPoint3 particle) {
+  aran_newton3d_m2p_pot(src_node, src, particle);
+  aran_newton3d_m2p_field(src_node, src, particle);
+}
+
 // ---- the solve
 
 template <class A, class B> static bool same_fn(A a, B b) {
@@ -1107,16 +1258,36 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
     solver->status = ARAN_FMM3D_EINVAL;
     return;
   }
-  if (solver->p2l || solver->m2p) {
-    critical("aran_solver3d_solve: semifar P2L/M2P is not supported");
-    solver->status = ARAN_FMM3D_EINVAL;
-    return;
-  }
   if (!dev_ok(solver->devel, "aran_solver3d_solve")) {
     solver->status = ARAN_FMM3D_EINVAL;
     return;
   }
   const int order = solver->devel->negdeg;
+  // semifar pairs when both P2L and M2P are set (aran/aransolver3d.c:
+  // 1152-1161): the library's P2L with the M2P of the same outputs as P2P
+  long semifar = -1;
+  if (solver->p2l && solver->m2p) {
+    int kind = -1;
+    if (solver->m2p == aran_newton3d_m2p_pot) kind = 1;
+    if (solver->m2p == aran_newton3d_m2p_field) kind = 2;
+    if (solver->m2p == aran_newton3d_m2p_potfield) kind = 3;
+    if (solver->p2l != aran_newton3d_p2l || kind != pot_p2p) {
+      critical("aran_solver3d_solve: unsupported semifar callbacks; the GPU "
+               "engine runs aran_newton3d_p2l with the aran_newton3d_m2p_* "
+               "callback of the same outputs as the P2P callback");
+      solver->status = ARAN_FMM3D_EINVAL;
+      return;
+    }
+    if (solver->semifar_threshold == 0) {
+      // automatic threshold (the reference derives it from profiled
+      // operator times, aran/aransolver3d.c:1090-1127; here from the
+      // operators' algorithmic costs)
+      solver->semifar_threshold =
+          (guint)aran_fmm3d_semifar_threshold_auto(order);
+    }
+    if (solver->semifar_threshold != 0xffffffffu)
+      semifar = (long)solver->semifar_threshold;
+  }
   const bool want_pot = (pot_p2p & 1) != 0, want_field = (pot_p2p & 2) != 0;
   const long n = (long)(solver->soa ? solver->soa_x.size() : solver->points.size());
   if (!solver->ctx) {
@@ -1171,6 +1342,7 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
                         solver->prtree->lb.z};
   const double ub[3] = {solver->prtree->ub.x, solver->prtree->ub.y,
                         solver->prtree->ub.z};
+  aran_fmm3d_set_semifar(solver->ctx, semifar);
   int rc = aran_fmm3d_solve(solver->ctx, n, solver->dx.as<double>(),
                             solver->dy.as<double>(), solver->dz.as<double>(),
                             solver->dq.as<double>(), lb, ub, order,
@@ -1208,8 +1380,10 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
       }
     }
   }
-  long st[12];
+  long st[15];
   aran_fmm3d_get_stats(solver->ctx, st);
   for (int k = 0; k < 7; k++) solver->stats[k] += st[k];
+  solver->stats[7] += st[12];  // p2l
+  solver->stats[8] += st[13];  // m2p
   solver->depth = (guint)st[7];
 }

@@ -53,11 +53,14 @@ struct OpTables {
   const double *ynorm;   // [tri(p+2)]  N_l^m
   const double *ogain;   // [tri(p+1)]  N_l^m (l+m)!
   const double *pgain;   // [tri(p)]    4 pi / (2l+1) N_l^m (l+m)!
+  const double *igain;   // [tri(p+1)]  N_l^m / (l-m)!
+  const double *xgain;   // [tri(p+1)]  4 pi / (2l+1) N_l^m / (l-m)!
   int p, rb;
 };
 
 struct Stats {
   long zero = 0, p2p = 0, p2m = 0, m2m = 0, m2l = 0, l2l = 0, l2p = 0;
+  long p2l = 0, m2p = 0, semifar = 0;  // semifar: (leaf, node) pairs
   int depth = 0;
   long nnodes = 0, nleaves = 0;
   int m2l_split = 0, m2l_tiles = 0;  // mixed form: tiled far-field launch
@@ -79,7 +82,8 @@ struct DevBuf {
 struct Tables {
   int order = -1;
   int ntheta = 0;
-  DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, ogain, pgain, dscr, thetas;
+  DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, ogain, pgain, igain, xgain,
+      dscr, thetas;
   // tiled far field (mixed form): FP32 copies and octant offset lists
   DevBuf rot32, ephi32, vfac32, oct_codes, code_to_k;
   std::vector<DirEntry> dir_host;
@@ -94,6 +98,10 @@ struct Workspace {
   DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
       colleagues;
+  // semifar lists: per leaf the nodes of its M2P (w_*), per node the leaves
+  // of its P2L (x_*); s_part: per-leaf P2L/M2P particle counts
+  DevBuf w_cnt, w_ofs, w_node, x_cnt, x_ofs, x_leaf, s_part;
+  long semifar_nodes = -1;  // nodes covered by the semifar lists, -1: none
   DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
       tile_meta, mult32, loc32;
@@ -127,6 +135,9 @@ struct Problem {
   double *grad = nullptr;  // device, user order [n][3], may be null
   bool timings = false;
   int precision = 0;  // 0: plain FP64, 1: mixed (FP32 far field and P2P)
+  // semifar pairs (P2L / M2P) for nodes holding more than this many
+  // particles; < 0: none (every leaf / deeper node pair is P2P)
+  long semifar = -1;
 };
 
 struct TreeInfo {
@@ -198,6 +209,15 @@ hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
                         int *ubeg, int *uend, long long *pairs, hipStream_t s);
 hipError_t launch_sum_ll(const long long *v, int n, long long *out,
                          hipStream_t s);
+hipError_t launch_ulist_semifar(const NodeArrays &nd, int nleaves,
+                                const int *leaves, const int *col, long thr,
+                                int pass, int *cnt, const int *ofs, int *ubeg,
+                                int *uend, long long *pairs, int *wcnt,
+                                const int *wofs, int *wnode, long long *spart,
+                                hipStream_t s);
+hipError_t launch_xlist(const NodeArrays &nd, int nnodes, const int *col,
+                        long thr, int pass, int *cnt, const int *ofs,
+                        int *xleaf, hipStream_t s);
 
 // ---- operators.hip
 hipError_t launch_wigner_tables(const double *thetas, int ntheta, int L,
@@ -221,6 +241,20 @@ hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
                       double *pot, double *grad, hipStream_t s);
+
+// M2P: for every leaf, the multipoles of its semifar nodes evaluated at its
+// sorted particles, added into pot / grad (potential and gradient).
+hipError_t launch_m2p(const OpTables &tb, const NodeArrays &nd,
+                      const int *leaves, int nleaves, const int *wofs,
+                      const int *wnode, const double *xs, const double *ys,
+                      const double *zs, const double2 *M, double *pot,
+                      double *grad, hipStream_t s);
+// P2L: for every node, the particles of its semifar leaves added into its
+// local.
+hipError_t launch_p2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
+                      const int *xofs, const int *xleaf, const double *xs,
+                      const double *ys, const double *zs, const double *qs,
+                      double2 *L, hipStream_t s);
 
 hipError_t launch_translate_one(int kind, const OpTables &tb, double r,
                                 const double2 *src, double2 *dst,

@@ -90,6 +90,8 @@ OpTables Tables::view() const {
   t.ynorm = ynorm.as<double>();
   t.ogain = ogain.as<double>();
   t.pgain = pgain.as<double>();
+  t.igain = igain.as<double>();
+  t.xgain = xgain.as<double>();
   t.p = order;
   t.rb = rot_block(order);
   return t;
@@ -123,10 +125,11 @@ Workspace::~Workspace() {
                    &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz,
                    &split_bounds, &split_cnt, &split_ofs, &flags, &leaf_list,
                    &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
-                   &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &mult, &loc,
+                   &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &w_cnt, &w_ofs,
+                   &w_node, &x_cnt, &x_ofs, &x_leaf, &s_part, &mult, &loc,
                    &small, &pleaf,
                    &tab.rot, &tab.eiphi, &tab.dir, &tab.tm2l, &tab.tm2m,
-                   &tab.tl2l, &tab.ynorm, &tab.ogain, &tab.pgain, &tab.dscr, &tab.thetas, &tab.rot32,
+                   &tab.tl2l, &tab.ynorm, &tab.ogain, &tab.pgain, &tab.igain, &tab.xgain, &tab.dscr, &tab.thetas, &tab.rot32,
                    &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
                    &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &n_sys, &sys_box, &sys_off, &sys_mm,
@@ -313,6 +316,16 @@ hipError_t ensure_tables(Workspace &ws, int p) {
       og[tri(l) + m] = yn[tri(l) + m] * f;
       if (l < p) pg[tri(l) + m] = 4. * M_PI / (l + l + 1.) * og[tri(l) + m];
     }
+  // irregular solid harmonics (semifar M2P / P2L): r^-(l+1) Y_l^m =
+  // N_l^m / (l-m)! I_l^m with I_l^m = (l-m)! P_l^m e^{i m phi} / r^(l+1)
+  std::vector<double> ig(tri(p + 1)), xg(tri(p + 1));
+  for (int l = 0; l <= p; l++)
+    for (int m = 0; m <= l; m++) {
+      double f = 1.;
+      for (int i = 2; i <= l - m; i++) f *= i;
+      ig[tri(l) + m] = yn[tri(l) + m] / f;
+      xg[tri(l) + m] = 4. * M_PI / (l + l + 1.) * ig[tri(l) + m];
+    }
 
   int rb = rot_block(p);
   int w = 2 * p + 1;
@@ -327,6 +340,8 @@ hipError_t ensure_tables(Workspace &ws, int p) {
   CK(T.ynorm.reserve(sizeof(double) * yn.size()));
   CK(T.ogain.reserve(sizeof(double) * og.size()));
   CK(T.pgain.reserve(sizeof(double) * pg.size()));
+  CK(T.igain.reserve(sizeof(double) * ig.size()));
+  CK(T.xgain.reserve(sizeof(double) * xg.size()));
   CK(hipMemcpyAsync(T.thetas.ptr, thetas.data(), sizeof(double) * nth,
                     hipMemcpyHostToDevice, s));
   CK(hipMemsetAsync(T.dscr.ptr, 0, sizeof(double) * (size_t)nth * (p + 1) * w * w,
@@ -348,6 +363,10 @@ hipError_t ensure_tables(Workspace &ws, int p) {
   CK(hipMemcpyAsync(T.ogain.ptr, og.data(), sizeof(double) * og.size(),
                     hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(T.pgain.ptr, pg.data(), sizeof(double) * pg.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.igain.ptr, ig.data(), sizeof(double) * ig.size(),
+                    hipMemcpyHostToDevice, s));
+  CK(hipMemcpyAsync(T.xgain.ptr, xg.data(), sizeof(double) * xg.size(),
                     hipMemcpyHostToDevice, s));
   CK(launch_wigner_tables(T.thetas.as<double>(), nth, p, T.dscr.as<double>(),
                           T.rot.as<double>(), s));
@@ -601,25 +620,71 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   int nfar = 0;
   ws.far_csr_nodes = -1;
   if (pb.precision == 0) CK(build_far_lists(ws, nn, &nfar));
-  // ---- near list
+  // ---- near list (and, with a semifar threshold, the semifar lists)
+  const bool semifar = pb.semifar >= 0;
   CK(ws.u_cnt.reserve(sizeof(int) * nleaves));
   CK(ws.u_ofs.reserve(sizeof(int) * (nleaves + 1)));
   CK(ws.u_pairs.reserve(sizeof(long long) * (nleaves + 1)));
-  CK(launch_ulist(nd, nleaves, leaves, col, 0, ws.u_cnt.as<int>(), nullptr,
-                  nullptr, nullptr, nullptr, s));
+  int nnear = 0, nsemi = 0;
+  ws.semifar_nodes = -1;
+  if (!semifar) {
+    CK(launch_ulist(nd, nleaves, leaves, col, 0, ws.u_cnt.as<int>(), nullptr,
+                    nullptr, nullptr, nullptr, s));
+  } else {
+    CK(ws.w_cnt.reserve(sizeof(int) * nleaves));
+    CK(ws.w_ofs.reserve(sizeof(int) * (nleaves + 1)));
+    CK(ws.s_part.reserve(sizeof(long long) * (nleaves + 1)));
+    CK(ws.x_cnt.reserve(sizeof(int) * nn));
+    CK(ws.x_ofs.reserve(sizeof(int) * (nn + 1)));
+    CK(launch_ulist_semifar(nd, nleaves, leaves, col, pb.semifar, 0,
+                            ws.u_cnt.as<int>(), nullptr, nullptr, nullptr,
+                            nullptr, ws.w_cnt.as<int>(), nullptr, nullptr,
+                            nullptr, s));
+    CK(launch_xlist(nd, nn, col, pb.semifar, 0, ws.x_cnt.as<int>(), nullptr,
+                    nullptr, s));
+    CK(scan_exclusive(ws.w_cnt.as<int>(), ws.w_ofs.as<int>(), nleaves,
+                      ws.w_ofs.as<int>() + nleaves, ws.scan_tmp, s));
+    CK(scan_exclusive(ws.x_cnt.as<int>(), ws.x_ofs.as<int>(), nn,
+                      ws.x_ofs.as<int>() + nn, ws.scan_tmp, s));
+  }
   CK(scan_exclusive(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
                     ws.u_ofs.as<int>() + nleaves, ws.scan_tmp, s));
-  int nnear = 0;
   CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
   CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
   CK(ws.u_end.reserve(sizeof(int) * std::max(1, nnear)));
-  CK(launch_ulist(nd, nleaves, leaves, col, 1, ws.u_cnt.as<int>(),
-                  ws.u_ofs.as<int>(), ws.u_beg.as<int>(), ws.u_end.as<int>(),
-                  ws.u_pairs.as<long long>(), s));
+  if (!semifar) {
+    CK(launch_ulist(nd, nleaves, leaves, col, 1, ws.u_cnt.as<int>(),
+                    ws.u_ofs.as<int>(), ws.u_beg.as<int>(), ws.u_end.as<int>(),
+                    ws.u_pairs.as<long long>(), s));
+  } else {
+    int nx = 0;
+    CK(read_int(ws, ws.w_ofs.as<int>() + nleaves, &nsemi));
+    CK(read_int(ws, ws.x_ofs.as<int>() + nn, &nx));
+    if (nx != nsemi) {
+      set_error("semifar lists disagree (internal error)");
+      return hipErrorUnknown;
+    }
+    CK(ws.w_node.reserve(sizeof(int) * std::max(1, nsemi)));
+    CK(ws.x_leaf.reserve(sizeof(int) * std::max(1, nsemi)));
+    CK(launch_ulist_semifar(nd, nleaves, leaves, col, pb.semifar, 1,
+                            ws.u_cnt.as<int>(), ws.u_ofs.as<int>(),
+                            ws.u_beg.as<int>(), ws.u_end.as<int>(),
+                            ws.u_pairs.as<long long>(), ws.w_cnt.as<int>(),
+                            ws.w_ofs.as<int>(), ws.w_node.as<int>(),
+                            ws.s_part.as<long long>(), s));
+    CK(launch_xlist(nd, nn, col, pb.semifar, 1, ws.x_cnt.as<int>(),
+                    ws.x_ofs.as<int>(), ws.x_leaf.as<int>(), s));
+    ws.semifar_nodes = nn;
+  }
   long long *d_pairs = reinterpret_cast<long long *>(ws.small.as<char>() + 1024);
-  // (d_pairs is followed by the 64 partial sums of launch_sum_ll)
+  // (d_pairs and d_spart are each followed by the 64 partial sums of
+  // launch_sum_ll)
   int *d_far = reinterpret_cast<int *>(ws.small.as<char>() + 2048);
+  long long *d_spart =
+      reinterpret_cast<long long *>(ws.small.as<char>() + 2560);
   CK(launch_sum_ll(ws.u_pairs.as<long long>(), nleaves, d_pairs, s));
+  if (semifar)
+    CK(launch_sum_ll(ws.s_part.as<long long>(), nleaves, d_spart, s));
   // ---- tiles of the far field (mixed form)
   int ntiles = 0;
   if (pb.precision != 0 && depth >= 2) {
@@ -736,6 +801,11 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     S.m2l_split = nsplit;
     S.m2l_tiles = ntiles;
   }
+  // ---- semifar P2L into the locals (before they move down)
+  if (semifar && nsemi > 0)
+    CK(launch_p2l(tb, nd, nn, ws.x_ofs.as<int>(), ws.x_leaf.as<int>(),
+                  ws.xs.as<double>(), ws.ys.as<double>(), ws.zs.as<double>(),
+                  ws.qs.as<double>(), L, s));
   // ---- downward pass
   for (int d = 1; d <= depth; d++)
     CK(launch_l2l(tb, nd, lstart[d], lcount[d], L, s));
@@ -745,6 +815,11 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                 ws.u_beg.as<int>(), ws.u_end.as<int>(), ws.xs.as<double>(),
                 ws.ys.as<double>(), ws.zs.as<double>(), ws.qs.as<double>(),
                 ws.pot_s.as<double>(), ws.grad_s.as<double>(), s));
+  if (semifar && nsemi > 0)
+    CK(launch_m2p(tb, nd, leaves, nleaves, ws.w_ofs.as<int>(),
+                  ws.w_node.as<int>(), ws.xs.as<double>(), ws.ys.as<double>(),
+                  ws.zs.as<double>(), M, ws.pot_s.as<double>(),
+                  ws.grad_s.as<double>(), s));
   CK(ws.pleaf.reserve(sizeof(int) * std::max<long>(1, n)));
   CK(launch_particle_leaf(nd, leaves, nleaves, ws.pleaf.as<int>(), s));
   CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
@@ -760,6 +835,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   int *hfar = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                       2048);
   CK(hipMemcpyAsync(hp, d_pairs, sizeof(long long), hipMemcpyDeviceToHost, s));
+  long long *hsp = reinterpret_cast<long long *>(
+      static_cast<char *>(ws.host_small) + 2560);
+  *hsp = 0;
+  if (semifar)
+    CK(hipMemcpyAsync(hsp, d_spart, sizeof(long long), hipMemcpyDeviceToHost,
+                      s));
   if (pb.precision != 0 && depth >= 2)
     CK(hipMemcpyAsync(hfar, d_far, sizeof(int), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
@@ -771,6 +852,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   S.m2l = nfar;
   S.l2l = nn - nsys;
   S.l2p = n;
+  S.p2l = (long)*hsp;
+  S.m2p = (long)*hsp;
+  S.semifar = nsemi;
   S.depth = depth;
   S.nnodes = nn;
   S.nleaves = nleaves;

@@ -544,6 +544,184 @@ __global__ __launch_bounds__(256) void k_l2p(
   grad[3 * (size_t)i + 2] += dz;
 }
 
+// ------------------------------------------------------------ M2P / P2L
+//
+// Semifar operators (aran_development3d_m2pv / multipole_evaluate and
+// aran_development3d_p2l, arandevelopment3d.c:227-277,485-560) through the
+// irregular solid harmonics
+//   I_l^m = (l-m)! P_l^m(cos theta) e^{i m phi} / r^(l+1),
+//   I_0^0 = 1/r,  I_m^m = -(2m-1) (x + iy) / r^2 I_{m-1}^{m-1},
+//   I_l^m = ((2l-1) z I_{l-1}^m - ((l-1)^2 - m^2) I_{l-2}^m) / r^2,
+// whose derivatives are harmonics of one degree more:
+//   d/dz I_l^m = -I_{l+1}^m,  (d/dx + i d/dy) I_l^m = I_{l+1}^{m+1},
+//   (d/dx - i d/dy) I_l^m = -I_{l+1}^{m-1}.
+// With h_l^m = N_l^m / (l-m)!, r^-(l+1) Y_l^m = h_l^m I_l^m.
+
+// One wave per leaf, one lane per particle: for every semifar node D of the
+// leaf, with c_l^m = h_l^m M_l^m of D (degrees l < p),
+//   phi = sum_l [c_l^0 I_l^0 + 2 Re sum_{m>0} c_l^m I_l^m],
+//   d/dz phi = -(same sum with I_{l+1}^m),
+//   (d/dx + i d/dy) phi = sum_l [sum_{m>=0} c_l^m I_{l+1}^{m+1}
+//                                - sum_{m>0} conj(c_l^m I_{l+1}^{m-1})],
+// accumulated per harmonic I_j^m as it is generated (column by column).
+__global__ __launch_bounds__(64) void k_m2p(
+    OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
+    const int *__restrict__ wofs, const int *__restrict__ wnode,
+    const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ zs, const double2 *__restrict__ M,
+    double *__restrict__ pot, double *__restrict__ grad) {
+  const int t = blockIdx.x;
+  const int w0 = wofs[t], w1 = wofs[t + 1];
+  if (w0 == w1) return;
+  const int A = leaves[t];
+  const int p = tb.p, mc = tri(p);
+  const double *__restrict__ g = tb.igain;
+  for (int i = nd.pbegin[A] + threadIdx.x; i < nd.pend[A]; i += 64) {
+    double phi = 0., dz = 0., gx = 0., gy = 0.;
+    for (int w = w0; w < w1; w++) {
+      const int D = wnode[w];
+      const double2 *__restrict__ Md = M + (size_t)D * mc;
+      const double vx = xs[i] - nd.cx[D], vy = ys[i] - nd.cy[D],
+                   vz = zs[i] - nd.cz[D];
+      const double r2 = vx * vx + vy * vy + vz * vz;
+      const double ir2 = 1. / r2;
+      double mx = sqrt(ir2), my = 0.;  // I_m^m
+      for (int m = 0; m <= p; m++) {
+        if (m > 0) {
+          const double f = -(2. * m - 1.) * ir2;
+          const double ax = f * vx, ay = f * vy;
+          const double nx = mx * ax - my * ay, ny = mx * ay + my * ax;
+          mx = nx;
+          my = ny;
+        }
+        const double wm = m ? 2. : 1.;
+        double ox = mx, oy = my, qx = 0., qy = 0.;  // I_j^m, I_{j-1}^m
+        for (int j = m; j <= p; j++) {
+          if (j > m) {
+            const double a = (2. * j - 1.) * vz;
+            const double b = (double)((j - 1) * (j - 1) - m * m);
+            const double nx = (a * ox - b * qx) * ir2,
+                         ny = (a * oy - b * qy) * ir2;
+            qx = ox;
+            qy = oy;
+            ox = nx;
+            oy = ny;
+          }
+          if (j < p) {  // c_j^m I_j^m
+            const int k = tri(j) + m;
+            const double cx = g[k] * Md[k].x, cy = g[k] * Md[k].y;
+            phi += wm * (cx * ox - cy * oy);
+          }
+          if (j >= 1 && j - 1 <= p - 1) {
+            const int l = j - 1;
+            if (l >= m) {  // -c_l^m I_{l+1}^m
+              const int k = tri(l) + m;
+              const double cx = g[k] * Md[k].x, cy = g[k] * Md[k].y;
+              dz -= wm * (cx * ox - cy * oy);
+            }
+            if (m >= 1) {  // c_l^{m-1} I_{l+1}^m
+              const int k = tri(l) + m - 1;
+              const double cx = g[k] * Md[k].x, cy = g[k] * Md[k].y;
+              gx += cx * ox - cy * oy;
+              gy += cx * oy + cy * ox;
+            }
+            if (l >= m + 1) {  // - conj(c_l^{m+1} I_{l+1}^m)
+              const int k = tri(l) + m + 1;
+              const double cx = g[k] * Md[k].x, cy = g[k] * Md[k].y;
+              gx -= cx * ox - cy * oy;
+              gy += cx * oy + cy * ox;
+            }
+          }
+        }
+      }
+    }
+    pot[i] += phi;
+    grad[3 * (size_t)i + 0] += gx;
+    grad[3 * (size_t)i + 1] += gy;
+    grad[3 * (size_t)i + 2] += dz;
+  }
+}
+
+// One wave per node: L_l^m += (4pi/(2l+1)) h_l^m conj(sum_j q_j I_l^m(x_j))
+// over the particles of its semifar leaves (aran_development3d_p2l).  As in
+// k_p2m, lanes take (particle, m) columns of a chunk into LDS, then lanes take
+// coefficients and add the chunk's particles in order.
+__global__ __launch_bounds__(64) void k_p2l(
+    OpTables tb, NodeArrays nd, const int *__restrict__ xofs,
+    const int *__restrict__ xleaf, const double *__restrict__ xs,
+    const double *__restrict__ ys, const double *__restrict__ zs,
+    const double *__restrict__ qs, double2 *__restrict__ L) {
+  __shared__ double2 T[kP2mSlots];
+  const int D = blockIdx.x;
+  const int x0 = xofs[D], x1 = xofs[D + 1];
+  if (x0 == x1) return;  // uniform per block: no barrier is skipped
+  const int p = tb.p, lc = tri(p + 1), np1 = p + 1;
+  const int lane = threadIdx.x;
+  const double cx = nd.cx[D], cy = nd.cy[D], cz = nd.cz[D];
+  const int chunk = min(64, kP2mSlots / lc);
+  double2 acc[3] = {make_double2(0., 0.), make_double2(0., 0.),
+                    make_double2(0., 0.)};
+  for (int x = x0; x < x1; x++) {
+    const int J = xleaf[x];
+    const int b = nd.pbegin[J], e = nd.pend[J];
+    for (int c0 = b; c0 < e; c0 += chunk) {
+      const int cnt = min(chunk, e - c0);
+      __syncthreads();
+      for (int t = lane; t < cnt * np1; t += 64) {
+        const int j = t / np1, m = t - j * np1;
+        const int i = c0 + j;
+        const double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
+        const double r2 = vx * vx + vy * vy + vz * vz;
+        const double ir2 = 1. / r2;
+        double ox = qs[i] * sqrt(ir2), oy = 0.;  // q I_m^m
+        for (int k = 1; k <= m; k++) {
+          const double f = -(2. * k - 1.) * ir2;
+          const double ax = f * vx, ay = f * vy;
+          const double nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
+          ox = nx;
+          oy = ny;
+        }
+        double2 *row = T + j * lc;
+        row[tri(m) + m] = make_double2(ox, oy);
+        double px = ox, py = oy, qx = 0., qy = 0.;
+        for (int l = m + 1; l <= p; l++) {
+          const double a = (2. * l - 1.) * vz;
+          const double bb = (double)((l - 1) * (l - 1) - m * m);
+          const double nx = (a * px - bb * qx) * ir2,
+                       ny = (a * py - bb * qy) * ir2;
+          qx = px;
+          qy = py;
+          px = nx;
+          py = ny;
+          row[tri(l) + m] = make_double2(nx, ny);
+        }
+      }
+      __syncthreads();
+      for (int slot = 0; slot < 3; slot++) {
+        const int k = lane + 64 * slot;
+        if (k >= lc) break;
+        double2 a = acc[slot];
+        for (int j = 0; j < cnt; j++) {
+          const double2 v = T[j * lc + k];
+          a.x += v.x;
+          a.y += v.y;
+        }
+        acc[slot] = a;
+      }
+    }
+  }
+  for (int slot = 0; slot < 3; slot++) {
+    const int k = lane + 64 * slot;
+    if (k < lc) {
+      const double g = tb.xgain[k];
+      double2 v = L[(size_t)D * lc + k];
+      v.x += g * acc[slot].x;
+      v.y -= g * acc[slot].y;
+      L[(size_t)D * lc + k] = v;
+    }
+  }
+}
+
 // pleaf[j] = leaf node of sorted particle j.
 __global__ void k_particle_leaf(NodeArrays nd, const int *__restrict__ leaves,
                                 int nleaves, int *__restrict__ pleaf) {
@@ -584,6 +762,26 @@ hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
                       int lvl_count, double2 *L, hipStream_t s) {
   if (lvl_count == 0) return hipSuccess;
   k_l2l<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, L);
+  return hipGetLastError();
+}
+
+hipError_t launch_m2p(const OpTables &tb, const NodeArrays &nd,
+                      const int *leaves, int nleaves, const int *wofs,
+                      const int *wnode, const double *xs, const double *ys,
+                      const double *zs, const double2 *M, double *pot,
+                      double *grad, hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  k_m2p<<<nleaves, 64, 0, s>>>(tb, nd, leaves, wofs, wnode, xs, ys, zs, M, pot,
+                               grad);
+  return hipGetLastError();
+}
+
+hipError_t launch_p2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
+                      const int *xofs, const int *xleaf, const double *xs,
+                      const double *ys, const double *zs, const double *qs,
+                      double2 *L, hipStream_t s) {
+  if (nnodes == 0) return hipSuccess;
+  k_p2l<<<nnodes, 64, 0, s>>>(tb, nd, xofs, xleaf, xs, ys, zs, qs, L);
   return hipGetLastError();
 }
 

@@ -699,6 +699,184 @@ __global__ void k_ulist(NodeArrays nd, int nleaves,
     pairs[t] = np;
 }
 
+// ---- semifar lists (leaf / deeper node pairs, aran/aransolver3d.c:357-396)
+//
+// The recursion of the near/far traversal pairs a leaf A with the children
+// of a same-depth non-leaf neighbour, and goes on down while the deeper node
+// touches A.  A deeper node D that does not touch A and holds more than thr
+// particles forms a semifar pair (A, D): P2L of A's particles into D's local
+// and M2P of D's multipole onto them; the recursion stops there.  Any other
+// non-touching node, and every touching leaf, is near: P2P with all of its
+// particles.
+
+// Boxes of node a and of node b (b at least as deep as a) touch or overlap.
+__device__ inline bool touching(const NodeArrays &nd, int a, int b) {
+  const int s = nd.depth[b] - nd.depth[a];
+  const int w = 1 << s;
+  const int ax = nd.ix[a] << s, ay = nd.iy[a] << s, az = nd.iz[a] << s;
+  return nd.ix[b] >= ax - 1 && nd.ix[b] <= ax + w && nd.iy[b] >= ay - 1 &&
+         nd.iy[b] <= ay + w && nd.iz[b] >= az - 1 && nd.iz[b] <= az + w;
+}
+
+__device__ inline int node_count(const NodeArrays &nd, int a) {
+  return nd.pend[a] - nd.pbegin[a];
+}
+
+// Near list (particle ranges) and semifar list (deeper nodes, M2P onto this
+// leaf) of every leaf; pass 0 counts, pass 1 fills.  pairs[t]: ordered P2P
+// pair count (as k_ulist); spart[t]: particles x semifar nodes of the leaf.
+__global__ void k_ulist_semifar(NodeArrays nd, int nleaves,
+                                const int *__restrict__ leaves,
+                                const int *__restrict__ col, long long thr,
+                                int pass, int *__restrict__ cnt,
+                                const int *__restrict__ ofs,
+                                int *__restrict__ ubeg, int *__restrict__ uend,
+                                long long *__restrict__ pairs,
+                                int *__restrict__ wcnt,
+                                const int *__restrict__ wofs,
+                                int *__restrict__ wnode,
+                                long long *__restrict__ spart) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nleaves) return;
+  const int A = leaves[t];
+  const int dA = nd.depth[A];
+  const int nA = node_count(nd, A);
+  int anc[kMaxDepth + 1];
+  anc[dA] = A;
+  for (int d = dA - 1; d >= 0; d--) anc[d] = nd.parent[anc[d + 1]];
+  int c = 0, cw = 0;
+  int pos = pass ? ofs[t] : 0, wpos = pass ? wofs[t] : 0;
+  long long np = 0;
+  auto near = [&](int j) {
+    if (pass) {
+      const int b = nd.pbegin[j], e = nd.pend[j];
+      ubeg[pos] = b;
+      uend[pos] = e;
+      pos++;
+      if (j == A)
+        np += 2LL * ((long long)nA * (nA + 1) / 2);
+      else
+        np += (long long)nA * (e - b);
+    } else {
+      c++;
+    }
+  };
+  // same-depth neighbours, descending into non-leaf ones (no stack: the
+  // children of a node are contiguous, so the walk moves to the next
+  // sibling or back up)
+  const int *cA = col + (size_t)A * 27;
+  for (int s = 0; s < 27; s++) {
+    const int j = cA[s];
+    if (j < 0) continue;
+    if (nd.leaf[j]) {
+      near(j);
+      continue;
+    }
+    int C = nd.child[j];
+    for (;;) {
+      bool down = false;
+      if (!touching(nd, A, C)) {
+        if (node_count(nd, C) > thr) {
+          if (pass) wnode[wpos++] = C;
+          else cw++;
+        } else {
+          near(C);
+        }
+      } else if (nd.leaf[C]) {
+        near(C);
+      } else {
+        down = true;
+      }
+      if (down) {
+        C = nd.child[C];
+        continue;
+      }
+      for (;;) {
+        const int P = nd.parent[C];
+        if (C + 1 < nd.child[P] + nd.nchild[P]) {
+          C++;
+          break;
+        }
+        C = P;
+        if (C == j) break;
+      }
+      if (C == j) break;
+    }
+  }
+  // shallower leaves next to an ancestor: near unless the first ancestor
+  // that does not touch them took them as a semifar pair
+  for (int d = 0; d < dA; d++) {
+    const int *cx = col + (size_t)anc[d] * 27;
+    for (int s = 0; s < 27; s++) {
+      const int J = cx[s];
+      if (J < 0 || s == 13 || !nd.leaf[J]) continue;
+      int k = d + 1;
+      while (k <= dA && touching(nd, J, anc[k])) k++;
+      if (k > dA || node_count(nd, anc[k]) <= thr) near(J);
+    }
+  }
+  if (!pass) {
+    cnt[t] = c;
+    wcnt[t] = cw;
+  } else {
+    pairs[t] = np;
+    spart[t] = (long long)nA * (wpos - wofs[t]);
+  }
+}
+
+// Semifar list of every node holding more than thr particles: the leaves
+// whose particles add to its local by P2L.  pass 0 counts, pass 1 fills.
+__global__ void k_xlist(NodeArrays nd, int nnodes, const int *__restrict__ col,
+                        long long thr, int pass, int *__restrict__ cnt,
+                        const int *__restrict__ ofs,
+                        int *__restrict__ xleaf) {
+  const int C = blockIdx.x * blockDim.x + threadIdx.x;
+  if (C >= nnodes) return;
+  const int k = nd.depth[C];
+  int c = 0, pos = pass ? ofs[C] : 0;
+  if (k >= 1 && node_count(nd, C) > thr) {
+    int anc[kMaxDepth + 1];
+    anc[k] = C;
+    for (int d = k - 1; d >= 0; d--) anc[d] = nd.parent[anc[d + 1]];
+    for (int d = 0; d < k; d++) {
+      const int *cx = col + (size_t)anc[d] * 27;
+      for (int s = 0; s < 27; s++) {
+        const int J = cx[s];
+        if (J < 0 || s == 13 || !nd.leaf[J]) continue;
+        int m = d + 1;
+        while (m < k && touching(nd, J, anc[m])) m++;
+        if (m == k && !touching(nd, J, C)) {
+          if (pass) xleaf[pos++] = J;
+          else c++;
+        }
+      }
+    }
+  }
+  if (!pass) cnt[C] = c;
+}
+
+hipError_t launch_ulist_semifar(const NodeArrays &nd, int nleaves,
+                                const int *leaves, const int *col, long thr,
+                                int pass, int *cnt, const int *ofs, int *ubeg,
+                                int *uend, long long *pairs, int *wcnt,
+                                const int *wofs, int *wnode, long long *spart,
+                                hipStream_t s) {
+  if (nleaves == 0) return hipSuccess;
+  k_ulist_semifar<<<(nleaves + 127) / 128, 128, 0, s>>>(
+      nd, nleaves, leaves, col, thr, pass, cnt, ofs, ubeg, uend, pairs, wcnt,
+      wofs, wnode, spart);
+  return hipGetLastError();
+}
+
+hipError_t launch_xlist(const NodeArrays &nd, int nnodes, const int *col,
+                        long thr, int pass, int *cnt, const int *ofs,
+                        int *xleaf, hipStream_t s) {
+  if (nnodes == 0) return hipSuccess;
+  k_xlist<<<(nnodes + 127) / 128, 128, 0, s>>>(nd, nnodes, col, thr, pass, cnt,
+                                               ofs, xleaf);
+  return hipGetLastError();
+}
+
 hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
                         int pass, int *cnt, const int *ofs, int *src,
                         short *code, hipStream_t s) {

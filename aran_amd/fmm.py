@@ -11,7 +11,8 @@ import numpy as np
 from ._lib import check, lib
 
 STAT_NAMES = ("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p", "depth",
-              "nodes", "leaves", "far_entries", "near_entries")
+              "nodes", "leaves", "far_entries", "near_entries", "p2l", "m2p",
+              "semifar_pairs")
 TIMER_NAMES = ("tree", "lists", "up", "m2l", "down", "near", "total")
 
 
@@ -49,6 +50,17 @@ class Fmm3d:
         code = {"plain": 0, "mixed": 1}.get(precision, precision)
         check(lib().aran_fmm3d_set_precision(self._ctx, int(code)),
               "set_precision")
+
+    def set_semifar(self, threshold):
+        """Semifar P2L / M2P pairs for nodes holding more than `threshold`
+        particles; None or a negative value disables them (the cost-model
+        threshold of an order is semifar_threshold_auto(order))."""
+        t = -1 if threshold is None else int(threshold)
+        check(lib().aran_fmm3d_set_semifar(self._ctx, t), "set_semifar")
+
+    @staticmethod
+    def semifar_threshold_auto(order):
+        return int(lib().aran_fmm3d_semifar_threshold_auto(int(order)))
 
     def set_timings(self, on=True):
         check(lib().aran_fmm3d_set_timings(self._ctx, int(on)), "set_timings")
@@ -98,7 +110,7 @@ class Fmm3d:
         return pot, grad
 
     def stats(self):
-        v = (C.c_long * 12)()
+        v = (C.c_long * len(STAT_NAMES))()
         check(lib().aran_fmm3d_get_stats(self._ctx, v), "get_stats")
         return dict(zip(STAT_NAMES, list(v)))
 
@@ -139,6 +151,22 @@ class Fmm3d:
         return dict(far_ofs=far_ofs, far_src=far_src[:nf],
                     far_code=far_code[:nf], leaves=leaves[:nl],
                     near_ofs=near_ofs, near_begin=nb[:nu], near_end=ne[:nu])
+
+    def export_semifar(self):
+        """Semifar lists of the last solve: per leaf (leaf order of
+        export_lists) the nodes it evaluates by M2P, per node the leaves it
+        takes by P2L."""
+        st = self.stats()
+        nn, nl, ns = st["nodes"], st["leaves"], st["semifar_pairs"]
+        w_ofs = np.zeros(nl + 1, dtype=np.int32)
+        w_node = np.zeros(max(ns, 1), dtype=np.int32)
+        x_ofs = np.zeros(nn + 1, dtype=np.int32)
+        x_leaf = np.zeros(max(ns, 1), dtype=np.int32)
+        check(lib().aran_fmm3d_export_semifar(
+            self._ctx, w_ofs.ctypes.data, w_node.ctypes.data,
+            x_ofs.ctypes.data, x_leaf.ctypes.data), "export_semifar")
+        return dict(w_ofs=w_ofs, w_node=w_node[:ns], x_ofs=x_ofs,
+                    x_leaf=x_leaf[:ns])
 
     def direct(self, x, y, z, q, targets):
         import torch

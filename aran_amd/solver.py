@@ -86,11 +86,17 @@ def lib():
     L.aran_development3d_local_evaluate.argtypes = [ni_p, dev_p, v3_p]
     L.aran_development3d_local_gradient_evaluate.argtypes = [ni_p, dev_p, v3_p,
                                                              v3_p]
+    L.aran_development3d_p2l.argtypes = [v3_p, d, ni_p, dev_p]
+    L.aran_development3d_multipole_evaluate.restype = Complex128
+    L.aran_development3d_multipole_evaluate.argtypes = [ni_p, dev_p, v3_p]
+    L.aran_development3d_multipole_gradient_evaluate.argtypes = [
+        ni_p, dev_p, v3_p, v3_p]
     L.aran_solver3d_new.restype = vp
     L.aran_solver3d_new.argtypes = [vp, C.c_size_t, vp, vp]
     L.aran_solver3d_free.argtypes = [vp]
     L.aran_solver3d_set_functions.argtypes = [vp] + [vp] * 6
     L.aran_solver3d_set_functions_full.argtypes = [vp] + [vp] * 8 + [u]
+    L.aran_solver3d_reinit_stats.argtypes = [vp]
     L.aran_solver3d_set_particle_layout.argtypes = [vp, C.c_ssize_t,
                                                     C.c_ssize_t, C.c_int,
                                                     C.c_ssize_t]

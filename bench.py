@@ -108,6 +108,111 @@ def error_vs_direct(f, dev, pot, grad, n, seed):
             "targets": 4096}
 
 
+def batch_systems(nsys, seed):
+    """nsys independent systems, sizes round(10^(3 + 2u)) (log-uniform in
+    [1e3, 1e5]), each a uniform cube or a Plummer sphere with probability
+    1/2, masses in (0, 1]."""
+    from aran_amd import workloads as W
+    u = W.uniform(seed, 20, 2 * nsys).reshape(nsys, 2)
+    xs, ys, zs, ms, lbs, ubs, offs = [], [], [], [], [], [], [0]
+    for k in range(nsys):
+        n = int(round(10 ** (3 + 2 * u[k, 0])))
+        if u[k, 1] < 0.5:
+            x, y, z, _ = W.uniform_cube(n, seed * 1000 + k)
+            lbs.append((0, 0, 0))
+            ubs.append((1, 1, 1))
+        else:
+            x, y, z, _ = W.plummer(n, seed * 1000 + k)
+            lbs.append((-1, -1, -1))
+            ubs.append((1, 1, 1))
+        m = 1.0 - W.uniform(seed * 1000 + k, 9, n)
+        for a, b in ((xs, x), (ys, y), (zs, z), (ms, m)):
+            a.append(b)
+        offs.append(offs[-1] + n)
+    cat = [np.concatenate(a) for a in (xs, ys, zs, ms)]
+    return cat, offs, np.array(lbs, float), np.array(ubs, float)
+
+
+def batch_session(nsys, order, leafcap, steps, warmup, seed,
+                  precision="mixed"):
+    """One batched solve of nsys independent systems (shared launches, one
+    octree per system), timed over `steps` solves; error against the FP64
+    direct sum of each target's own system on 4096 sampled targets."""
+    import torch
+
+    from aran_amd import workloads as W
+    from aran_amd.fmm import Fmm3d
+    cat, offs, lbs, ubs = batch_systems(nsys, seed)
+    dev = [torch.from_numpy(a).cuda() for a in cat]
+    n = offs[-1]
+    f = Fmm3d()
+    f.set_precision(precision)
+    pot = torch.empty(n, dtype=torch.float64, device="cuda")
+    grad = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    for _ in range(warmup):
+        f.solve_batch(offs, *dev, lbs, ubs, order, leafcap, pot, grad)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        f.solve_batch(offs, *dev, lbs, ubs, order, leafcap, pot, grad)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    tg = W.sample_targets(n, 4096, seed)
+    sysid = np.searchsorted(np.array(offs), tg, side="right") - 1
+    pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
+    dp, dg = np.zeros(len(tg)), np.zeros((len(tg), 3))
+    for s in np.unique(sysid):
+        sel = np.nonzero(sysid == s)[0]
+        a, b = offs[s], offs[s + 1]
+        loc = torch.from_numpy(tg[sel] - a)
+        p_, g_ = f.direct(*[t[a:b] for t in dev], loc)
+        dp[sel], dg[sel] = p_.cpu().numpy(), g_.cpu().numpy()
+    st = f.stats()
+    return {"systems": nsys, "particles": int(n), "order": order,
+            "leafcap": leafcap, "form": precision,
+            "ms_per_solve": 1e3 * dt, "particles_s": n / dt,
+            "pot_l2": float(np.linalg.norm(pp - dp) / np.linalg.norm(dp)),
+            "pot_max": float(np.max(np.abs(pp - dp) / np.abs(dp))),
+            "grad_l2": float(np.linalg.norm(gg - dg) / np.linalg.norm(dg)),
+            "nodes": st["nodes"], "m2l": st["m2l"]}
+
+
+def precision_sweep(n, orders, steps, seed):
+    """Uniform cube, charges in [-1, 1), leaf 64: per order and form,
+    particles/s and L2 / max errors of potential and gradient against the
+    FP64 direct sum on 4096 sampled targets."""
+    import torch
+
+    from aran_amd import workloads as W
+    from aran_amd.fmm import Fmm3d
+    x, y, z, qv = W.signed_cube(n, seed)
+    dev = [torch.from_numpy(a).cuda() for a in (x, y, z, qv)]
+    f = Fmm3d()
+    pot = torch.empty(n, dtype=torch.float64, device="cuda")
+    grad = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    tg = W.sample_targets(n, 4096, seed)
+    dp, dg = f.direct(*dev, torch.from_numpy(tg))
+    dp, dg = dp.cpu().numpy(), dg.cpu().numpy()
+    out = []
+    for p in orders:
+        for form in ("plain", "mixed"):
+            r = run_form(f, dev, pot, grad, (0, 0, 0), (1, 1, 1), p, 64, form,
+                         steps, 1)
+            pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
+            out.append({"order": p, "form": form,
+                        "dtype": "f64" if form == "plain" else "f32 M2L/P2P",
+                        "particles_s": r["value"],
+                        "ms_per_solve": r["ms_per_step"],
+                        "m2l_ms": r["m2l_ms"],
+                        "pot_l2": float(np.linalg.norm(pp - dp) /
+                                        np.linalg.norm(dp)),
+                        "pot_max": float(np.max(np.abs(pp - dp) /
+                                                np.abs(dp))),
+                        "grad_l2": float(np.linalg.norm(gg - dg) /
+                                         np.linalg.norm(dg))})
+    return out
+
+
 def serve_session(n_requests, rate, seed, precision="mixed"):
     """Poisson arrivals at `rate` requests/s; per request N log-uniform in
     [1e3, 2e6], p in {6, 8, 12}, leaf 64, half potential-only, half
@@ -190,6 +295,12 @@ def main():
                     help="also run one elementwise kernel that reads and "
                          "writes 512 MiB (the reference point for HBM byte "
                          "counters)")
+    ap.add_argument("--batch-systems", type=int, default=512,
+                    help="independent systems of the batched solve (0: skip)")
+    ap.add_argument("--sweep", action="store_true",
+                    help="only the precision sweep: N=4,194,304 uniform "
+                         "cube, signed charges, leaf 64, p in {6, 10, 14}, "
+                         "plain and mixed forms (one JSON line)")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="offered load of the serving session, requests/s")
@@ -205,6 +316,13 @@ def main():
     from aran_amd.fmm import Fmm3d
 
     torch.cuda.set_device(0)
+    if args.sweep:
+        pts = precision_sweep(4_194_304, (6, 10, 14), max(2, args.steps // 3),
+                              0xA2A0 + 3)
+        print(json.dumps({"sweep": "uniform cube N=4194304, charges in "
+                                   "[-1,1), leaf 64, pot+grad",
+                          "points": pts}))
+        return
     seed = 0xA2A0 + 2
     n, p, q = args.n, args.order, args.leafcap
     x, y, z, m = W.plummer(n, seed)
@@ -254,6 +372,10 @@ def main():
         torch.add(a, 1.0, out=b)
         torch.cuda.synchronize()
         del a, b
+    batch = None
+    if args.batch_systems > 0:
+        batch = batch_session(args.batch_systems, 8, 32, max(2, args.steps // 2),
+                              1, 0xA2A0 + 4, args.precision)
     serve = None
     if args.serve_requests > 0:
         serve = serve_session(args.serve_requests, args.serve_rate, seed + 3,
@@ -278,6 +400,7 @@ def main():
         "error": err,
         "phases_ms": head["phases_ms"],
         "plain_fp64": plain,
+        "batched": batch,
         "serving": serve,
         "tree": {k: st[k] for k in ("depth", "nodes", "leaves", "m2l", "p2p")},
     }

@@ -78,6 +78,28 @@ aran_development3d_local_gradient_evaluate (const VsgPRTree3dNodeInfo *devel_nod
                                             AranDevelopment3d *devel,
                                             const VsgVector3d *pos,
                                             VsgVector3d *grad);
+/* semifar operators (aran/arandevelopment3d.h:64, :84, :92, :112;
+ * aran/arandevelopment3d.c:227-277, :485-560) */
+void aran_development3d_p2l (const VsgVector3d *position,
+                             const gdouble charge,
+                             const VsgPRTree3dNodeInfo *dst_node,
+                             AranDevelopment3d *dst);
+gcomplex128
+aran_development3d_multipole_evaluate (const VsgPRTree3dNodeInfo *devel_node,
+                                       AranDevelopment3d *devel,
+                                       const VsgVector3d *pos);
+void
+aran_development3d_multipole_gradient_evaluate (const VsgPRTree3dNodeInfo *devel_node,
+                                                AranDevelopment3d *devel,
+                                                const VsgVector3d *pos,
+                                                VsgVector3d *grad);
+gcomplex128 aran_development3d_m2p (const VsgPRTree3dNodeInfo *devel_node,
+                                    AranDevelopment3d *devel,
+                                    const VsgVector3d *pos);
+void aran_development3d_m2pv (const VsgPRTree3dNodeInfo *devel_node,
+                              AranDevelopment3d *devel,
+                              const VsgVector3d *pos, VsgVector3d *grad);
+
 gcomplex128 aran_development3d_l2p (const VsgPRTree3dNodeInfo *devel_node,
                                     AranDevelopment3d *devel,
                                     const VsgVector3d *pos);

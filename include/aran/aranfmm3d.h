@@ -59,8 +59,28 @@ int aran_fmm3d_set_precision (AranFmm3d *ctx, int precision);
 
 /* Counters of the last solve: zero, p2p, p2m, m2m, m2l, l2l, l2p (the
  * reference's semantics), then depth, nodes, leaves, far-list entries,
- * near-list entries. */
-int aran_fmm3d_get_stats (const AranFmm3d *ctx, long *out12);
+ * near-list entries, then p2l, m2p (particles in semifar pairs, the
+ * reference's counters) and the number of semifar (leaf, node) pairs. */
+int aran_fmm3d_get_stats (const AranFmm3d *ctx, long *out15);
+
+/* Semifar pairs of the next solves (aran/aransolver3d.c:357-396): when the
+ * near/far recursion pairs a leaf with a deeper node whose box does not
+ * touch the leaf's and which holds more than `threshold` particles, the
+ * leaf's particles add to the node's local (P2L) and take its multipole's
+ * potential and gradient (M2P) instead of P2P with the node's particles.
+ * threshold < 0 disables them (the default). */
+int aran_fmm3d_set_semifar (AranFmm3d *ctx, long threshold);
+
+/* Threshold at which P2L + M2P per particle cost what P2P with the node's
+ * particles costs, from the operators' algorithmic flops at this order. */
+long aran_fmm3d_semifar_threshold_auto (int order);
+
+/* Semifar lists of the last solve (host arrays sized from get_stats):
+ * w_ofs[nleaves+1], w_node[npairs] (per leaf, in the leaf order of
+ * export_lists, the nodes whose multipoles it evaluates), x_ofs[nnodes+1],
+ * x_leaf[npairs] (per node, the leaves whose particles it takes). */
+int aran_fmm3d_export_semifar (const AranFmm3d *ctx, int *w_ofs, int *w_node,
+                               int *x_ofs, int *x_leaf);
 
 /* Phase timers (HIP events) of the next solves when on != 0; read back as
  * tree, lists, up, m2l, down, near, total in ms. */

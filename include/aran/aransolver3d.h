@@ -13,7 +13,11 @@
  *   m2l : aran_development3d_m2l_rotate
  *   l2l : aran_development3d_l2l_rotate
  *   l2p : aran_newton3d_l2p_pot / _field / _potfield
- * and the particle layout (offsets of charge, potential, field inside the
+ * optionally with the semifar pair (aran_solver3d_set_functions_full)
+ *   p2l : aran_newton3d_p2l
+ *   m2p : aran_newton3d_m2p_pot / _field / _potfield (the outputs of p2p)
+ * and a semifar threshold (G_MAXUINT: never semifar, 0: automatic from the
+ * operators' algorithmic costs), and the particle layout (offsets of charge, potential, field inside the
  * caller's struct whose first member is a VsgVector3d) is declared with
  * aran_solver3d_set_particle_layout().  Other callbacks are rejected with a
  * critical message; there is no host fallback.
@@ -143,6 +147,16 @@ void aran_newton3d_l2p_pot (const VsgPRTree3dNodeInfo *src_node, gpointer src,
 void aran_newton3d_l2p_field (const VsgPRTree3dNodeInfo *src_node, gpointer src,
                               VsgPoint3 particle);
 void aran_newton3d_l2p_potfield (const VsgPRTree3dNodeInfo *src_node,
+                                 gpointer src, VsgPoint3 particle);
+/* semifar callbacks (tests/newtonpot3.c:93-104, tests/newtonfield3.c:
+ * 139-153) */
+void aran_newton3d_p2l (VsgPoint3 particle, const VsgPRTree3dNodeInfo *dst_node,
+                        gpointer dst);
+void aran_newton3d_m2p_pot (const VsgPRTree3dNodeInfo *src_node, gpointer src,
+                            VsgPoint3 particle);
+void aran_newton3d_m2p_field (const VsgPRTree3dNodeInfo *src_node, gpointer src,
+                              VsgPoint3 particle);
+void aran_newton3d_m2p_potfield (const VsgPRTree3dNodeInfo *src_node,
                                  gpointer src, VsgPoint3 particle);
 
 /* Adds nparticles particles given as structure-of-arrays copies (the solver

@@ -1,0 +1,268 @@
+"""GPU parity of the semifar (P2L / M2P) path against the FP64 oracle,
+through the C-ABI (aran/aransolver3d.c:357-396 semifar_func; operators
+aran/arandevelopment3d.c:227-277, :485-560).
+
+Bars: semifar and near lists bit-exact with the oracle traversal (as pair
+sets); FP64 solves within 1e-10 relative of the oracle FMM with identical
+counters; single P2L / M2P operators within 1e-12 of the oracle's.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch  # noqa: F401
+
+import oracle as O
+from aran_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(torch, *arrs):
+    return [torch.from_numpy(np.ascontiguousarray(a, dtype=np.float64)).cuda()
+            for a in arrs]
+
+
+def _relerr_pot(a, b):
+    return np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+
+
+def _relerr_grad(a, b):
+    nb = np.linalg.norm(b, axis=1)
+    floor = 1e-8 * np.sqrt(np.mean(nb ** 2))
+    return np.max(np.linalg.norm(a - b, axis=1) / np.maximum(nb, floor))
+
+
+def _clustered(n, seed):
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.random((3, n)) ** np.array([[1.0], [3.0], [5.0]])
+    m = 1.0 - rng.random(n)
+    return x, y, z, m
+
+
+def _case(kind, n, seed):
+    if kind == "plummer":
+        return W.plummer(n, seed), (-1, -1, -1), (1, 1, 1)
+    return _clustered(n, seed), (0, 0, 0), (1, 1, 1)
+
+
+@pytest.mark.parametrize("kind,n,q,thr", [("clustered", 3000, 6, 0),
+                                          ("clustered", 5000, 8, 20),
+                                          ("plummer", 20000, 32, 0),
+                                          ("plummer", 20000, 32, 145)])
+def test_semifar_lists_bit_exact(torch_cuda, kind, n, q, thr):
+    torch = torch_cuda
+    from aran_amd.fmm import Fmm3d
+    (x, y, z, m), lb, ub = _case(kind, n, 3)
+    f = Fmm3d()
+    f.set_semifar(thr)
+    f.solve(*_dev(torch, x, y, z, m), lb, ub, 4, q)
+    _, onodes = O.tree(x, y, z, np.array(lb, float), np.array(ub, float), q)
+    (fs, fd), (na, nb), (sl, sd) = O.traverse_semifar(onodes, thr)
+    assert len(sl) > 0
+    lists = f.export_lists()
+    semi = f.export_semifar()
+    leaves = lists["leaves"]
+    # per leaf: its M2P nodes; per node: its P2L leaves; both == the pairs
+    gw, gx = set(), set()
+    wo = semi["w_ofs"]
+    for t, A in enumerate(leaves):
+        for e in range(wo[t], wo[t + 1]):
+            key = (int(A), int(semi["w_node"][e]))
+            assert key not in gw
+            gw.add(key)
+    xo = semi["x_ofs"]
+    for D in range(len(xo) - 1):
+        for e in range(xo[D], xo[D + 1]):
+            key = (int(semi["x_leaf"][e]), D)
+            assert key not in gx
+            gx.add(key)
+    ref = set(zip(sl.tolist(), sd.tolist()))
+    assert gw == ref and gx == ref and len(ref) == len(sl)
+    # near pairs: every leaf-pair of the oracle, each listed once per side
+    leaf_of = np.full(n, -1)
+    for i in np.nonzero(onodes["isleaf"])[0]:
+        leaf_of[onodes["pbegin"][i]:onodes["pend"][i]] = i
+    gnear = set()
+    no = lists["near_ofs"]
+    for t, A in enumerate(leaves):
+        for e in range(no[t], no[t + 1]):
+            b, en = lists["near_begin"][e], lists["near_end"][e]
+            for B in np.unique(leaf_of[b:en]):
+                key = (int(A), int(B))
+                assert key not in gnear
+                gnear.add(key)
+    ref_near = set()
+    for a, b in zip(na.tolist(), nb.tolist()):
+        ref_near.add((a, b))
+        ref_near.add((b, a))
+    assert gnear == ref_near
+    st = f.stats()
+    assert st["semifar_pairs"] == len(sl)
+
+
+@pytest.mark.parametrize("kind,n,p,q,thr", [("clustered", 4000, 8, 8, 0),
+                                            ("plummer", 12000, 10, 32, 145),
+                                            ("clustered", 3000, 15, 10, 30),
+                                            ("plummer", 5000, 1, 16, 0)])
+def test_semifar_solve_matches_oracle(torch_cuda, kind, n, p, q, thr):
+    """FP64 solve with semifar pairs vs the oracle FMM with the same
+    threshold: 1e-10 pointwise (potential and gradient); counters (p2p, p2l,
+    m2p and the rest) identical."""
+    torch = torch_cuda
+    from aran_amd.fmm import Fmm3d
+    (x, y, z, m), lb, ub = _case(kind, n, 9)
+    f = Fmm3d()
+    f.set_semifar(thr)
+    pot, grad = f.solve(*_dev(torch, x, y, z, m), lb, ub, p, q)
+    pot, grad = pot.cpu().numpy(), grad.cpu().numpy()
+    opot, ograd, ost = O.fmm_solve(x, y, z, m, np.array(lb, float),
+                                   np.array(ub, float), p, q, semifar=thr)
+    assert _relerr_pot(pot, opot) <= 1e-10
+    assert _relerr_grad(grad, ograd) <= 1e-10
+    st = f.stats()
+    for i, k in enumerate(("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p",
+                           "p2l", "m2p")):
+        assert st[k] == ost[i], k
+    assert st["p2l"] > 0
+
+
+def test_semifar_mixed_form_against_direct_sum(torch_cuda):
+    """Mixed form (FP32 M2L / P2P) with semifar pairs at the cost-model
+    threshold: Plummer N=2^17, p=10, leaf 64; potential L2 error against the
+    FP64 direct sum on 4096 targets <= 1e-6, gradient <= 1e-5."""
+    torch = torch_cuda
+    from aran_amd.fmm import Fmm3d
+    n = 1 << 17
+    x, y, z, m = W.plummer(n, 77)
+    m = np.full(n, 1.0 / n)
+    dev = _dev(torch, x, y, z, m)
+    f = Fmm3d()
+    f.set_precision("mixed")
+    f.set_semifar(Fmm3d.semifar_threshold_auto(10))
+    pot, grad = f.solve(*dev, (-1, -1, -1), (1, 1, 1), 10, 64)
+    assert f.stats()["p2l"] > 0
+    tg = W.sample_targets(n, 4096, 77)
+    dpot, dgrad = f.direct(*dev, torch.from_numpy(tg))
+    dpot, dgrad = dpot.cpu().numpy(), dgrad.cpu().numpy()
+    pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
+    assert np.linalg.norm(pp - dpot) / np.linalg.norm(dpot) <= 1e-6
+    assert np.linalg.norm(gg - dgrad) / np.linalg.norm(dgrad) <= 1e-5
+
+
+@pytest.mark.parametrize("p", [1, 6, 10, 15])
+def test_p2l_m2p_operators_match_oracle(torch_cuda, p):
+    """aran_development3d_p2l / multipole_evaluate / multipole_gradient_
+    evaluate (one particle, on the GPU) vs the oracle at 1e-12."""
+    from aran_amd import solver as S
+    L = S.lib()
+    rng = np.random.default_rng(40 + p)
+    c = np.array([0.2, -0.1, 0.05])
+    ni = S.node_info(c)
+    d = L.aran_development3d_new(0, p)
+    # multipole of a few charges, set through P2M
+    for _ in range(4):
+        pos = c + rng.uniform(-0.3, 0.3, 3)
+        L.aran_development3d_p2m(C.byref(S.VsgVector3d(*pos)),
+                                 float(rng.uniform(0.1, 1)), C.byref(ni), d)
+    mc, lc = O.tri(p), O.tri(p + 1)
+    M = np.ctypeslib.as_array(d.contents.multipole, shape=(2 * mc,)).copy()
+    for _ in range(5):
+        t = c + rng.normal(size=3) * 2.0
+        v = L.aran_development3d_multipole_evaluate(
+            C.byref(ni), d, C.byref(S.VsgVector3d(*t)))
+        ref = O.multipole_eval(p, M.view(np.complex128), c, t)
+        assert abs(v.re - ref) <= 1e-12 * abs(ref)
+        g = S.VsgVector3d()
+        L.aran_development3d_multipole_gradient_evaluate(
+            C.byref(ni), d, C.byref(S.VsgVector3d(*t)), C.byref(g))
+        gref = O.multipole_grad(p, M.view(np.complex128), c, t)
+        assert np.linalg.norm(np.array([g.x, g.y, g.z]) - gref) <= \
+            1e-12 * np.linalg.norm(gref)
+    # P2L adds into the local
+    L.aran_development3d_set_zero(d)
+    Lref = np.zeros(lc, dtype=np.complex128)
+    for _ in range(3):
+        s = c + rng.normal(size=3) * 3.0
+        q = float(rng.uniform(-1, 1))
+        L.aran_development3d_p2l(C.byref(S.VsgVector3d(*s)), q, C.byref(ni), d)
+        O.p2l(p, s, q, c, Lref)
+    Lg = np.ctypeslib.as_array(d.contents.local, shape=(2 * lc,)).view(
+        np.complex128)
+    assert np.max(np.abs(Lg - Lref)) <= 1e-12 * np.max(np.abs(Lref))
+    L.aran_development3d_free(d)
+
+
+def test_solver_api_semifar_newtonpot3_style(torch_cuda):
+    """AranSolver3d with aran_solver3d_set_functions_full and the library
+    P2L / M2P callbacks (tests/newtonpot3.c:620-632 with -semifar): an
+    explicit threshold matches the oracle at 1e-10 with its p2l / m2p
+    counters; threshold 0 takes the automatic one; G_MAXUINT gives the solve
+    without semifar pairs."""
+    from aran_amd import solver as S
+    from aran_amd.fmm import Fmm3d
+    L = S.lib()
+    n, p, q = 3000, 8, 10
+    x, y, z, m = _clustered(n, 21)
+    pts = (S.PointAccumBoth * n)()
+    for i in range(n):
+        pts[i].vector = S.VsgVector3d(x[i], y[i], z[i])
+        pts[i].density = m[i]
+    lb, ub = S.VsgVector3d(0, 0, 0), S.VsgVector3d(1, 1, 1)
+    tree = L.vsg_prtree3d_new_full(C.byref(lb), C.byref(ub),
+                                   S.fn("vsg_vector3d_vector3d_locfunc"),
+                                   S.fn("vsg_vector3d_dist"), None, q)
+    dev = L.aran_development3d_new(0, p)
+    solver = L.aran_solver3d_new(tree, L.aran_development3d_get_type(),
+                                 C.cast(dev, C.c_void_p),
+                                 S.fn("aran_development3d_set_zero"))
+    L.aran_solver3d_set_particle_layout(solver, S.PointAccumBoth.density.offset,
+                                        S.PointAccumBoth.pot.offset, 0,
+                                        S.PointAccumBoth.field.offset)
+    for i in range(n):
+        L.aran_solver3d_insert_point(solver, C.addressof(pts[i]))
+
+    def run(thr):
+        for i in range(n):
+            pts[i].pot = 0.0
+            pts[i].field = S.VsgVector3d(0, 0, 0)
+        L.aran_solver3d_reinit_stats(solver)
+        L.aran_solver3d_set_functions_full(
+            solver, S.fn("aran_newton3d_p2p_potfield"),
+            S.fn("aran_newton3d_p2m"), S.fn("aran_development3d_m2m_rotate"),
+            S.fn("aran_development3d_m2l_rotate"),
+            S.fn("aran_development3d_l2l_rotate"),
+            S.fn("aran_newton3d_l2p_potfield"), S.fn("aran_newton3d_p2l"),
+            S.fn("aran_newton3d_m2p_potfield"), thr)
+        L.aran_solver3d_solve(solver)
+        assert L.aran_solver3d_status(solver) == 0
+        pot = np.array([pts[i].pot for i in range(n)])
+        grad = np.array([[pts[i].field.x, pts[i].field.y, pts[i].field.z]
+                         for i in range(n)])
+        return pot, grad, S.get_stats(solver)
+
+    for thr, othr in ((5, 5), (0, Fmm3d.semifar_threshold_auto(p)),
+                      (0xFFFFFFFF, None)):
+        pot, grad, st = run(thr)
+        opot, ograd, ost = O.fmm_solve(x, y, z, m, np.zeros(3), np.ones(3), p,
+                                       q, semifar=othr)
+        assert _relerr_pot(pot, opot) <= 1e-10
+        assert _relerr_grad(grad, ograd) <= 1e-10
+        keys = ("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p", "p2l", "m2p")
+        for i, k in enumerate(keys[:len(ost)]):
+            assert st[k] == ost[i], k
+        if othr is None:
+            assert st["p2l"] == 0 and st["m2p"] == 0
+        elif thr == 5:
+            assert st["p2l"] > 0
+    # a foreign M2P callback is refused
+    L.aran_solver3d_set_functions_full(
+        solver, S.fn("aran_newton3d_p2p_potfield"), S.fn("aran_newton3d_p2m"),
+        S.fn("aran_development3d_m2m_rotate"),
+        S.fn("aran_development3d_m2l_rotate"),
+        S.fn("aran_development3d_l2l_rotate"),
+        S.fn("aran_newton3d_l2p_potfield"), S.fn("aran_newton3d_p2l"),
+        S.fn("aran_newton3d_m2p_pot"), 5)
+    L.aran_solver3d_solve(solver)
+    assert L.aran_solver3d_status(solver) != 0
+    L.aran_solver3d_free(solver)
