@@ -23,6 +23,9 @@ SOURCES = [("tree.hip", "tree", []), ("operators.hip", "operators", []),
            ("server.cpp", "server", [])] + [
     ("m2l_tile.hip", "m2l_tile_%d" % a,
      ["-DARAN_M2L_PMIN=%d" % a, "-DARAN_M2L_PMAX=%d" % b])
+    for a, b in M2L_RANGES] + [
+    ("m2l_mfma.hip", "m2l_mfma_%d" % a,
+     ["-DARAN_MFMA_PMIN=%d" % a, "-DARAN_MFMA_PMAX=%d" % b])
     for a, b in M2L_RANGES]
 ARCH = "gfx950"
 

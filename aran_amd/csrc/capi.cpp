@@ -255,7 +255,7 @@ extern "C" long aran_fmm3d_semifar_threshold_auto(int order) {
 
 
 extern "C" int aran_fmm3d_set_precision(AranFmm3d *ctx, int precision) {
-  if (!ctx || (precision != ARAN_FMM3D_PLAIN && precision != ARAN_FMM3D_MIXED)) {
+  if (!ctx || precision < ARAN_FMM3D_PLAIN || precision > ARAN_FMM3D_BF16X6) {
     aran::set_error("aran_fmm3d_set_precision: invalid precision");
     return ARAN_FMM3D_EINVAL;
   }

@@ -86,6 +86,10 @@ struct Tables {
       dscr, thetas;
   // tiled far field (mixed form): FP32 copies and octant offset lists
   DevBuf rot32, ephi32, vfac32, oct_codes, code_to_k;
+  // MFMA far field: dense BF16 operators of every offset code (built on
+  // first use for this order)
+  DevBuf mfma_ops;
+  int mfma_order = -1;
   std::vector<DirEntry> dir_host;
   OpTables view() const;
 };
@@ -104,7 +108,7 @@ struct Workspace {
   long semifar_nodes = -1;  // nodes covered by the semifar lists, -1: none
   DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
-      tile_meta, mult32, loc32;
+      tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma;
   DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos;
   long node_cap = 0;
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
@@ -134,7 +138,9 @@ struct Problem {
   double *pot = nullptr;   // device, user order, may be null
   double *grad = nullptr;  // device, user order [n][3], may be null
   bool timings = false;
-  int precision = 0;  // 0: plain FP64, 1: mixed (FP32 far field and P2P)
+  // 0: plain FP64; 1: mixed (FP32 far field on VALU, FP32 P2P); 2 / 3: far
+  // field on MFMA with BF16 operands in 3 / 6 products, FP32 P2P
+  int precision = 0;
   // semifar pairs (P2L / M2P) for nodes holding more than this many
   // particles; < 0: none (every leaf / deeper node pair is P2P)
   long semifar = -1;
@@ -153,6 +159,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti);
 // CSR far lists (v_ofs, v_src, v_code) of the last tree, from its colleagues.
 hipError_t build_far_lists(Workspace &ws, long nn, int *nfar);
 hipError_t ensure_tables(Workspace &ws, int order);
+hipError_t ensure_mfma_tables(Workspace &ws, int order);
 hipError_t workspace_init(Workspace &ws, hipStream_t stream);
 const char *last_error();
 void set_error(const std::string &msg);
@@ -255,6 +262,9 @@ hipError_t launch_p2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const int *xofs, const int *xleaf, const double *xs,
                       const double *ys, const double *zs, const double *qs,
                       double2 *L, hipStream_t s);
+
+// Dense scaled M2L operators O[343][2lc-p-1][2mc-p] in FP64.
+hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s);
 
 hipError_t launch_translate_one(int kind, const OpTables &tb, double r,
                                 const double2 *src, double2 *dst,

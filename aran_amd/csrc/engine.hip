@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "m2l_mfma.h"
 #include "m2l_tile.h"
 
 namespace aran {
@@ -132,7 +133,8 @@ Workspace::~Workspace() {
                    &tab.tl2l, &tab.ynorm, &tab.ogain, &tab.pgain, &tab.igain, &tab.xgain, &tab.dscr, &tab.thetas, &tab.rot32,
                    &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
                    &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
-                   &tile_rank, &tile_meta, &mult32, &loc32, &n_sys, &sys_box, &sys_off, &sys_mm,
+                   &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
+                   &mult_bf, &loc_mfma, &tab.mfma_ops, &n_sys, &sys_box, &sys_off, &sys_mm,
                    &skey, &spos};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
@@ -443,6 +445,26 @@ hipError_t build_far_lists(Workspace &ws, long nn, int *nfar) {
   return hipSuccess;
 }
 
+hipError_t ensure_mfma_tables(Workspace &ws, int order) {
+  CK(ensure_tables(ws, order));
+  Tables &T = ws.tab;
+  if (T.mfma_order == order) return hipSuccess;
+  hipStream_t s = ws.stream;
+  const MfmaDims D = mfma_dims(order);
+  DevBuf tmp;
+  hipError_t e = tmp.reserve(sizeof(double) * (size_t)kNumCodes * D.ro * D.ki);
+  if (e == hipSuccess) e = launch_m2l_dense(T.view(), tmp.as<double>(), s);
+  if (e == hipSuccess)
+    e = T.mfma_ops.reserve(16 * (size_t)kNumCodes * mfma_ops_units(order));
+  if (e == hipSuccess)
+    e = launch_mfma_pack_ops(order, tmp.as<double>(), T.mfma_ops.ptr, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  tmp.release();
+  if (e != hipSuccess) return e;
+  T.mfma_order = order;
+  return hipSuccess;
+}
+
 hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   const long n = pb.n;
   const int p = pb.order;
@@ -454,8 +476,14 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     set_error("leaf capacity must be >= 1 and 0 <= n <= 2^30");
     return hipErrorInvalidValue;
   }
+  if (pb.precision < 0 || pb.precision > 3) {
+    set_error("precision must be 0 (plain), 1 (mixed), 2 (BF16x3) or 3 "
+              "(BF16x6)");
+    return hipErrorInvalidValue;
+  }
   CK(workspace_init(ws, ws.stream));
   CK(ensure_tables(ws, p));
+  if (pb.precision >= 2) CK(ensure_mfma_tables(ws, p));
   hipStream_t s = ws.stream;
   Stats local_st;
   Stats &S = st ? *st : local_st;
@@ -686,7 +714,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (semifar)
     CK(launch_sum_ll(ws.s_part.as<long long>(), nleaves, d_spart, s));
   // ---- tiles of the far field (mixed form)
-  int ntiles = 0;
+  int ntiles = 0, nwg = 0;
   if (pb.precision != 0 && depth >= 2) {
     // Tiles: 64 targets of one (level, octant) group each, in Morton order
     // inside the group; within a level the tiles take chunk j of every
@@ -741,6 +769,30 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.node_slot.reserve(sizeof(int) * nn));
     CK(hipMemcpyAsync(ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles,
                       hipMemcpyHostToDevice, s));
+    // MFMA far field: workgroups of consecutive chunks of one (level,
+    // octant) group (they share every offset code), in the order level,
+    // chunk block, octant (neighbouring workgroups are neighbours in space)
+    std::vector<int> wgt;
+    if (pb.precision >= 2) {
+      const int tpw = mfma_wg_tiles(p);
+      for (int d = 2; d <= depth; d++) {
+        int maxch = 0;
+        for (int o = 0; o < 8; o++)
+          maxch = std::max(maxch, (hcnt[d * 8 + o] + 63) / 64);
+        for (int jb = 0; jb < maxch; jb += tpw)
+          for (int o = 0; o < 8; o++) {
+            const int nch = (hcnt[d * 8 + o] + 63) / 64;
+            if (jb >= nch) continue;
+            for (int j = jb; j < jb + tpw; j++)
+              wgt.push_back(j < nch ? tile_of[gofs[d * 8 + o] + j] : -1);
+          }
+      }
+      nwg = (int)(wgt.size() / tpw);
+      CK(ws.tile_perm.reserve(sizeof(int) * std::max<size_t>(1, wgt.size())));
+      if (!wgt.empty())
+        CK(hipMemcpyAsync(ws.tile_perm.ptr, wgt.data(),
+                          sizeof(int) * wgt.size(), hipMemcpyHostToDevice, s));
+    }
     CK(hipMemcpyAsync(d_meta + kMaxDepth + 2, meta.data() + kMaxDepth + 2,
                       sizeof(int) * (meta.size() - kMaxDepth - 2),
                       hipMemcpyHostToDevice, s));
@@ -775,6 +827,26 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
                   ws.v_code.as<short>(), M, L, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+  } else if (pb.precision >= 2) {
+    const MfmaDims D = mfma_dims(p);
+    const int nsplit = m2l_mfma_split(nwg, p);
+    CK(ws.mult_bf.reserve(16 * (size_t)(nn + 1) * mfma_mult_units(p)));
+    CK(ws.loc_mfma.reserve(sizeof(float) * (size_t)nsplit *
+                           std::max(1, ntiles) * 64 * D.ro));
+    CK(launch_mfma_pack_mult(nd, nn, p, M, ws.mult_bf.ptr, s));
+    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+    if (ntiles > 0)
+      CK(launch_m2l_mfma(pb.precision == 2 ? 3 : 6, p, nwg,
+                         ws.tile_perm.as<int>(), ntiles, ws.tile_src.as<int>(),
+                         ws.tile_oct.as<int>(), ws.tab.oct_codes.as<short>(),
+                         ws.tab.mfma_ops.ptr, ws.mult_bf.ptr, nn, nsplit,
+                         ws.loc_mfma.as<float>(), s));
+    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+    const int first = depth >= 2 ? lstart[2] : nn;
+    CK(launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
+                           ws.loc_mfma.as<float>(), nsplit, L, s));
+    S.m2l_split = nsplit;
+    S.m2l_tiles = ntiles;
   } else {
     CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
     const int nsplit = m2l_choose_split(ntiles, p);

@@ -7,6 +7,7 @@
 #include <cstdlib>
 
 #include "engine.h"
+#include "m2l_mfma.h"
 #include "m2l_tile.h"
 
 namespace aran {
@@ -31,6 +32,33 @@ hipError_t launch_m2l_tile_12(int p, int ntiles, const M2LTileArgs &a,
                               const void *Ms, void *Ls, hipStream_t s);
 hipError_t launch_m2l_tile_14(int p, int ntiles, const M2LTileArgs &a,
                               const void *Ms, void *Ls, hipStream_t s);
+
+#define ARAN_MFMA_DECL(n)                                                    \
+  hipError_t launch_m2l_mfma_##n(int nprod, int p, int nwg, const int *wgt,  \
+                                 int ntiles, const int *tsrc,                \
+                                 const int *toct, const short *codes,        \
+                                 const void *ops, const void *mb,            \
+                                 int zero_node, int nsplit, float *Lsm,      \
+                                 hipStream_t s);
+ARAN_MFMA_DECL(1)
+ARAN_MFMA_DECL(7)
+ARAN_MFMA_DECL(10)
+ARAN_MFMA_DECL(12)
+ARAN_MFMA_DECL(14)
+
+hipError_t launch_m2l_mfma(int nprod, int p, int nwg, const int *wg_tiles,
+                           int ntiles, const int *tsrc, const int *toct,
+                           const short *codes, const void *ops,
+                           const void *mb, int zero_node, int nsplit,
+                           float *Lsm, hipStream_t s) {
+  if (nwg <= 0 || ntiles <= 0) return hipSuccess;
+  if (nprod != 3 && nprod != 6) return hipErrorInvalidValue;
+  auto f = p <= 6 ? launch_m2l_mfma_1 : p <= 9 ? launch_m2l_mfma_7
+           : p <= 11 ? launch_m2l_mfma_10 : p <= 13 ? launch_m2l_mfma_12
+                                                    : launch_m2l_mfma_14;
+  return f(nprod, p, nwg, wg_tiles, ntiles, tsrc, toct, codes, ops, mb,
+           zero_node, nsplit, Lsm, s);
+}
 
 hipError_t launch_m2l_tile(int precision, int p, int ntiles,
                            const M2LTileArgs &a, const void *Ms, void *Ls,
@@ -351,6 +379,224 @@ hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int p,
         nd, p, ntiles, tile_tgt, static_cast<const float2 *>(Ls), nsplit, L);
   return hipGetLastError();
 }
+// ------------------------------------------------------------ MFMA form
+
+__device__ inline unsigned bf16_rn(float f) {  // round to nearest even
+  unsigned u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+
+// x = h + m + l in BF16 (after rounding x to FP32; each residual is exact)
+__device__ inline void split3(double x, unsigned &h, unsigned &m,
+                              unsigned &l) {
+  const float f = (float)x;
+  h = bf16_rn(f);
+  const float r1 = f - __uint_as_float(h << 16);
+  m = bf16_rn(r1);
+  const float r2 = r1 - __uint_as_float(m << 16);
+  l = bf16_rn(r2);
+}
+
+// 8 values -> three 16-byte BF16 parts
+__device__ inline void pack8(const double *v, uint4 *dst) {
+  unsigned hw[4] = {0, 0, 0, 0}, mw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+  for (int j = 0; j < 8; j++) {
+    unsigned h, m, l;
+    split3(v[j], h, m, l);
+    const int sh = (j & 1) * 16;
+    hw[j / 2] |= h << sh;
+    mw[j / 2] |= m << sh;
+    lw[j / 2] |= l << sh;
+  }
+  dst[0] = make_uint4(hw[0], hw[1], hw[2], hw[3]);
+  dst[1] = make_uint4(mw[0], mw[1], mw[2], mw[3]);
+  dst[2] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+}
+
+// ------------------------------------------------------------ tables
+
+// One thread per (code, ks, rb, lane): 8 operator entries of row
+// 32 rb + lane % 32, columns 16 ks + 8 (lane / 32) + j; table layout
+// [code][ks][rb][part][lane] (16-byte units), so that one k step of one
+// code is contiguous.
+__global__ void k_mfma_pack_ops(int p, const double *__restrict__ O,
+                                uint4 *__restrict__ ops) {
+  const MfmaDims D = mfma_dims(p);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  const long total = (long)kNumCodes * D.rb * D.ks * 64;
+  if (t >= total) return;
+  const int lane = (int)(t % 64);
+  const int rb = (int)((t / 64) % D.rb);
+  const int s = (int)((t / 64 / D.rb) % D.ks);
+  const int code = (int)(t / 64 / D.rb / D.ks);
+  const int row = 32 * rb + lane % 32;
+  double v[8];
+  for (int j = 0; j < 8; j++) {
+    const int k = 16 * s + 8 * (lane / 32) + j;
+    v[j] = (row < D.ro && k < D.ki)
+               ? O[((size_t)code * D.ro + row) * D.ki + k]
+               : 0.;
+  }
+  uint4 parts[3];
+  pack8(v, parts);
+  uint4 *dst = ops + (((size_t)code * D.ks + s) * D.rb + rb) * 3 * 64 + lane;
+  for (int q = 0; q < 3; q++) dst[q * 64] = parts[q];
+}
+
+// One thread per (node, ks, half): 8 scaled multipole entries 16 ks +
+// 8 half + j of the real input vector.
+__global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
+                                 const double2 *__restrict__ M,
+                                 uint4 *__restrict__ mb) {
+  const MfmaDims D = mfma_dims(p);
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= (long)(nnodes + 1) * D.ks * 2) return;
+  const int node = (int)(t / (2 * D.ks));
+  const int rest = (int)(t % (2 * D.ks));
+  const int s = rest / 2, g = rest % 2;
+  if (node == nnodes) {  // the all-zero node that stands in for no source
+    const uint4 z = make_uint4(0, 0, 0, 0);
+    mb[t * 3] = z;
+    mb[t * 3 + 1] = z;
+    mb[t * 3 + 2] = z;
+    return;
+  }
+  const double w = node_width(nd, node);
+  const double2 *Mn = M + (size_t)node * D.mc;
+  double v[8];
+  for (int j = 0; j < 8; j++) {
+    const int k = 16 * s + 8 * g + j;
+    double x = 0.;
+    int l = 0, idx = -1;
+    bool im = false;
+    if (k < D.mc) {
+      while (tri(l + 1) <= k) l++;
+      idx = k;
+    } else if (k < D.ki) {
+      const int jj = k - D.mc;  // jj = tri(l-1) + m - 1
+      l = 1;
+      while (tri(l) <= jj) l++;
+      idx = tri(l) + (jj - tri(l - 1)) + 1;
+      im = true;
+    }
+    if (idx >= 0) {
+      double f = 1.;
+      for (int i = 0; i < l; i++) f /= w;
+      x = (im ? Mn[idx].y : Mn[idx].x) * f;
+    }
+    v[j] = x;
+  }
+  pack8(v, mb + t * 3);
+}
+
+hipError_t launch_mfma_pack_ops(int p, const double *O, void *ops,
+                                hipStream_t s) {
+  const MfmaDims D = mfma_dims(p);
+  const long total = (long)kNumCodes * D.rb * D.ks * 64;
+  k_mfma_pack_ops<<<(total + 255) / 256, 256, 0, s>>>(
+      p, O, static_cast<uint4 *>(ops));
+  return hipGetLastError();
+}
+
+hipError_t launch_mfma_pack_mult(const NodeArrays &nd, int nnodes, int p,
+                                 const double2 *M, void *mb, hipStream_t s) {
+  const long total = (long)(nnodes + 1) * mfma_dims(p).ks * 2;
+  k_mfma_pack_mult<<<(total + 255) / 256, 256, 0, s>>>(
+      nd, nnodes, p, M, static_cast<uint4 *>(mb));
+  return hipGetLastError();
+}
+
+
+// One workgroup per tile: sum the nsplit FP32 slices [split][tile][row][64]
+// in slice order, transpose through LDS and write L[node][k] =
+// (Re row k, Im row of (l, m >= 1)) / w^(l+1), kUnscaleRows coefficients at
+// a time.
+__global__ __launch_bounds__(256) void k_unscale_mfma(
+    NodeArrays nd, int p, int ntiles, const int *__restrict__ tile_tgt,
+    const float *__restrict__ Lsm, int nsplit, double2 *__restrict__ L) {
+  __shared__ double2 sl[64 * (kUnscaleRows + 1)];
+  __shared__ double sw[64];
+  __shared__ int snode[64];
+  const MfmaDims D = mfma_dims(p);
+  const int lc = D.lc;
+  const int row = kUnscaleRows + 1;  // odd row stride in LDS
+  const int tile = blockIdx.x;
+  const long slice = (long)ntiles * D.ro * 64;
+  const float *src = Lsm + (long)tile * D.ro * 64;
+  if (threadIdx.x < 64) {
+    const int node = tile_tgt[tile * 64 + threadIdx.x];
+    snode[threadIdx.x] = node;
+    sw[threadIdx.x] = node >= 0 ? node_width(nd, node) : 1.;
+  }
+  for (int k0 = 0; k0 < lc; k0 += kUnscaleRows) {
+    const int nk = min(kUnscaleRows, lc - k0);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
+      const int kk = e / 64, lane = e % 64, k = k0 + kk;
+      int l = 0;
+      while (tri_p(l + 1) <= k) l++;
+      const int m = k - tri_p(l);
+      const int rim = m > 0 ? lc + tri_p(l - 1) + m - 1 : -1;
+      double sx = 0., sy = 0.;
+      for (int s = 0; s < nsplit; s++) {
+        sx += (double)src[s * slice + (long)k * 64 + lane];
+        if (rim >= 0) sy += (double)src[s * slice + (long)rim * 64 + lane];
+      }
+      sl[lane * row + kk] = make_double2(sx, sy);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
+      const int lane = e / nk, kk = e % nk, k = k0 + kk;
+      const int node = snode[lane];
+      if (node < 0) continue;
+      int l = 0;
+      while (tri_p(l + 1) <= k) l++;
+      const double w = sw[lane];
+      double f = 1. / w;
+      for (int i = 0; i < l; i++) f /= w;
+      const double2 v = sl[lane * row + kk];
+      L[(long)node * lc + k] = make_double2(v.x * f, v.y * f);
+    }
+  }
+}
+
+hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
+                               int ntiles, const int *tile_tgt,
+                               const float *Lsm, int nsplit, double2 *L,
+                               hipStream_t s) {
+  const int lc = tri_p(p + 1);
+  const long nz = (long)first * lc;
+  if (nz > 0) k_zero_double2<<<(nz + 255) / 256, 256, 0, s>>>(L, nz);
+  if (ntiles <= 0) return hipGetLastError();
+  k_unscale_mfma<<<ntiles, 256, 0, s>>>(nd, p, ntiles, tile_tgt, Lsm, nsplit,
+                                        L);
+  return hipGetLastError();
+}
+
+// Offset ranges per workgroup of the MFMA far field: one workgroup of 8
+// waves fills a CU (two waves per SIMD), so the duration is modelled as
+// rounds of 256 workgroups times the offsets per range (plus two offsets'
+// worth for the output rows and the slice reduction).
+int m2l_mfma_split(int nwg, int p) {
+  (void)p;
+  if (const char *e = getenv("ARAN_M2L_SPLIT")) {
+    int v = atoi(e);
+    if (v >= 1 && v <= kOctOffsets) return v;
+  }
+  int best = 1;
+  long best_cost = -1;
+  for (int ns = 1; ns <= 24; ns++) {
+    const long rounds = ((long)std::max(1, nwg) * ns + 255) / 256;
+    const long cost = rounds * ((kOctOffsets + ns - 1) / ns + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = ns;
+    }
+  }
+  return best;
+}
+
 // ------------------------------------------------------------ tables
 
 // Per offset code: (-1)^(l+m) c^(l+n) T(l,m,n) / |o|^(l+n+1) in type T.

@@ -415,6 +415,96 @@ __global__ __launch_bounds__(kOpThreads) void k_m2l(
   }
 }
 
+// Dense scaled M2L operators: for offset code `code` (box units, width 1)
+// and real basis input k, column k of the real matrix mapping
+//   (Re M^_l^m for all (l,m), Im M^_l^m for m >= 1)   [2 mc - p inputs]
+// to (Re L^_l^m for all (l,m), Im L^_l^m for m >= 1)  [2 lc - p - 1 outputs]
+// through the same rotate / z-translate / rotate-back operator as k_m2l
+// (M^_n = M_n / w^n, L^_l = L_l w^(l+1), so w = 1).  Near codes (every
+// |component| <= 1) give zero columns.  O is [343][ro][ki] FP64.
+__global__ __launch_bounds__(kOpThreads) void k_m2l_dense(OpTables tb,
+                                                          double *__restrict__ O) {
+  __shared__ double2 U[kMaxMc], X[kMaxMc], Z[kMaxLc], Lo[kMaxLc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  const int p = tb.p, mc = tri(p), lc = tri(p + 1), rb = tb.rb;
+  const int ki = 2 * mc - p, ro = 2 * lc - p - 1;
+  const int code = blockIdx.x / ki, kb = blockIdx.x % ki;
+  const int dx = code / 49 - 3, dy = (code / 7) % 7 - 3, dz = code % 7 - 3;
+  double *col = O + (size_t)code * ro * ki + kb;
+  if (abs(dx) <= 1 && abs(dy) <= 1 && abs(dz) <= 1) {
+    for (int r = threadIdx.x; r < ro; r += kOpThreads) col[(size_t)r * ki] = 0.;
+    return;
+  }
+  // basis input: Re of coefficient kb (kb < mc) or Im of the (l, m >= 1)
+  // coefficient number kb - mc in tri order without m = 0
+  int bk = kb, bim = 0;
+  if (kb >= mc) {
+    int j = kb - mc, l = 1;
+    while (tri(l) <= j) l++;  // j = tri(l-1) + m - 1
+    bk = tri(l) + (j - tri(l - 1)) + 1;
+    bim = 1;
+  }
+  const DirEntry de = tb.dir[code];
+  const double2 *ph = tb.eiphi + code * (p + 1);
+  const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+  for (int k = threadIdx.x; k < mc; k += kOpThreads) {
+    int l, m;
+    lm_of(k, l, m);
+    const double2 b = make_double2(k == bk && !bim ? 1. : 0.,
+                                   k == bk && bim ? 1. : 0.);
+    U[k] = cmul(ph[m], b);
+  }
+  if (threadIdx.x < 2 * p + 2) {
+    const double rho = 1. / de.norm;
+    const int kk = threadIdx.x;
+    double v = 1.;
+    for (int i = 0; i < kk; i++) v *= rho;
+    if (kk > 0 && ((kk - 1) & 1)) v *= de.c;  // c^{l+n} with kk = l+n+1
+    rp[kk] = v;
+  }
+  __syncthreads();
+  rotate_fwd(R, rb, p - 1, U, X);
+  __syncthreads();
+  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
+    int l, m;
+    lm_of(k, l, m);
+    const double *T = tb.tm2l + ((size_t)l * (p + 1) + m) * p;
+    double re = 0., im = 0.;
+    for (int n = m; n < p; n++) {
+      const double f = T[n] * rp[l + n + 1];
+      re += f * X[tri(n) + m].x;
+      im += f * X[tri(n) + m].y;
+    }
+    if ((l + m) & 1) {
+      re = -re;
+      im = -im;
+    }
+    Z[k] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < lc; k += kOpThreads)
+    Lo[k] = rotate_inv_one(R, rb, ph, Z, k);
+  __syncthreads();
+  for (int r = threadIdx.x; r < ro; r += kOpThreads) {
+    double v;
+    if (r < lc) {
+      v = Lo[r].x;
+    } else {
+      const int j = r - lc;
+      int l = 1;
+      while (tri(l) <= j) l++;
+      v = Lo[tri(l) + (j - tri(l - 1)) + 1].y;
+    }
+    col[(size_t)r * ki] = v;
+  }
+}
+
+hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s) {
+  const int p = tb.p, ki = 2 * tri(p) - p;
+  k_m2l_dense<<<kNumCodes * ki, kOpThreads, 0, s>>>(tb, O);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------ L2L
 
 // One block per node of one level (depth >= 1): L(A) += T L(parent).

@@ -222,7 +222,7 @@ extern "C" AranServer3d *aran_server3d_new(double hbm_budget_bytes,
                                            int max_batch) {
   if (!(hbm_budget_bytes > 0.) || nworkers < 1 || nworkers > 8 ||
       max_batch < 1 ||
-      (precision != ARAN_FMM3D_PLAIN && precision != ARAN_FMM3D_MIXED)) {
+      precision < ARAN_FMM3D_PLAIN || precision > ARAN_FMM3D_BF16X6) {
     aran::set_error("aran_server3d_new: invalid argument");
     return nullptr;
   }

@@ -46,8 +46,11 @@ class Fmm3d:
     PLAIN, MIXED = 0, 1
 
     def set_precision(self, precision):
-        """"plain" (FP64 throughout) or "mixed" (FP32 M2L and P2P)."""
-        code = {"plain": 0, "mixed": 1}.get(precision, precision)
+        """"plain" (FP64 throughout), "mixed" (FP32 M2L and P2P), "bf16x3" /
+        "bf16x6" (M2L on MFMA with BF16 operands in 3 / 6 products, FP32
+        P2P)."""
+        code = {"plain": 0, "mixed": 1, "bf16x3": 2, "bf16x6": 3}.get(
+            precision, precision)
         check(lib().aran_fmm3d_set_precision(self._ctx, int(code)),
               "set_precision")
 

@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 MI355X_FP64_TFLOPS = 78.6
 MI355X_FP32_TFLOPS = 157.3
 MI355X_HBM_GBS = 8000.0
+MI355X_BF16_TFLOPS = 2500.0  # dense MFMA
 
 
 def rot_macs(L):
@@ -285,7 +286,9 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--order", type=int, default=10)
     ap.add_argument("--leafcap", type=int, default=64)
-    ap.add_argument("--precision", choices=("mixed", "plain"), default="mixed")
+    ap.add_argument("--precision",
+                    choices=("mixed", "plain", "bf16x3", "bf16x6"),
+                    default="mixed")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
@@ -337,7 +340,7 @@ def main():
     err = None if args.no_check else error_vs_direct(f, dev, pot, grad, n,
                                                      seed)
     plain = None
-    if args.precision == "mixed" and not args.no_plain:
+    if args.precision != "plain" and not args.no_plain:
         r = run_form(f, dev, pot, grad, lb, ub, p, q, "plain",
                      max(2, args.steps // 2), 1)
         plain = {"value": r["value"], "ms_per_step": r["ms_per_step"],
@@ -347,14 +350,22 @@ def main():
         plain["mixed_over_plain"] = head["value"] / r["value"]
 
     st = head["stats"]
-    mixed = args.precision == "mixed"
-    peak = MI355X_FP32_TFLOPS if mixed else MI355X_FP64_TFLOPS
+    form = args.precision
+    mixed = form != "plain"
+    # peak of the arithmetic the M2L kernel issues: FP64 VALU (plain), FP32
+    # VALU (mixed), dense BF16 MFMA (bf16x3 / bf16x6; the split products are
+    # not counted as algorithmic work, so frac is against the BF16 peak)
+    peak = {"plain": MI355X_FP64_TFLOPS, "mixed": MI355X_FP32_TFLOPS,
+            "bf16x3": MI355X_BF16_TFLOPS, "bf16x6": MI355X_BF16_TFLOPS}[form]
+    kname = {"plain": "k_m2l (FP64)", "mixed": "k_m2l_tile (FP32)",
+             "bf16x3": "k_m2l_mfma (BF16 MFMA, 3 products)",
+             "bf16x6": "k_m2l_mfma (BF16 MFMA, 6 products)"}[form]
     flops = st["m2l"] * m2l_flops(p)
     achieved = flops / (head["m2l_ms"] * 1e-3) / 1e12
     workload = "plummer_2p20_p10_q64_%s" % args.precision
     tr = measured_traffic(workload) if (n, p, q) == (1 << 20, 10, 64) else None
     roof = {"bound": "mfma",
-            "kernel": "k_m2l_tile (FP32)" if mixed else "k_m2l (FP64)",
+            "kernel": kname,
             "achieved": round(achieved, 4), "peak": peak, "unit": "TFLOP/s",
             "frac": round(achieved / peak, 5),
             "traffic": tr["bytes_per_launch"] if tr else None,
@@ -388,13 +399,22 @@ def main():
         "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32" if mixed else "f64",
+        "dtype": {"plain": "f64", "mixed": "f32", "bf16x3": "bf16x3",
+                  "bf16x6": "bf16x6"}[form],
         "data": "synthetic (seeded Plummer sphere, a=0.1, r<=1)",
         "config": {"workload": workload,
                    "n": n, "order": p, "leafcap": q,
-                   "form": ("mixed: FP32 M2L and P2P on per-level scaled "
-                            "expansions, FP64 elsewhere") if mixed else
-                   "plain FP64", "outputs": "potential+gradient"},
+                   "form": {
+                       "plain": "plain FP64",
+                       "mixed": "mixed: FP32 M2L and P2P on per-level "
+                                "scaled expansions, FP64 elsewhere",
+                       "bf16x3": "M2L on MFMA, BF16 operands in 3 products "
+                                 "with FP32 accumulation; FP32 P2P; FP64 "
+                                 "elsewhere",
+                       "bf16x6": "M2L on MFMA, BF16 operands in 6 products "
+                                 "with FP32 accumulation; FP32 P2P; FP64 "
+                                 "elsewhere"}[form],
+                   "outputs": "potential+gradient"},
         "roofline": roof,
         "cpu_baseline": cpu,
         "error": err,

@@ -55,6 +55,13 @@ int aran_fmm3d_solve_batch (AranFmm3d *ctx, int nsys, const long *offsets,
  * and accumulation across staged source passes). */
 #define ARAN_FMM3D_PLAIN 0
 #define ARAN_FMM3D_MIXED 1
+/* Far field on MFMA: per offset code the M2L is one dense real matrix on
+ * the scaled expansions, applied with BF16 operands split in parts
+ * (x = h + m + l) and FP32 accumulation: 3 products (h h + h m + m h) or
+ * 6 products (+ h l + l h + m m, FP32-level accuracy); P2P in FP32 as in
+ * ARAN_FMM3D_MIXED. */
+#define ARAN_FMM3D_BF16X3 2
+#define ARAN_FMM3D_BF16X6 3
 int aran_fmm3d_set_precision (AranFmm3d *ctx, int precision);
 
 /* Counters of the last solve: zero, p2p, p2m, m2m, m2l, l2l, l2p (the
