@@ -140,6 +140,7 @@ struct _AranFmm3d {
   bool timings = false;
   int precision = ARAN_FMM3D_PLAIN;
   long semifar = -1;
+  bool autotune = false;
   long n_last = 0;
 };
 
@@ -187,6 +188,7 @@ extern "C" int aran_fmm3d_solve(AranFmm3d *ctx, long n, const double *x,
   pb.timings = ctx->timings;
   pb.precision = ctx->precision;
   pb.semifar = ctx->semifar;
+  pb.autotune = ctx->autotune;
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
   return code_of_err(e);
@@ -224,6 +226,7 @@ extern "C" int aran_fmm3d_solve_batch(AranFmm3d *ctx, int nsys,
   pb.timings = ctx->timings;
   pb.precision = ctx->precision;
   pb.semifar = ctx->semifar;
+  pb.autotune = ctx->autotune;
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
   return code_of_err(e);
@@ -236,6 +239,12 @@ extern "C" int aran_fmm3d_get_stats(const AranFmm3d *ctx, long *o) {
                 s.l2l,    s.l2p,     (long)s.depth, s.nnodes,     s.nleaves,
                 ctx->ti.nfar, ctx->ti.nnear, s.p2l, s.m2p,        s.semifar};
   memcpy(o, v, sizeof v);
+  return 0;
+}
+
+extern "C" int aran_fmm3d_set_autotune(AranFmm3d *ctx, int on) {
+  if (!ctx) return ARAN_FMM3D_EINVAL;
+  ctx->autotune = on != 0;
   return 0;
 }
 

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -106,6 +107,8 @@ struct Workspace {
   // of its P2L (x_*); s_part: per-leaf P2L/M2P particle counts
   DevBuf w_cnt, w_ofs, w_node, x_cnt, x_ofs, x_leaf, s_part;
   long semifar_nodes = -1;  // nodes covered by the semifar lists, -1: none
+  // autotuned far-field splits, by (form, order, size bucket)
+  std::map<long, int> tuned_split;
   DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
       tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma;
@@ -144,6 +147,8 @@ struct Problem {
   // semifar pairs (P2L / M2P) for nodes holding more than this many
   // particles; < 0: none (every leaf / deeper node pair is P2P)
   long semifar = -1;
+  // time the far-field split candidates on first use (see tune_split)
+  bool autotune = false;
 };
 
 struct TreeInfo {

@@ -445,6 +445,58 @@ hipError_t build_far_lists(Workspace &ws, long nn, int *nfar) {
   return hipSuccess;
 }
 
+// ---- far-field split autotuner
+//
+// The offset range count (nsplit) of the tiled far-field kernels trades
+// parallelism against partial-sum traffic; the best value depends on the
+// form, the order and the number of tiles or workgroups.  With
+// Problem::autotune the first solve of each (form, order, size bucket) times
+// every candidate on the device (HIP events on the engine stream) and keeps
+// the fastest for later solves of that bucket.
+constexpr int kTuneMaxSplit = 16;
+static const int kTuneCandidates[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
+
+static long tune_key(int precision, int p, int units) {
+  int bucket = 0;  // units rounded down to a power of two
+  while ((2 << bucket) <= units) bucket++;
+  return ((long)precision * 64 + p) * 64 + bucket;
+}
+
+template <class Run>
+static hipError_t tune_split(Workspace &ws, long key, Run run, int *nsplit) {
+  auto it = ws.tuned_split.find(key);
+  if (it != ws.tuned_split.end()) {
+    *nsplit = it->second;
+    return hipSuccess;
+  }
+  hipStream_t s = ws.stream;
+  hipEvent_t e0, e1;
+  CK(hipEventCreate(&e0));
+  CK(hipEventCreate(&e1));
+  float best_ms = -1.f;
+  int best = *nsplit;
+  hipError_t err = run(best);  // warm the caches once
+  for (int c : kTuneCandidates) {
+    if (err != hipSuccess) break;
+    float ms = 0.f;
+    if ((err = hipEventRecord(e0, s)) != hipSuccess) break;
+    if ((err = run(c)) != hipSuccess) break;
+    if ((err = hipEventRecord(e1, s)) != hipSuccess) break;
+    if ((err = hipEventSynchronize(e1)) != hipSuccess) break;
+    if ((err = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) break;
+    if (best_ms < 0.f || ms < best_ms) {
+      best_ms = ms;
+      best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (err != hipSuccess) return err;
+  ws.tuned_split[key] = best;
+  *nsplit = best;
+  return hipSuccess;
+}
+
 hipError_t ensure_mfma_tables(Workspace &ws, int order) {
   CK(ensure_tables(ws, order));
   Tables &T = ws.tab;
@@ -829,31 +881,43 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
   } else if (pb.precision >= 2) {
     const MfmaDims D = mfma_dims(p);
-    const int nsplit = m2l_mfma_split(nwg, p);
-    CK(ws.mult_bf.reserve(16 * (size_t)(nn + 1) * mfma_mult_units(p)));
-    CK(ws.loc_mfma.reserve(sizeof(float) * (size_t)nsplit *
-                           std::max(1, ntiles) * 64 * D.ro));
-    CK(launch_mfma_pack_mult(nd, nn, p, M, ws.mult_bf.ptr, s));
-    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
-    if (ntiles > 0)
-      CK(launch_m2l_mfma(pb.precision == 2 ? 3 : 6, p, nwg,
-                         ws.tile_perm.as<int>(), ntiles, ws.tile_src.as<int>(),
-                         ws.tile_oct.as<int>(), ws.tab.oct_codes.as<short>(),
-                         ws.tab.mfma_ops.ptr, ws.mult_bf.ptr, nn, nsplit,
-                         ws.loc_mfma.as<float>(), s));
-    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     const int first = depth >= 2 ? lstart[2] : nn;
+    auto run = [&](int ns) {
+      return launch_m2l_mfma(pb.precision == 2 ? 3 : 6, p, nwg,
+                             ws.tile_perm.as<int>(), ntiles,
+                             ws.tile_src.as<int>(), ws.tile_oct.as<int>(),
+                             ws.tab.oct_codes.as<short>(), ws.tab.mfma_ops.ptr,
+                             ws.mult_bf.ptr, nn, ns, ws.loc_mfma.as<float>(),
+                             s);
+    };
+    // the tuner times the far field with its slice reduction
+    auto run_all = [&](int ns) {
+      hipError_t e = run(ns);
+      if (e != hipSuccess) return e;
+      return launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
+                                 ws.loc_mfma.as<float>(), ns, L, s);
+    };
+    CK(ws.mult_bf.reserve(16 * (size_t)(nn + 1) * mfma_mult_units(p)));
+    const size_t slice = sizeof(float) * std::max(1, ntiles) * 64 * D.ro;
+    CK(launch_mfma_pack_mult(nd, nn, p, M, ws.mult_bf.ptr, s));
+    int nsplit = m2l_mfma_split(nwg, p);
+    if (pb.autotune && ntiles > 0) {
+      CK(ws.loc_mfma.reserve(slice * kTuneMaxSplit));
+      CK(tune_split(ws, tune_key(pb.precision, p, nwg), run_all, &nsplit));
+    }
+    CK(ws.loc_mfma.reserve(slice * nsplit));
+    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+    if (ntiles > 0) CK(run(nsplit));
+    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     CK(launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
                            ws.loc_mfma.as<float>(), nsplit, L, s));
     S.m2l_split = nsplit;
     S.m2l_tiles = ntiles;
   } else {
     CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
-    const int nsplit = m2l_choose_split(ntiles, p);
-    CK(ws.loc32.reserve(sizeof(float2) * (size_t)nsplit *
-                        std::max(1, ntiles) * 64 * lc));
+    int nsplit = m2l_choose_split(ntiles, p);
+    const size_t slice = sizeof(float2) * std::max(1, ntiles) * 64 * lc;
     CK(launch_scale_mult(1, nd, nn, p, M, ws.mult32.ptr, s));
-    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
     M2LTileArgs ta;
     ta.tile_src = ws.tile_src.as<int>();
     ta.tile_oct = ws.tile_oct.as<int>();
@@ -862,12 +926,29 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     ta.ephi = ws.tab.ephi32.ptr;
     ta.rot = ws.tab.rot32.ptr;
     ta.vfac = ws.tab.vfac32.ptr;
-    ta.nsplit = nsplit;
     ta.tile_major = m2l_tile_major();
-    if (ntiles > 0)
-      CK(launch_m2l_tile(1, p, ntiles, ta, ws.mult32.ptr, ws.loc32.ptr, s));
-    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     const int first = depth >= 2 ? lstart[2] : nn;
+    auto run = [&](int ns) {
+      M2LTileArgs a = ta;
+      a.nsplit = ns;
+      return launch_m2l_tile(1, p, ntiles, a, ws.mult32.ptr, ws.loc32.ptr, s);
+    };
+    auto run_all = [&](int ns) {
+      hipError_t e = run(ns);
+      if (e != hipSuccess) return e;
+      return launch_unscale_local(1, nd, p, first, ntiles,
+                                  ws.tile_tgt.as<int>(), ws.loc32.ptr, ns, L,
+                                  s);
+    };
+    if (pb.autotune && ntiles > 0) {
+      CK(ws.loc32.reserve(slice * kTuneMaxSplit));
+      CK(tune_split(ws, tune_key(1, p, ntiles), run_all, &nsplit));
+    }
+    CK(ws.loc32.reserve(slice * nsplit));
+    ta.nsplit = nsplit;
+    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+    if (ntiles > 0) CK(run(nsplit));
+    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     CK(launch_unscale_local(1, nd, p, first, ntiles, ws.tile_tgt.as<int>(),
                             ws.loc32.ptr, nsplit, L, s));
     S.m2l_split = nsplit;

@@ -58,11 +58,12 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   constexpr int PER_T = (CHUNK + 64 * kMfmaWaves - 1) / (64 * kMfmaWaves);
   __shared__ uint4 ring[kMfmaStages][CHUNK];
   // workgroups are dealt round-robin to the 8 XCDs: give each XCD one
-  // contiguous range (one offset range, consecutive tile groups)
+  // contiguous range of tile groups, with all their offset ranges side by
+  // side, so that the sources a group gathers stay in that XCD's L2
   const int per_xcd = gridDim.x / 8;
   const int logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (logical >= nwg * nsplit) return;  // whole workgroup
-  const int split = logical / nwg, wgi = logical % nwg;
+  const int split = logical % nsplit, wgi = logical / nsplit;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int tid = threadIdx.x, lane = tid % 64;
   const int g = lane / 32, c32 = lane % 32;
@@ -142,11 +143,19 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
       for (int q = 0; q < NPART; q++) bb[cb][q] = as_bf16x8(pb[q]);
     }
   };
+  // whether any column of the wave has a source at this slot (wave-uniform)
+  auto alive = [&](const int *srcv, bool ok) -> bool {
+    bool any = false;
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) any |= ok && srcv[cb] >= 0;
+    return __ballot(any) != 0ull;
+  };
   int src_c[CB], src_n[CB];
   bf16x8_t bA[CB][NPART], bB[CB][NPART];
   load_src(k0, src_c);
   load_src(k0 + 1, src_n);
   load_b(src_c, has_tile, 0, bA);
+  bool live = alive(src_c, has_tile && k0 < k1);
   // one step: MFMAs of step t from `cur`, B of step t + 1 into `nxt`
   auto step = [&](int t, bf16x8_t (&cur)[CB][NPART],
                   bf16x8_t (&nxt)[CB][NPART]) {
@@ -165,6 +174,9 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
     } else {
       load_b(src_c, has_tile, s + 1, nxt);
     }
+    const bool live_t = live;
+    if (next_slot) live = alive(src_c, has_tile && k + 1 < k1);
+    if (!live_t) return;  // no source in any column: skip the products
     const uint4 *ra = ring[t % kMfmaStages];
     bf16x8_t a[RB][NPART];
 #pragma unroll

@@ -283,63 +283,90 @@ __global__ __launch_bounds__(64) void k_p2m(
 
 // ------------------------------------------------------------ M2M
 
-// One block per parent node of one level: M(P) = sum over children.
-__global__ __launch_bounds__(kOpThreads) void k_m2m(
+// One block per parent node of one level, one wave per child (children in
+// parallel): each wave translates its child's multipole (rotate, vertical
+// M2M, rotate back) into LDS, then the block adds the children in child
+// order, so the sum has the order of a serial loop over the children.
+constexpr int kM2mWaves = 8;
+
+__global__ __launch_bounds__(64 * kM2mWaves) void k_m2m(
     OpTables tb, NodeArrays nd, int lvl_begin, double2 *__restrict__ M) {
-  __shared__ double2 U[kMaxMc], X[kMaxMc], Z[kMaxMc];
-  __shared__ double rp[2 * kMaxOrder + 2];
+  __shared__ double2 U[kM2mWaves][kMaxMc], X[kM2mWaves][kMaxMc],
+      Z[kM2mWaves][kMaxMc];
+  __shared__ double rp[kM2mWaves][2 * kMaxOrder + 2];
   const int P = lvl_begin + blockIdx.x;
-  if (nd.leaf[P]) return;
+  if (nd.leaf[P]) return;  // uniform per block
   const int p = tb.p, mc = tri(p), rb = tb.rb;
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const double half = node_width(nd, P) * 0.25;  // half a child width
-  double2 acc[2] = {make_double2(0., 0.), make_double2(0., 0.)};
-  const int cb = nd.child[P], ce = cb + nd.nchild[P];
-  for (int C = cb; C < ce; C++) {
-    int bx = nd.ix[C] & 1, by = nd.iy[C] & 1, bz = nd.iz[C] & 1;
-    int code = code_of(1 - 2 * bx, 1 - 2 * by, 1 - 2 * bz);
-    DirEntry de = tb.dir[code];
-    const double2 *ph = tb.eiphi + code * (p + 1);
-    const double *R = tb.rot + (size_t)de.theta * 4 * rb;
-    for (int k = threadIdx.x; k < mc; k += kOpThreads) {
+  const int cb = nd.child[P], nch = nd.nchild[P];
+  const int C = cb + w;
+  const bool active = w < nch;  // uniform per wave
+  int code = 0;
+  DirEntry de;
+  const double2 *ph = tb.eiphi;
+  const double *R = tb.rot;
+  if (active) {
+    const int bx = nd.ix[C] & 1, by = nd.iy[C] & 1, bz = nd.iz[C] & 1;
+    code = code_of(1 - 2 * bx, 1 - 2 * by, 1 - 2 * bz);
+    de = tb.dir[code];
+    ph = tb.eiphi + code * (p + 1);
+    R = tb.rot + (size_t)de.theta * 4 * rb;
+    for (int k = lane; k < mc; k += 64) {
       int l, m;
       lm_of(k, l, m);
-      U[k] = cmul(ph[m], M[(size_t)C * mc + k]);
+      U[w][k] = cmul(ph[m], M[(size_t)C * mc + k]);
     }
-    if (threadIdx.x < 2 * p + 2) {
-      double r = de.norm * half;
+    if (lane < 2 * p + 2) {
+      const double r = de.norm * half;
       double v = 1.;
-      for (int i = 0; i < (int)threadIdx.x; i++) v *= -de.c * r;
-      rp[threadIdx.x] = v;  // ((-c) r)^k
+      for (int i = 0; i < lane; i++) v *= -de.c * r;
+      rp[w][lane] = v;  // ((-c) r)^k
     }
-    __syncthreads();
-    rotate_fwd(R, rb, p - 1, U, X);
-    __syncthreads();
-    for (int k = threadIdx.x; k < mc; k += kOpThreads) {
+  }
+  __syncthreads();
+  if (active) {
+    for (int k = lane; k < mc; k += 64) {  // forward rotation, degrees < p
+      int l, mp;
+      lm_of(k, l, mp);
+      const double *ep = R + rot_ofs(l) + mp * (l + 1);
+      const double *em = ep + rb;
+      const double2 *u = U[w] + tri(l);
+      double re = 0., im = 0.;
+      for (int m = 0; m <= l; m++) {
+        re += ep[m] * u[m].x;
+        im += em[m] * u[m].y;
+      }
+      X[w][k] = make_double2(re, im);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (int k = lane; k < mc; k += 64) {
       int l, m;
       lm_of(k, l, m);
       const double *T = tb.tm2m + ((size_t)l * p + m) * p;
       double re = 0., im = 0.;
       for (int n = m; n <= l; n++) {
-        double f = T[n] * rp[l - n];
-        re += f * X[tri(n) + m].x;
-        im += f * X[tri(n) + m].y;
+        const double f = T[n] * rp[w][l - n];
+        re += f * X[w][tri(n) + m].x;
+        im += f * X[w][tri(n) + m].y;
       }
-      Z[k] = make_double2(re, im);
+      Z[w][k] = make_double2(re, im);
     }
-    __syncthreads();
-    for (int slot = 0; slot < 2; slot++) {
-      int k = threadIdx.x + slot * kOpThreads;
-      if (k < mc) {
-        double2 v = rotate_inv_one(R, rb, ph, Z, k);
-        acc[slot].x += v.x;
-        acc[slot].y += v.y;
-      }
-    }
-    __syncthreads();
   }
-  for (int slot = 0; slot < 2; slot++) {
-    int k = threadIdx.x + slot * kOpThreads;
-    if (k < mc) M[(size_t)P * mc + k] = acc[slot];
+  __syncthreads();
+  if (active)  // rotate back; U[w] now holds the child's contribution
+    for (int k = lane; k < mc; k += 64)
+      U[w][k] = rotate_inv_one(R, rb, ph, Z[w], k);
+  __syncthreads();
+  for (int k = threadIdx.x; k < mc; k += 64 * kM2mWaves) {
+    double2 acc = make_double2(0., 0.);
+    for (int c = 0; c < nch; c++) {
+      acc.x += U[c][k].x;
+      acc.y += U[c][k].y;
+    }
+    M[(size_t)P * mc + k] = acc;
   }
 }
 
@@ -836,7 +863,7 @@ hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
 hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
                       int lvl_count, double2 *M, hipStream_t s) {
   if (lvl_count == 0) return hipSuccess;
-  k_m2m<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, M);
+  k_m2m<<<lvl_count, 64 * kM2mWaves, 0, s>>>(tb, nd, lvl_begin, M);
   return hipGetLastError();
 }
 

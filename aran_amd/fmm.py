@@ -65,6 +65,12 @@ class Fmm3d:
     def semifar_threshold_auto(order):
         return int(lib().aran_fmm3d_semifar_threshold_auto(int(order)))
 
+    def set_autotune(self, on=True):
+        """Time the far-field split candidates on first use of each (form,
+        order, size bucket) and keep the fastest."""
+        check(lib().aran_fmm3d_set_autotune(self._ctx, int(on)),
+              "set_autotune")
+
     def set_timings(self, on=True):
         check(lib().aran_fmm3d_set_timings(self._ctx, int(on)), "set_timings")
 

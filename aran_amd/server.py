@@ -45,7 +45,8 @@ def footprint(n, order, leafcap):
 class Server:
     def __init__(self, budget_bytes=200e9, nworkers=2, precision="mixed",
                  max_batch=64):
-        code = {"plain": PLAIN, "mixed": MIXED}.get(precision, precision)
+        code = {"plain": PLAIN, "mixed": MIXED, "bf16x3": 2,
+                "bf16x6": 3}.get(precision, precision)
         self._s = _L().aran_server3d_new(float(budget_bytes), int(nworkers),
                                          int(code), int(max_batch))
         if not self._s:

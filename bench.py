@@ -72,6 +72,8 @@ def cpu_baseline(n_sample, order, leafcap, seed):
 
 
 def run_form(f, dev, pot, grad, lb, ub, p, q, precision, steps, warmup):
+    """Warm-up solves (the first one also runs the far-field split autotuner
+    when it is on), then `steps` timed solves."""
     import torch
     n = dev[0].numel()
     f.set_precision(precision)
@@ -195,13 +197,16 @@ def precision_sweep(n, orders, steps, seed):
     dp, dg = f.direct(*dev, torch.from_numpy(tg))
     dp, dg = dp.cpu().numpy(), dg.cpu().numpy()
     out = []
+    dtypes = {"plain": "f64", "mixed": "f32 M2L/P2P",
+              "bf16x3": "bf16x3 M2L (MFMA), f32 P2P",
+              "bf16x6": "bf16x6 M2L (MFMA), f32 P2P"}
     for p in orders:
-        for form in ("plain", "mixed"):
+        for form in ("plain", "mixed", "bf16x3", "bf16x6"):
             r = run_form(f, dev, pot, grad, (0, 0, 0), (1, 1, 1), p, 64, form,
                          steps, 1)
             pp, gg = pot.cpu().numpy()[tg], grad.cpu().numpy()[tg]
             out.append({"order": p, "form": form,
-                        "dtype": "f64" if form == "plain" else "f32 M2L/P2P",
+                        "dtype": dtypes[form],
                         "particles_s": r["value"],
                         "ms_per_solve": r["ms_per_step"],
                         "m2l_ms": r["m2l_ms"],
@@ -288,12 +293,20 @@ def main():
     ap.add_argument("--leafcap", type=int, default=64)
     ap.add_argument("--precision",
                     choices=("mixed", "plain", "bf16x3", "bf16x6"),
-                    default="mixed")
+                    default="bf16x3",
+                    help="form of the timed solves: bf16x3 (M2L on MFMA, "
+                         "BF16 operands in 3 products, FP32 P2P; the "
+                         "default), bf16x6 (6 products), mixed (FP32 VALU "
+                         "M2L), plain (FP64)")
     ap.add_argument("--cpu-sample", type=int, default=131072)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparison run")
+    ap.add_argument("--autotune", action="store_true",
+                    help="far-field split from the on-device autotuner (its "
+                         "candidate launches run in the warm-up) instead of "
+                         "the cost model")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run one elementwise kernel that reads and "
                          "writes 512 MiB (the reference point for HBM byte "
@@ -303,7 +316,7 @@ def main():
     ap.add_argument("--sweep", action="store_true",
                     help="only the precision sweep: N=4,194,304 uniform "
                          "cube, signed charges, leaf 64, p in {6, 10, 14}, "
-                         "plain and mixed forms (one JSON line)")
+                         "every form (one JSON line)")
     ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=100.0,
                     help="offered load of the serving session, requests/s")
@@ -332,6 +345,7 @@ def main():
     dev = [torch.from_numpy(a).cuda() for a in (x, y, z, m)]
     lb, ub = (-1.0, -1.0, -1.0), (1.0, 1.0, 1.0)
     f = Fmm3d()
+    f.set_autotune(args.autotune)
     pot = torch.empty(n, dtype=torch.float64, device="cuda")
     grad = torch.empty((n, 3), dtype=torch.float64, device="cuda")
 

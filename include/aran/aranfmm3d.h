@@ -64,6 +64,13 @@ int aran_fmm3d_solve_batch (AranFmm3d *ctx, int nsys, const long *offsets,
 #define ARAN_FMM3D_BF16X6 3
 int aran_fmm3d_set_precision (AranFmm3d *ctx, int precision);
 
+/* Far-field autotuner: when on, the first solve of each (form, order,
+ * problem-size bucket) times every candidate split of the tiled far-field
+ * kernel over its offset ranges on the device and later solves of that
+ * bucket use the fastest (kept in the context).  Off by default (the split
+ * then comes from a cost model). */
+int aran_fmm3d_set_autotune (AranFmm3d *ctx, int on);
+
 /* Counters of the last solve: zero, p2p, p2m, m2m, m2l, l2l, l2p (the
  * reference's semantics), then depth, nodes, leaves, far-list entries,
  * near-list entries, then p2l, m2p (particles in semifar pairs, the
