@@ -119,6 +119,10 @@ struct Workspace {
   Tables tab;
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // second stream (default priority) for the near field, which runs
+  // alongside the upward pass, the far field and the downward pass
+  hipStream_t aux = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_near = nullptr;
   hipEvent_t ev[8] = {};
   bool events_ready = false;
   int device = -1;
