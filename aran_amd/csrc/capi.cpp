@@ -1124,6 +1124,78 @@ extern "C" gint aran_solver3d_insert_points_soa(AranSolver3d *solver,
   return 0;
 }
 
+// ---- FMA particle files (the formats of tests/newtonfield3parallel.c)
+
+extern "C" gint aran_solver3d_write_fma(AranSolver3d *solver, FILE *file) {
+  if (!solver || !file) {
+    critical("aran_solver3d_write_fma: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  // header: count, half-width of the bounds' box and its centre
+  // (newtonfield3parallel.c:1193-1212)
+  const VsgVector3d lb = solver->prtree->lb, ub = solver->prtree->ub;
+  const double cx = 0.5 * (lb.x + ub.x), cy = 0.5 * (lb.y + ub.y),
+               cz = 0.5 * (lb.z + ub.z);
+  double box = ub.x - cx;
+  box = std::max(box, ub.y - cy);
+  box = std::max(box, ub.z - cz);
+  const unsigned long np = aran_solver3d_point_count(solver);
+  if (fprintf(file, "%lu %g %g %g %g\n", np, box, cx, cy, cz) < 0)
+    return ARAN_FMM3D_EINVAL;
+  // one line per particle: position and charge (newtonfield3parallel.c:
+  // 1186-1191), in insertion order
+  for (unsigned long i = 0; i < np; i++) {
+    double x, y, z, q;
+    if (solver->soa) {
+      x = solver->soa_x[i];
+      y = solver->soa_y[i];
+      z = solver->soa_z[i];
+      q = solver->soa_q[i];
+    } else {
+      const char *pt = static_cast<const char *>(solver->points[i]);
+      const VsgVector3d *v = reinterpret_cast<const VsgVector3d *>(pt);
+      x = v->x;
+      y = v->y;
+      z = v->z;
+      q = *reinterpret_cast<const double *>(pt + solver->layout.charge);
+    }
+    if (fprintf(file, "%g %g %g %g\n", x, y, z, q) < 0)
+      return ARAN_FMM3D_EINVAL;
+  }
+  return 0;
+}
+
+extern "C" glong aran_fma_read_points(FILE *file, glong max_points,
+                                      gdouble *x, gdouble *y, gdouble *z,
+                                      gdouble *charge) {
+  if (!file || max_points < 0 ||
+      (max_points > 0 && (!x || !y || !z || !charge))) {
+    critical("aran_fma_read_points: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  // lines "index x y z charge" (the -dist load fill,
+  // newtonfield3parallel.c:1020-1086); the index is read and not used
+  glong n = 0;
+  unsigned long idx;
+  double a, b, c, d;
+  for (;;) {
+    const int got = fscanf(file, "%lu %lf %lf %lf %lf", &idx, &a, &b, &c, &d);
+    if (got == EOF) break;
+    if (got != 5) {
+      critical("aran_fma_read_points: malformed line after %ld points", n);
+      return ARAN_FMM3D_EINVAL;
+    }
+    if (n < max_points) {
+      x[n] = a;
+      y[n] = b;
+      z[n] = c;
+      charge[n] = d;
+    }
+    n++;
+  }
+  return n;
+}
+
 extern "C" gint aran_solver3d_get_results_soa(AranSolver3d *solver,
                                               gdouble *pot, gdouble *field) {
   if (!solver || !solver->soa) return ARAN_FMM3D_EINVAL;

@@ -118,6 +118,10 @@ def lib():
     L.aran_solver3d_set_tolerance.argtypes = [vp, d]
     L.aran_solver3d_find_point.restype = vp
     L.aran_solver3d_find_point.argtypes = [vp, vp]
+    L.aran_solver3d_write_fma.restype = C.c_int
+    L.aran_solver3d_write_fma.argtypes = [vp, vp]
+    L.aran_fma_read_points.restype = C.c_long
+    L.aran_fma_read_points.argtypes = [vp, C.c_long] + [vp] * 4
     _declared = True
     return L
 
@@ -141,3 +145,57 @@ def get_stats(solver):
     keys = ("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p", "p2l", "m2p",
             "p2p_remote", "m2l_remote")
     return dict(zip(keys, [v.value for v in vals]))
+
+
+_libc = None
+
+
+def _stdio():
+    global _libc
+    if _libc is None:
+        _libc = C.CDLL(None)
+        _libc.fopen.restype = C.c_void_p
+        _libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+        _libc.fclose.restype = C.c_int
+        _libc.fclose.argtypes = [C.c_void_p]
+    return _libc
+
+
+def _open(path, mode):
+    f = _stdio().fopen(str(path).encode(), mode)
+    if not f:
+        raise OSError("cannot open %s" % path)
+    return f
+
+
+def write_fma(solver, path):
+    """Write the solver's particles as an FMA file (aran_solver3d_write_fma)."""
+    f = _open(path, b"w")
+    try:
+        rc = lib().aran_solver3d_write_fma(solver, f)
+    finally:
+        _stdio().fclose(f)
+    if rc != 0:
+        raise RuntimeError("aran_solver3d_write_fma failed (%d)" % rc)
+
+
+def read_fma_points(path):
+    """Read an "index x y z charge" particle file (aran_fma_read_points):
+    returns x, y, z, charge as float64 numpy arrays."""
+    import numpy as np
+    f = _open(path, b"r")
+    try:
+        n = lib().aran_fma_read_points(f, 0, None, None, None, None)
+    finally:
+        _stdio().fclose(f)
+    if n < 0:
+        raise ValueError("malformed particle file %s" % path)
+    out = [np.zeros(n) for _ in range(4)]
+    f = _open(path, b"r")
+    try:
+        m = lib().aran_fma_read_points(f, n, *[a.ctypes.data for a in out])
+    finally:
+        _stdio().fclose(f)
+    if m != n:
+        raise ValueError("particle file %s changed while reading" % path)
+    return tuple(out)

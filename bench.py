@@ -98,6 +98,42 @@ def run_form(f, dev, pot, grad, lb, ub, p, q, precision, steps, warmup):
             "stats": f.stats()}
 
 
+def host_inclusive(f, host, lb, ub, p, q, precision, steps):
+    """The same solve with inputs and outputs in pinned host memory: each
+    step copies x, y, z, q to the GPU, solves, copies potential and
+    gradient back and waits for them (the PCIe-inclusive rate of a caller
+    that hands over host buffers)."""
+    import torch
+    n = host[0].numel()
+    hin = [t.pin_memory() for t in host]
+    hpot = torch.empty(n, dtype=torch.float64).pin_memory()
+    hgrad = torch.empty((n, 3), dtype=torch.float64).pin_memory()
+    din = [torch.empty_like(t, device="cuda") for t in hin]
+    pot = torch.empty(n, dtype=torch.float64, device="cuda")
+    grad = torch.empty((n, 3), dtype=torch.float64, device="cuda")
+    f.set_precision(precision)
+    f.set_timings(False)
+
+    def step():
+        for d, h in zip(din, hin):
+            d.copy_(h, non_blocking=True)
+        f.solve(*din, lb, ub, p, q, pot, grad)
+        hpot.copy_(pot, non_blocking=True)
+        hgrad.copy_(grad, non_blocking=True)
+        torch.cuda.current_stream().synchronize()
+
+    step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    elapsed = time.perf_counter() - t0
+    return {"value": n * steps / elapsed,
+            "ms_per_step": 1e3 * elapsed / steps,
+            "bytes_per_step": 8 * n * (4 + 4),
+            "note": "pinned host inputs and outputs, copies and the wait for "
+                    "the results inside each step"}
+
+
 def error_vs_direct(f, dev, pot, grad, n, seed):
     import torch
     from aran_amd import workloads as W
@@ -363,6 +399,10 @@ def main():
             plain["error"] = error_vs_direct(f, dev, pot, grad, n, seed)
         plain["mixed_over_plain"] = head["value"] / r["value"]
 
+    hostinc = host_inclusive(f, [torch.from_numpy(a) for a in (x, y, z, m)],
+                             lb, ub, p, q, args.precision,
+                             max(2, args.steps // 2))
+
     st = head["stats"]
     form = args.precision
     mixed = form != "plain"
@@ -434,6 +474,7 @@ def main():
         "error": err,
         "phases_ms": head["phases_ms"],
         "plain_fp64": plain,
+        "host_buffers": hostinc,
         "batched": batch,
         "serving": serve,
         "tree": {k: st[k] for k in ("depth", "nodes", "leaves", "m2l", "p2p")},

@@ -25,6 +25,8 @@
 #ifndef ARAN_SOLVER3D_H
 #define ARAN_SOLVER3D_H
 
+#include <stdio.h>
+
 #include <aran/aran.h>
 #include <aran/arandevelopment3d.h>
 #include <aran/arantypes.h>
@@ -167,6 +169,20 @@ gint aran_solver3d_insert_points_soa (AranSolver3d *solver, glong nparticles,
                                       const gdouble *z, const gdouble *charge);
 gint aran_solver3d_get_results_soa (AranSolver3d *solver, gdouble *pot,
                                     gdouble *field);
+
+/* FMA particle files, the formats of the reference's parallel Newton test
+ * (tests/newtonfield3parallel.c).  aran_solver3d_write_fma writes the
+ * header "count half_width cx cy cz" of the solver's bounds, then one line
+ * "x y z charge" per particle in insertion order, with %g as the reference
+ * does (newtonfield3parallel.c:1186-1212); returns 0 or a negative code.
+ * aran_fma_read_points reads lines "index x y z charge" (its -dist load
+ * fill, newtonfield3parallel.c:1020-1086) into the first max_points slots of
+ * the arrays and returns the number of lines read (which may exceed
+ * max_points: call with max_points 0 to size the arrays), or a negative code
+ * on a malformed line. */
+gint aran_solver3d_write_fma (AranSolver3d *solver, FILE *file);
+glong aran_fma_read_points (FILE *file, glong max_points, gdouble *x,
+                            gdouble *y, gdouble *z, gdouble *charge);
 
 #ifdef __cplusplus
 }
