@@ -7,9 +7,13 @@
  * (an engine context on its own HIP stream) admits the oldest waiting
  * requests that share the oldest one's order and leaf capacity while the
  * estimated device footprint in flight stays under the budget, and solves
- * them together as independent systems (aran_fmm3d_solve_batch).  A served
- * result is bitwise the result of solving that request alone in the same
- * precision form.
+ * them together as independent systems (aran_fmm3d_solve_batch).  In the
+ * plain form a served result is bitwise the result of solving that request
+ * alone.  In the other forms each local expansion is accumulated in a number
+ * of partial sums (the far-field split) chosen from the size of the whole
+ * batch: a served result is bitwise the standalone one when both choose the
+ * same split (as they do while the batch's far-field work fits one round of
+ * workgroups on the GPU), and otherwise equal to within that form's rounding.
  */
 #ifndef ARAN_SERVER3D_H
 #define ARAN_SERVER3D_H
@@ -27,7 +31,7 @@ double aran_server3d_footprint (long n, int order, int leafcap);
 
 /* hbm_budget_bytes: admission budget (64 MB per worker is set aside for its
  * tables; the rest bounds the requests in flight); nworkers: concurrent batches (1..8);
- * precision: ARAN_FMM3D_PLAIN or ARAN_FMM3D_MIXED; max_batch: requests per
+ * precision: ARAN_FMM3D_PLAIN, _MIXED, _BF16X3 or _BF16X6; max_batch: requests per
  * batch.  Binds the current HIP device. */
 AranServer3d *aran_server3d_new (double hbm_budget_bytes, int nworkers,
                                  int precision, int max_batch);

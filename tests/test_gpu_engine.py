@@ -418,7 +418,9 @@ def test_batch_equals_individual_solves(torch_cuda, precision):
 def test_served_results_equal_standalone(torch_cuda):
     """Requests of mixed sizes, orders and outputs submitted together are
     served in batches; every served result is bitwise the standalone solve
-    of that request (mixed form); a request over the budget is refused."""
+    of that request (mixed form: these batches are small enough that the
+    batch and the request alone choose the same far-field split); a request
+    over the budget is refused."""
     torch = torch_cuda
     from aran_amd.server import EBUDGET, Server, footprint
     reqs = []
