@@ -19,6 +19,7 @@ LIB = os.path.join(HERE, "libaran_amd.so")
 M2L_RANGES = [(1, 6), (7, 9), (10, 11), (12, 13), (14, 15)]
 SOURCES = [("tree.hip", "tree", []), ("operators.hip", "operators", []),
            ("m2l_prep.hip", "m2l_prep", []), ("p2p.hip", "p2p", []),
+           ("translate.hip", "translate", []),
            ("engine.hip", "engine", []), ("capi.cpp", "capi", []),
            ("server.cpp", "server", [])] + [
     ("m2l_tile.hip", "m2l_tile_%d" % a,

@@ -101,6 +101,8 @@ struct Workspace {
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
   DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
+  DevBuf internal_list;  // internal nodes in node order (level by level)
+  DevBuf trans_rows;     // TransRow of each internal node (same order)
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
       colleagues;
   // semifar lists: per leaf the nodes of its M2P (w_*), per node the leaves
@@ -214,7 +216,12 @@ hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
 hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
                              hipStream_t s);
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
-                               const int *pos, int *leaves, hipStream_t s);
+                               const int *pos, int *leaves, int *internal,
+                               hipStream_t s);
+// out[0] = leaf count, out[1 + d] = internal nodes of level d (d < nlevels)
+hipError_t launch_level_internal(const int *pos, const int *total,
+                                 const int *lstart, int nlevels, int nnodes,
+                                 int *out, hipStream_t s);
 hipError_t launch_colleagues(const NodeArrays &nd, int first, int count,
                              int *col, hipStream_t s);
 hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
@@ -242,13 +249,9 @@ hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
                       const int *leaves, int nleaves, const double *xs,
                       const double *ys, const double *zs, const double *qs,
                       double2 *M, hipStream_t s);
-hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double2 *M, hipStream_t s);
 hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const int *vofs, const int *vsrc, const short *vcode,
                       const double2 *M, double2 *L, hipStream_t s);
-hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double2 *L, hipStream_t s);
 // pleaf[j] = leaf node of sorted particle j (every particle of the leaves)
 hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
                                 int nleaves, int *pleaf, hipStream_t s);

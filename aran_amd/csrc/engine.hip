@@ -20,6 +20,7 @@
 
 #include "engine.h"
 #include "m2l_mfma.h"
+#include "translate.h"
 #include "m2l_tile.h"
 
 namespace aran {
@@ -135,7 +136,8 @@ Workspace::~Workspace() {
                    &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
                    &mult_bf, &loc_mfma, &tab.mfma_ops, &n_sys, &sys_box, &sys_off, &sys_mm,
-                   &skey, &spos};
+                   &skey, &spos, &internal_list,
+                   &trans_rows};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
   if (events_ready) {
@@ -288,6 +290,14 @@ hipError_t ensure_tables(Workspace &ws, int p) {
         }
       }
   if (thetas.empty()) thetas.push_back(0.);
+  // the level L2L stages one rotation table for its eight offsets
+  // (+-1, +-1, +-1): after the z flip they share one angle
+  for (int o = 1; o < 8; o++)
+    if (dirs[code_of(2 * (o & 1) - 1, 2 * ((o >> 1) & 1) - 1,
+                     2 * (o >> 2) - 1)].theta != dirs[code_of(-1, -1, -1)].theta) {
+      set_error("internal error: diagonal offsets with different angles");
+      return hipErrorUnknown;
+    }
   int nth = (int)thetas.size();
   // translation factor tables
   std::vector<double> al = alpha_table(2 * p + 2);
@@ -683,23 +693,44 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   const int nn = (int)nnodes;
 
   // ---- leaves
+  // ---- leaves and internal nodes (the latter level by level, for the
+  // M2M / L2L level products); one read brings both counts
   CK(ws.flags.reserve(sizeof(int) * nn));
   CK(ws.leaf_list.reserve(sizeof(int) * (nn + 1)));
+  CK(ws.internal_list.reserve(sizeof(int) * (nn + 1)));
   CK(ws.split_ofs.reserve(sizeof(int) * (nn + 1)));
-  CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
-  CK(scan_exclusive(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
-                    ws.scan_tmp, s));
-  int nleaves = 0;
-  CK(read_int(ws, d_total, &nleaves));
-  CK(launch_leaf_compact(nd, nn, ws.split_ofs.as<int>(), ws.leaf_list.as<int>(),
-                         s));
-  const int *leaves = ws.leaf_list.as<int>();
   CK(ws.lstart.reserve(sizeof(int) * (kMaxDepth + 2)));
   CK(ws.lcount.reserve(sizeof(int) * (kMaxDepth + 2)));
   CK(hipMemcpyAsync(ws.lstart.ptr, lstart.data(), sizeof(int) * lstart.size(),
                     hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
                     hipMemcpyHostToDevice, s));
+  CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
+  CK(scan_exclusive(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
+                    ws.scan_tmp, s));
+  int *d_lvl = reinterpret_cast<int *>(ws.small.as<char>() + 3584);
+  int *h_lvl = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
+                                       3584);
+  CK(launch_level_internal(ws.split_ofs.as<int>(), d_total,
+                           ws.lstart.as<int>(), depth + 1, nn, d_lvl, s));
+  CK(hipMemcpyAsync(h_lvl, d_lvl, sizeof(int) * (depth + 2),
+                    hipMemcpyDeviceToHost, s));
+  CK(hipStreamSynchronize(s));
+  const int nleaves = h_lvl[0];
+  std::vector<int> icount(depth + 1), ibase(depth + 1);
+  for (int d = 0, b = 0; d <= depth; d++) {
+    icount[d] = h_lvl[1 + d];
+    ibase[d] = b;
+    b += icount[d];
+  }
+  CK(launch_leaf_compact(nd, nn, ws.split_ofs.as<int>(), ws.leaf_list.as<int>(),
+                         ws.internal_list.as<int>(), s));
+  const int *leaves = ws.leaf_list.as<int>();
+  const int ninternal = nn - nleaves;
+  CK(ws.trans_rows.reserve(sizeof(TransRow) * std::max(1, ninternal)));
+  CK(launch_trans_rows(nd, ws.internal_list.as<int>(), ninternal,
+                       ws.trans_rows.as<TransRow>(), s));
+  const TransRow *trows = ws.trans_rows.as<TransRow>();
   if (pb.timings) CK(hipEventRecord(ws.ev[1], s));
 
   // ---- colleagues, level by level
@@ -901,7 +932,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(launch_p2m(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
                 ws.zs.as<double>(), ws.qs.as<double>(), M, s));
   for (int d = depth - 1; d >= 0; d--)
-    CK(launch_m2m(tb, nd, lstart[d], lcount[d], M, s));
+    CK(launch_m2m_level(tb, trows + ibase[d], icount[d], M, s));
 
   // ---- far field (the M2L timer brackets the M2L kernel alone)
   if (pb.precision == 0) {
@@ -991,7 +1022,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                   ws.qs.as<double>(), L, s));
   // ---- downward pass
   for (int d = 1; d <= depth; d++)
-    CK(launch_l2l(tb, nd, lstart[d], lcount[d], L, s));
+    CK(launch_l2l_level(tb, trows + ibase[d - 1], icount[d - 1], L, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
   // ---- join the near field, then M2P and L2P add to its results
   CK(hipStreamWaitEvent(s, ws.ev_join, 0));

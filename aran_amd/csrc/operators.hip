@@ -4,12 +4,8 @@
 //                     in l for fixed (a, b), FP64, one block per theta;
 //                     folded into the real rotation matrices E+-, F+-
 //   k_p2m           : aran_development3d_p2m (arandevelopment3d.c:169-216)
-//   k_m2m           : aran_development3d_m2m_rotate (arandevelopment3d.c:
-//                     402-409, aransphericalseriesd-rotate.c:219-263)
 //   k_m2l           : aran_development3d_m2l_rotate (arandevelopment3d.c:
 //                     423-430, rotate.c:165-207, translate.c:609-676)
-//   k_l2l           : aran_development3d_l2l_rotate (arandevelopment3d.c:
-//                     444-451, translate.c:301-364)
 //   k_l2p           : local evaluate + gradient (aransphericalseriesd.c:
 //                     313-546)
 //
@@ -25,6 +21,8 @@
 // (d(-theta) is the transpose of d(theta)).
 #include <hip/hip_runtime.h>
 #include <math.h>
+
+#include <type_traits>
 
 #include "engine.h"
 #include "tree_rules.h"
@@ -208,165 +206,93 @@ constexpr InvTable make_inv_table() {
 }
 __constant__ InvTable kInv = make_inv_table();
 
-constexpr int kP2mSlots = 1024;  // LDS staging, double2 (16 KiB)
+constexpr int kP2mSlots = 1024;  // P2L LDS staging, double2 (16 KiB)
+
+template <int B, int E, class F>
+__device__ __forceinline__ void static_loop(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_loop<B + 1, E>(f);
+  }
+}
 
 // One wave per leaf: M_l^m = (4pi/(2l+1)) g_l^m conj(sum_j q_j O_l^m(x_j))
-// (aran_development3d_p2m, arandevelopment3d.c:169-216).  Per chunk of
-// particles, lanes take (particle, m) columns and store q O_l^m in LDS (the
-// reciprocals are computed, so no memory access sits in the recurrence),
-// then lanes take coefficients and add the chunk's particles in index order.
+// (aran_development3d_p2m, arandevelopment3d.c:169-216).  Lane s P + m
+// (s < 64 / P particle slots) runs column m of the degree recurrence for the
+// leaf's particles s, s + S, ... in registers and accumulates q O_l^m for
+// l >= m; the slots are then added in slot order through LDS.  No memory
+// access sits in the recurrence (the reciprocals come from a constant
+// table), and the LDS is small enough that several leaves share a CU with
+// the near field.
+template <int P>
 __global__ __launch_bounds__(64) void k_p2m(
     OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double *__restrict__ qs,
     double2 *__restrict__ M) {
-  __shared__ double2 T[kP2mSlots];
-  const int p = tb.p;
-  const int mc = tri(p);
+  constexpr int MC = P * (P + 1) / 2, S = 64 / P;
+  __shared__ double2 T[S][MC];
   const int lane = threadIdx.x;
   const int A = leaves[blockIdx.x];
   const int b = nd.pbegin[A], e = nd.pend[A];
   const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
-  const int chunk = min(64, kP2mSlots / mc);
-  double2 acc[2] = {make_double2(0., 0.), make_double2(0., 0.)};
-  for (int c0 = b; c0 < e; c0 += chunk) {
-    const int cnt = min(chunk, e - c0);
-    __syncthreads();
-    for (int t = lane; t < cnt * p; t += 64) {
-      const int j = t / p, m = t - j * p;
-      const int i = c0 + j;
+  const int s = lane / P, m = lane % P;
+  double ar[P], ai[P];
+  static_loop<0, P>([&](auto lc) {
+    ar[decltype(lc)::value] = 0.;
+    ai[decltype(lc)::value] = 0.;
+  });
+  if (s < S) {
+    for (int i = b + s; i < e; i += S) {
       const double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
       const double r2 = vx * vx + vy * vy + vz * vz;
       double ox = qs[i], oy = 0.;  // q O_m^m
-      for (int k = 1; k <= m; k++) {
-        const double f = -0.5 / (double)k;
-        const double ax = f * vx, ay = f * vy;
-        const double nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
-        ox = nx;
-        oy = ny;
-      }
-      double2 *row = T + j * mc;
-      row[tri(m) + m] = make_double2(ox, oy);
-      double px = ox, py = oy, qx = 0., qy = 0.;  // degrees l-1, l-2
-      for (int l = m + 1; l < p; l++) {
-        const double a = (2. * l - 1.) * vz;
-        const double g = 1. / (double)(l * l - m * m);
-        const double nx = (a * px - r2 * qx) * g, ny = (a * py - r2 * qy) * g;
-        qx = px;
-        qy = py;
-        px = nx;
-        py = ny;
-        row[tri(l) + m] = make_double2(nx, ny);
-      }
+      static_loop<1, P>([&](auto kc) {
+        constexpr int k = decltype(kc)::value;
+        if (k <= m) {
+          const double f = -0.5 * kInv.v[k];
+          const double ax = f * vx, ay = f * vy;
+          const double nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
+          ox = nx;
+          oy = ny;
+        }
+      });
+      double px = 0., py = 0., qx = 0., qy = 0.;  // degrees l-1, l-2
+      static_loop<0, P>([&](auto lc) {
+        constexpr int l = decltype(lc)::value;
+        if (l == m) {
+          px = ox;
+          py = oy;
+          ar[l] += ox;
+          ai[l] += oy;
+        } else if (l > m) {
+          const double a = (2. * l - 1.) * vz;
+          const double g = kInv.v[l * l - m * m];
+          const double nx = (a * px - r2 * qx) * g,
+                       ny = (a * py - r2 * qy) * g;
+          qx = px;
+          qy = py;
+          px = nx;
+          py = ny;
+          ar[l] += nx;
+          ai[l] += ny;
+        }
+      });
     }
-    __syncthreads();
-    for (int slot = 0; slot < 2; slot++) {
-      const int k = lane + 64 * slot;
-      if (k >= mc) break;
-      double2 a = acc[slot];
-      for (int j = 0; j < cnt; j++) {
-        const double2 v = T[j * mc + k];
-        a.x += v.x;
-        a.y += v.y;
-      }
-      acc[slot] = a;
-    }
-  }
-  for (int slot = 0; slot < 2; slot++) {
-    const int k = lane + 64 * slot;
-    if (k < mc) {
-      const double g = tb.pgain[k];
-      M[(size_t)A * mc + k] = make_double2(g * acc[slot].x, -g * acc[slot].y);
-    }
-  }
-}
-
-// ------------------------------------------------------------ M2M
-
-// One block per parent node of one level, one wave per child (children in
-// parallel): each wave translates its child's multipole (rotate, vertical
-// M2M, rotate back) into LDS, then the block adds the children in child
-// order, so the sum has the order of a serial loop over the children.
-constexpr int kM2mWaves = 8;
-
-__global__ __launch_bounds__(64 * kM2mWaves) void k_m2m(
-    OpTables tb, NodeArrays nd, int lvl_begin, double2 *__restrict__ M) {
-  __shared__ double2 U[kM2mWaves][kMaxMc], X[kM2mWaves][kMaxMc],
-      Z[kM2mWaves][kMaxMc];
-  __shared__ double rp[kM2mWaves][2 * kMaxOrder + 2];
-  const int P = lvl_begin + blockIdx.x;
-  if (nd.leaf[P]) return;  // uniform per block
-  const int p = tb.p, mc = tri(p), rb = tb.rb;
-  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
-  const double half = node_width(nd, P) * 0.25;  // half a child width
-  const int cb = nd.child[P], nch = nd.nchild[P];
-  const int C = cb + w;
-  const bool active = w < nch;  // uniform per wave
-  int code = 0;
-  DirEntry de;
-  const double2 *ph = tb.eiphi;
-  const double *R = tb.rot;
-  if (active) {
-    const int bx = nd.ix[C] & 1, by = nd.iy[C] & 1, bz = nd.iz[C] & 1;
-    code = code_of(1 - 2 * bx, 1 - 2 * by, 1 - 2 * bz);
-    de = tb.dir[code];
-    ph = tb.eiphi + code * (p + 1);
-    R = tb.rot + (size_t)de.theta * 4 * rb;
-    for (int k = lane; k < mc; k += 64) {
-      int l, m;
-      lm_of(k, l, m);
-      U[w][k] = cmul(ph[m], M[(size_t)C * mc + k]);
-    }
-    if (lane < 2 * p + 2) {
-      const double r = de.norm * half;
-      double v = 1.;
-      for (int i = 0; i < lane; i++) v *= -de.c * r;
-      rp[w][lane] = v;  // ((-c) r)^k
-    }
+    static_loop<0, P>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      if (l >= m) T[s][tri(l) + m] = make_double2(ar[l], ai[l]);
+    });
   }
   __syncthreads();
-  if (active) {
-    for (int k = lane; k < mc; k += 64) {  // forward rotation, degrees < p
-      int l, mp;
-      lm_of(k, l, mp);
-      const double *ep = R + rot_ofs(l) + mp * (l + 1);
-      const double *em = ep + rb;
-      const double2 *u = U[w] + tri(l);
-      double re = 0., im = 0.;
-      for (int m = 0; m <= l; m++) {
-        re += ep[m] * u[m].x;
-        im += em[m] * u[m].y;
-      }
-      X[w][k] = make_double2(re, im);
-    }
-  }
-  __syncthreads();
-  if (active) {
-    for (int k = lane; k < mc; k += 64) {
-      int l, m;
-      lm_of(k, l, m);
-      const double *T = tb.tm2m + ((size_t)l * p + m) * p;
-      double re = 0., im = 0.;
-      for (int n = m; n <= l; n++) {
-        const double f = T[n] * rp[w][l - n];
-        re += f * X[w][tri(n) + m].x;
-        im += f * X[w][tri(n) + m].y;
-      }
-      Z[w][k] = make_double2(re, im);
-    }
-  }
-  __syncthreads();
-  if (active)  // rotate back; U[w] now holds the child's contribution
-    for (int k = lane; k < mc; k += 64)
-      U[w][k] = rotate_inv_one(R, rb, ph, Z[w], k);
-  __syncthreads();
-  for (int k = threadIdx.x; k < mc; k += 64 * kM2mWaves) {
+  for (int k = lane; k < MC; k += 64) {
     double2 acc = make_double2(0., 0.);
-    for (int c = 0; c < nch; c++) {
-      acc.x += U[c][k].x;
-      acc.y += U[c][k].y;
+    for (int t = 0; t < S; t++) {
+      acc.x += T[t][k].x;
+      acc.y += T[t][k].y;
     }
-    M[(size_t)P * mc + k] = acc;
+    const double g = tb.pgain[k];
+    M[(size_t)A * MC + k] = make_double2(g * acc.x, -g * acc.y);
   }
 }
 
@@ -530,56 +456,6 @@ hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s) {
   const int p = tb.p, ki = 2 * tri(p) - p;
   k_m2l_dense<<<kNumCodes * ki, kOpThreads, 0, s>>>(tb, O);
   return hipGetLastError();
-}
-
-// ------------------------------------------------------------ L2L
-
-// One block per node of one level (depth >= 1): L(A) += T L(parent).
-__global__ __launch_bounds__(kOpThreads) void k_l2l(
-    OpTables tb, NodeArrays nd, int lvl_begin, double2 *__restrict__ L) {
-  __shared__ double2 U[kMaxLc], X[kMaxLc], Z[kMaxLc];
-  __shared__ double rp[2 * kMaxOrder + 2];
-  const int A = lvl_begin + blockIdx.x;
-  const int P = nd.parent[A];
-  const int p = tb.p, lc = tri(p + 1), rb = tb.rb;
-  const double half = node_width(nd, A) * 0.5;
-  int bx = nd.ix[A] & 1, by = nd.iy[A] & 1, bz = nd.iz[A] & 1;
-  int code = code_of(2 * bx - 1, 2 * by - 1, 2 * bz - 1);
-  DirEntry de = tb.dir[code];
-  const double2 *ph = tb.eiphi + code * (p + 1);
-  const double *R = tb.rot + (size_t)de.theta * 4 * rb;
-  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
-    int l, m;
-    lm_of(k, l, m);
-    U[k] = cmul(ph[m], L[(size_t)P * lc + k]);
-  }
-  if (threadIdx.x < 2 * p + 2) {
-    double r = de.norm * half;
-    double v = 1.;
-    for (int i = 0; i < (int)threadIdx.x; i++) v *= de.c * r;
-    rp[threadIdx.x] = v;  // (c r)^k
-  }
-  __syncthreads();
-  rotate_fwd(R, rb, p, U, X);
-  __syncthreads();
-  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
-    int l, m;
-    lm_of(k, l, m);
-    const double *T = tb.tl2l + ((size_t)l * (p + 1) + m) * (p + 1);
-    double re = 0., im = 0.;
-    for (int n = l; n <= p; n++) {
-      double f = T[n] * rp[n - l];
-      re += f * X[tri(n) + m].x;
-      im += f * X[tri(n) + m].y;
-    }
-    Z[k] = make_double2(re, im);
-  }
-  __syncthreads();
-  for (int k = threadIdx.x; k < lc; k += kOpThreads) {
-    double2 v = rotate_inv_one(R, rb, ph, Z, k);
-    double2 o = L[(size_t)A * lc + k];
-    L[(size_t)A * lc + k] = make_double2(o.x + v.x, o.y + v.y);
-  }
 }
 
 // ------------------------------------------------------------ L2P
@@ -851,20 +727,29 @@ __global__ void k_particle_leaf(NodeArrays nd, const int *__restrict__ leaves,
 
 // ------------------------------------------------------------ launchers
 
+template <int P>
+static hipError_t p2m_dispatch(const OpTables &tb, const NodeArrays &nd,
+                               const int *leaves, int nleaves,
+                               const double *xs, const double *ys,
+                               const double *zs, const double *qs, double2 *M,
+                               hipStream_t s) {
+  if constexpr (P > kMaxOrder) {
+    return hipErrorInvalidValue;
+  } else {
+    if (tb.p != P)
+      return p2m_dispatch<P + 1>(tb, nd, leaves, nleaves, xs, ys, zs, qs, M,
+                                 s);
+    k_p2m<P><<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
+    return hipGetLastError();
+  }
+}
+
 hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
                       const int *leaves, int nleaves, const double *xs,
                       const double *ys, const double *zs, const double *qs,
                       double2 *M, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
-  k_p2m<<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
-  return hipGetLastError();
-}
-
-hipError_t launch_m2m(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double2 *M, hipStream_t s) {
-  if (lvl_count == 0) return hipSuccess;
-  k_m2m<<<lvl_count, 64 * kM2mWaves, 0, s>>>(tb, nd, lvl_begin, M);
-  return hipGetLastError();
+  return p2m_dispatch<1>(tb, nd, leaves, nleaves, xs, ys, zs, qs, M, s);
 }
 
 hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
@@ -872,13 +757,6 @@ hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const double2 *M, double2 *L, hipStream_t s) {
   if (nnodes == 0) return hipSuccess;
   k_m2l<<<nnodes, kOpThreads, 0, s>>>(tb, nd, vofs, vsrc, vcode, M, L);
-  return hipGetLastError();
-}
-
-hipError_t launch_l2l(const OpTables &tb, const NodeArrays &nd, int lvl_begin,
-                      int lvl_count, double2 *L, hipStream_t s) {
-  if (lvl_count == 0) return hipSuccess;
-  k_l2l<<<lvl_count, kOpThreads, 0, s>>>(tb, nd, lvl_begin, L);
   return hipGetLastError();
 }
 

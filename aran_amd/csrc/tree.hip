@@ -558,11 +558,34 @@ __global__ void k_leaf_flags(NodeArrays nd, int nnodes, int *__restrict__ flag) 
   if (i < nnodes) flag[i] = nd.leaf[i];
 }
 
+// leaves[pos[i]] = i for leaves; internal[i - pos[i]] = i for the others
+// (both in node order, so grouped by level)
 __global__ void k_leaf_compact(NodeArrays nd, int nnodes,
                                const int *__restrict__ pos,
-                               int *__restrict__ leaves) {
+                               int *__restrict__ leaves,
+                               int *__restrict__ internal) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nnodes && nd.leaf[i]) leaves[pos[i]] = i;
+  if (i >= nnodes) return;
+  if (nd.leaf[i])
+    leaves[pos[i]] = i;
+  else
+    internal[i - pos[i]] = i;
+}
+
+// out[0] = leaves; out[1 + d] = internal nodes of level d, d < nlevels
+// (pos: exclusive scan of the leaf flags, total: its sum)
+__global__ void k_level_internal(const int *__restrict__ pos,
+                                 const int *__restrict__ total,
+                                 const int *__restrict__ lstart, int nlevels,
+                                 int nnodes, int *__restrict__ out) {
+  const int d = threadIdx.x;
+  if (d == 0) out[0] = *total;
+  if (d >= nlevels) return;
+  const int a = lstart[d];
+  const int b = d + 1 < nlevels ? lstart[d + 1] : nnodes;
+  const int la = pos[a];
+  const int lb = b < nnodes ? pos[b] : *total;
+  out[1 + d] = (b - a) - (lb - la);
 }
 
 hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
@@ -572,8 +595,18 @@ hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
 }
 
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
-                               const int *pos, int *leaves, hipStream_t s) {
-  k_leaf_compact<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, pos, leaves);
+                               const int *pos, int *leaves, int *internal,
+                               hipStream_t s) {
+  k_leaf_compact<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, pos, leaves,
+                                                      internal);
+  return hipGetLastError();
+}
+
+hipError_t launch_level_internal(const int *pos, const int *total,
+                                 const int *lstart, int nlevels, int nnodes,
+                                 int *out, hipStream_t s) {
+  if (nlevels > 64) return hipErrorInvalidValue;
+  k_level_internal<<<1, 64, 0, s>>>(pos, total, lstart, nlevels, nnodes, out);
   return hipGetLastError();
 }
 
