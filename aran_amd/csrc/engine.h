@@ -195,6 +195,9 @@ hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
                              double *out, hipStream_t s);
 hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
                           DevBuf &tmp, hipStream_t s);
+// the same in one launch for n <= 2^17 (falls back to scan_exclusive)
+hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
+                      hipStream_t s);
 hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
                       int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
                       int *total, int passes, hipStream_t s);

@@ -451,7 +451,7 @@ hipError_t build_far_lists(Workspace &ws, long nn, int *nfar) {
   CK(ws.v_ofs.reserve(sizeof(int) * (nn + 1)));
   CK(launch_vlist(nd, (int)nn, col, 0, ws.v_cnt.as<int>(), nullptr, nullptr,
                   nullptr, s));
-  CK(scan_exclusive(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
+  CK(scan_small(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
                     ws.v_ofs.as<int>() + nn, ws.scan_tmp, s));
   int nf = 0;
   CK(read_int(ws, ws.v_ofs.as<int>() + nn, &nf));
@@ -674,7 +674,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.split_ofs.reserve(sizeof(int) * ((size_t)lc + 1)));
     CK(launch_split_count(nd, lb, lc, keys, pb.leafcap,
                           ws.split_bounds.as<int>(), ws.split_cnt.as<int>(), s));
-    CK(scan_exclusive(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), lc,
+    CK(scan_small(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), lc,
                       d_total, ws.scan_tmp, s));
     int total = 0;
     CK(read_int(ws, d_total, &total));
@@ -706,7 +706,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(hipMemcpyAsync(ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
                     hipMemcpyHostToDevice, s));
   CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
-  CK(scan_exclusive(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
+  CK(scan_small(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
                     ws.scan_tmp, s));
   int *d_lvl = reinterpret_cast<int *>(ws.small.as<char>() + 3584);
   int *h_lvl = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
@@ -765,12 +765,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                             nullptr, s));
     CK(launch_xlist(nd, nn, col, pb.semifar, 0, ws.x_cnt.as<int>(), nullptr,
                     nullptr, s));
-    CK(scan_exclusive(ws.w_cnt.as<int>(), ws.w_ofs.as<int>(), nleaves,
+    CK(scan_small(ws.w_cnt.as<int>(), ws.w_ofs.as<int>(), nleaves,
                       ws.w_ofs.as<int>() + nleaves, ws.scan_tmp, s));
-    CK(scan_exclusive(ws.x_cnt.as<int>(), ws.x_ofs.as<int>(), nn,
+    CK(scan_small(ws.x_cnt.as<int>(), ws.x_ofs.as<int>(), nn,
                       ws.x_ofs.as<int>() + nn, ws.scan_tmp, s));
   }
-  CK(scan_exclusive(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
+  CK(scan_small(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
                     ws.u_ofs.as<int>() + nleaves, ws.scan_tmp, s));
   CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
   CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
@@ -847,7 +847,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)count + 1)));
     int *rank = ws.tile_rank.as<int>();
     CK(launch_octant_flags(nd, first, count, rank, s));
-    CK(scan_exclusive(rank, rank, 8L * count, rank + 8L * count, ws.scan_tmp,
+    CK(scan_small(rank, rank, 8L * count, rank + 8L * count, ws.scan_tmp,
                       s));
     CK(launch_group_counts(first, count, depth, d_meta, rank, cnt, s));
     std::vector<int> hcnt(ngroups);

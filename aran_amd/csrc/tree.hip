@@ -260,6 +260,60 @@ hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
   return e == hipSuccess ? hipGetLastError() : e;
 }
 
+// One-workgroup exclusive scan for up to kScan1Max entries (in and out may
+// alias); writes the grand total to *total.  Thread t scans entries
+// [kScan1Per t, kScan1Per (t + 1)) held in registers (all of its loads in
+// flight at once), so the whole scan is one launch and two memory round
+// trips; the chunk is short because a single CU serves every access.
+constexpr int kScan1Threads = 1024;
+constexpr int kScan1Per = 16;
+constexpr long kScan1Max = (long)kScan1Threads * kScan1Per;
+
+__global__ __launch_bounds__(kScan1Threads) void k_scan_single(
+    const int *in, int *out, int n, int *__restrict__ total) {
+  __shared__ int ws[kScan1Threads / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int b = t * kScan1Per;
+  int v[kScan1Per];
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kScan1Per; k++) {
+    v[k] = b + k < n ? in[b + k] : 0;
+    sum += v[k];
+  }
+  int x = sum;  // inclusive scan over the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int u = lane < kScan1Threads / 64 ? ws[lane] : 0;
+    for (int off = 1; off < kScan1Threads / 64; off <<= 1) {
+      const int y = __shfl_up(u, off);
+      if (lane >= off) u += y;
+    }
+    if (lane < kScan1Threads / 64) ws[lane] = u;
+  }
+  __syncthreads();
+  int run = x - sum + (w > 0 ? ws[w - 1] : 0);
+#pragma unroll
+  for (int k = 0; k < kScan1Per; k++) {
+    if (b + k < n) out[b + k] = run;
+    run += v[k];
+  }
+  if (t == kScan1Threads - 1) *total = run;
+}
+
+// Exclusive scan in one launch when n is small enough, else scan_exclusive.
+hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
+                      hipStream_t s) {
+  if (n > kScan1Max) return scan_exclusive(in, out, n, total, tmp, s);
+  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, (int)n, total);
+  return hipGetLastError();
+}
+
 // ------------------------------------------------------------ radix sort
 
 constexpr int kRsThreads = 256;
@@ -282,16 +336,64 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_hist(
   hist[threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Stable scatter: elements keep their input order inside each digit.
+// Block-wide exclusive scan of one int per thread (kRsThreads threads);
+// returns the thread's prefix, *tot the block total.
+__device__ inline int rs_block_scan(int v, int *wsum, int *tot) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int x = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  int pre = 0, all = 0;
+  for (int k = 0; k < kRsWaves; k++) {
+    if (k < w) pre += wsum[k];
+    all += wsum[k];
+  }
+  __syncthreads();
+  *tot = all;
+  return x - v + pre;
+}
+
+// One block per digit: exclusive scan of the digit's per-block counts
+// (hist[d][b] -> offs[d][b]) and the digit total dtot[d].
+__global__ __launch_bounds__(kRsThreads) void k_rs_scan_digits(
+    const int *__restrict__ hist, int *__restrict__ offs, int nblocks,
+    int *__restrict__ dtot) {
+  __shared__ int wsum[kRsWaves];
+  const int d = blockIdx.x;
+  int carry = 0;
+  for (int b0 = 0; b0 < nblocks; b0 += kRsThreads) {
+    const int b = b0 + threadIdx.x;
+    const int v = b < nblocks ? hist[(size_t)d * nblocks + b] : 0;
+    int tot;
+    const int pre = rs_block_scan(v, wsum, &tot);
+    if (b < nblocks) offs[(size_t)d * nblocks + b] = carry + pre;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) dtot[d] = carry;
+}
+
+// Stable scatter: elements keep their input order inside each digit.  The
+// block's base of digit d is the total of the digits below d plus its
+// offset among the blocks.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const int *__restrict__ vin,
     uint64_t *__restrict__ kout, int *__restrict__ vout, long n, int shift,
-    const int *__restrict__ offs, int nblocks) {
+    const int *__restrict__ offs, const int *__restrict__ dtot,
+    int nblocks) {
   __shared__ int base[256];
   __shared__ int wcnt[kRsWaves][256];
+  __shared__ int wsum[kRsWaves];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  base[threadIdx.x] = offs[threadIdx.x * nblocks + blockIdx.x];
+  {
+    int tot;
+    const int below = rs_block_scan(dtot[threadIdx.x], wsum, &tot);
+    base[threadIdx.x] = below + offs[threadIdx.x * nblocks + blockIdx.x];
+  }
   for (int w = 0; w < kRsWaves; w++) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -335,18 +437,21 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
                       int *total, int passes, hipStream_t s) {
   if (n <= 1) return hipSuccess;
   int nb = (int)((n + kRsTile - 1) / kRsTile);
-  hipError_t e = hist.reserve(sizeof(int) * 256 * (size_t)nb * 2);
+  hipError_t e = hist.reserve(sizeof(int) * (256 * (size_t)nb * 2 + 256));
   if (e != hipSuccess) return e;
   int *h = hist.as<int>();
   int *o = h + 256 * (size_t)nb;
+  int *dt = o + 256 * (size_t)nb;
+  (void)scan_tmp;
+  (void)total;
   uint64_t *ka = keys, *kb = keys_tmp;
   int *va = idx, *vb = idx_tmp;
   for (int pass = 0; pass < passes; pass++) {
     int shift = 8 * pass;
     k_rs_hist<<<nb, kRsThreads, 0, s>>>(ka, n, shift, h, nb);
-    e = scan_exclusive(h, o, 256L * nb, total, scan_tmp, s);
-    if (e != hipSuccess) return e;
-    k_rs_scatter<<<nb, kRsThreads, 0, s>>>(ka, va, kb, vb, n, shift, o, nb);
+    k_rs_scan_digits<<<256, kRsThreads, 0, s>>>(h, o, nb, dt);
+    k_rs_scatter<<<nb, kRsThreads, 0, s>>>(ka, va, kb, vb, n, shift, o, dt,
+                                           nb);
     uint64_t *tk = ka;
     ka = kb;
     kb = tk;
