@@ -97,6 +97,8 @@ struct Tables {
 
 struct Workspace {
   DevBuf keys_a, keys_b, idx_a, idx_b, xs, ys, zs, qs, pot_s, grad_s;
+  DevBuf packed;     // (x, y, z, q) per particle in input order
+  DevBuf lvl_table;  // per level: first node, node count (device)
   DevBuf scan_tmp, hist, part;
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
@@ -182,8 +184,9 @@ void root_box(const double *lb, const double *ub, const double *pmin,
 
 // ---- tree.hip
 hipError_t launch_keys(const double *x, const double *y, const double *z,
-                       long n, const long *soff, int nsys, const double *box,
-                       uint64_t *keys, int *idx, hipStream_t s);
+                       const double *q, long n, const long *soff, int nsys,
+                       const double *box, uint64_t *keys, int *idx,
+                       double4 *packed, hipStream_t s);
 hipError_t launch_sys_keys(const int *idx, long n, const long *soff, int nsys,
                            uint64_t *skey, int *pos, hipStream_t s);
 hipError_t launch_permute_keys(const int *pos, long n, const uint64_t *kin,
@@ -201,8 +204,7 @@ hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
 hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
                       int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
                       int *total, int passes, hipStream_t s);
-hipError_t launch_gather(const int *idx, long n, const double *x,
-                         const double *y, const double *z, const double *q,
+hipError_t launch_gather(const int *idx, long n, const double4 *packed,
                          double *xs, double *ys, double *zs, double *qs,
                          hipStream_t s);
 hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
@@ -210,12 +212,20 @@ hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
                           hipStream_t s);
 hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
                         hipStream_t s);
-hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
-                              int lvl_count, const uint64_t *keys, int leafcap,
-                              int *bounds, int *cnt, hipStream_t s);
-hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
-                              int lvl_count, int next_begin, const int *bounds,
+// Level table on the device: dl[2 d] first node of level d, dl[2 d + 1] its
+// node count; max_count bounds the count for the grid.  split_count also
+// sets dl[2 d + 2] (the next level's first node); the scan of cnt writes
+// dl[2 d + 3].
+hipError_t launch_split_count(const NodeArrays &nd, int *dl, int d,
+                              long max_count, const uint64_t *keys,
+                              int leafcap, int *bounds, int *cnt,
+                              hipStream_t s);
+hipError_t launch_split_write(const NodeArrays &nd, const int *dl, int d,
+                              long max_count, const int *bounds,
                               const int *cnt, const int *ofs, hipStream_t s);
+// Exclusive scan of (*n_dev) entries, one launch.
+hipError_t scan_device_count(const int *in, int *out, const int *n_dev,
+                             int *total, hipStream_t s);
 hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
                              hipStream_t s);
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,

@@ -137,7 +137,8 @@ Workspace::~Workspace() {
                    &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
                    &mult_bf, &loc_mfma, &tab.mfma_ops, &n_sys, &sys_box, &sys_off, &sys_mm,
                    &skey, &spos, &internal_list,
-                   &trans_rows};
+                   &trans_rows, &packed,
+                   &lvl_table};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
   if (events_ready) {
@@ -633,9 +634,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   for (DevBuf *b : pbufs) CK(b->reserve(sizeof(double) * n));
   CK(ws.grad_s.reserve(sizeof(double) * 3 * n));
   int *d_total = ws.small.as<int>() + 64;
-  CK(launch_keys(pb.x, pb.y, pb.z, n, ws.sys_off.as<long>(), nsys,
+  CK(ws.packed.reserve(sizeof(double4) * n));
+  CK(launch_keys(pb.x, pb.y, pb.z, pb.q, n, ws.sys_off.as<long>(), nsys,
                  ws.sys_box.as<double>(), ws.keys_a.as<uint64_t>(),
-                 ws.idx_a.as<int>(), s));
+                 ws.idx_a.as<int>(), ws.packed.as<double4>(), s));
   CK(radix_sort(ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(),
                 ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), n, ws.hist,
                 ws.scan_tmp, d_total, 8, s));
@@ -655,7 +657,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     std::swap(ws.keys_a, ws.keys_b);
     std::swap(ws.idx_a, ws.idx_b);
   }
-  CK(launch_gather(ws.idx_a.as<int>(), n, pb.x, pb.y, pb.z, pb.q,
+  CK(launch_gather(ws.idx_a.as<int>(), n, ws.packed.as<double4>(),
                    ws.xs.as<double>(), ws.ys.as<double>(), ws.zs.as<double>(),
                    ws.qs.as<double>(), s));
   const uint64_t *keys = ws.keys_a.as<uint64_t>();
@@ -665,29 +667,69 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (ws.node_cap < cap0) CK(reserve_nodes(ws, cap0, 0));
   NodeArrays nd = ws.nodes();
   CK(launch_roots(nd, nsys, ws.sys_off.as<long>(), s));
+  // Levels in batches of kLevelBatch without a host round trip: the level
+  // table lives on the device and grids are sized by upper bounds (a level
+  // has at most 8 nodes per split node of the level above, and a split node
+  // holds more than leafcap particles); one read per batch brings the
+  // counts back.
   std::vector<int> lstart = {0}, lcount = {nsys};
   long nnodes = nsys;
-  for (int d = 0;; d++) {
-    int lc = lcount[d], lb = lstart[d];
-    CK(ws.split_bounds.reserve(sizeof(int) * 9 * (size_t)lc));
-    CK(ws.split_cnt.reserve(sizeof(int) * (size_t)lc));
-    CK(ws.split_ofs.reserve(sizeof(int) * ((size_t)lc + 1)));
-    CK(launch_split_count(nd, lb, lc, keys, pb.leafcap,
-                          ws.split_bounds.as<int>(), ws.split_cnt.as<int>(), s));
-    CK(scan_small(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), lc,
-                      d_total, ws.scan_tmp, s));
-    int total = 0;
-    CK(read_int(ws, d_total, &total));
-    if (nnodes + total > ws.node_cap) {
-      CK(reserve_nodes(ws, 2 * (nnodes + total), nnodes));
-      nd = ws.nodes();
+  {
+    constexpr int kLevelBatch = 4;
+    const long max_split = n / ((long)pb.leafcap + 1);
+    CK(ws.lvl_table.reserve(sizeof(int) * 2 * (kMaxDepth + 3)));
+    int *d_lvl = ws.lvl_table.as<int>();
+    const int lvl0[2] = {0, nsys};
+    CK(hipMemcpyAsync(d_lvl, lvl0, sizeof(lvl0), hipMemcpyHostToDevice, s));
+    long ub = nsys;  // bound on the count of level d
+    bool done = false;
+    for (int d0 = 0; !done && d0 <= kMaxDepth; d0 += kLevelBatch) {
+      const int d1 = std::min(kMaxDepth, d0 + kLevelBatch - 1);
+      // bounds of the levels in this batch and of the nodes they add
+      std::vector<long> ubs;
+      long b = ub, add = 0, bmax = 1;
+      for (int d = d0; d <= d1; d++) {
+        ubs.push_back(b);
+        bmax = std::max(bmax, b);
+        b = std::min(8 * b, 8 * max_split);
+        add += b;
+      }
+      CK(ws.split_bounds.reserve(sizeof(int) * 9 * (size_t)bmax));
+      CK(ws.split_cnt.reserve(sizeof(int) * (size_t)bmax));
+      CK(ws.split_ofs.reserve(sizeof(int) * ((size_t)bmax + 1)));
+      if (nnodes + add > ws.node_cap) {
+        CK(reserve_nodes(ws, std::max(2 * ws.node_cap, nnodes + add), nnodes));
+        nd = ws.nodes();
+      }
+      for (int d = d0; d <= d1; d++) {
+        const long ubd = ubs[d - d0];
+        CK(launch_split_count(nd, d_lvl, d, ubd, keys, pb.leafcap,
+                              ws.split_bounds.as<int>(),
+                              ws.split_cnt.as<int>(), s));
+        CK(scan_device_count(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(),
+                             d_lvl + 2 * d + 1, d_lvl + 2 * d + 3, s));
+        CK(launch_split_write(nd, d_lvl, d, ubd, ws.split_bounds.as<int>(),
+                              ws.split_cnt.as<int>(), ws.split_ofs.as<int>(),
+                              s));
+      }
+      int *h_tab = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
+                                           3072);
+      const int nget = 2 * (d1 - d0 + 1);
+      CK(hipMemcpyAsync(h_tab, d_lvl + 2 * d0 + 2, sizeof(int) * nget,
+                        hipMemcpyDeviceToHost, s));
+      CK(hipStreamSynchronize(s));
+      for (int d = d0; d <= d1; d++) {
+        const int cnt = h_tab[2 * (d - d0) + 1];
+        if (cnt == 0) {
+          done = true;
+          break;
+        }
+        lstart.push_back(h_tab[2 * (d - d0)]);
+        lcount.push_back(cnt);
+        nnodes += cnt;
+      }
+      ub = std::min(8 * ubs.back(), 8 * max_split);
     }
-    CK(launch_split_write(nd, lb, lc, (int)nnodes, ws.split_bounds.as<int>(),
-                          ws.split_cnt.as<int>(), ws.split_ofs.as<int>(), s));
-    if (total == 0) break;
-    lstart.push_back((int)nnodes);
-    lcount.push_back(total);
-    nnodes += total;
   }
   const int depth = (int)lstart.size() - 1;
   const int nn = (int)nnodes;

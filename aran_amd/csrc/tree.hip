@@ -31,28 +31,36 @@ __device__ inline int system_of(const long *__restrict__ soff, int nsys,
   return lo;
 }
 
+// Morton keys and identity indices; also packs (x, y, z, q) per particle
+// so that the gather after the sort reads one 32-byte record per particle
+// instead of four scattered words.
 __global__ void k_keys(const double *__restrict__ x,
                        const double *__restrict__ y,
-                       const double *__restrict__ z, long n,
+                       const double *__restrict__ z,
+                       const double *__restrict__ q, long n,
                        const long *__restrict__ soff, int nsys,
                        const double *__restrict__ box,
-                       uint64_t *__restrict__ keys, int *__restrict__ idx) {
+                       uint64_t *__restrict__ keys, int *__restrict__ idx,
+                       double4 *__restrict__ packed) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
   const int sy = system_of(soff, nsys, i);
   const double *b = box + 4 * sy;
   const double scale = 2097152.0 / b[3];
-  keys[i] = morton_key(icoord(x[i], b[0], scale), icoord(y[i], b[1], scale),
-                       icoord(z[i], b[2], scale));
+  const double xi = x[i], yi = y[i], zi = z[i];
+  keys[i] = morton_key(icoord(xi, b[0], scale), icoord(yi, b[1], scale),
+                       icoord(zi, b[2], scale));
   idx[i] = (int)i;
+  packed[i] = make_double4(xi, yi, zi, q[i]);
 }
 
 hipError_t launch_keys(const double *x, const double *y, const double *z,
-                       long n, const long *soff, int nsys, const double *box,
-                       uint64_t *keys, int *idx, hipStream_t s) {
+                       const double *q, long n, const long *soff, int nsys,
+                       const double *box, uint64_t *keys, int *idx,
+                       double4 *packed, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_keys<<<(n + 255) / 256, 256, 0, s>>>(x, y, z, n, soff, nsys, box, keys,
-                                         idx);
+  k_keys<<<(n + 255) / 256, 256, 0, s>>>(x, y, z, q, n, soff, nsys, box, keys,
+                                         idx, packed);
   return hipGetLastError();
 }
 
@@ -133,14 +141,16 @@ __global__ void k_sys_minmax(const double *__restrict__ x,
     part[((size_t)sy * nb + blockIdx.y) * 6 + threadIdx.x] = red[threadIdx.x][0];
 }
 
+// One block per system: its nb partial extents combined (a NaN minimum
+// marks a non-finite coordinate and survives).
 __global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
                                    int nb, double *__restrict__ out) {
-  int sy = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sy >= nsys) return;
+  __shared__ double red[6][256];
+  const int sy = blockIdx.x;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
                  -INFINITY};
   bool bad = false;
-  for (int b = 0; b < nb; b++) {
+  for (int b = threadIdx.x; b < nb; b += 256) {
     const double *p = part + ((size_t)sy * nb + b) * 6;
     for (int k = 0; k < 3; k++) {
       bad |= isnan(p[k]);
@@ -148,12 +158,27 @@ __global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
     }
     for (int k = 3; k < 6; k++) v[k] = fmax(v[k], p[k]);
   }
-  for (int k = 0; k < 6; k++) out[6 * sy + k] = v[k];
-  if (bad) out[6 * sy] = NAN;
+  if (bad) v[0] = NAN;
+  for (int k = 0; k < 6; k++) red[k][threadIdx.x] = v[k];
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) {
+      for (int k = 0; k < 3; k++) {
+        double a = red[k][threadIdx.x], b = red[k][threadIdx.x + st];
+        red[k][threadIdx.x] = (isnan(a) || isnan(b)) ? NAN : fmin(a, b);
+      }
+      for (int k = 3; k < 6; k++)
+        red[k][threadIdx.x] = fmax(red[k][threadIdx.x], red[k][threadIdx.x + st]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x < 6) out[6 * sy + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // part must hold 6 * nsys * minmax_blocks(nsys) doubles.
-int minmax_blocks(int nsys) { return std::max(1, std::min(128, 2048 / std::max(nsys, 1))); }
+int minmax_blocks(int nsys) {
+  return std::max(1, std::min(1024, 8192 / std::max(nsys, 1)));
+}
 
 hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
                              const long *soff, int nsys, double *part,
@@ -161,7 +186,7 @@ hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
   if (nsys == 0) return hipSuccess;
   const int nb = minmax_blocks(nsys);
   k_sys_minmax<<<dim3(nsys, nb), 256, 0, s>>>(x, y, z, soff, part);
-  k_sys_minmax_final<<<(nsys + 63) / 64, 64, 0, s>>>(part, nsys, nb, out);
+  k_sys_minmax_final<<<nsys, 256, 0, s>>>(part, nsys, nb, out);
   return hipGetLastError();
 }
 
@@ -260,57 +285,72 @@ hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
   return e == hipSuccess ? hipGetLastError() : e;
 }
 
-// One-workgroup exclusive scan for up to kScan1Max entries (in and out may
-// alias); writes the grand total to *total.  Thread t scans entries
-// [kScan1Per t, kScan1Per (t + 1)) held in registers (all of its loads in
-// flight at once), so the whole scan is one launch and two memory round
+// One-workgroup exclusive scan (in and out may alias); writes the grand
+// total to *total.  Tiles of kScan1Max entries in turn: thread t scans
+// entries [kScan1Per t, kScan1Per (t + 1)) of the tile held in registers
+// (all of its loads in flight at once), so a tile costs two memory round
 // trips; the chunk is short because a single CU serves every access.
 constexpr int kScan1Threads = 1024;
 constexpr int kScan1Per = 16;
 constexpr long kScan1Max = (long)kScan1Threads * kScan1Per;
 
 __global__ __launch_bounds__(kScan1Threads) void k_scan_single(
-    const int *in, int *out, int n, int *__restrict__ total) {
+    const int *in, int *out, int n, const int *__restrict__ n_dev,
+    int *__restrict__ total) {
   __shared__ int ws[kScan1Threads / 64];
+  if (n_dev) n = *n_dev;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  const int b = t * kScan1Per;
-  int v[kScan1Per];
-  int sum = 0;
+  int carry = 0;
+  for (int base = 0; base < n; base += (int)kScan1Max) {
+    const int b = base + t * kScan1Per;
+    int v[kScan1Per];
+    int sum = 0;
 #pragma unroll
-  for (int k = 0; k < kScan1Per; k++) {
-    v[k] = b + k < n ? in[b + k] : 0;
-    sum += v[k];
-  }
-  int x = sum;  // inclusive scan over the wave
-  for (int off = 1; off < 64; off <<= 1) {
-    const int y = __shfl_up(x, off);
-    if (lane >= off) x += y;
-  }
-  if (lane == 63) ws[w] = x;
-  __syncthreads();
-  if (w == 0) {
-    int u = lane < kScan1Threads / 64 ? ws[lane] : 0;
-    for (int off = 1; off < kScan1Threads / 64; off <<= 1) {
-      const int y = __shfl_up(u, off);
-      if (lane >= off) u += y;
+    for (int k = 0; k < kScan1Per; k++) {
+      v[k] = b + k < n ? in[b + k] : 0;
+      sum += v[k];
     }
-    if (lane < kScan1Threads / 64) ws[lane] = u;
-  }
-  __syncthreads();
-  int run = x - sum + (w > 0 ? ws[w - 1] : 0);
+    int x = sum;  // inclusive scan over the wave
+    for (int off = 1; off < 64; off <<= 1) {
+      const int y = __shfl_up(x, off);
+      if (lane >= off) x += y;
+    }
+    if (lane == 63) ws[w] = x;
+    __syncthreads();
+    if (w == 0) {
+      int u = lane < kScan1Threads / 64 ? ws[lane] : 0;
+      for (int off = 1; off < kScan1Threads / 64; off <<= 1) {
+        const int y = __shfl_up(u, off);
+        if (lane >= off) u += y;
+      }
+      if (lane < kScan1Threads / 64) ws[lane] = u;
+    }
+    __syncthreads();
+    int run = carry + x - sum + (w > 0 ? ws[w - 1] : 0);
 #pragma unroll
-  for (int k = 0; k < kScan1Per; k++) {
-    if (b + k < n) out[b + k] = run;
-    run += v[k];
+    for (int k = 0; k < kScan1Per; k++) {
+      if (b + k < n) out[b + k] = run;
+      run += v[k];
+    }
+    carry += ws[kScan1Threads / 64 - 1];
+    __syncthreads();
   }
-  if (t == kScan1Threads - 1) *total = run;
+  if (t == 0) *total = carry;
+}
+
+// Exclusive scan of (*n_dev) entries in one launch (the count is read on
+// the device, so the host need not know it).
+hipError_t scan_device_count(const int *in, int *out, const int *n_dev,
+                             int *total, hipStream_t s) {
+  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, 0, n_dev, total);
+  return hipGetLastError();
 }
 
 // Exclusive scan in one launch when n is small enough, else scan_exclusive.
 hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
                       hipStream_t s) {
   if (n > kScan1Max) return scan_exclusive(in, out, n, total, tmp, s);
-  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, (int)n, total);
+  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, (int)n, nullptr, total);
   return hipGetLastError();
 }
 
@@ -465,28 +505,23 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
 // ------------------------------------------------------------ gather / scatter
 
 __global__ void k_gather(const int *__restrict__ idx, long n,
-                         const double *__restrict__ x,
-                         const double *__restrict__ y,
-                         const double *__restrict__ z,
-                         const double *__restrict__ q, double *__restrict__ xs,
-                         double *__restrict__ ys, double *__restrict__ zs,
-                         double *__restrict__ qs) {
+                         const double4 *__restrict__ packed,
+                         double *__restrict__ xs, double *__restrict__ ys,
+                         double *__restrict__ zs, double *__restrict__ qs) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= n) return;
-  int j = idx[i];
-  xs[i] = x[j];
-  ys[i] = y[j];
-  zs[i] = z[j];
-  qs[i] = q[j];
+  const double4 v = packed[idx[i]];
+  xs[i] = v.x;
+  ys[i] = v.y;
+  zs[i] = v.z;
+  qs[i] = v.w;
 }
 
-hipError_t launch_gather(const int *idx, long n, const double *x,
-                         const double *y, const double *z, const double *q,
+hipError_t launch_gather(const int *idx, long n, const double4 *packed,
                          double *xs, double *ys, double *zs, double *qs,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
-  k_gather<<<(n + 255) / 256, 256, 0, s>>>(idx, n, x, y, z, q, xs, ys, zs,
-                                           qs);
+  k_gather<<<(n + 255) / 256, 256, 0, s>>>(idx, n, packed, xs, ys, zs, qs);
   return hipGetLastError();
 }
 
@@ -554,27 +589,29 @@ __device__ inline int digit_lower_bound(const uint64_t *__restrict__ keys,
   return b;
 }
 
-// Eight lanes per node of the level: lane o finds where octant digit o
-// starts inside the node's sorted key range (lane 0: the range start); the
-// node splits when it holds more than leafcap particles below the maximum
-// depth, and cnt counts its non-empty octants.
-__global__ void k_split_count(NodeArrays nd, int lvl_begin, int lvl_count,
+// Level table dl[2 d] = first node of level d, dl[2 d + 1] = its node count
+// (device), so that consecutive levels launch without a host round trip;
+// grids are sized by host upper bounds and threads beyond the count exit.
+
+// Eight lanes per node of level d: lane o finds where octant digit o starts
+// inside the node's sorted key range (lane 0: the range start); the node
+// splits when it holds more than leafcap particles below the maximum depth,
+// and cnt counts its non-empty octants.  Also sets the next level's first
+// node.
+__global__ void k_split_count(NodeArrays nd, int *__restrict__ dl, int d,
                               const uint64_t *__restrict__ keys, int leafcap,
                               int *__restrict__ bounds, int *__restrict__ cnt) {
+  const int lvl_begin = dl[2 * d], lvl_count = dl[2 * d + 1];
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t == 0) dl[2 * d + 2] = lvl_begin + lvl_count;
+  if (t >= 8L * lvl_count) return;  // whole groups of 8 (lvl_count * 8)
   const int i = (int)(t / 8), o = (int)(t % 8);
-  const bool valid = i < lvl_count;
-  int b = 0, e = 0, d = kMaxDepth;
-  if (valid) {
-    const int node = lvl_begin + i;
-    b = nd.pbegin[node];
-    e = nd.pend[node];
-    d = nd.depth[node];
-  }
-  const bool split = valid && e - b > leafcap && d < kMaxDepth;
+  const int node = lvl_begin + i;
+  const int b = nd.pbegin[node], e = nd.pend[node], dep = nd.depth[node];
+  const bool split = e - b > leafcap && dep < kMaxDepth;
   int x = b;
   if (split && o > 0)
-    x = digit_lower_bound(keys, b, e, 3 * (kMaxDepth - d - 1), o);
+    x = digit_lower_bound(keys, b, e, 3 * (kMaxDepth - dep - 1), o);
   // next boundary: the lane above in the group of 8, the range end for o = 7
   int nx = __shfl_down(x, 1, 8);
   if (o == 7) nx = e;
@@ -582,7 +619,6 @@ __global__ void k_split_count(NodeArrays nd, int lvl_begin, int lvl_count,
   c += __shfl_xor(c, 1, 8);
   c += __shfl_xor(c, 2, 8);
   c += __shfl_xor(c, 4, 8);
-  if (!valid) return;
   if (split) {
     bounds[9 * i + o] = x;
     if (o == 7) bounds[9 * i + 8] = e;
@@ -590,10 +626,12 @@ __global__ void k_split_count(NodeArrays nd, int lvl_begin, int lvl_count,
   if (o == 0) cnt[i] = c;
 }
 
-__global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
-                              int next_begin, const int *__restrict__ bounds,
+__global__ void k_split_write(NodeArrays nd, const int *__restrict__ dl,
+                              int d, const int *__restrict__ bounds,
                               const int *__restrict__ cnt,
                               const int *__restrict__ ofs) {
+  const int lvl_begin = dl[2 * d], lvl_count = dl[2 * d + 1];
+  const int next_begin = lvl_begin + lvl_count;
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= lvl_count) return;
   int node = lvl_begin + i;
@@ -607,10 +645,10 @@ __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
   nd.leaf[node] = 0;
   nd.child[node] = first;
   nd.nchild[node] = cnt[i];
-  const int d = nd.depth[node] + 1;
+  const int dd = nd.depth[node] + 1;
   const int sy = nd.sys[node];
   const double *b = nd.box + 4 * sy;
-  const double h = ldexp(b[3], -d), lo0 = b[0], lo1 = b[1], lo2 = b[2];
+  const double h = ldexp(b[3], -dd), lo0 = b[0], lo1 = b[1], lo2 = b[2];
   int k = 0;
   for (int o = 0; o < 8; o++) {
     int bb = bounds[9 * i + o], e = bounds[9 * i + o + 1];
@@ -619,7 +657,7 @@ __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
     int ix = 2 * nd.ix[node] + (o & 1);
     int iy = 2 * nd.iy[node] + ((o >> 1) & 1);
     int iz = 2 * nd.iz[node] + ((o >> 2) & 1);
-    nd.depth[c] = d;
+    nd.depth[c] = dd;
     nd.ix[c] = ix;
     nd.iy[c] = iy;
     nd.iz[c] = iz;
@@ -637,22 +675,22 @@ __global__ void k_split_write(NodeArrays nd, int lvl_begin, int lvl_count,
   }
 }
 
-hipError_t launch_split_count(const NodeArrays &nd, int lvl_begin,
-                              int lvl_count, const uint64_t *keys, int leafcap,
-                              int *bounds, int *cnt, hipStream_t s) {
-  if (lvl_count == 0) return hipSuccess;
-  const long t = 8L * lvl_count;
-  k_split_count<<<(t + 255) / 256, 256, 0, s>>>(nd, lvl_begin, lvl_count, keys,
-                                                leafcap, bounds, cnt);
+hipError_t launch_split_count(const NodeArrays &nd, int *dl, int d,
+                              long max_count, const uint64_t *keys,
+                              int leafcap, int *bounds, int *cnt,
+                              hipStream_t s) {
+  const long t = 8 * std::max(1L, max_count);
+  k_split_count<<<(t + 255) / 256, 256, 0, s>>>(nd, dl, d, keys, leafcap,
+                                                bounds, cnt);
   return hipGetLastError();
 }
 
-hipError_t launch_split_write(const NodeArrays &nd, int lvl_begin,
-                              int lvl_count, int next_begin, const int *bounds,
+hipError_t launch_split_write(const NodeArrays &nd, const int *dl, int d,
+                              long max_count, const int *bounds,
                               const int *cnt, const int *ofs, hipStream_t s) {
-  if (lvl_count == 0) return hipSuccess;
-  k_split_write<<<(lvl_count + 127) / 128, 128, 0, s>>>(
-      nd, lvl_begin, lvl_count, next_begin, bounds, cnt, ofs);
+  if (max_count <= 0) return hipSuccess;
+  k_split_write<<<(max_count + 127) / 128, 128, 0, s>>>(nd, dl, d, bounds,
+                                                        cnt, ofs);
   return hipGetLastError();
 }
 
