@@ -19,6 +19,8 @@
 // summed in slice order by k_unscale_mfma.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "engine.h"
 #include "m2l_mfma.h"
 #include "m2l_tile.h"
@@ -36,11 +38,28 @@ __device__ inline bf16x8_t as_bf16x8(uint4 u) {
 
 // Workgroup: kMfmaWaves waves on kMfmaWaves / HALVES tiles of one (level,
 // octant) group, consecutive in Morton order, so they share the offset codes
-// of every step; the operator slice of a step (one 16-wide k step of every
-// row block) goes through a ring of kMfmaStages LDS buffers, loaded by the
-// whole workgroup kMfmaStages - 1 steps ahead.
+// of every step.  The operator slices (one 16-wide k step of every row block
+// per step) go through LDS in groups of G steps, double buffered: the whole
+// workgroup loads group j + 1 into registers while it multiplies group j,
+// stores it, and meets at one barrier per group.  B fragments (the sources)
+// are loaded one step ahead, and the first step of the next offset slot PF
+// steps ahead (a slot's first step gathers new source nodes; its later steps
+// reread the same nodes' next bytes).
 constexpr int kMfmaWaves = 8;
-constexpr int kMfmaStages = 3;
+constexpr int kMfmaRingUnits = 8192;  // 16-byte LDS units of the two buffers
+
+template <int B, int E, class F>
+__device__ __forceinline__ void mfma_static_for(F &&f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    mfma_static_for<B + 1, E>(f);
+  }
+}
+
+constexpr __host__ __device__ int mfma_group_steps(int chunk) {
+  const int g = 2 * (kMfmaRingUnits / (4 * chunk));
+  return g < 2 ? 2 : (g > 4 ? 4 : g);
+}
 
 template <int P, int NPROD>
 __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
@@ -55,8 +74,12 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   constexpr int HALVES = 2 / CB;
   constexpr int NPART = NPROD == 6 ? 3 : 2;
   constexpr int CHUNK = RB * NPART * 64;  // 16-byte units of one step
-  constexpr int PER_T = (CHUNK + 64 * kMfmaWaves - 1) / (64 * kMfmaWaves);
-  __shared__ uint4 ring[kMfmaStages][CHUNK];
+  constexpr int NT = 64 * kMfmaWaves;
+  constexpr int G = NPROD == 6 ? 2 : mfma_group_steps(CHUNK), GU = G * CHUNK;
+  constexpr int PER = (GU + NT - 1) / NT;
+  // steps of early B prefetch for a slot's first step (registers allowing)
+  constexpr int PF = NPROD == 3 ? (KS >= 4 ? 3 : (KS == 3 ? 1 : 0)) : 0;
+  __shared__ uint4 ring[2][GU];
   // workgroups are dealt round-robin to the 8 XCDs: give each XCD one
   // contiguous range of tile groups, with all their offset ranges side by
   // side, so that the sources a group gathers stay in that XCD's L2
@@ -82,36 +105,43 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   __syncthreads();
   // global 16-byte unit of step t for copy slot u (layout of the table:
   // [code][ks][rb][part 3][lane])
-  auto src_unit = [&](int t, int u) -> const uint4 * {
+  auto src_unit = [&](int t, int u) __attribute__((always_inline)) -> const uint4 * {
     const int k = k0 + t / KS, s = t % KS;
     const int code = codes[k];
     const int rb = u / (NPART * 64), part = (u / 64) % NPART, ln = u % 64;
     return ops + ((((size_t)code * KS + s) * RB + rb) * 3 + part) * 64 + ln;
   };
-  // prologue: steps 0 .. kMfmaStages - 2 straight into the ring
-  for (int t = 0; t < kMfmaStages - 1 && t < nsteps; t++)
-    for (int u = tid; u < CHUNK; u += 64 * kMfmaWaves)
-      ring[t][u] = *src_unit(t, u);
-  // the next ring slot is staged in registers (up to three units per thread)
-  static_assert(PER_T <= 3, "operator chunk larger than three units/thread");
-  constexpr int NT = 64 * kMfmaWaves;
-  const uint4 zero4 = make_uint4(0, 0, 0, 0);
-  uint4 pre0 = zero4, pre1 = zero4, pre2 = zero4;
-  auto fetch = [&](int t) {
-    if (CHUNK >= NT || tid < CHUNK) pre0 = *src_unit(t, tid);
-    if constexpr (PER_T > 1)
-      if (tid + NT < CHUNK) pre1 = *src_unit(t, tid + NT);
-    if constexpr (PER_T > 2)
-      if (tid + 2 * NT < CHUNK) pre2 = *src_unit(t, tid + 2 * NT);
+  // operator unit u of step group grp (steps past the end read the last
+  // step: a valid address whose value is never used)
+  auto group_unit = [&](int grp, int u) __attribute__((always_inline)) -> const uint4 * {
+    const int t = min(grp * G + u / CHUNK, nsteps - 1);
+    return src_unit(t, u % CHUNK);
   };
-  auto store = [&](uint4 *slot) {
-    if (CHUNK >= NT || tid < CHUNK) slot[tid] = pre0;
-    if constexpr (PER_T > 1)
-      if (tid + NT < CHUNK) slot[tid + NT] = pre1;
-    if constexpr (PER_T > 2)
-      if (tid + 2 * NT < CHUNK) slot[tid + 2 * NT] = pre2;
+  const int ngroups = (nsteps + G - 1) / G;
+  if (nsteps > 0)
+    for (int u = tid; u < GU; u += NT) ring[0][u] = *group_unit(0, u);
+  // staging registers of the next group (named variables: an array here
+  // is moved to LDS by the compiler)
+  static_assert(PER <= 8, "operator group larger than eight units/thread");
+  uint4 p0, p1, p2, p3, p4, p5, p6, p7;
+#define ARAN_GFETCH(i, v)                                     \
+  if constexpr (PER > i)                                      \
+    if (tid + i * NT < GU) v = *group_unit(grp, tid + i * NT);
+#define ARAN_GSTORE(i, v)                                     \
+  if constexpr (PER > i)                                      \
+    if (tid + i * NT < GU) slot[tid + i * NT] = v;
+  auto gfetch = [&](int grp) __attribute__((always_inline)) {
+    ARAN_GFETCH(0, p0) ARAN_GFETCH(1, p1) ARAN_GFETCH(2, p2)
+    ARAN_GFETCH(3, p3) ARAN_GFETCH(4, p4) ARAN_GFETCH(5, p5)
+    ARAN_GFETCH(6, p6) ARAN_GFETCH(7, p7)
   };
-  if (kMfmaStages - 1 < nsteps) fetch(kMfmaStages - 1);
+  auto gstore = [&](uint4 *slot) __attribute__((always_inline)) {
+    ARAN_GSTORE(0, p0) ARAN_GSTORE(1, p1) ARAN_GSTORE(2, p2)
+    ARAN_GSTORE(3, p3) ARAN_GSTORE(4, p4) ARAN_GSTORE(5, p5)
+    ARAN_GSTORE(6, p6) ARAN_GSTORE(7, p7)
+  };
+#undef ARAN_GFETCH
+#undef ARAN_GSTORE
   f32x16_t acc[RB][CB];
 #pragma unroll
   for (int rb = 0; rb < RB; rb++)
@@ -124,7 +154,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   // for them until they are used a whole slot later).
   const int tclamp = max(tile, 0);
   const bool has_tile = tile >= 0;
-  auto load_src = [&](int k, int *dst) {
+  auto load_src = [&](int k, int *dst) __attribute__((always_inline)) {
     const int kc = min(k, kOctOffsets - 1);
 #pragma unroll
     for (int cb = 0; cb < CB; cb++)
@@ -134,7 +164,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   // B fragments (k step s) of the sources srcv; a missing source reads the
   // all-zero node zero_node
   auto load_b = [&](const int *srcv, bool ok, int s,
-                    bf16x8_t (&bb)[CB][NPART]) {
+                    bf16x8_t (&bb)[CB][NPART]) __attribute__((always_inline)) {
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
       const int node = (ok && srcv[cb] >= 0) ? srcv[cb] : zero_node;
@@ -144,40 +174,44 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
     }
   };
   // whether any column of the wave has a source at this slot (wave-uniform)
-  auto alive = [&](const int *srcv, bool ok) -> bool {
+  auto alive = [&](const int *srcv, bool ok) __attribute__((always_inline)) -> bool {
     bool any = false;
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) any |= ok && srcv[cb] >= 0;
     return __ballot(any) != 0ull;
   };
   int src_c[CB], src_n[CB];
-  bf16x8_t bA[CB][NPART], bB[CB][NPART];
+  bf16x8_t bA[CB][NPART], bB[CB][NPART], bN[CB][PF > 0 ? NPART : 1];
   load_src(k0, src_c);
   load_src(k0 + 1, src_n);
   load_b(src_c, has_tile, 0, bA);
   bool live = alive(src_c, has_tile && k0 < k1);
-  // one step: MFMAs of step t from `cur`, B of step t + 1 into `nxt`
-  auto step = [&](int t, bf16x8_t (&cur)[CB][NPART],
-                  bf16x8_t (&nxt)[CB][NPART]) {
-    __syncthreads();
-    // ring slot of step t + kMfmaStages - 1 (read last at step t - 1)
-    const int tw = t + kMfmaStages - 1;
-    if (tw < nsteps) store(ring[tw % kMfmaStages]);
-    fetch(min(tw + 1, nsteps - 1));
+  // one step: MFMAs of step t from `cur` and the LDS slice ra, B of step
+  // t + 1 into `nxt`
+  auto step = [&](int t, const uint4 *ra, bf16x8_t (&cur)[CB][NPART],
+                  bf16x8_t (&nxt)[CB][NPART]) __attribute__((always_inline)) {
     const int k = k0 + t / KS, s = t % KS;
     const bool next_slot = s + 1 == KS;
     if (next_slot) {
-      load_b(src_n, has_tile && k + 1 < k1, 0, nxt);
+      if constexpr (PF > 0) {
+#pragma unroll
+        for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+          for (int q = 0; q < NPART; q++) nxt[cb][q] = bN[cb][q];
+      } else {
+        load_b(src_n, has_tile && k + 1 < k1, 0, nxt);
+      }
 #pragma unroll
       for (int cb = 0; cb < CB; cb++) src_c[cb] = src_n[cb];
       load_src(k + 2, src_n);
     } else {
       load_b(src_c, has_tile, s + 1, nxt);
+      if constexpr (PF > 0)
+        if (s == KS - 1 - PF) load_b(src_n, has_tile && k + 1 < k1, 0, bN);
     }
     const bool live_t = live;
     if (next_slot) live = alive(src_c, has_tile && k + 1 < k1);
     if (!live_t) return;  // no source in any column: skip the products
-    const uint4 *ra = ring[t % kMfmaStages];
     bf16x8_t a[RB][NPART];
 #pragma unroll
     for (int rb = 0; rb < RB; rb++)
@@ -199,13 +233,26 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
               a[rb][QA[q]], cur[cb][QB[q]], acc[rb][cb], 0, 0, 0);
     }
   };
-  // two steps per iteration with ping-pong B buffers (no register copies)
-  int t = 0;
-  for (; t + 1 < nsteps; t += 2) {
-    step(t, bA, bB);
-    step(t + 1, bB, bA);
+  __syncthreads();  // group 0 in the ring
+  // G is even, so the B buffer of step t is fixed by t's parity within the
+  // group (no register copies between buffers).  Steps past the end would
+  // fall in the slot after the range, whose products are skipped (no live
+  // source), with operator units repeating the last step, so they may run;
+  // the three-product form skips them (the register allocation of the two
+  // forms prefers opposite choices).
+  for (int grp = 0; grp < ngroups; grp++) {
+    if (grp + 1 < ngroups) gfetch(grp + 1);
+    const uint4 *rg = ring[grp & 1];
+#pragma unroll
+    for (int gg = 0; gg < G; gg += 2) {
+      const int t = grp * G + gg;
+      if (NPROD == 6 || t < nsteps) step(t, rg + gg * CHUNK, bA, bB);
+      if (NPROD == 6 || t + 1 < nsteps)
+        step(t + 1, rg + (gg + 1) * CHUNK, bB, bA);
+    }
+    if (grp + 1 < ngroups) gstore(ring[(grp + 1) & 1]);
+    __syncthreads();
   }
-  if (t < nsteps) step(t, bA, bB);
   if (tile < 0) return;
   // slice layout [split][tile][row][64]: each store is two 128-byte rows
   float *out = Lsm + ((size_t)split * ntiles + tile) * RO * 64;
