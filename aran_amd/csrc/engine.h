@@ -235,8 +235,10 @@ hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
 hipError_t launch_level_internal(const int *pos, const int *total,
                                  const int *lstart, int nlevels, int nnodes,
                                  int *out, hipStream_t s);
-hipError_t launch_colleagues(const NodeArrays &nd, int first, int count,
-                             int *col, hipStream_t s);
+// colleagues of every node (lstart / lcount of the levels on the device)
+hipError_t launch_colleagues_all(const NodeArrays &nd, int nnodes,
+                                 const int *lstart, const int *lcount,
+                                 int *col, hipStream_t s);
 hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
                         int pass, int *cnt, const int *ofs, int *src,
                         short *code, hipStream_t s);

@@ -778,8 +778,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // ---- colleagues, level by level
   CK(ws.colleagues.reserve(sizeof(int) * 27 * (size_t)nn));
   int *col = ws.colleagues.as<int>();
-  for (int d = 0; d <= depth; d++)
-    CK(launch_colleagues(nd, lstart[d], lcount[d], col, s));
+  CK(launch_colleagues_all(nd, nn, ws.lstart.as<int>(), ws.lcount.as<int>(),
+                           col, s));
   // ---- far list (CSR; the mixed form reads its far field straight from
   // the colleagues and builds this only when the lists are exported)
   int nfar = 0;

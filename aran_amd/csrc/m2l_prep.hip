@@ -253,9 +253,9 @@ __global__ void k_scale_mult(NodeArrays nd, int nnodes, int p,
   int node = (int)(idx / mc), k = (int)(idx % mc);
   int l = 0;
   while (tri_p(l + 1) <= k) l++;
-  double w = node_width(nd, node);
+  const double iw = 1. / node_width(nd, node);
   double f = 1.;
-  for (int i = 0; i < l; i++) f /= w;
+  for (int i = 0; i < l; i++) f *= iw;
   double2 v = M[idx];
   typename V2<T>::type o;
   o.x = (T)(v.x * f);
@@ -275,8 +275,9 @@ __global__ __launch_bounds__(256) void k_unscale_local(
     const typename V2<T>::type *__restrict__ Ls, int nsplit,
     double2 *__restrict__ L) {
   __shared__ double2 sl[64 * (kUnscaleRows + 1)];
-  __shared__ double sw[64];
+  __shared__ double spw[64][kMaxOrder + 2];  // 1 / w^(l+1)
   __shared__ int snode[64];
+  __shared__ unsigned char sdeg[(kMaxOrder + 2) * (kMaxOrder + 3) / 2];
   const int lc = tri_p(p + 1);
   const int row = kUnscaleRows + 1;  // odd row stride in LDS
   const int tile = blockIdx.x;
@@ -285,7 +286,17 @@ __global__ __launch_bounds__(256) void k_unscale_local(
   if (threadIdx.x < 64) {
     const int node = tile_tgt[tile * 64 + threadIdx.x];
     snode[threadIdx.x] = node;
-    sw[threadIdx.x] = node >= 0 ? node_width(nd, node) : 1.;
+    const double iw = node >= 0 ? 1. / node_width(nd, node) : 1.;
+    double f = iw;
+    for (int l = 0; l <= p; l++) {
+      spw[threadIdx.x][l] = f;
+      f *= iw;
+    }
+  }
+  for (int k = threadIdx.x; k < lc; k += blockDim.x) {
+    int l = 0;
+    while (tri_p(l + 1) <= k) l++;
+    sdeg[k] = (unsigned char)l;
   }
   for (int k0 = 0; k0 < lc; k0 += kUnscaleRows) {
     const int nk = min(kUnscaleRows, lc - k0);
@@ -305,11 +316,7 @@ __global__ __launch_bounds__(256) void k_unscale_local(
       const int lane = e / nk, kk = e % nk, k = k0 + kk;
       const int node = snode[lane];
       if (node < 0) continue;
-      int l = 0;
-      while (tri_p(l + 1) <= k) l++;
-      const double w = sw[lane];
-      double f = 1. / w;
-      for (int i = 0; i < l; i++) f /= w;
+      const double f = spw[lane][sdeg[k]];
       const double2 v = sl[lane * row + kk];
       L[(long)node * lc + k] = make_double2(v.x * f, v.y * f);
     }
@@ -462,7 +469,7 @@ __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
     mb[t * 3 + 2] = z;
     return;
   }
-  const double w = node_width(nd, node);
+  const double iw = 1. / node_width(nd, node);
   const double2 *Mn = M + (size_t)node * D.mc;
   double v[8];
   for (int j = 0; j < 8; j++) {
@@ -482,7 +489,7 @@ __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
     }
     if (idx >= 0) {
       double f = 1.;
-      for (int i = 0; i < l; i++) f /= w;
+      for (int i = 0; i < l; i++) f *= iw;
       x = (im ? Mn[idx].y : Mn[idx].x) * f;
     }
     v[j] = x;
@@ -516,8 +523,10 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
     NodeArrays nd, int p, int ntiles, const int *__restrict__ tile_tgt,
     const float *__restrict__ Lsm, int nsplit, double2 *__restrict__ L) {
   __shared__ double2 sl[64 * (kUnscaleRows + 1)];
-  __shared__ double sw[64];
+  __shared__ double spw[64][kMaxOrder + 2];  // 1 / w^(l+1)
   __shared__ int snode[64];
+  __shared__ short srim[(kMaxOrder + 2) * (kMaxOrder + 3) / 2];
+  __shared__ unsigned char sdeg[(kMaxOrder + 2) * (kMaxOrder + 3) / 2];
   const MfmaDims D = mfma_dims(p);
   const int lc = D.lc;
   const int row = kUnscaleRows + 1;  // odd row stride in LDS
@@ -527,17 +536,27 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
   if (threadIdx.x < 64) {
     const int node = tile_tgt[tile * 64 + threadIdx.x];
     snode[threadIdx.x] = node;
-    sw[threadIdx.x] = node >= 0 ? node_width(nd, node) : 1.;
+    const double iw = node >= 0 ? 1. / node_width(nd, node) : 1.;
+    double f = iw;
+    for (int l = 0; l <= p; l++) {
+      spw[threadIdx.x][l] = f;
+      f *= iw;
+    }
+  }
+  // per coefficient: degree, and the row of its imaginary part (-1: m = 0)
+  for (int k = threadIdx.x; k < lc; k += blockDim.x) {
+    int l = 0;
+    while (tri_p(l + 1) <= k) l++;
+    const int m = k - tri_p(l);
+    sdeg[k] = (unsigned char)l;
+    srim[k] = (short)(m > 0 ? lc + tri_p(l - 1) + m - 1 : -1);
   }
   for (int k0 = 0; k0 < lc; k0 += kUnscaleRows) {
     const int nk = min(kUnscaleRows, lc - k0);
     __syncthreads();
     for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
       const int kk = e / 64, lane = e % 64, k = k0 + kk;
-      int l = 0;
-      while (tri_p(l + 1) <= k) l++;
-      const int m = k - tri_p(l);
-      const int rim = m > 0 ? lc + tri_p(l - 1) + m - 1 : -1;
+      const int rim = srim[k];
       double sx = 0., sy = 0.;
       for (int s = 0; s < nsplit; s++) {
         sx += (double)src[s * slice + (long)k * 64 + lane];
@@ -550,11 +569,7 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
       const int lane = e / nk, kk = e % nk, k = k0 + kk;
       const int node = snode[lane];
       if (node < 0) continue;
-      int l = 0;
-      while (tri_p(l + 1) <= k) l++;
-      const double w = sw[lane];
-      double f = 1. / w;
-      for (int i = 0; i < l; i++) f /= w;
+      const double f = spw[lane][sdeg[k]];
       const double2 v = sl[lane * row + kk];
       L[(long)node * lc + k] = make_double2(v.x * f, v.y * f);
     }

@@ -756,43 +756,51 @@ hipError_t launch_level_internal(const int *pos, const int *total,
 // ------------------------------------------------------------ lists
 
 // Colleagues: the 27 same-depth nodes adjacent to (or equal to) a node,
-// stored by offset slot (dx+1)*9 + (dy+1)*3 + (dz+1), -1 when absent.  The
-// neighbour of node A (octant b in its parent P) at offset d has parent
-// P + floor((b + d) / 2), a colleague of P, and octant (b + d) mod 2 there;
-// so one launch per level, root first, fills the slots with one thread per
-// (node, slot) and a scan of at most 8 children.
-__global__ void k_colleagues(NodeArrays nd, int first, int count,
-                             int *__restrict__ col) {
+// stored by offset slot (dx+1)*9 + (dy+1)*3 + (dz+1), -1 when absent,
+// for every node in one launch: the neighbour at offset d has the node's
+// level, system and coordinates plus d, and a level's nodes are ordered by
+// (system, level key), so each slot is one binary search in that level
+// (lstart / lcount: the levels' first node and node count).
+__global__ void k_colleagues_all(NodeArrays nd, int nnodes,
+                                 const int *__restrict__ lstart,
+                                 const int *__restrict__ lcount,
+                                 int *__restrict__ col) {
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= 27L * count) return;
-  const int A = first + (int)(t / 27), slot = (int)(t % 27);
+  if (t >= 27L * nnodes) return;
+  const int A = (int)(t / 27), slot = (int)(t % 27);
+  const int d = nd.depth[A];
+  const int dx = slot / 9 - 1, dy = (slot / 3) % 3 - 1, dz = slot % 3 - 1;
+  const int nx = nd.ix[A] + dx, ny = nd.iy[A] + dy, nz = nd.iz[A] + dz;
+  const int side = 1 << d;
   int res = -1;
-  if (nd.depth[A] == 0) {
-    res = slot == 13 ? A : -1;
-  } else {
-    const int dx = slot / 9 - 1, dy = (slot / 3) % 3 - 1, dz = slot % 3 - 1;
-    const int ux = (nd.ix[A] & 1) + dx, uy = (nd.iy[A] & 1) + dy,
-              uz = (nd.iz[A] & 1) + dz;  // in [-1, 2]
-    const int ps = ((ux >> 1) + 1) * 9 + ((uy >> 1) + 1) * 3 + ((uz >> 1) + 1);
-    const int C = col[(size_t)nd.parent[A] * 27 + ps];
-    if (C >= 0 && !nd.leaf[C]) {
-      const int ox = ux & 1, oy = uy & 1, oz = uz & 1;
-      for (int K = nd.child[C]; K < nd.child[C] + nd.nchild[C]; K++)
-        if ((nd.ix[K] & 1) == ox && (nd.iy[K] & 1) == oy &&
-            (nd.iz[K] & 1) == oz) {
-          res = K;
-          break;
-        }
+  if (slot == 13) {
+    res = A;
+  } else if (nx >= 0 && ny >= 0 && nz >= 0 && nx < side && ny < side &&
+             nz < side) {
+    const uint64_t key = morton_key(nx, ny, nz);
+    const int sy = nd.sys[A];
+    int lo = lstart[d], hi = lstart[d] + lcount[d];
+    while (lo < hi) {  // first node not below (sy, key)
+      const int mid = lo + ((hi - lo) >> 1);
+      const int ms = nd.sys[mid];
+      if (ms < sy || (ms == sy && nd.key[mid] < key))
+        lo = mid + 1;
+      else
+        hi = mid;
     }
+    if (lo < lstart[d] + lcount[d] && nd.sys[lo] == sy && nd.key[lo] == key)
+      res = lo;
   }
   col[(size_t)A * 27 + slot] = res;
 }
 
-hipError_t launch_colleagues(const NodeArrays &nd, int first, int count,
-                             int *col, hipStream_t s) {
-  if (count == 0) return hipSuccess;
-  const long t = 27L * count;
-  k_colleagues<<<(t + 255) / 256, 256, 0, s>>>(nd, first, count, col);
+hipError_t launch_colleagues_all(const NodeArrays &nd, int nnodes,
+                                 const int *lstart, const int *lcount,
+                                 int *col, hipStream_t s) {
+  if (nnodes == 0) return hipSuccess;
+  const long t = 27L * nnodes;
+  k_colleagues_all<<<(t + 255) / 256, 256, 0, s>>>(nd, nnodes, lstart, lcount,
+                                                   col);
   return hipGetLastError();
 }
 
