@@ -75,10 +75,13 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   constexpr int NPART = NPROD == 6 ? 3 : 2;
   constexpr int CHUNK = RB * NPART * 64;  // 16-byte units of one step
   constexpr int NT = 64 * kMfmaWaves;
-  constexpr int G = NPROD == 6 ? 2 : mfma_group_steps(CHUNK), GU = G * CHUNK;
+  // B buffers: three for the three-product form (B loaded two steps
+  // ahead), two for the six-product form (one step ahead; its registers
+  // allow no more); a group holds a multiple of NB steps
+  constexpr int NB = NPROD == 3 ? 3 : 2;
+  constexpr int G = NB == 3 ? (2 * 6 * CHUNK <= kMfmaRingUnits ? 6 : 3) : 2;
+  constexpr int GU = G * CHUNK;
   constexpr int PER = (GU + NT - 1) / NT;
-  // steps of early B prefetch for a slot's first step (registers allowing)
-  constexpr int PF = NPROD == 3 ? (KS >= 4 ? 3 : (KS == 3 ? 1 : 0)) : 0;
   __shared__ uint4 ring[2][GU];
   // workgroups are dealt round-robin to the 8 XCDs: give each XCD one
   // contiguous range of tile groups, with all their offset ranges side by
@@ -180,37 +183,43 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
     for (int cb = 0; cb < CB; cb++) any |= ok && srcv[cb] >= 0;
     return __ballot(any) != 0ull;
   };
-  int src_c[CB], src_n[CB];
-  bf16x8_t bA[CB][NPART], bB[CB][NPART], bN[CB][PF > 0 ? NPART : 1];
+  // sources of the current slot and of the next two
+  int src_c[CB], src_n[CB], src_nn[CB];
+  bf16x8_t bq[NB][CB][NPART];
   load_src(k0, src_c);
   load_src(k0 + 1, src_n);
-  load_b(src_c, has_tile, 0, bA);
+  load_src(k0 + 2, src_nn);
+  // B fragments of step t + d (d < 3) from the step t of slot k = k0 + t / KS
+  auto load_ahead = [&](int k, int s, int d, bf16x8_t (&dst)[CB][NPART])
+      __attribute__((always_inline)) {
+    const int sa = s + d;
+    if (sa < KS)
+      load_b(src_c, has_tile && k < k1, sa, dst);
+    else if (sa < 2 * KS)
+      load_b(src_n, has_tile && k + 1 < k1, sa - KS, dst);
+    else
+      load_b(src_nn, has_tile && k + 2 < k1, sa - 2 * KS, dst);
+  };
+  load_ahead(k0, 0, 0, bq[0]);
+  if constexpr (NB == 3) load_ahead(k0, 0, 1, bq[1]);
   bool live = alive(src_c, has_tile && k0 < k1);
   // one step: MFMAs of step t from `cur` and the LDS slice ra, B of step
-  // t + 1 into `nxt`
+  // t + NB - 1 into `nxt`
   auto step = [&](int t, const uint4 *ra, bf16x8_t (&cur)[CB][NPART],
                   bf16x8_t (&nxt)[CB][NPART]) __attribute__((always_inline)) {
     const int k = k0 + t / KS, s = t % KS;
+    load_ahead(k, s, NB - 1, nxt);
     const bool next_slot = s + 1 == KS;
-    if (next_slot) {
-      if constexpr (PF > 0) {
-#pragma unroll
-        for (int cb = 0; cb < CB; cb++)
-#pragma unroll
-          for (int q = 0; q < NPART; q++) nxt[cb][q] = bN[cb][q];
-      } else {
-        load_b(src_n, has_tile && k + 1 < k1, 0, nxt);
-      }
-#pragma unroll
-      for (int cb = 0; cb < CB; cb++) src_c[cb] = src_n[cb];
-      load_src(k + 2, src_n);
-    } else {
-      load_b(src_c, has_tile, s + 1, nxt);
-      if constexpr (PF > 0)
-        if (s == KS - 1 - PF) load_b(src_n, has_tile && k + 1 < k1, 0, bN);
-    }
     const bool live_t = live;
-    if (next_slot) live = alive(src_c, has_tile && k + 1 < k1);
+    if (next_slot) {
+#pragma unroll
+      for (int cb = 0; cb < CB; cb++) {
+        src_c[cb] = src_n[cb];
+        src_n[cb] = src_nn[cb];
+      }
+      load_src(k + 3, src_nn);
+      live = alive(src_c, has_tile && k + 1 < k1);
+    }
     if (!live_t) return;  // no source in any column: skip the products
     bf16x8_t a[RB][NPART];
 #pragma unroll
@@ -234,22 +243,21 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
     }
   };
   __syncthreads();  // group 0 in the ring
-  // G is even, so the B buffer of step t is fixed by t's parity within the
-  // group (no register copies between buffers).  Steps past the end would
-  // fall in the slot after the range, whose products are skipped (no live
-  // source), with operator units repeating the last step, so they may run;
-  // the three-product form skips them (the register allocation of the two
-  // forms prefers opposite choices).
+  // G is a multiple of NB, so the B buffer of step t is fixed by its place
+  // in the group (no register copies between buffers).  Steps past the end
+  // would fall in the slot after the range, whose products are skipped (no
+  // live source), with operator units repeating the last step, so they may
+  // run; the three-product form skips them (the register allocation of the
+  // two forms prefers opposite choices).
   for (int grp = 0; grp < ngroups; grp++) {
     if (grp + 1 < ngroups) gfetch(grp + 1);
     const uint4 *rg = ring[grp & 1];
-#pragma unroll
-    for (int gg = 0; gg < G; gg += 2) {
+    mfma_static_for<0, G>([&](auto gc) {
+      constexpr int gg = decltype(gc)::value;
       const int t = grp * G + gg;
-      if (NPROD == 6 || t < nsteps) step(t, rg + gg * CHUNK, bA, bB);
-      if (NPROD == 6 || t + 1 < nsteps)
-        step(t + 1, rg + (gg + 1) * CHUNK, bB, bA);
-    }
+      if (NPROD == 6 || t < nsteps)
+        step(t, rg + gg * CHUNK, bq[gg % NB], bq[(gg + NB - 1) % NB]);
+    });
     if (grp + 1 < ngroups) gstore(ring[(grp + 1) & 1]);
     __syncthreads();
   }

@@ -208,8 +208,6 @@ __constant__ InvTable kInv = make_inv_table();
 
 constexpr int kP2mSlots = 1024;  // P2L LDS staging, double2 (16 KiB)
 
-constexpr __host__ __device__ int tri_l(int n) { return n * (n + 1) / 2; }
-
 template <int B, int E, class F>
 __device__ __forceinline__ void static_loop(F &&f) {
   if constexpr (B < E) {
@@ -470,43 +468,38 @@ hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s) {
 //   (d/dx + i d/dy) phi = sum_l [c_l^0 O_{l-1}^1
 //       + sum_{m>0} (c_l^m O_{l-1}^{m+1} - conj(c_l^m O_{l-1}^{m-1}))],
 // accumulated per harmonic O_j^m as it is generated (column by column).
-template <int P>
 __global__ __launch_bounds__(256) void k_l2p(
     OpTables tb, NodeArrays nd, const int *__restrict__ pleaf, int n,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double2 *__restrict__ L,
     double *__restrict__ pot, double *__restrict__ grad) {
-  constexpr int LC = (P + 1) * (P + 2) / 2;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const int p = tb.p, lc = tri(p + 1);
   const int A = pleaf[i];
-  const double2 *__restrict__ La = L + (size_t)A * LC;
+  const double2 *__restrict__ La = L + (size_t)A * lc;
   const double *__restrict__ g = tb.ogain;
   const double vx = xs[i] - nd.cx[A], vy = ys[i] - nd.cy[A],
                vz = zs[i] - nd.cz[A];
   const double r2 = vx * vx + vy * vy + vz * vz;
   double phi = 0., dz = 0., gx = 0., gy = 0.;
   double mx = 1., my = 0.;  // O_m^m
-  // the order is a template parameter: every loop unrolls, so the leaf's
-  // coefficients are loaded ahead of the recurrence that uses them
-  static_loop<0, P + 1>([&](auto mc) {
-    constexpr int m = decltype(mc)::value;
-    if constexpr (m > 0) {
+  for (int m = 0; m <= p; m++) {
+    if (m > 0) {
       const double f = -0.5 * kInv.v[m];
       const double ax = f * vx, ay = f * vy;
       const double nx = mx * ax - my * ay, ny = mx * ay + my * ax;
       mx = nx;
       my = ny;
     }
-    constexpr double w = m ? 2. : 1.;
+    const double w = m ? 2. : 1.;
     double ox = mx, oy = my, qx = 0., qy = 0.;  // O_j^m, O_{j-1}^m
-    double2 c = La[tri_l(m) + m];
-    const double gc = g[tri_l(m) + m];
+    double2 c = La[tri(m) + m];
+    double gc = g[tri(m) + m];
     c.x *= gc;
     c.y *= gc;
-    static_loop<m, P + 1>([&](auto jc) {
-      constexpr int j = decltype(jc)::value;
-      if constexpr (j > m) {
+    for (int j = m; j <= p; j++) {
+      if (j > m) {
         const double a = (2. * j - 1.) * vz, s = kInv.v[j * j - m * m];
         const double nx = (a * ox - r2 * qx) * s, ny = (a * oy - r2 * qy) * s;
         qx = ox;
@@ -515,29 +508,29 @@ __global__ __launch_bounds__(256) void k_l2p(
         oy = ny;
       }
       phi += w * (c.x * ox - c.y * oy);
-      if constexpr (j < P) {
-        constexpr int k1 = tri_l(j + 1) + m;
+      if (j < p) {
+        const int k1 = tri(j + 1) + m;
         double2 c1 = La[k1];  // c_{j+1}^m
         const double g1 = g[k1];
         c1.x *= g1;
         c1.y *= g1;
         dz += w * (c1.x * ox - c1.y * oy);
-        if constexpr (m > 0) {  // c_{j+1}^{m-1} O_j^m
-          const double2 c2 = La[k1 - 1];
+        if (m > 0) {  // c_{j+1}^{m-1} O_j^m
+          double2 c2 = La[k1 - 1];
           const double g2 = g[k1 - 1];
           gx += g2 * (c2.x * ox - c2.y * oy);
           gy += g2 * (c2.x * oy + c2.y * ox);
         }
         {  // - conj(c_{j+1}^{m+1} O_j^m)
-          const double2 c3 = La[k1 + 1];
+          double2 c3 = La[k1 + 1];
           const double g3 = g[k1 + 1];
           gx -= g3 * (c3.x * ox - c3.y * oy);
           gy += g3 * (c3.x * oy + c3.y * ox);
         }
         c = c1;
       }
-    });
-  });
+    }
+  }
   pot[i] += phi;
   grad[3 * (size_t)i + 0] += gx;
   grad[3 * (size_t)i + 1] += gy;
@@ -795,30 +788,14 @@ hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
   return hipGetLastError();
 }
 
-template <int P>
-static hipError_t l2p_dispatch(const OpTables &tb, const NodeArrays &nd,
-                               const int *pleaf, int n, const double *xs,
-                               const double *ys, const double *zs,
-                               const double2 *L, double *pot, double *grad,
-                               hipStream_t s) {
-  if constexpr (P > kMaxOrder) {
-    return hipErrorInvalidValue;
-  } else {
-    if (tb.p != P)
-      return l2p_dispatch<P + 1>(tb, nd, pleaf, n, xs, ys, zs, L, pot, grad,
-                                 s);
-    k_l2p<P><<<(n + 255) / 256, 256, 0, s>>>(tb, nd, pleaf, n, xs, ys, zs, L,
-                                             pot, grad);
-    return hipGetLastError();
-  }
-}
-
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
                       double *pot, double *grad, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  return l2p_dispatch<1>(tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, s);
+  k_l2p<<<(n + 255) / 256, 256, 0, s>>>(tb, nd, pleaf, n, xs, ys, zs, L, pot,
+                                        grad);
+  return hipGetLastError();
 }
 
 }  // namespace aran
