@@ -63,10 +63,17 @@ __global__ __launch_bounds__(kNearThreads) void k_near(
     const double *__restrict__ zs, const double *__restrict__ qs,
     double *__restrict__ pot, double *__restrict__ grad) {
   typedef typename Src4<T>::type S4;
-  __shared__ S4 src[kStage];  // x, y, z relative to the leaf centre; q
+  // staged sources (x, y, z relative to the leaf centre; q), and after the
+  // last pass of a target chunk the per-slice partial sums in the same LDS
+  // (18 KB in FP32, so that more leaves run per CU)
+  constexpr size_t kSrcBytes = sizeof(S4) * kStage,
+                   kRedBytes = sizeof(double) * 4 * kNearThreads;
+  __shared__ __align__(16) unsigned char
+      smem[kSrcBytes > kRedBytes ? kSrcBytes : kRedBytes];
+  S4 *src = reinterpret_cast<S4 *>(smem);
+  double(*red)[kNearThreads] = reinterpret_cast<double(*)[kNearThreads]>(smem);
   __shared__ int lbeg[kListChunk], lpre[kListChunk + 1];
   __shared__ int own;  // staged position of the target leaf's first particle
-  __shared__ double red[4][kNearThreads];
   const int tid = threadIdx.x;
   const int A = leaves[blockIdx.x];
   const int tb = nd.pbegin[A], te = nd.pend[A];
@@ -83,16 +90,31 @@ __global__ __launch_bounds__(kNearThreads) void k_near(
     for (int c0 = e0; c0 < e1; c0 += kListChunk) {
       const int nc = min(kListChunk, e1 - c0);
       __syncthreads();
-      if (tid < nc) lbeg[tid] = ubeg[c0 + tid];
-      if (tid == 0) {
-        int s = 0, o = -(1 << 30);
-        for (int k = 0; k < nc; k++) {
-          lpre[k] = s;
-          if (ubeg[c0 + k] == tb) o = s;
-          s += uend[c0 + k] - ubeg[c0 + k];
+      if (tid < 64) {
+        // wave 0: staged position of each range (prefix of the lengths, by
+        // shuffles, 64 ranges at a time) and of the target leaf's own range
+        if (tid == 0) own = -(1 << 30);
+        int carry = 0;
+        for (int b0 = 0; b0 < nc; b0 += 64) {
+          const int k = b0 + tid;
+          int beg = 0, len = 0;
+          if (k < nc) {
+            beg = ubeg[c0 + k];
+            len = uend[c0 + k] - beg;
+          }
+          int x = len;  // inclusive scan over the wave
+          for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (tid >= off) x += y;
+          }
+          if (k < nc) {
+            lbeg[k] = beg;
+            lpre[k] = carry + x - len;
+            if (beg == tb) own = carry + x - len;
+          }
+          carry += __shfl(x, 63);
         }
-        lpre[nc] = s;
-        own = o;
+        if (tid == 0) lpre[nc] = carry;
       }
       __syncthreads();
       const int total = lpre[nc];
