@@ -838,49 +838,72 @@ __global__ void k_vlist(NodeArrays nd, int nnodes, const int *__restrict__ col,
 
 // P2P (near) list of every leaf as particle ranges: its colleagues (whole
 // subtrees for non-leaf ones) and the leaf colleagues of its ancestors.
-__global__ void k_ulist(NodeArrays nd, int nleaves,
-                        const int *__restrict__ leaves,
-                        const int *__restrict__ col, int pass,
-                        int *__restrict__ cnt, const int *__restrict__ ofs,
-                        int *__restrict__ ubeg, int *__restrict__ uend,
-                        long long *__restrict__ pairs) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nleaves) return;
+// Near list of every leaf: the same-depth colleagues of the leaf and the
+// leaf colleagues of its ancestors (not the ancestors themselves), in the
+// order (depth, slot); pass 0 counts, pass 1 fills.  One wave per leaf:
+// lanes 0-26 take the slots of one ancestor level and lanes 32-58 those of
+// the next, so a level costs one round of loads; positions come from a
+// ballot prefix, which keeps the (depth, slot) order.  pairs[t]: reference
+// pair counter (n(n+1)/2 for the leaf itself, half of the ordered count
+// otherwise: each unordered pair is seen twice).
+constexpr int kUlistWaves = 4;
+
+__global__ __launch_bounds__(64 * kUlistWaves) void k_ulist(
+    NodeArrays nd, int nleaves, const int *__restrict__ leaves,
+    const int *__restrict__ col, int pass, int *__restrict__ cnt,
+    const int *__restrict__ ofs, int *__restrict__ ubeg, int *__restrict__ uend,
+    long long *__restrict__ pairs) {
+  __shared__ int anc[kUlistWaves][kMaxDepth + 1];
+  const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
+  const int t = blockIdx.x * kUlistWaves + w;
+  if (t >= nleaves) return;  // whole wave; no barrier below
   const int A = leaves[t];
   const int dA = nd.depth[A];
-  int anc[kMaxDepth + 1];
-  anc[dA] = A;
-  for (int d = dA - 1; d >= 0; d--) anc[d] = nd.parent[anc[d + 1]];
-  int c = 0;
-  int pos = pass ? ofs[t] : 0;
-  long long np = 0;
-  const int nA = nd.pend[A] - nd.pbegin[A];
-  for (int d = 0; d <= dA; d++) {
-    const int *cx = col + (size_t)anc[d] * 27;
-    for (int s = 0; s < 27; s++) {
-      int j = cx[s];
-      if (j < 0) continue;
-      if (d < dA && (s == 13 || !nd.leaf[j])) continue;
-      int b = nd.pbegin[j], e = nd.pend[j];
-      if (pass) {
-        ubeg[pos] = b;
-        uend[pos] = e;
-        pos++;
-        // reference pair counter: n(n+1)/2 for the leaf itself, half of
-        // the ordered count otherwise (each unordered pair is seen twice)
-        if (j == A)
-          np += 2LL * ((long long)nA * (nA + 1) / 2);
-        else
-          np += (long long)nA * (e - b);
-      } else {
-        c++;
-      }
+  if (lane == 0) {  // ancestors, root first
+    int v = A;
+    for (int d = dA; d >= 0; d--) {
+      anc[w][d] = v;
+      if (d > 0) v = nd.parent[v];
     }
   }
-  if (!pass)
-    cnt[t] = c;
-  else
-    pairs[t] = np;
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int half = lane / 32, s = lane % 32;
+  const int nA = nd.pend[A] - nd.pbegin[A];
+  int pos = pass ? ofs[t] : 0;
+  int c = 0;
+  long long np = 0;
+  for (int d0 = 0; d0 <= dA; d0 += 2) {
+    const int d = d0 + half;
+    bool take = false;
+    int j = -1, b = 0, e = 0;
+    if (s < 27 && d <= dA) {
+      j = col[(size_t)anc[w][d] * 27 + s];
+      if (j >= 0 && !(d < dA && (s == 13 || !nd.leaf[j]))) {
+        take = true;
+        b = nd.pbegin[j];
+        e = nd.pend[j];
+      }
+    }
+    const unsigned long long m = __ballot(take);
+    if (pass && take) {
+      const int k = pos + __popcll(m & ((1ull << lane) - 1ull));
+      ubeg[k] = b;
+      uend[k] = e;
+      if (j == A)
+        np += 2LL * ((long long)nA * (nA + 1) / 2);
+      else
+        np += (long long)nA * (e - b);
+    }
+    pos += __popcll(m);
+    c += __popcll(m);
+  }
+  if (!pass) {
+    if (lane == 0) cnt[t] = c;
+  } else {
+    for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
+    if (lane == 0) pairs[t] = np;
+  }
 }
 
 // ---- semifar lists (leaf / deeper node pairs, aran/aransolver3d.c:357-396)
@@ -1074,8 +1097,8 @@ hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
                         const int *col, int pass, int *cnt, const int *ofs,
                         int *ubeg, int *uend, long long *pairs, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
-  k_ulist<<<(nleaves + 127) / 128, 128, 0, s>>>(nd, nleaves, leaves, col, pass,
-                                                 cnt, ofs, ubeg, uend, pairs);
+  k_ulist<<<(nleaves + kUlistWaves - 1) / kUlistWaves, 64 * kUlistWaves, 0,
+            s>>>(nd, nleaves, leaves, col, pass, cnt, ofs, ubeg, uend, pairs);
   return hipGetLastError();
 }
 
