@@ -551,7 +551,10 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
     sdeg[k] = (unsigned char)l;
     srim[k] = (short)(m > 0 ? lc + tri_p(l - 1) + m - 1 : -1);
   }
-  for (int k0 = 0; k0 < lc; k0 += kUnscaleRows) {
+  // blockIdx.y takes one chunk of kUnscaleRows coefficients (the chunks of
+  // a tile run in parallel)
+  for (int k0 = blockIdx.y * kUnscaleRows; k0 < lc;
+       k0 += gridDim.y * kUnscaleRows) {
     const int nk = min(kUnscaleRows, lc - k0);
     __syncthreads();
     for (int e = threadIdx.x; e < nk * 64; e += blockDim.x) {
@@ -584,8 +587,8 @@ hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
   const long nz = (long)first * lc;
   if (nz > 0) k_zero_double2<<<(nz + 255) / 256, 256, 0, s>>>(L, nz);
   if (ntiles <= 0) return hipGetLastError();
-  k_unscale_mfma<<<ntiles, 256, 0, s>>>(nd, p, ntiles, tile_tgt, Lsm, nsplit,
-                                        L);
+  const dim3 grid(ntiles, (lc + kUnscaleRows - 1) / kUnscaleRows);
+  k_unscale_mfma<<<grid, 256, 0, s>>>(nd, p, ntiles, tile_tgt, Lsm, nsplit, L);
   return hipGetLastError();
 }
 
