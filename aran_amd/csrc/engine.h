@@ -118,6 +118,7 @@ struct Workspace {
       tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma;
   DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos;
   long node_cap = 0;
+  int depth_hint = 0;  // depth of the last tree (levels of the first batch)
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
   void *host_small = nullptr;  // pinned, 4 KiB
   Tables tab;

@@ -683,8 +683,14 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(hipMemcpyAsync(d_lvl, lvl0, sizeof(lvl0), hipMemcpyHostToDevice, s));
     long ub = nsys;  // bound on the count of level d
     bool done = false;
-    for (int d0 = 0; !done && d0 <= kMaxDepth; d0 += kLevelBatch) {
-      const int d1 = std::min(kMaxDepth, d0 + kLevelBatch - 1);
+    // the first batch reaches the previous solve's depth (repeated solves of
+    // similar systems then read the level counts back once)
+    const int first_batch =
+        std::max(kLevelBatch, std::min(kMaxDepth, ws.depth_hint) + 1);
+    for (int d0 = 0; !done && d0 <= kMaxDepth;
+         d0 += d0 == 0 ? first_batch : kLevelBatch) {
+      const int d1 = std::min(
+          kMaxDepth, d0 + (d0 == 0 ? first_batch : kLevelBatch) - 1);
       // bounds of the levels in this batch and of the nodes they add
       std::vector<long> ubs;
       long b = ub, add = 0, bmax = 1;
@@ -733,6 +739,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   }
   const int depth = (int)lstart.size() - 1;
   const int nn = (int)nnodes;
+  ws.depth_hint = depth;
 
   // ---- leaves
   // ---- leaves and internal nodes (the latter level by level, for the
