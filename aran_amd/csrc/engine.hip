@@ -138,7 +138,7 @@ Workspace::~Workspace() {
                    &mult_bf, &loc_mfma, &tab.mfma_ops, &n_sys, &sys_box, &sys_off, &sys_mm,
                    &skey, &spos, &internal_list,
                    &trans_rows, &packed,
-                   &lvl_table};
+                   &lvl_table, &child_map};
   for (DevBuf *b : all) b->release();
   if (host_small) (void)hipHostFree(host_small);
   if (events_ready) {
@@ -962,8 +962,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                         d_meta + kMaxDepth + 2 + ngroups,
                         ws.tile_tgt.as<int>(), ws.node_slot.as<int>(), s));
     CK(launch_fill_int(d_far, 1, 0, s));
+    CK(ws.child_map.reserve(sizeof(int) * 8 * (size_t)nn));
+    CK(launch_child_map(nd, nn, ws.child_map.as<int>(), s));
     CK(launch_tile_src(nd, ntiles, ws.tile_tgt.as<int>(),
-                       ws.tile_oct.as<int>(), col,
+                       ws.tile_oct.as<int>(), col, ws.child_map.as<int>(),
                        ws.tab.oct_codes.as<short>(), ws.tile_src.as<int>(),
                        d_far, s));
     // the host vectors above must outlive the copies

@@ -139,30 +139,44 @@ __global__ void k_tile_fill(NodeArrays nd, int first, int count,
 // children of a node are stored in octant order), then the 189 x 64 entries
 // are written with coalesced stores.  *nfar receives the number of sources
 // (the far-field operation count).
+// childmap[8 node + o]: child of node in octant o (bit 0: x, 1: y, 2: z),
+// -1 when absent (every octant of a leaf).
+__global__ void k_child_map(NodeArrays nd, int nnodes,
+                            int *__restrict__ childmap) {
+  const int v = blockIdx.x * blockDim.x + threadIdx.x;
+  if (v >= nnodes) return;
+  int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+  const int cb = nd.child[v], nch = nd.leaf[v] ? 0 : nd.nchild[v];
+  for (int k = cb; k < cb + nch; k++)
+    c[(nd.ix[k] & 1) | (nd.iy[k] & 1) << 1 | (nd.iz[k] & 1) << 2] = k;
+  int4 *dst = reinterpret_cast<int4 *>(childmap + 8 * (size_t)v);
+  dst[0] = make_int4(c[0], c[1], c[2], c[3]);
+  dst[1] = make_int4(c[4], c[5], c[6], c[7]);
+}
+
+hipError_t launch_child_map(const NodeArrays &nd, int nnodes, int *childmap,
+                            hipStream_t s) {
+  if (nnodes <= 0) return hipSuccess;
+  k_child_map<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, childmap);
+  return hipGetLastError();
+}
+
+// Sources of every (tile, offset slot, lane): the child in octant
+// (b - offset) mod 2 of the parent's colleague at slot floor((b - offset) /
+// 2), from the child map; the parents' colleagues of the tile go through LDS
+// first.  Every load of a phase is independent of the others.
 __global__ __launch_bounds__(256) void k_tile_src(
     NodeArrays nd, int ntiles, const int *__restrict__ tile_tgt,
     const int *__restrict__ tile_oct, const int *__restrict__ col,
-    const short *__restrict__ oct_codes, int *__restrict__ tile_src,
-    int *__restrict__ nfar) {
-  __shared__ int sbase[27 * 64];
-  __shared__ unsigned char smask[27 * 64];
+    const int *__restrict__ childmap, const short *__restrict__ oct_codes,
+    int *__restrict__ tile_src, int *__restrict__ nfar) {
+  __shared__ int scol[27 * 64];
   __shared__ int wcount[4];
   const int tile = blockIdx.x;
   for (int e = threadIdx.x; e < 27 * 64; e += blockDim.x) {
     const int ps = e / 64, lane = e % 64;
     const int A = tile_tgt[tile * 64 + lane];
-    int base = 0, mask = 0;
-    if (A >= 0) {
-      const int C = col[(size_t)nd.parent[A] * 27 + ps];
-      if (C >= 0 && !nd.leaf[C]) {
-        base = nd.child[C];
-        for (int K = base; K < base + nd.nchild[C]; K++)
-          mask |= 1 << ((nd.ix[K] & 1) | (nd.iy[K] & 1) << 1 |
-                        (nd.iz[K] & 1) << 2);
-      }
-    }
-    sbase[e] = base;
-    smask[e] = (unsigned char)mask;
+    scol[e] = A >= 0 ? col[(size_t)nd.parent[A] * 27 + ps] : -1;
   }
   __syncthreads();
   const int b = tile_oct[tile];
@@ -178,9 +192,8 @@ __global__ __launch_bounds__(256) void k_tile_src(
       const int ps =
           ((ux >> 1) + 1) * 9 + ((uy >> 1) + 1) * 3 + ((uz >> 1) + 1);
       const int co = (ux & 1) | (uy & 1) << 1 | (uz & 1) << 2;
-      const int mask = smask[ps * 64 + lane];
-      if (mask >> co & 1)
-        res = sbase[ps * 64 + lane] + __popc(mask & ((1 << co) - 1));
+      const int C = scol[ps * 64 + lane];
+      if (C >= 0) res = childmap[8 * (size_t)C + co];
     }
     tile_src[(size_t)tile * kOctOffsets * 64 + e] = res;
     found += res >= 0;
@@ -226,11 +239,12 @@ hipError_t launch_tile_fill(const NodeArrays &nd, int first, int count,
 
 hipError_t launch_tile_src(const NodeArrays &nd, int ntiles,
                            const int *tile_tgt, const int *tile_oct,
-                           const int *col, const short *oct_codes,
-                           int *tile_src, int *nfar, hipStream_t s) {
+                           const int *col, const int *childmap,
+                           const short *oct_codes, int *tile_src, int *nfar,
+                           hipStream_t s) {
   if (ntiles <= 0) return hipSuccess;
   k_tile_src<<<ntiles, 256, 0, s>>>(nd, ntiles, tile_tgt, tile_oct, col,
-                                    oct_codes, tile_src, nfar);
+                                    childmap, oct_codes, tile_src, nfar);
   return hipGetLastError();
 }
 
