@@ -1072,7 +1072,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                   ws.xs.as<double>(), ws.ys.as<double>(), ws.zs.as<double>(),
                   ws.qs.as<double>(), L, s));
   // ---- downward pass
-  for (int d = 1; d <= depth; d++)
+  // locals of levels 0 and 1 receive nothing without semifar pairs (far
+  // interactions start at level 2), so their translations would add zeros
+  for (int d = semifar ? 1 : 3; d <= depth; d++)
     CK(launch_l2l_level(tb, trows + ibase[d - 1], icount[d - 1], L, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
   // ---- join the near field, then M2P and L2P add to its results
