@@ -93,7 +93,12 @@ struct Tables {
   int mfma_order = -1;
   std::vector<DirEntry> dir_host;
   OpTables view() const;
+  void release();
 };
+
+// Orders whose tables a workspace keeps besides the current one (switching
+// back to a cached order swaps its tables in instead of rebuilding them).
+constexpr int kCachedOrders = 3;
 
 struct Workspace {
   DevBuf keys_a, keys_b, idx_a, idx_b, xs, ys, zs, qs, pot_s, grad_s;
@@ -123,6 +128,7 @@ struct Workspace {
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
   void *host_small = nullptr;  // pinned, 4 KiB
   Tables tab;
+  std::vector<Tables> cached_tabs;  // other orders, least recently used first
   hipStream_t stream = nullptr;
   bool own_stream = false;
   // second stream (default priority) for the near field, which runs

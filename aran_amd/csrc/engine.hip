@@ -81,6 +81,17 @@ void DevBuf::release() {
   bytes = 0;
 }
 
+void Tables::release() {
+  DevBuf *all[] = {&rot,    &eiphi,  &dir,       &tm2l,    &tm2m,  &tl2l,
+                   &ynorm,  &ogain,  &pgain,     &igain,   &xgain, &dscr,
+                   &thetas, &rot32,  &ephi32,    &vfac32,  &oct_codes,
+                   &code_to_k, &mfma_ops};
+  for (DevBuf *b : all) b->release();
+  order = -1;
+  mfma_order = -1;
+  dir_host.clear();
+}
+
 OpTables Tables::view() const {
   OpTables t;
   t.rot = rot.as<double>();
@@ -129,17 +140,15 @@ Workspace::~Workspace() {
                    &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &w_cnt, &w_ofs,
                    &w_node, &x_cnt, &x_ofs, &x_leaf, &s_part, &mult, &loc,
-                   &small, &pleaf,
-                   &tab.rot, &tab.eiphi, &tab.dir, &tab.tm2l, &tab.tm2m,
-                   &tab.tl2l, &tab.ynorm, &tab.ogain, &tab.pgain, &tab.igain, &tab.xgain, &tab.dscr, &tab.thetas, &tab.rot32,
-                   &tab.ephi32, &tab.vfac32, &tab.oct_codes, &tab.code_to_k,
-                   &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
+                   &small, &pleaf, &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
-                   &mult_bf, &loc_mfma, &tab.mfma_ops, &n_sys, &sys_box, &sys_off, &sys_mm,
+                   &mult_bf, &loc_mfma, &n_sys, &sys_box, &sys_off, &sys_mm,
                    &skey, &spos, &internal_list,
                    &trans_rows, &packed,
                    &lvl_table, &child_map};
   for (DevBuf *b : all) b->release();
+  tab.release();
+  for (Tables &t : cached_tabs) t.release();
   if (host_small) (void)hipHostFree(host_small);
   if (events_ready) {
     for (auto &e : ev) (void)hipEventDestroy(e);
@@ -244,6 +253,23 @@ void octant_offsets(const std::vector<DirEntry> &dirs, std::vector<short> &codes
 // overrides, 1..16).
 hipError_t ensure_tables(Workspace &ws, int p) {
   if (ws.tab.order == p) return hipSuccess;
+  // the current order's tables join the cache; a cached order is swapped in
+  // (Tables hold plain device pointers: a moved-from Tables is reset so
+  // that each buffer has one owner)
+  if (ws.tab.order >= 0) {
+    ws.cached_tabs.push_back(std::move(ws.tab));
+    ws.tab = Tables();
+  }
+  for (size_t k = 0; k < ws.cached_tabs.size(); k++)
+    if (ws.cached_tabs[k].order == p) {
+      ws.tab = std::move(ws.cached_tabs[k]);
+      ws.cached_tabs.erase(ws.cached_tabs.begin() + k);
+      return hipSuccess;
+    }
+  while ((int)ws.cached_tabs.size() > kCachedOrders) {
+    ws.cached_tabs.front().release();
+    ws.cached_tabs.erase(ws.cached_tabs.begin());
+  }
   hipStream_t s = ws.stream;
   Tables &T = ws.tab;
   // directions and theta values

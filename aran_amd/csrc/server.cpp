@@ -84,8 +84,9 @@ struct _AranServer3d {
   std::deque<Request *> queue;
   std::map<long, std::unique_ptr<Request>> all;
   long next_id = 0;
-  bool stop = false, paused = false;
+  bool stop = false, paused = false, preparing = false;
   long batches = 0, served = 0, max_batch_seen = 0;
+  int running = 0;  // batches taken and not yet finished
   std::vector<std::unique_ptr<Worker>> workers;
 };
 
@@ -101,8 +102,18 @@ extern "C" double aran_server3d_footprint(long n, int order, int leafcap) {
   return 1.25 * (n * per_particle + nodes * per_node) + 1e6;
 }
 
-// Tables and fixed buffers of one worker context, outside the requests.
-static const double kWorkerBytes = 64e6;
+// Tables (of up to four orders) and fixed buffers of one worker context,
+// outside the requests.
+static const double kWorkerBytes = ARAN_SERVER3D_WORKER_BYTES;
+
+// Host and device staging of a batch of n particles (inputs x, y, z, q;
+// outputs potential and gradient).
+static bool reserve_io(Worker *W, long n) {
+  const size_t nb = sizeof(double) * std::max<long>(n, 1);
+  return W->hin.reserve(4 * nb) && W->hout.reserve(4 * nb) &&
+         W->din.reserve(4 * nb) == hipSuccess &&
+         W->dout.reserve(4 * nb) == hipSuccess;
+}
 
 static void worker_loop(AranServer3d *S, Worker *W) {
   (void)hipSetDevice(S->device);
@@ -113,7 +124,7 @@ static void worker_loop(AranServer3d *S, Worker *W) {
       std::unique_lock<std::mutex> lock(S->mu);
       for (;;) {
         if (S->stop && S->queue.empty()) return;
-        if (!S->queue.empty() && (!S->paused || S->stop)) {
+        if (!S->queue.empty() && !S->preparing && (!S->paused || S->stop)) {
           Request *r0 = S->queue.front();
           const double cap = S->budget / (double)S->workers.size();
           if (S->inflight + r0->bytes <= S->budget || S->inflight == 0.) {
@@ -131,6 +142,7 @@ static void worker_loop(AranServer3d *S, Worker *W) {
             for (Request *r : batch)
               S->queue.erase(std::find(S->queue.begin(), S->queue.end(), r));
             S->inflight += bytes;
+            S->running++;
             S->peak_inflight = std::max(S->peak_inflight, S->inflight);
             break;
           }
@@ -151,12 +163,8 @@ static void worker_loop(AranServer3d *S, Worker *W) {
     }
     const long n = off[nsys];
     int rc = 0;
-    const size_t nb = sizeof(double) * std::max<long>(n, 1);
     hipStream_t st = static_cast<hipStream_t>(aran_fmm3d_stream(W->ctx));
-    if (!W->hin.reserve(4 * nb) || !W->hout.reserve(4 * nb) ||
-        W->din.reserve(4 * nb) != hipSuccess ||
-        W->dout.reserve(4 * nb) != hipSuccess)
-      rc = ARAN_FMM3D_ENOMEM;
+    if (!reserve_io(W, n)) rc = ARAN_FMM3D_ENOMEM;
     double *hin = static_cast<double *>(W->hin.ptr);
     double *din = W->din.as<double>();
     double *hout = static_cast<double *>(W->hout.ptr);
@@ -208,6 +216,7 @@ static void worker_loop(AranServer3d *S, Worker *W) {
         r->done = true;
       }
       S->inflight -= bytes;
+      S->running--;
       S->batches++;
       S->served += nsys;
       S->max_batch_seen = std::max<long>(S->max_batch_seen, nsys);
@@ -339,6 +348,69 @@ extern "C" int aran_server3d_wait(AranServer3d *S, long id, double *pot,
     times3[2] = sec(r->t_done - r->t_start).count();        // batch service
   }
   return 0;
+}
+
+extern "C" int aran_server3d_prepare(AranServer3d *S, long n_max,
+                                     const int *orders, int norders,
+                                     int leafcap) {
+  if (!S || n_max < 1 || norders < 0 || (norders > 0 && !orders) ||
+      leafcap < 1) {
+    aran::set_error("aran_server3d_prepare: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  for (int k = 0; k < norders; k++)
+    if (orders[k] < 1 || orders[k] > aran::kMaxOrder) {
+      aran::set_error("aran_server3d_prepare: invalid order");
+      return ARAN_FMM3D_EINVAL;
+    }
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    if (S->preparing || !S->queue.empty() || S->running != 0) {
+      aran::set_error("aran_server3d_prepare: requests queued or in flight");
+      return ARAN_FMM3D_EINVAL;
+    }
+    S->preparing = true;  // workers take no batch until it is cleared
+  }
+  (void)hipSetDevice(S->device);
+  int rc = 0;
+  for (auto &w : S->workers) {
+    Worker *W = w.get();
+    if (!reserve_io(W, n_max)) {
+      aran::set_error("aran_server3d_prepare: staging allocation failed");
+      rc = ARAN_FMM3D_ENOMEM;
+      break;
+    }
+    // a uniform system of n_max particles (fixed LCG) sizes the context's
+    // workspace; one solve per order builds that order's tables
+    double *h = static_cast<double *>(W->hin.ptr);
+    unsigned long long st = 0x9E3779B97F4A7C15ull;
+    for (long i = 0; i < 3 * n_max; i++) {
+      st = st * 6364136223846793005ull + 1442695040888963407ull;
+      h[i] = (double)(st >> 11) * (1.0 / 9007199254740992.0);
+    }
+    for (long i = 0; i < n_max; i++) h[3 * n_max + i] = 1.0 / (double)n_max;
+    hipStream_t hs = static_cast<hipStream_t>(aran_fmm3d_stream(W->ctx));
+    double *din = W->din.as<double>(), *dout = W->dout.as<double>();
+    if (hipMemcpyAsync(din, h, 4 * sizeof(double) * n_max,
+                       hipMemcpyHostToDevice, hs) != hipSuccess) {
+      rc = ARAN_FMM3D_EHIP;
+      break;
+    }
+    const long off[2] = {0, n_max};
+    const double lb[3] = {0., 0., 0.}, ub[3] = {1., 1., 1.};
+    for (int k = 0; k < norders && rc == 0; k++)
+      rc = aran_fmm3d_solve_batch(W->ctx, 1, off, din, din + n_max,
+                                  din + 2 * n_max, din + 3 * n_max, lb, ub,
+                                  orders[k], leafcap, dout, dout + n_max);
+    if (rc == 0 && hipStreamSynchronize(hs) != hipSuccess) rc = ARAN_FMM3D_EHIP;
+    if (rc != 0) break;
+  }
+  {
+    std::lock_guard<std::mutex> lock(S->mu);
+    S->preparing = false;
+  }
+  S->cv_work.notify_all();
+  return rc;
 }
 
 extern "C" int aran_server3d_pause(AranServer3d *S, int paused) {

@@ -10,6 +10,7 @@ from ._lib import check, last_error, lib
 
 PLAIN, MIXED = 0, 1
 EBUDGET = -4
+WORKER_BYTES = 512e6  # ARAN_SERVER3D_WORKER_BYTES
 _declared = False
 
 
@@ -30,6 +31,8 @@ def _L():
                                            ip]
         L.aran_server3d_wait.restype = ip
         L.aran_server3d_wait.argtypes = [vp, lp, vp, vp, vp]
+        L.aran_server3d_prepare.restype = ip
+        L.aran_server3d_prepare.argtypes = [vp, lp, C.POINTER(ip), ip, ip]
         L.aran_server3d_pause.restype = ip
         L.aran_server3d_pause.argtypes = [vp, ip]
         L.aran_server3d_stats.restype = ip
@@ -89,6 +92,13 @@ class Server:
         check(rc, "aran_server3d_wait")
         return pot, grad, {"latency": times[0], "queue": times[1],
                            "service": times[2]}
+
+    def prepare(self, n_max, orders, leafcap):
+        """Reserve staging and workspace for batches of up to n_max particles
+        and build the tables of each order, on every worker, before traffic."""
+        ov = (C.c_int * max(1, len(orders)))(*[int(o) for o in orders])
+        check(_L().aran_server3d_prepare(self._s, int(n_max), ov, len(orders),
+                                         int(leafcap)), "aran_server3d_prepare")
 
     def pause(self, paused=True):
         check(_L().aran_server3d_pause(self._s, int(bool(paused))), "pause")

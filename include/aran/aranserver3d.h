@@ -25,12 +25,16 @@ extern "C" {
 typedef struct _AranServer3d AranServer3d;
 
 #define ARAN_SERVER3D_EBUDGET (-4)
+/* Device bytes set aside per worker for its engine context: operator tables
+ * of up to four orders (the BF16 MFMA operators alone take about 57 MB at
+ * order 12 and 126 MB at order 15) and fixed buffers. */
+#define ARAN_SERVER3D_WORKER_BYTES 512e6
 
 /* Estimated device bytes of one request (upper bound used for admission). */
 double aran_server3d_footprint (long n, int order, int leafcap);
 
-/* hbm_budget_bytes: admission budget (64 MB per worker is set aside for its
- * tables; the rest bounds the requests in flight); nworkers: concurrent batches (1..8);
+/* hbm_budget_bytes: admission budget (ARAN_SERVER3D_WORKER_BYTES per worker
+ * are set aside for its context; the rest bounds the requests in flight); nworkers: concurrent batches (1..8);
  * precision: ARAN_FMM3D_PLAIN, _MIXED, _BF16X3 or _BF16X6; max_batch: requests per
  * batch.  Binds the current HIP device. */
 AranServer3d *aran_server3d_new (double hbm_budget_bytes, int nworkers,
@@ -52,6 +56,17 @@ long aran_server3d_submit (AranServer3d *srv, long n, const double *x,
  * service time in seconds.  Each id is waited for once. */
 int aran_server3d_wait (AranServer3d *srv, long id, double *pot, double *grad,
                         double *times3);
+
+/* Readies the server for traffic before the first request: every worker
+ * reserves host (pinned) and device staging for batches of up to n_max
+ * particles, sizes its solver workspace with a synthetic uniform system of
+ * n_max particles, and builds its tables for each of the norders orders
+ * (one solve per order at leaf capacity leafcap).  Without it the first
+ * requests of each size class and order pay for those allocations and table
+ * builds.  Returns ARAN_FMM3D_EINVAL when requests are queued or in flight;
+ * blocks until done. */
+int aran_server3d_prepare (AranServer3d *srv, long n_max, const int *orders,
+                           int norders, int leafcap);
 
 /* paused != 0: workers finish their batch and take no new one until
  * resumed (queued requests wait and can then be batched together). */

@@ -422,7 +422,7 @@ def test_served_results_equal_standalone(torch_cuda):
     batch and the request alone choose the same far-field split); a request
     over the budget is refused."""
     torch = torch_cuda
-    from aran_amd.server import EBUDGET, Server, footprint
+    from aran_amd.server import EBUDGET, WORKER_BYTES, Server, footprint
     reqs = []
     for k in range(12):
         n = [1200, 5000, 800, 20000, 3000, 1, 9000, 2500, 700, 15000, 4000,
@@ -435,7 +435,7 @@ def test_served_results_equal_standalone(torch_cuda):
             x, y, z, m = W.uniform_cube(n, 500 + k)
             lb, ub = (0, 0, 0), (1, 1, 1)
         reqs.append((x, y, z, m, lb, ub, order, 32, k % 2 == 0))
-    srv = Server(budget_bytes=footprint(40000, 8, 32) * 2 + 128e6, nworkers=2,
+    srv = Server(budget_bytes=footprint(40000, 8, 32) * 2 + 2 * WORKER_BYTES, nworkers=2,
                  precision="mixed", max_batch=8)
     srv.pause()
     ids = [srv.submit(*r[:4], r[4], r[5], r[6], r[7], want_grad=r[8])
@@ -461,7 +461,58 @@ def test_served_results_equal_standalone(torch_cuda):
         else:
             assert grad is None
         assert times["latency"] >= times["service"] > 0
-    small = Server(budget_bytes=footprint(1000, 8, 32) * 2 + 128e6, nworkers=2)
+    small = Server(budget_bytes=footprint(1000, 8, 32) * 2 + 2 * WORKER_BYTES, nworkers=2)
     rid = small.submit(*W.uniform_cube(5000, 3), (0, 0, 0), (1, 1, 1), 8, 32)
     assert rid == EBUDGET
     small.close()
+
+
+def test_prepared_server(torch_cuda):
+    """prepare() sizes every worker for 30000 particles and builds the tables
+    of orders 6 and 8; requests served afterwards are still bitwise their
+    standalone solves (mixed form), and prepare() is refused while a request
+    is queued."""
+    torch = torch_cuda
+    from aran_amd.server import Server
+    srv = Server(budget_bytes=4e9, nworkers=2, precision="mixed", max_batch=4)
+    srv.prepare(30000, (6, 8), 32)
+    reqs = []
+    for k, n in enumerate([25000, 300, 7000, 1]):
+        x, y, z, m = W.uniform_cube(n, 700 + k)
+        reqs.append((x, y, z, m, (0, 0, 0), (1, 1, 1), (6, 8)[k % 2], 32))
+    ids = [srv.submit(*r[:4], r[4], r[5], r[6], r[7]) for r in reqs]
+    out = [srv.wait(i) for i in ids]
+    srv.pause()
+    rid = srv.submit(*W.uniform_cube(50, 9), (0, 0, 0), (1, 1, 1), 8, 32)
+    with pytest.raises(RuntimeError):
+        srv.prepare(1000, (8,), 32)
+    srv.pause(False)
+    srv.wait(rid)
+    srv.close()
+    f = _fmm(torch)
+    f.set_precision("mixed")
+    for r, (pot, grad, _) in zip(reqs, out):
+        d = _dev(torch, *r[:4])
+        p1, g1 = [t.cpu().numpy() for t in f.solve(*d, r[4], r[5], r[6], r[7])]
+        assert np.array_equal(pot, p1) and np.array_equal(grad, g1)
+
+
+@pytest.mark.parametrize("form", ["plain", "bf16x3"])
+def test_order_switches_reuse_tables(torch_cuda, form):
+    """A context keeps the tables of recent orders: solves that cycle over
+    five orders (more than it caches, so some are rebuilt after eviction)
+    and then alternate over two (swapped in from the cache) give, for every
+    repeat of an order, bitwise the result of that order's first solve."""
+    torch = torch_cuda
+    d = _dev(torch, *W.uniform_cube(3000, 808))
+    f = _fmm(torch)
+    f.set_precision(form)
+    first = {}
+    for p in [6, 8, 10, 12, 14, 6, 8, 10, 12, 14, 6, 8, 6, 8, 14]:
+        pot, grad = [t.cpu().numpy() for t in f.solve(*d, (0, 0, 0),
+                                                       (1, 1, 1), p, 32)]
+        if p not in first:
+            first[p] = (pot, grad)
+        else:
+            assert np.array_equal(pot, first[p][0]), p
+            assert np.array_equal(grad, first[p][1]), p
