@@ -27,8 +27,11 @@ template <typename T> __device__ inline T rsqrt_t(T x);
 template <> __device__ inline double rsqrt_t<double>(double x) {
   return 1.0 / sqrt(x);
 }
+// v_rsq_f32 directly: rsqrtf adds a rescaling for denormal arguments, which
+// a squared distance here never is (two particles that close have equal
+// FP32 coordinates relative to the leaf centre, r2 == 0)
 template <> __device__ inline float rsqrt_t<float>(float x) {
-  return rsqrtf(x);
+  return __builtin_amdgcn_rsqf(x);
 }
 
 template <typename T> struct Src4;
@@ -46,7 +49,10 @@ __device__ __forceinline__ void near_one(const typename Src4<T>::type &s,
                                          T &gx, T &gy, T &gz) {
   const T dx = xi - s.x, dy = yi - s.y, dz = zi - s.z;
   const T r2 = dx * dx + dy * dy + dz * dz;
-  const T ir = self ? (T)0 : rsqrt_t<T>(r2);
+  // computed for every source and then selected (a branch around it costs
+  // more than the reciprocal square root)
+  const T ir0 = rsqrt_t<T>(r2);
+  const T ir = self ? (T)0 : ir0;
   const T qr = s.w * ir;
   const T qr3 = qr * (ir * ir);
   p += qr;
