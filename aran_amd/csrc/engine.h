@@ -126,7 +126,14 @@ struct Workspace {
   long node_cap = 0;
   int depth_hint = 0;  // depth of the last tree (levels of the first batch)
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
-  void *host_small = nullptr;  // pinned, 4 KiB
+  void *host_small = nullptr;  // pinned, 8 KiB
+  // pinned staging of the host-to-device copies (and the larger reads) of
+  // one solve: a bump arena reset when the next solve starts (a solve ends
+  // with its stream synchronized, so no copy from it is pending then);
+  // blocks it outgrows are kept until then
+  char *host_stage = nullptr;
+  size_t host_stage_bytes = 0, host_stage_used = 0;
+  std::vector<char *> host_stage_old;
   Tables tab;
   std::vector<Tables> cached_tabs;  // other orders, least recently used first
   hipStream_t stream = nullptr;

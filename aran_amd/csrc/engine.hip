@@ -150,6 +150,8 @@ Workspace::~Workspace() {
   tab.release();
   for (Tables &t : cached_tabs) t.release();
   if (host_small) (void)hipHostFree(host_small);
+  if (host_stage) (void)hipHostFree(host_stage);
+  for (char *b : host_stage_old) (void)hipHostFree(b);
   if (events_ready) {
     for (auto &e : ev) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ev_fork);
@@ -162,7 +164,7 @@ Workspace::~Workspace() {
 
 hipError_t workspace_init(Workspace &ws, hipStream_t stream) {
   if (ws.device < 0) CK(hipGetDevice(&ws.device));
-  if (!ws.host_small) CK(hipHostMalloc(&ws.host_small, 4096));
+  if (!ws.host_small) CK(hipHostMalloc(&ws.host_small, 8192));
   if (stream) {
     ws.stream = stream;
   } else if (!ws.stream) {
@@ -179,6 +181,42 @@ hipError_t workspace_init(Workspace &ws, hipStream_t stream) {
   if (!ws.aux) CK(hipStreamCreateWithFlags(&ws.aux, hipStreamNonBlocking));
   CK(ws.small.reserve(4096));
   return hipSuccess;
+}
+
+// Pinned host memory for one copy of this solve (see Workspace::host_stage).
+static hipError_t stage_alloc(Workspace &ws, size_t bytes, void **out) {
+  bytes = (bytes + 255) & ~(size_t)255;
+  if (ws.host_stage_used + bytes > ws.host_stage_bytes) {
+    if (ws.host_stage) ws.host_stage_old.push_back(ws.host_stage);
+    const size_t want =
+        std::max(std::max<size_t>(65536, 2 * ws.host_stage_bytes), 2 * bytes);
+    ws.host_stage = nullptr;
+    ws.host_stage_bytes = ws.host_stage_used = 0;
+    CK(hipHostMalloc(&ws.host_stage, want));
+    ws.host_stage_bytes = want;
+  }
+  *out = ws.host_stage + ws.host_stage_used;
+  ws.host_stage_used += bytes;
+  return hipSuccess;
+}
+
+// Host array to device memory through the pinned staging (so the copy is
+// asynchronous: a copy from pageable memory makes the host wait).
+static hipError_t stage_copy(Workspace &ws, void *dst, const void *src,
+                             size_t bytes, hipStream_t s) {
+  if (bytes == 0) return hipSuccess;
+  void *h = nullptr;
+  CK(stage_alloc(ws, bytes, &h));
+  memcpy(h, src, bytes);
+  return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s);
+}
+
+static void stage_reset(Workspace &ws) {
+  // (a solve that returned an error may have left copies in flight)
+  (void)hipStreamSynchronize(ws.stream);
+  for (char *b : ws.host_stage_old) (void)hipHostFree(b);
+  ws.host_stage_old.clear();
+  ws.host_stage_used = 0;
 }
 
 void root_box(const double *lb, const double *ub, const double *pmin,
@@ -622,13 +660,16 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(ws.sys_off.reserve(sizeof(long) * (nsys + 1)));
   CK(ws.sys_mm.reserve(sizeof(double) * 6 * nsys));
   CK(ws.sys_box.reserve(sizeof(double) * 4 * nsys));
-  CK(hipMemcpyAsync(ws.sys_off.ptr, soff.data(), sizeof(long) * (nsys + 1),
-                    hipMemcpyHostToDevice, s));
+  stage_reset(ws);
+  CK(stage_copy(ws, ws.sys_off.ptr, soff.data(), sizeof(long) * (nsys + 1),
+                s));
   CK(ws.part.reserve(sizeof(double) * 6 * (size_t)nsys * minmax_blocks(nsys)));
   CK(launch_sys_minmax(pb.x, pb.y, pb.z, ws.sys_off.as<long>(), nsys,
                        ws.part.as<double>(), ws.sys_mm.as<double>(), s));
-  std::vector<double> mm(6 * nsys), box(4 * nsys);
-  CK(hipMemcpyAsync(mm.data(), ws.sys_mm.ptr, sizeof(double) * 6 * nsys,
+  std::vector<double> box(4 * nsys);
+  double *mm = nullptr;
+  CK(stage_alloc(ws, sizeof(double) * 6 * nsys, reinterpret_cast<void **>(&mm)));
+  CK(hipMemcpyAsync(mm, ws.sys_mm.ptr, sizeof(double) * 6 * nsys,
                     hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
   for (int k = 0; k < nsys; k++) {
@@ -648,8 +689,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     }
     root_box(&lbs[3 * k], &ubs[3 * k], v, v + 3, &box[4 * k], &box[4 * k + 3]);
   }
-  CK(hipMemcpyAsync(ws.sys_box.ptr, box.data(), sizeof(double) * 4 * nsys,
-                    hipMemcpyHostToDevice, s));
+  CK(stage_copy(ws, ws.sys_box.ptr, box.data(), sizeof(double) * 4 * nsys,
+                s));
 
   // ---- keys and sort (by system, then Morton key)
   CK(ws.keys_a.reserve(sizeof(uint64_t) * n));
@@ -706,7 +747,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.lvl_table.reserve(sizeof(int) * 2 * (kMaxDepth + 3)));
     int *d_lvl = ws.lvl_table.as<int>();
     const int lvl0[2] = {0, nsys};
-    CK(hipMemcpyAsync(d_lvl, lvl0, sizeof(lvl0), hipMemcpyHostToDevice, s));
+    CK(stage_copy(ws, d_lvl, lvl0, sizeof(lvl0), s));
     long ub = nsys;  // bound on the count of level d
     bool done = false;
     // the first batch reaches the previous solve's depth (repeated solves of
@@ -776,10 +817,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(ws.split_ofs.reserve(sizeof(int) * (nn + 1)));
   CK(ws.lstart.reserve(sizeof(int) * (kMaxDepth + 2)));
   CK(ws.lcount.reserve(sizeof(int) * (kMaxDepth + 2)));
-  CK(hipMemcpyAsync(ws.lstart.ptr, lstart.data(), sizeof(int) * lstart.size(),
-                    hipMemcpyHostToDevice, s));
-  CK(hipMemcpyAsync(ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
-                    hipMemcpyHostToDevice, s));
+  CK(stage_copy(ws, ws.lstart.ptr, lstart.data(), sizeof(int) * lstart.size(),
+                s));
+  CK(stage_copy(ws, ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
+                s));
   CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
   CK(scan_small(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
                     ws.scan_tmp, s));
@@ -790,6 +831,36 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                            ws.lstart.as<int>(), depth + 1, nn, d_lvl, s));
   CK(hipMemcpyAsync(h_lvl, d_lvl, sizeof(int) * (depth + 2),
                     hipMemcpyDeviceToHost, s));
+  // Far-field tiles (all but the plain form): 64 targets of one (level,
+  // octant) group each.  The group sizes are counted here, so that they
+  // come back with the leaf counts.
+  const bool tiled = pb.precision != 0 && depth >= 2;
+  const int ngroups = (kMaxDepth + 1) * 8;
+  const int tfirst = tiled ? lstart[2] : 0;
+  const int tcount = tiled ? (int)nn - lstart[2] : 0;
+  int *d_meta = nullptr, *trank = nullptr;
+  const int *h_cnt =
+      reinterpret_cast<int *>(static_cast<char *>(ws.host_small) + 4096);
+  if (tiled) {
+    // device tables: lstart[kMaxDepth + 2], gofs[ngroups] (first chunk of
+    // each group in tile_of), tile_of[chunks] (tile of each chunk)
+    const size_t nmeta = kMaxDepth + 2 + ngroups + tcount / 64 + ngroups;
+    CK(ws.tile_meta.reserve(sizeof(int) * nmeta));
+    d_meta = ws.tile_meta.as<int>();
+    std::vector<int> ls(kMaxDepth + 2, 0);
+    for (int d = 0; d <= depth; d++) ls[d] = lstart[d];
+    CK(stage_copy(ws, d_meta, ls.data(), sizeof(int) * (kMaxDepth + 2), s));
+    CK(ws.tile_cnt.reserve(sizeof(int) * ngroups));
+    int *cnt = ws.tile_cnt.as<int>();
+    CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)tcount + 1)));
+    trank = ws.tile_rank.as<int>();
+    CK(launch_octant_flags(nd, tfirst, tcount, trank, s));
+    CK(scan_small(trank, trank, 8L * tcount, trank + 8L * tcount,
+                  ws.scan_tmp, s));
+    CK(launch_group_counts(tfirst, tcount, depth, d_meta, trank, cnt, s));
+    CK(hipMemcpyAsync(const_cast<int *>(h_cnt), cnt, sizeof(int) * ngroups,
+                      hipMemcpyDeviceToHost, s));
+  }
   CK(hipStreamSynchronize(s));
   const int nleaves = h_lvl[0];
   std::vector<int> icount(depth + 1), ibase(depth + 1);
@@ -847,6 +918,64 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   }
   CK(scan_small(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
                     ws.u_ofs.as<int>() + nleaves, ws.scan_tmp, s));
+  // ---- tile tables from the group counts (host, while the lists run);
+  // within a level the tiles take chunk j of every octant in turn, so that
+  // consecutive tiles cover the same region
+  int ntiles = 0, nwg = 0;
+  if (tiled) {
+    std::vector<int> meta(ngroups, 0), toct;  // gofs, then tile_of
+    int *gofs = meta.data();
+    int nchunks = 0;
+    for (int g = 0; g < ngroups; g++) {
+      gofs[g] = nchunks;
+      nchunks += (h_cnt[g] + 63) / 64;
+    }
+    meta.resize(ngroups + nchunks);
+    gofs = meta.data();
+    int *tile_of = gofs + ngroups;
+    for (int d = 2; d <= depth; d++) {
+      int maxch = 0;
+      for (int o = 0; o < 8; o++)
+        maxch = std::max(maxch, (h_cnt[d * 8 + o] + 63) / 64);
+      for (int j = 0; j < maxch; j++)
+        for (int o = 0; o < 8; o++)
+          if (j < (h_cnt[d * 8 + o] + 63) / 64) {
+            tile_of[gofs[d * 8 + o] + j] = ntiles++;
+            toct.push_back(o);
+          }
+    }
+    CK(ws.tile_oct.reserve(sizeof(int) * std::max(1, ntiles)));
+    CK(ws.tile_tgt.reserve(sizeof(int) * 64 * (size_t)std::max(1, ntiles)));
+    CK(ws.tile_src.reserve(sizeof(int) * 64 * (size_t)kOctOffsets *
+                           std::max(1, ntiles)));
+    CK(ws.node_slot.reserve(sizeof(int) * nn));
+    CK(stage_copy(ws, ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles, s));
+    // MFMA far field: workgroups of consecutive chunks of one (level,
+    // octant) group (they share every offset code), in the order level,
+    // chunk block, octant (neighbouring workgroups are neighbours in space)
+    if (pb.precision >= 2) {
+      std::vector<int> wgt;
+      const int tpw = mfma_wg_tiles(p);
+      for (int d = 2; d <= depth; d++) {
+        int maxch = 0;
+        for (int o = 0; o < 8; o++)
+          maxch = std::max(maxch, (h_cnt[d * 8 + o] + 63) / 64);
+        for (int jb = 0; jb < maxch; jb += tpw)
+          for (int o = 0; o < 8; o++) {
+            const int nch = (h_cnt[d * 8 + o] + 63) / 64;
+            if (jb >= nch) continue;
+            for (int j = jb; j < jb + tpw; j++)
+              wgt.push_back(j < nch ? tile_of[gofs[d * 8 + o] + j] : -1);
+          }
+      }
+      nwg = (int)(wgt.size() / tpw);
+      CK(ws.tile_perm.reserve(sizeof(int) * std::max<size_t>(1, wgt.size())));
+      CK(stage_copy(ws, ws.tile_perm.ptr, wgt.data(), sizeof(int) * wgt.size(),
+                    s));
+    }
+    CK(stage_copy(ws, d_meta + kMaxDepth + 2, meta.data(),
+                  sizeof(int) * meta.size(), s));
+  }
   CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
   CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
   CK(ws.u_end.reserve(sizeof(int) * std::max(1, nnear)));
@@ -900,93 +1029,13 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (pb.timings) CK(hipEventRecord(ws.ev_near, ws.aux));
   CK(launch_particle_leaf(nd, leaves, nleaves, ws.pleaf.as<int>(), ws.aux));
   CK(hipEventRecord(ws.ev_join, ws.aux));
-  // ---- tiles of the far field (mixed form)
-  int ntiles = 0, nwg = 0;
-  if (pb.precision != 0 && depth >= 2) {
-    // Tiles: 64 targets of one (level, octant) group each, in Morton order
-    // inside the group; within a level the tiles take chunk j of every
-    // octant in turn, so that consecutive tiles cover the same region.
-    const int first = lstart[2], count = nn - lstart[2];
-    const int ngroups = (kMaxDepth + 1) * 8;
-    // device tables: lstart[kMaxDepth + 2], gofs[ngroups] (first chunk of
-    // each group in tile_of), tile_of[chunks] (tile of each chunk)
-    const size_t nmeta = kMaxDepth + 2 + ngroups + count / 64 + ngroups;
-    CK(ws.tile_meta.reserve(sizeof(int) * nmeta));
-    std::vector<int> meta(kMaxDepth + 2 + ngroups, 0), toct;
-    for (int d = 0; d <= depth; d++) meta[d] = lstart[d];
-    int *d_meta = ws.tile_meta.as<int>();
-    CK(hipMemcpyAsync(d_meta, meta.data(), sizeof(int) * (kMaxDepth + 2),
-                      hipMemcpyHostToDevice, s));
-    CK(ws.tile_cnt.reserve(sizeof(int) * ngroups));
-    int *cnt = ws.tile_cnt.as<int>();
-    CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)count + 1)));
-    int *rank = ws.tile_rank.as<int>();
-    CK(launch_octant_flags(nd, first, count, rank, s));
-    CK(scan_small(rank, rank, 8L * count, rank + 8L * count, ws.scan_tmp,
-                      s));
-    CK(launch_group_counts(first, count, depth, d_meta, rank, cnt, s));
-    std::vector<int> hcnt(ngroups);
-    CK(hipMemcpyAsync(hcnt.data(), cnt, sizeof(int) * ngroups,
-                      hipMemcpyDeviceToHost, s));
-    CK(hipStreamSynchronize(s));
-    int *gofs = meta.data() + kMaxDepth + 2;
-    int nchunks = 0;
-    for (int g = 0; g < ngroups; g++) {
-      gofs[g] = nchunks;
-      nchunks += (hcnt[g] + 63) / 64;
-    }
-    meta.resize(meta.size() + nchunks);
-    gofs = meta.data() + kMaxDepth + 2;
-    int *tile_of = gofs + ngroups;
-    for (int d = 2; d <= depth; d++) {
-      int maxch = 0;
-      for (int o = 0; o < 8; o++)
-        maxch = std::max(maxch, (hcnt[d * 8 + o] + 63) / 64);
-      for (int j = 0; j < maxch; j++)
-        for (int o = 0; o < 8; o++)
-          if (j < (hcnt[d * 8 + o] + 63) / 64) {
-            tile_of[gofs[d * 8 + o] + j] = ntiles++;
-            toct.push_back(o);
-          }
-    }
-    CK(ws.tile_oct.reserve(sizeof(int) * std::max(1, ntiles)));
-    CK(ws.tile_tgt.reserve(sizeof(int) * 64 * (size_t)std::max(1, ntiles)));
-    CK(ws.tile_src.reserve(sizeof(int) * 64 * (size_t)kOctOffsets *
-                           std::max(1, ntiles)));
-    CK(ws.node_slot.reserve(sizeof(int) * nn));
-    CK(hipMemcpyAsync(ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles,
-                      hipMemcpyHostToDevice, s));
-    // MFMA far field: workgroups of consecutive chunks of one (level,
-    // octant) group (they share every offset code), in the order level,
-    // chunk block, octant (neighbouring workgroups are neighbours in space)
-    std::vector<int> wgt;
-    if (pb.precision >= 2) {
-      const int tpw = mfma_wg_tiles(p);
-      for (int d = 2; d <= depth; d++) {
-        int maxch = 0;
-        for (int o = 0; o < 8; o++)
-          maxch = std::max(maxch, (hcnt[d * 8 + o] + 63) / 64);
-        for (int jb = 0; jb < maxch; jb += tpw)
-          for (int o = 0; o < 8; o++) {
-            const int nch = (hcnt[d * 8 + o] + 63) / 64;
-            if (jb >= nch) continue;
-            for (int j = jb; j < jb + tpw; j++)
-              wgt.push_back(j < nch ? tile_of[gofs[d * 8 + o] + j] : -1);
-          }
-      }
-      nwg = (int)(wgt.size() / tpw);
-      CK(ws.tile_perm.reserve(sizeof(int) * std::max<size_t>(1, wgt.size())));
-      if (!wgt.empty())
-        CK(hipMemcpyAsync(ws.tile_perm.ptr, wgt.data(),
-                          sizeof(int) * wgt.size(), hipMemcpyHostToDevice, s));
-    }
-    CK(hipMemcpyAsync(d_meta + kMaxDepth + 2, meta.data() + kMaxDepth + 2,
-                      sizeof(int) * (meta.size() - kMaxDepth - 2),
-                      hipMemcpyHostToDevice, s));
+  // ---- tiles of the far field: targets and sources of every tile
+  if (tiled) {
+    const int *tofs = d_meta + kMaxDepth + 2;
     CK(launch_fill_int(ws.tile_tgt.as<int>(), 64L * ntiles, -1, s));
-    CK(launch_tile_fill(nd, first, count, rank, d_meta, d_meta + kMaxDepth + 2,
-                        d_meta + kMaxDepth + 2 + ngroups,
-                        ws.tile_tgt.as<int>(), ws.node_slot.as<int>(), s));
+    CK(launch_tile_fill(nd, tfirst, tcount, trank, d_meta, tofs,
+                        tofs + ngroups, ws.tile_tgt.as<int>(),
+                        ws.node_slot.as<int>(), s));
     CK(launch_fill_int(d_far, 1, 0, s));
     CK(ws.child_map.reserve(sizeof(int) * 8 * (size_t)nn));
     CK(launch_child_map(nd, nn, ws.child_map.as<int>(), s));
@@ -994,8 +1043,6 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                        ws.tile_oct.as<int>(), col, ws.child_map.as<int>(),
                        ws.tab.oct_codes.as<short>(), ws.tile_src.as<int>(),
                        d_far, s));
-    // the host vectors above must outlive the copies
-    CK(hipStreamSynchronize(s));
   }
   if (pb.timings) CK(hipEventRecord(ws.ev[2], s));
 
