@@ -15,6 +15,8 @@
 // form, where each staged pass is summed in FP32 and passes in FP64).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "engine.h"
 
 namespace aran {
@@ -62,7 +64,9 @@ __device__ __forceinline__ void near_one(const typename Src4<T>::type &s,
 }
 
 template <typename T>
-__global__ __launch_bounds__(kNearThreads) void k_near(
+// seven blocks per CU fit the LDS; the register cap keeps them resident
+__global__ __launch_bounds__(kNearThreads) __attribute__((
+    amdgpu_waves_per_eu(7))) void k_near(
     const int *__restrict__ leaves, NodeArrays nd, const int *__restrict__ uofs,
     const int *__restrict__ ubeg, const int *__restrict__ uend,
     const double *__restrict__ xs, const double *__restrict__ ys,
@@ -149,18 +153,33 @@ __global__ __launch_bounds__(kNearThreads) void k_near(
           // the target's own staged position in this pass (never matches
           // when the target leaf is not in it)
           const int fself = own - s0 + (i - tb);
+          // the target leaf's own particles are staged at [o0, o1) of this
+          // pass: only that range needs the self test.  The ranges are taken
+          // in order, so each lane still sums its sources by increasing f.
+          const int o0 = min(max(own - s0, 0), ns);
+          const int o1 = min(max(own - s0 + (te - tb), 0), ns);
           T p = 0, gx = 0, gy = 0, gz = 0;
-          int f = my_s;
-          for (; f + 3 * nsl < ns; f += 4 * nsl) {
-            const S4 a = src[f], b = src[f + nsl], c = src[f + 2 * nsl],
-                     d = src[f + 3 * nsl];
-            near_one<T>(a, xi, yi, zi, f == fself, p, gx, gy, gz);
-            near_one<T>(b, xi, yi, zi, f + nsl == fself, p, gx, gy, gz);
-            near_one<T>(c, xi, yi, zi, f + 2 * nsl == fself, p, gx, gy, gz);
-            near_one<T>(d, xi, yi, zi, f + 3 * nsl == fself, p, gx, gy, gz);
-          }
-          for (; f < ns; f += nsl)
-            near_one<T>(src[f], xi, yi, zi, f == fself, p, gx, gy, gz);
+          auto range = [&](int lo, int hi, auto chk)
+              __attribute__((always_inline)) {
+            constexpr bool C = decltype(chk)::value;
+            // first f >= lo with f = my_s (mod nsl)
+            int f = lo + ((my_s - lo) % nsl + nsl) % nsl;
+            for (; f + 3 * nsl < hi; f += 4 * nsl) {
+              const S4 a = src[f], b = src[f + nsl], c = src[f + 2 * nsl],
+                       d = src[f + 3 * nsl];
+              near_one<T>(a, xi, yi, zi, C && f == fself, p, gx, gy, gz);
+              near_one<T>(b, xi, yi, zi, C && f + nsl == fself, p, gx, gy, gz);
+              near_one<T>(c, xi, yi, zi, C && f + 2 * nsl == fself, p, gx, gy,
+                          gz);
+              near_one<T>(d, xi, yi, zi, C && f + 3 * nsl == fself, p, gx, gy,
+                          gz);
+            }
+            for (; f < hi; f += nsl)
+              near_one<T>(src[f], xi, yi, zi, C && f == fself, p, gx, gy, gz);
+          };
+          range(0, o0, std::false_type());
+          range(o0, o1, std::true_type());
+          range(o1, ns, std::false_type());
           ap += (double)p;
           ax += (double)gx;
           ay += (double)gy;
