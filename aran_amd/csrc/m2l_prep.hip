@@ -143,6 +143,9 @@ __global__ void k_tile_fill(NodeArrays nd, int first, int count,
 // -1 when absent (every octant of a leaf).
 __global__ void k_child_map(NodeArrays nd, int nnodes,
                             int *__restrict__ childmap) {
+  // issue priority over the near field's waves (second stream), which
+  // would otherwise hold up this latency-bound step of the far-field chain
+  __builtin_amdgcn_s_setprio(2);
   const int v = blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= nnodes) return;
   int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
@@ -170,6 +173,9 @@ __global__ __launch_bounds__(256) void k_tile_src(
     const int *__restrict__ tile_oct, const int *__restrict__ col,
     const int *__restrict__ childmap, const short *__restrict__ oct_codes,
     int *__restrict__ tile_src, int *__restrict__ nfar) {
+  // issue priority over the near field's waves (second stream), which
+  // would otherwise hold up this latency-bound step of the far-field chain
+  __builtin_amdgcn_s_setprio(2);
   __shared__ int scol[27 * 64];
   __shared__ int wcount[4];
   const int tile = blockIdx.x;
@@ -470,6 +476,9 @@ __global__ void k_mfma_pack_ops(int p, const double *__restrict__ O,
 __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
                                  const double2 *__restrict__ M,
                                  uint4 *__restrict__ mb) {
+  // issue priority over the near field's waves (second stream), which
+  // would otherwise hold up this latency-bound step of the far-field chain
+  __builtin_amdgcn_s_setprio(2);
   const MfmaDims D = mfma_dims(p);
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (t >= (long)(nnodes + 1) * D.ks * 2) return;

@@ -230,6 +230,9 @@ __global__ __launch_bounds__(64) void k_p2m(
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double *__restrict__ qs,
     double2 *__restrict__ M) {
+  // issue priority over the near field's waves (second stream), which
+  // would otherwise hold up this latency-bound step of the far-field chain
+  __builtin_amdgcn_s_setprio(2);
   constexpr int MC = P * (P + 1) / 2, S = 64 / P;
   __shared__ double2 T[S][MC];
   const int lane = threadIdx.x;

@@ -134,6 +134,9 @@ template <int P>
 __global__ __launch_bounds__(512) void k_m2m_level(
     OpTables tb, const TransRow *__restrict__ rows,
     double2 *__restrict__ M) {
+  // issue priority over the near field's waves (second stream), which
+  // would otherwise hold up this latency-bound step of the far-field chain
+  __builtin_amdgcn_s_setprio(2);
   constexpr int MC = tri_c(P), NS = (MC + 63) / 64;
   __shared__ double2 U[8][MC], X[8][MC];
   __shared__ double rp[8][2 * P + 2];
