@@ -418,15 +418,25 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scan_digits(
 
 // Stable scatter: elements keep their input order inside each digit.  The
 // block's base of digit d is the total of the digits below d plus its
-// offset among the blocks.
+// offset among the blocks.  Wave w of the block owns the w-th quarter of the
+// block's tile (its items in rounds of 64): it ranks them with its own
+// per-digit counters (wave-synchronous, no block barrier); one pass over
+// (digit, wave) turns the counters into block-local run starts, the items
+// are placed in LDS in block-local sorted order, and the block then writes
+// each digit's run to consecutive addresses (a direct scatter would write
+// 64 different buckets per wave store).  Every position stays below the
+// digit's base plus the block's count of it, the count the histogram kernel
+// took over the same items.
 __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
     const uint64_t *__restrict__ kin, const int *__restrict__ vin,
     uint64_t *__restrict__ kout, int *__restrict__ vout, long n, int shift,
     const int *__restrict__ offs, const int *__restrict__ dtot,
     int nblocks) {
-  __shared__ int base[256];
+  __shared__ int base[256], lstart[256];
   __shared__ int wcnt[kRsWaves][256];
   __shared__ int wsum[kRsWaves];
+  __shared__ uint64_t lkey[kRsTile];
+  __shared__ int lval[kRsTile];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   {
@@ -437,36 +447,74 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
   for (int w = 0; w < kRsWaves; w++) wcnt[w][threadIdx.x] = 0;
   __syncthreads();
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  long tile = blockIdx.x * (long)kRsTile;
+  const long first =
+      blockIdx.x * (long)kRsTile + (long)wave * (kRsItems * 64) + lane;
+  uint64_t key[kRsItems];
+  int val[kRsItems], loc[kRsItems];
+#pragma unroll
   for (int k = 0; k < kRsItems; k++) {
-    long i = tile + k * kRsThreads + threadIdx.x;
-    bool valid = i < n;
-    uint64_t key = valid ? kin[i] : 0;
-    int val = valid ? vin[i] : 0;
-    int d = (int)((key >> shift) & 255);
+    const long i = first + 64L * k;
+    key[k] = i < n ? kin[i] : 0ull;
+    val[k] = i < n ? vin[i] : 0;
+  }
+#pragma unroll
+  for (int k = 0; k < kRsItems; k++) {
+    const bool valid = first + 64L * k < n;
+    const int d = (int)((key[k] >> shift) & 255);
     uint64_t peers = __ballot(valid);
     for (int b = 0; b < 8; b++) {
-      uint64_t m = __ballot(valid && ((d >> b) & 1));
+      const uint64_t m = __ballot(valid && ((d >> b) & 1));
       peers &= ((d >> b) & 1) ? m : ~m;
     }
-    int rank = __popcll(peers & lt);
-    int cnt = __popcll(peers);
-    if (valid && rank == 0) wcnt[wave][d] = cnt;
-    __syncthreads();
-    if (valid) {
-      int pos = base[d] + rank;
-      for (int w = 0; w < wave; w++) pos += wcnt[w][d];
-      kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
-    int tot = 0;
+    const int rank = __popcll(peers & lt);
+    const int old = valid ? wcnt[wave][d] : 0;
+    // every lane has read its digit's count before the lowest lane of the
+    // digit advances it
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (valid && rank == 0) wcnt[wave][d] = old + __popcll(peers);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    loc[k] = old + rank;
+  }
+  __syncthreads();
+  {  // block-local start of digit d, and of each wave's run inside it
+    const int d = threadIdx.x;
+    int c = 0;
+    for (int w = 0; w < kRsWaves; w++) c += wcnt[w][d];
+    int tot;
+    int run = rs_block_scan(c, wsum, &tot);
+    lstart[d] = run;
     for (int w = 0; w < kRsWaves; w++) {
-      tot += wcnt[w][threadIdx.x];
-      wcnt[w][threadIdx.x] = 0;
+      const int cw = wcnt[w][d];
+      wcnt[w][d] = run;
+      run += cw;
     }
-    base[threadIdx.x] += tot;
-    __syncthreads();
+  }
+  __syncthreads();
+  // the block's items in block-local sorted order, in LDS
+#pragma unroll
+  for (int k = 0; k < kRsItems; k++) {
+    if (first + 64L * k < n) {
+      const int d = (int)((key[k] >> shift) & 255);
+      const int lp = wcnt[wave][d] + loc[k];
+      lkey[lp] = key[k];
+      lval[lp] = val[k];
+    }
+  }
+  __syncthreads();
+  // runs of one digit go to consecutive addresses: consecutive threads
+  // write consecutive words
+  const long tile = blockIdx.x * (long)kRsTile;
+  const int nvalid = (int)min((long)kRsTile, n - tile);
+  for (int t = threadIdx.x; t < nvalid; t += kRsThreads) {
+    const uint64_t kk = lkey[t];
+    const int d = (int)((kk >> shift) & 255);
+    const int pos = base[d] + (t - lstart[d]);
+    kout[pos] = kk;
+    vout[pos] = lval[t];
   }
 }
 
