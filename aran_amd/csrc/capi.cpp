@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 extern "C" {
@@ -66,6 +67,27 @@ struct HostBuf {
 };
 
 constexpr GType kDevelopmentType = 0x41524e33;  // "ARN3"
+
+// f(begin, end) over [0, n) on up to 16 host threads, at least 64K items
+// each (the gather from and the scatter to the caller's particle structs:
+// pointer-chasing loops that would otherwise take longer than the solve).
+template <class F> void parallel_ranges(long n, F f) {
+  constexpr long kMinItems = 1L << 16;
+  const long hw = std::max(1u, std::thread::hardware_concurrency());
+  const long nt = std::max(1L, std::min({hw, 16L, n / kMinItems}));
+  if (nt <= 1) {
+    f(0L, n);
+    return;
+  }
+  const long chunk = (n + nt - 1) / nt;
+  std::vector<std::thread> th;
+  for (long t = 1; t < nt; t++) {
+    const long a = t * chunk, b = std::min(n, a + chunk);
+    if (a < b) th.emplace_back(f, a, b);
+  }
+  f(0L, std::min(n, chunk));
+  for (auto &x : th) x.join();
+}
 
 }  // namespace
 
@@ -909,6 +931,7 @@ struct _AranSolver3d {
   long stats[11] = {0};
   guint depth = 0;
   gint status = 0;
+  int precision = ARAN_FMM3D_PLAIN;
   AranFmm3d *ctx = nullptr;
   DevBuf dx, dy, dz, dq, dpot, dgrad;
   HostBuf hx, hy, hz, hq, hpot, hgrad;
@@ -1084,6 +1107,17 @@ extern "C" void aran_solver3d_foreach_point(AranSolver3d *solver, GFunc func,
 
 extern "C" gint aran_solver3d_status(const AranSolver3d *solver) {
   return solver ? solver->status : ARAN_FMM3D_EINVAL;
+}
+
+extern "C" gint aran_solver3d_set_precision(AranSolver3d *solver,
+                                            gint precision) {
+  if (!solver || precision < ARAN_FMM3D_PLAIN ||
+      precision > ARAN_FMM3D_BF16X6) {
+    aran::set_error("aran_solver3d_set_precision: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  solver->precision = precision;
+  return 0;
 }
 
 extern "C" void aran_solver3d_set_particle_layout(AranSolver3d *solver,
@@ -1378,6 +1412,7 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
       return;
     }
   }
+  aran_fmm3d_set_precision(solver->ctx, solver->precision);
   hipStream_t s = solver->ctx->ws.stream;
   hipError_t e = hipSuccess;
   auto fail = [&](hipError_t err) {
@@ -1404,14 +1439,17 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
     memcpy(hq, solver->soa_q.data(), sizeof(double) * n);
   } else {
     const gssize oc = solver->layout.charge;
-    for (long i = 0; i < n; i++) {
-      const char *p = static_cast<const char *>(solver->points[i]);
-      const VsgVector3d *v = reinterpret_cast<const VsgVector3d *>(p);
-      hx[i] = v->x;
-      hy[i] = v->y;
-      hz[i] = v->z;
-      hq[i] = *reinterpret_cast<const double *>(p + oc);
-    }
+    void *const *pts = solver->points.data();
+    parallel_ranges(n, [=](long a, long b) {
+      for (long i = a; i < b; i++) {
+        const char *p = static_cast<const char *>(pts[i]);
+        const VsgVector3d *v = reinterpret_cast<const VsgVector3d *>(p);
+        hx[i] = v->x;
+        hy[i] = v->y;
+        hz[i] = v->z;
+        hq[i] = *reinterpret_cast<const double *>(p + oc);
+      }
+    });
   }
   const DevBuf *dsts[4] = {&solver->dx, &solver->dy, &solver->dz, &solver->dq};
   double *srcs[4] = {hx, hy, hz, hq};
@@ -1448,18 +1486,24 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
     solver->res_pot.assign(hp, hp + n);
     solver->res_field.assign(hg, hg + 3 * n);
   } else {
-    const Layout &L = solver->layout;
-    for (long i = 0; i < n; i++) {
-      char *p = static_cast<char *>(solver->points[i]);
-      if (want_pot && L.pot >= 0)
-        *reinterpret_cast<double *>(p + L.pot) += hp[i];
-      if (want_field && L.field >= 0) {
-        double *f = reinterpret_cast<double *>(p + L.field);
-        f[0] += hg[3 * i + 0];
-        f[1] += hg[3 * i + 1];
-        f[2] += hg[3 * i + 2];
+    // each inserted point is its own struct (a point inserted twice would
+    // interact with itself at distance 0), so the ranges write disjoint
+    // memory
+    const Layout L = solver->layout;
+    void *const *pts = solver->points.data();
+    parallel_ranges(n, [=](long a, long b) {
+      for (long i = a; i < b; i++) {
+        char *p = static_cast<char *>(pts[i]);
+        if (want_pot && L.pot >= 0)
+          *reinterpret_cast<double *>(p + L.pot) += hp[i];
+        if (want_field && L.field >= 0) {
+          double *f = reinterpret_cast<double *>(p + L.field);
+          f[0] += hg[3 * i + 0];
+          f[1] += hg[3 * i + 1];
+          f[2] += hg[3 * i + 2];
+        }
       }
-    }
+    });
   }
   long st[15];
   aran_fmm3d_get_stats(solver->ctx, st);

@@ -358,7 +358,7 @@ hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
 
 constexpr int kRsThreads = 256;
 constexpr int kRsWaves = kRsThreads / 64;
-constexpr int kRsItems = 16;
+constexpr int kRsItems = 8;
 constexpr int kRsTile = kRsThreads * kRsItems;
 
 __global__ __launch_bounds__(kRsThreads) void k_rs_hist(

@@ -110,6 +110,8 @@ def lib():
     L.aran_solver3d_solve.argtypes = [vp]
     L.aran_solver3d_status.restype = C.c_int
     L.aran_solver3d_status.argtypes = [vp]
+    L.aran_solver3d_set_precision.restype = C.c_int
+    L.aran_solver3d_set_precision.argtypes = [vp, C.c_int]
     L.aran_solver3d_get_stats.argtypes = [vp] + [C.POINTER(C.c_long)] * 11
     L.aran_solver3d_insert_points_soa.restype = C.c_int
     L.aran_solver3d_insert_points_soa.argtypes = [vp, C.c_long] + [vp] * 4

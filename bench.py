@@ -134,6 +134,61 @@ def host_inclusive(f, host, lb, ub, p, q, precision, steps):
                     "the results inside each step"}
 
 
+def struct_session(x, y, z, q, lb, ub, p, leafcap, precision, steps):
+    """The reference's own interface end to end: particles in caller structs
+    (VsgVector3d position, charge, potential, field; 64 bytes each) inserted
+    by pointer into an AranSolver3d with the Newton callbacks and the
+    declared layout.  Each step gathers through the layout into pinned host
+    memory (host threads), copies in, solves, copies out and adds the
+    results into the structs."""
+    import ctypes as C
+
+    from aran_amd import solver as S
+    L = S.lib()
+    n = len(x)
+    dt = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("density", "f8"),
+                   ("pot", "f8"), ("fx", "f8"), ("fy", "f8"), ("fz", "f8")])
+    pts = np.zeros(n, dtype=dt)
+    pts["x"], pts["y"], pts["z"], pts["density"] = x, y, z, q
+    vlb, vub = S.VsgVector3d(*lb), S.VsgVector3d(*ub)
+    tree = L.vsg_prtree3d_new_full(C.byref(vlb), C.byref(vub),
+                                   S.fn("vsg_vector3d_vector3d_locfunc"),
+                                   S.fn("vsg_vector3d_dist"), None, leafcap)
+    dev = L.aran_development3d_new(0, p)
+    solver = L.aran_solver3d_new(tree, L.aran_development3d_get_type(),
+                                 C.cast(dev, C.c_void_p),
+                                 S.fn("aran_development3d_set_zero"))
+    L.aran_solver3d_set_functions(
+        solver, S.fn("aran_newton3d_p2p_potfield"), S.fn("aran_newton3d_p2m"),
+        S.fn("aran_development3d_m2m_rotate"),
+        S.fn("aran_development3d_m2l_rotate"),
+        S.fn("aran_development3d_l2l_rotate"),
+        S.fn("aran_newton3d_l2p_potfield"))
+    L.aran_solver3d_set_particle_layout(solver, S.PointAccumBoth.density.offset,
+                                        S.PointAccumBoth.pot.offset, 0,
+                                        S.PointAccumBoth.field.offset)
+    code = {"plain": 0, "mixed": 1, "bf16x3": 2, "bf16x6": 3}[precision]
+    if L.aran_solver3d_set_precision(solver, code) != 0:
+        raise RuntimeError("aran_solver3d_set_precision failed")
+    base = pts.ctypes.data
+    for i in range(n):
+        L.aran_solver3d_insert_point(solver, base + i * dt.itemsize)
+    L.aran_solver3d_solve(solver)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        L.aran_solver3d_solve(solver)
+    elapsed = time.perf_counter() - t0
+    ok = L.aran_solver3d_status(solver) == 0
+    L.aran_solver3d_free(solver)
+    if not ok:
+        raise RuntimeError("aran_solver3d_solve failed")
+    return {"value": n * steps / elapsed, "ms_per_step": 1e3 * elapsed / steps,
+            "form": precision,
+            "note": "AranSolver3d with particles in 64-byte caller structs: "
+                    "gather through the layout into pinned memory, copies, "
+                    "solve, copies, results added into the structs"}
+
+
 def error_vs_direct(f, dev, pot, grad, n, seed):
     import torch
     from aran_amd import workloads as W
@@ -345,6 +400,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-plain", action="store_true",
                     help="skip the plain FP64 comparison run")
+    ap.add_argument("--no-structs", action="store_true",
+                    help="skip the end-to-end run through the AranSolver3d "
+                         "pointer interface (particles in caller structs)")
     ap.add_argument("--autotune", action="store_true",
                     help="far-field split from the on-device autotuner (its "
                          "candidate launches run in the warm-up) instead of "
@@ -408,6 +466,8 @@ def main():
     hostinc = host_inclusive(f, [torch.from_numpy(a) for a in (x, y, z, m)],
                              lb, ub, p, q, args.precision,
                              max(2, args.steps // 2))
+    structs = None if args.no_structs else struct_session(
+        x, y, z, m, lb, ub, p, q, args.precision, max(2, args.steps // 2))
 
     st = head["stats"]
     form = args.precision
@@ -481,6 +541,7 @@ def main():
         "phases_ms": head["phases_ms"],
         "plain_fp64": plain,
         "host_buffers": hostinc,
+        "caller_structs": structs,
         "batched": batch,
         "serving": serve,
         "tree": {k: st[k] for k in ("depth", "nodes", "leaves", "m2l", "p2p")},

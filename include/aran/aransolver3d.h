@@ -125,6 +125,14 @@ void aran_solver3d_solve (AranSolver3d *solver);
 /* Status of the last call on this solver: 0 or a negative error code. */
 gint aran_solver3d_status (const AranSolver3d *solver);
 
+/* Arithmetic form of the next solves (no counterpart in the reference,
+ * which computes in FP64 only; for callers of this interface that accept a
+ * far field of lower precision): 0 = ARAN_FMM3D_PLAIN (FP64, the
+ * default), 1 = ARAN_FMM3D_MIXED, 2 = ARAN_FMM3D_BF16X3 or
+ * 3 = ARAN_FMM3D_BF16X6, the forms of aran_fmm3d_set_precision()
+ * (aran/aranfmm3d.h).  Returns 0, or ARAN_FMM3D_EINVAL (-1). */
+gint aran_solver3d_set_precision (AranSolver3d *solver, gint precision);
+
 /* Byte offsets inside the caller's particle struct.  off_pot / off_field
  * < 0 disable that output.  pot_is_complex: the potential slot is a
  * gcomplex128 (its real part receives the potential, as in

@@ -516,3 +516,53 @@ def test_order_switches_reuse_tables(torch_cuda, form):
         else:
             assert np.array_equal(pot, first[p][0]), p
             assert np.array_equal(grad, first[p][1]), p
+
+
+@pytest.mark.parametrize("form,code", [("plain", 0), ("bf16x3", 2)])
+def test_solver_api_structs_match_device_solve(torch_cuda, form, code):
+    """150,000 particle structs (so that the gather from and the scatter to
+    them run on several host threads), solved through the AranSolver3d
+    pointer API in the given form: every potential and field is bitwise the
+    device-array solve of the same system; an unknown form is refused."""
+    torch = torch_cuda
+    from aran_amd import solver as S
+    L = S.lib()
+    n, p, q = 150_000, 8, 64
+    x, y, z, m = W.plummer(n, 91)
+    dt = np.dtype([("x", "f8"), ("y", "f8"), ("z", "f8"), ("density", "f8"),
+                   ("pot", "f8"), ("fx", "f8"), ("fy", "f8"), ("fz", "f8")])
+    assert dt.itemsize == C.sizeof(S.PointAccumBoth)
+    pts = np.zeros(n, dtype=dt)
+    pts["x"], pts["y"], pts["z"], pts["density"] = x, y, z, m
+    lb, ub = S.VsgVector3d(-1, -1, -1), S.VsgVector3d(1, 1, 1)
+    tree = L.vsg_prtree3d_new_full(C.byref(lb), C.byref(ub),
+                                   S.fn("vsg_vector3d_vector3d_locfunc"),
+                                   S.fn("vsg_vector3d_dist"), None, q)
+    dev = L.aran_development3d_new(0, p)
+    solver = L.aran_solver3d_new(tree, L.aran_development3d_get_type(),
+                                 C.cast(dev, C.c_void_p),
+                                 S.fn("aran_development3d_set_zero"))
+    L.aran_solver3d_set_functions(
+        solver, S.fn("aran_newton3d_p2p_potfield"), S.fn("aran_newton3d_p2m"),
+        S.fn("aran_development3d_m2m_rotate"),
+        S.fn("aran_development3d_m2l_rotate"),
+        S.fn("aran_development3d_l2l_rotate"),
+        S.fn("aran_newton3d_l2p_potfield"))
+    L.aran_solver3d_set_particle_layout(solver, S.PointAccumBoth.density.offset,
+                                        S.PointAccumBoth.pot.offset, 0,
+                                        S.PointAccumBoth.field.offset)
+    assert L.aran_solver3d_set_precision(solver, 7) == -1
+    assert L.aran_solver3d_set_precision(solver, code) == 0
+    base = pts.ctypes.data
+    for i in range(n):
+        L.aran_solver3d_insert_point(solver, base + i * dt.itemsize)
+    L.aran_solver3d_solve(solver)
+    assert L.aran_solver3d_status(solver) == 0
+    L.aran_solver3d_free(solver)
+    f = _fmm(torch)
+    f.set_precision(form)
+    d = _dev(torch, x, y, z, m)
+    p1, g1 = [t.cpu().numpy() for t in f.solve(*d, (-1, -1, -1), (1, 1, 1),
+                                               p, q)]
+    assert np.array_equal(pts["pot"], p1)
+    assert np.array_equal(np.stack([pts["fx"], pts["fy"], pts["fz"]], 1), g1)
