@@ -81,9 +81,15 @@ template <class F> void parallel_ranges(long n, F f) {
   }
   const long chunk = (n + nt - 1) / nt;
   std::vector<std::thread> th;
+  th.reserve(nt);
   for (long t = 1; t < nt; t++) {
     const long a = t * chunk, b = std::min(n, a + chunk);
-    if (a < b) th.emplace_back(f, a, b);
+    if (a >= b) continue;
+    try {
+      th.emplace_back(f, a, b);
+    } catch (...) {  // no thread to be had: this range on the caller's
+      f(a, b);
+    }
   }
   f(0L, std::min(n, chunk));
   for (auto &x : th) x.join();
