@@ -287,7 +287,7 @@ __global__ void k_scale_mult(NodeArrays nd, int nnodes, int p,
 // in slice order (coalesced reads), transpose through LDS, and write the 64
 // target rows L[node][k] = sum / w^(l+1) (coalesced writes), kUnscaleRows
 // coefficients at a time.
-constexpr int kUnscaleRows = 32;
+constexpr int kUnscaleRows = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void k_unscale_local(
