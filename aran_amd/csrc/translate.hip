@@ -244,15 +244,15 @@ __device__ inline double2 rot_inv_s(const double *Rs, double2 e, int l,
 // L2L of one parent per workgroup: wave o translates the parent's local to
 // the child in octant o and adds it there.  The eight offsets (+-1, +-1,
 // +-1) share one rotation angle, so the workgroup stages that angle's
-// rotation tables, the vertical L2L factors and the eight phases in LDS
-// once, and every phase then reads LDS only.
+// rotation tables and the eight phases in LDS once; the vertical L2L
+// factors are read from L2 (in LDS too they would leave room for one
+// workgroup less per CU).
 template <int P>
 __global__ __launch_bounds__(512) void k_l2l_level(
     OpTables tb, const TransRow *__restrict__ rows,
     double2 *__restrict__ L) {
-  constexpr int LC = tri_c(P + 1), NS = (LC + 63) / 64, RB = rofs_c(P + 1),
-                NT = (P + 1) * (P + 1) * (P + 1);
-  __shared__ double Rs[4 * RB], Ts[NT];
+  constexpr int LC = tri_c(P + 1), NS = (LC + 63) / 64, RB = rofs_c(P + 1);
+  __shared__ double Rs[4 * RB];
   __shared__ double2 Ph[8][P + 1];
   __shared__ double2 L0[LC], U[8][LC], X[8][LC];
   __shared__ double rp[8][2 * P + 2];
@@ -263,7 +263,6 @@ __global__ __launch_bounds__(512) void k_l2l_level(
   {
     const double *R = tb.rot + (size_t)tb.dir[code_of(-1, -1, -1)].theta * 4 * RB;
     for (int i = tid; i < 4 * RB; i += 512) Rs[i] = R[i];
-    for (int i = tid; i < NT; i += 512) Ts[i] = tb.tl2l[i];
     for (int i = tid; i < 8 * (P + 1); i += 512) {
       const int o = i / (P + 1), m = i % (P + 1);
       const int code = code_of(2 * (o & 1) - 1, 2 * ((o >> 1) & 1) - 1,
@@ -298,7 +297,7 @@ __global__ __launch_bounds__(512) void k_l2l_level(
       const int k = lane + 64 * decltype(sc)::value;
       if (k < LC) {
         const int l = kLm.l[k], m = kLm.m[k];
-        const double *T = Ts + (l * (P + 1) + m) * (P + 1);
+        const double *T = tb.tl2l + (l * (P + 1) + m) * (P + 1);
         double re = 0., im = 0.;
         static_loop<0, P + 1>([&](auto nc) {
           constexpr int n = decltype(nc)::value;
