@@ -303,13 +303,24 @@ __global__ __launch_bounds__(kScan1Threads) void k_scan_single(
   int carry = 0;
   for (int base = 0; base < n; base += (int)kScan1Max) {
     const int b = base + t * kScan1Per;
+    const bool full = b + kScan1Per <= n;  // 16-byte loads and stores
     int v[kScan1Per];
     int sum = 0;
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < kScan1Per; k++) {
-      v[k] = b + k < n ? in[b + k] : 0;
-      sum += v[k];
+      for (int k = 0; k < kScan1Per; k += 4) {
+        const int4 q = *reinterpret_cast<const int4 *>(in + b + k);
+        v[k] = q.x;
+        v[k + 1] = q.y;
+        v[k + 2] = q.z;
+        v[k + 3] = q.w;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScan1Per; k++) v[k] = b + k < n ? in[b + k] : 0;
     }
+#pragma unroll
+    for (int k = 0; k < kScan1Per; k++) sum += v[k];
     int x = sum;  // inclusive scan over the wave
     for (int off = 1; off < 64; off <<= 1) {
       const int y = __shfl_up(x, off);
@@ -327,10 +338,23 @@ __global__ __launch_bounds__(kScan1Threads) void k_scan_single(
     }
     __syncthreads();
     int run = carry + x - sum + (w > 0 ? ws[w - 1] : 0);
+    if (full) {
 #pragma unroll
-    for (int k = 0; k < kScan1Per; k++) {
-      if (b + k < n) out[b + k] = run;
-      run += v[k];
+      for (int k = 0; k < kScan1Per; k += 4) {
+        int4 q;
+        q.x = run;
+        q.y = q.x + v[k];
+        q.z = q.y + v[k + 1];
+        q.w = q.z + v[k + 2];
+        run = q.w + v[k + 3];
+        *reinterpret_cast<int4 *>(out + b + k) = q;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScan1Per; k++) {
+        if (b + k < n) out[b + k] = run;
+        run += v[k];
+      }
     }
     carry += ws[kScan1Threads / 64 - 1];
     __syncthreads();
