@@ -470,11 +470,12 @@ def test_served_results_equal_standalone(torch_cuda):
 def test_prepared_server(torch_cuda):
     """prepare() sizes every worker for 30000 particles and builds the tables
     of orders 6 and 8; requests served afterwards are still bitwise their
-    standalone solves (mixed form), and prepare() is refused while a request
-    is queued."""
+    standalone solves, and prepare() is refused while a request is queued.
+    Plain form: its served results are bitwise whatever batches the requests
+    land in (the other forms' far-field split follows the batch size)."""
     torch = torch_cuda
     from aran_amd.server import Server
-    srv = Server(budget_bytes=4e9, nworkers=2, precision="mixed", max_batch=4)
+    srv = Server(budget_bytes=4e9, nworkers=2, precision="plain", max_batch=4)
     srv.prepare(30000, (6, 8), 32)
     reqs = []
     for k, n in enumerate([25000, 300, 7000, 1]):
@@ -490,7 +491,7 @@ def test_prepared_server(torch_cuda):
     srv.wait(rid)
     srv.close()
     f = _fmm(torch)
-    f.set_precision("mixed")
+    f.set_precision("plain")
     for r, (pot, grad, _) in zip(reqs, out):
         d = _dev(torch, *r[:4])
         p1, g1 = [t.cpu().numpy() for t in f.solve(*d, r[4], r[5], r[6], r[7])]
