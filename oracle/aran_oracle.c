@@ -14,7 +14,7 @@
  * brute-force pair accounting on small inputs.
  *
  * Tree rules (LibVsg's VsgPRTree3d is not in the reference tree, so these are
- * this project's definition and this file is the judge of them):
+ * this project's definition, and this file is where they are fixed):
  *   - root cube: centre (lb+ub)/2, half-width max((ub-lb)/2), doubled about
  *     the same centre until every particle lies inside;
  *   - integer coordinates i = floor((x - lo) * 2^21 / width) clamped to
