@@ -49,6 +49,10 @@ def _declare(L):
     L.aran_fmm3d_direct.restype = ip
     L.aran_fmm3d_direct.argtypes = [vp, C.c_long, vp, vp, vp, vp, C.c_long, vp,
                                     vp, vp]
+    L.aran_fmm3d_set_translations.restype = ip
+    L.aran_fmm3d_set_translations.argtypes = [vp, ip, ip, ip]
+    L.aran_fmm3d_export_expansions.restype = ip
+    L.aran_fmm3d_export_expansions.argtypes = [vp, vp, vp]
 
 
 def lib():

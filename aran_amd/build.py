@@ -20,6 +20,7 @@ M2L_RANGES = [(1, 6), (7, 9), (10, 11), (12, 13), (14, 15)]
 SOURCES = [("tree.hip", "tree", []), ("operators.hip", "operators", []),
            ("m2l_prep.hip", "m2l_prep", []), ("p2p.hip", "p2p", []),
            ("translate.hip", "translate", []),
+           ("flavours.hip", "flavours", []),
            ("engine.hip", "engine", []), ("capi.cpp", "capi", []),
            ("server.cpp", "server", [])] + [
     ("m2l_tile.hip", "m2l_tile_%d" % a,
@@ -66,7 +67,27 @@ def build(verbose=False):
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_driver()
     return LIB
+
+
+DRIVER_SRC = os.path.join(ROOT, "tests", "c", "newton3_driver.c")
+DRIVER = os.path.join(HERE, "newton3_driver")
+
+
+def build_driver():
+    """The C test driver (tests/c/newton3_driver.c): plain C against
+    include/, linked to the engine library (found next to it at run time)."""
+    cmd = ["cc", "-std=gnu99", "-O2", "-Wall", "-I" + os.path.join(ROOT,
+                                                                  "include"),
+           DRIVER_SRC, "-o", DRIVER + ".tmp", "-L" + HERE, "-laran_amd",
+           "-Wl,-rpath,$ORIGIN", "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("cc failed on " + DRIVER_SRC)
+    os.replace(DRIVER + ".tmp", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":

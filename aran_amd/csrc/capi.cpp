@@ -18,11 +18,13 @@ extern "C" {
 #include <aran/aran.h>
 #include <aran/aranfmm3d.h>
 #include <aran/arandevelopment3d.h>
+#include <aran/aransphericalseriesd.h>
 #include <aran/aransolver3d.h>
 #include <vsg/vsgd.h>
 }
 
 #include "engine.h"
+#include "flavours.h"
 
 using aran::DevBuf;
 
@@ -159,6 +161,86 @@ extern "C" VsgPRTree3d *vsg_prtree3d_new_full(const VsgVector3d *lbound,
 
 extern "C" void vsg_prtree3d_free(VsgPRTree3d *prtree) { delete prtree; }
 
+extern "C" void vsg_prtree3d_write(VsgPRTree3d *prtree, FILE *file) {
+  if (!prtree || !file) return;
+  fprintf(file, "VsgPRTree3d { lbound: [%g, %g, %g], ubound: [%g, %g, %g], "
+                "maxbox: %u }\n",
+          prtree->lb.x, prtree->lb.y, prtree->lb.z, prtree->ub.x, prtree->ub.y,
+          prtree->ub.z, prtree->maxbox);
+}
+
+extern "C" void vsg_vector3d_set(VsgVector3d *vec, gdouble x, gdouble y,
+                                 gdouble z) {
+  vec->x = x;
+  vec->y = y;
+  vec->z = z;
+}
+
+extern "C" void vsg_vector3d_copy(const VsgVector3d *src, VsgVector3d *dst) {
+  *dst = *src;
+}
+
+extern "C" void vsg_vector3d_add(const VsgVector3d *src,
+                                 const VsgVector3d *other, VsgVector3d *dst) {
+  const VsgVector3d r = {src->x + other->x, src->y + other->y,
+                         src->z + other->z};
+  *dst = r;
+}
+
+extern "C" void vsg_vector3d_sub(const VsgVector3d *src,
+                                 const VsgVector3d *other, VsgVector3d *dst) {
+  const VsgVector3d r = {src->x - other->x, src->y - other->y,
+                         src->z - other->z};
+  *dst = r;
+}
+
+extern "C" void vsg_vector3d_scalp(const VsgVector3d *src, gdouble scal,
+                                   VsgVector3d *dst) {
+  const VsgVector3d r = {scal * src->x, scal * src->y, scal * src->z};
+  *dst = r;
+}
+
+extern "C" gdouble vsg_vector3d_dotp(const VsgVector3d *one,
+                                     const VsgVector3d *other) {
+  return one->x * other->x + one->y * other->y + one->z * other->z;
+}
+
+extern "C" gdouble vsg_vector3d_norm(const VsgVector3d *vec) {
+  return sqrt(vsg_vector3d_dotp(vec, vec));
+}
+
+extern "C" void vsg_vector3d_write(const VsgVector3d *vec, FILE *file) {
+  fprintf(file, "[%g, %g, %g]", vec->x, vec->y, vec->z);
+}
+
+extern "C" void vsg_vector3d_to_spherical_internal(const VsgVector3d *vec,
+                                                   gdouble *r, gdouble *cost,
+                                                   gdouble *sint,
+                                                   gdouble *cosp,
+                                                   gdouble *sinp) {
+  const double rho = sqrt(vec->x * vec->x + vec->y * vec->y);
+  const double rr = sqrt(rho * rho + vec->z * vec->z);
+  *r = rr;
+  *cost = rr > 0. ? vec->z / rr : 1.;
+  *sint = rr > 0. ? rho / rr : 0.;
+  *cosp = rho > 0. ? vec->x / rho : 1.;
+  *sinp = rho > 0. ? vec->y / rho : 0.;
+}
+
+extern "C" void vsg_vector3d_from_spherical_internal(VsgVector3d *vec,
+                                                     gdouble r, gdouble cost,
+                                                     gdouble sint,
+                                                     gdouble cosp,
+                                                     gdouble sinp) {
+  vec->x = r * sint * cosp;
+  vec->y = r * sint * sinp;
+  vec->z = r * cost;
+}
+
+extern "C" void vsg_packed_msg_trace_set_active(gboolean active) {
+  (void)active;
+}
+
 // ================================================================ device API
 
 struct _AranFmm3d {
@@ -169,7 +251,9 @@ struct _AranFmm3d {
   int precision = ARAN_FMM3D_PLAIN;
   long semifar = -1;
   bool autotune = false;
+  int flav[3] = {0, 0, 0};  // M2M, M2L, L2L translation flavours
   long n_last = 0;
+  int order_last = 0;
 };
 
 extern "C" AranFmm3d *aran_fmm3d_new(void *hip_stream) {
@@ -217,8 +301,12 @@ extern "C" int aran_fmm3d_solve(AranFmm3d *ctx, long n, const double *x,
   pb.precision = ctx->precision;
   pb.semifar = ctx->semifar;
   pb.autotune = ctx->autotune;
+  pb.flav_m2m = ctx->flav[0];
+  pb.flav_m2l = ctx->flav[1];
+  pb.flav_l2l = ctx->flav[2];
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
+  ctx->order_last = (e == hipSuccess) ? order : 0;
   return code_of_err(e);
 }
 
@@ -255,8 +343,12 @@ extern "C" int aran_fmm3d_solve_batch(AranFmm3d *ctx, int nsys,
   pb.precision = ctx->precision;
   pb.semifar = ctx->semifar;
   pb.autotune = ctx->autotune;
+  pb.flav_m2m = ctx->flav[0];
+  pb.flav_m2l = ctx->flav[1];
+  pb.flav_l2l = ctx->flav[2];
   hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
   ctx->n_last = (e == hipSuccess) ? n : 0;
+  ctx->order_last = (e == hipSuccess) ? order : 0;
   return code_of_err(e);
 }
 
@@ -274,6 +366,43 @@ extern "C" int aran_fmm3d_set_autotune(AranFmm3d *ctx, int on) {
   if (!ctx) return ARAN_FMM3D_EINVAL;
   ctx->autotune = on != 0;
   return 0;
+}
+
+extern "C" int aran_fmm3d_set_translations(AranFmm3d *ctx, int m2m, int m2l,
+                                           int l2l) {
+  const int f[3] = {m2m, m2l, l2l};
+  if (!ctx) return ARAN_FMM3D_EINVAL;
+  for (int v : f)
+    if (v < ARAN_FMM3D_ROTATE || v > ARAN_FMM3D_KYLIN) {
+      aran::set_error("aran_fmm3d_set_translations: invalid flavour");
+      return ARAN_FMM3D_EINVAL;
+    }
+  for (int k = 0; k < 3; k++) ctx->flav[k] = f[k];
+  return 0;
+}
+
+extern "C" int aran_fmm3d_export_expansions(const AranFmm3d *ctx,
+                                            double *multipoles,
+                                            double *locals) {
+  if (!ctx || ctx->order_last < 1 || ctx->n_last <= 0) {
+    aran::set_error("aran_fmm3d_export_expansions: no solve to export");
+    return ARAN_FMM3D_EINVAL;
+  }
+  const aran::Workspace &ws = ctx->ws;
+  const long nn = ctx->ti.nnodes;
+  const int p = ctx->order_last;
+  hipError_t e = hipSuccess;
+  if (multipoles)
+    e = hipMemcpyAsync(multipoles, ws.mult.ptr,
+                       sizeof(double) * 2 * nn * aran::tri(p),
+                       hipMemcpyDeviceToHost, ws.stream);
+  if (e == hipSuccess && locals)
+    e = hipMemcpyAsync(locals, ws.loc.ptr,
+                       sizeof(double) * 2 * nn * aran::tri(p + 1),
+                       hipMemcpyDeviceToHost, ws.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ws.stream);
+  if (e != hipSuccess) aran::set_error(hipGetErrorString(e));
+  return code_of_err(e);
 }
 
 extern "C" int aran_fmm3d_set_semifar(AranFmm3d *ctx, long threshold) {
@@ -431,10 +560,6 @@ extern "C" int aran_fmm3d_direct(AranFmm3d *ctx, long n, const double *x,
 // ================================================================ development
 
 namespace {
-
-int dev_local_degree(const AranDevelopment3d *d) {
-  return std::max(d->posdeg, d->negdeg);
-}
 
 // Process-wide context of the single-operator entry points.
 struct OpContext {
@@ -676,11 +801,183 @@ int leaf_op_one(int op, int p, const VsgVector3d &center,
   return 0;
 }
 
+// ---- spherical series (storage of the reference's AranSphericalSeriesd:
+// local terms l <= posdeg, then multipole terms l < negdeg, m >= 0, here
+// 16-byte aligned after the header)
+
+}  // namespace
+
+struct _AranSphericalSeriesd {
+  guint8 posdeg, negdeg;
+  guint8 pad_[14];
+};
+
+namespace {
+
+static_assert(sizeof(AranSphericalSeriesd) == 16, "series header");
+
+double *ser_pos(const AranSphericalSeriesd *a) {
+  return reinterpret_cast<double *>(
+      reinterpret_cast<char *>(const_cast<AranSphericalSeriesd *>(a)) + 16);
+}
+double *ser_neg(const AranSphericalSeriesd *a) {
+  return ser_pos(a) + 2 * aran::tri(a->posdeg + 1);
+}
+size_t ser_count(int posdeg, int negdeg) {
+  return (size_t)aran::tri(posdeg + 1) + aran::tri(negdeg);
+}
+
+bool all_zero(const double *v, size_t n) {
+  for (size_t i = 0; i < n; i++)
+    if (v[i] != 0.) return false;
+  return true;
+}
+
+// The translation engine's order for source / destination degrees: M2M
+// takes multipole degree counts, L2L local degrees, M2L a multipole count
+// and a local degree (kinds as aran::kFlav*).
+int xlate_order(int kind, int sdeg, int ddeg) {
+  return std::max(1, std::max(sdeg, ddeg));
+}
+
+// dst += T src on the GPU: one translation of the given kind and flavour
+// (0 rotate, 1 normal, 2 Kylin) from centre xs to centre xd; src has sdeg,
+// dst ddeg degrees (see xlate_order).  Inputs are zero-padded and outputs
+// truncated to the engine's order (the terms the reference's loops skip).
+int xlate(int kind, int flavour, int sdeg, const double *src, int ddeg,
+          double *dst, const VsgVector3d &xs, const VsgVector3d &xd) {
+  const int nin_s = kind == aran::kFlavL2L ? aran::tri(sdeg + 1)
+                                           : aran::tri(sdeg);
+  const int nout_d = kind == aran::kFlavM2M ? aran::tri(ddeg)
+                                            : aran::tri(ddeg + 1);
+  if (nin_s == 0 || nout_d == 0 || all_zero(src, 2 * (size_t)nin_s)) return 0;
+  const int p = xlate_order(kind, sdeg, ddeg);
+  if (p > aran::kMaxOrder) {
+    aran::set_error("expansion degrees above 30");
+    return ARAN_FMM3D_EINVAL;
+  }
+  const int nin = kind == aran::kFlavL2L ? aran::tri(p + 1) : aran::tri(p);
+  const int nout = kind == aran::kFlavM2M ? aran::tri(p) : aran::tri(p + 1);
+  std::vector<double> in(2 * (size_t)nin, 0.), out(2 * (size_t)nout, 0.);
+  std::copy(src, src + 2 * nin_s, in.begin());
+  if (flavour == aran::kFlavRotate) {
+    const int rc = translate_one(kind, p, in.data(), one_dir(xs, xd),
+                                 out.data());
+    if (rc != 0) return rc;
+  } else {
+    OpContext &oc = opctx();
+    std::lock_guard<std::mutex> lock(oc.mu);
+    aran::Workspace &ws = oc.ws;
+    hipError_t e = aran::workspace_init(ws, nullptr);
+    if (e == hipSuccess) e = aran::ensure_tables(ws, p);
+    if (e != hipSuccess) return code_of_err(e);
+    hipStream_t s = ws.stream;
+    e = oc.buf.reserve(sizeof(double) * 2 * ((size_t)nin + nout));
+    if (e != hipSuccess) return code_of_err(e);
+    double2 *din = oc.buf.as<double2>(), *dout = din + nin;
+    aran::FlavVec v;
+    const double d[3] = {xd.x - xs.x, xd.y - xs.y, xd.z - xs.z};
+    const double rho = sqrt(d[0] * d[0] + d[1] * d[1]);
+    v.r = sqrt(rho * rho + d[2] * d[2]);
+    v.cost = v.r > 0. ? d[2] / v.r : 1.;
+    v.sint = v.r > 0. ? rho / v.r : 0.;
+    v.cosp = rho > 0. ? d[0] / rho : 1.;
+    v.sinp = rho > 0. ? d[1] / rho : 0.;
+    // the engine's degrees: M2M counts and L2L degrees are p on both sides
+    // (Kylin's M2M needs equal counts); M2L reads a p-term multipole into a
+    // degree-p local
+    e = hipMemcpyAsync(din, in.data(), sizeof(double) * in.size(),
+                       hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = hipMemsetAsync(dout, 0, sizeof(double) * out.size(), s);
+    if (e == hipSuccess)
+      e = aran::launch_flavour_one(ws.tab.view(), kind, flavour, p, din, p,
+                                   dout, v, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(out.data(), dout, sizeof(double) * out.size(),
+                         hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      aran::set_error(hipGetErrorString(e));
+      return code_of_err(e);
+    }
+  }
+  for (int i = 0; i < 2 * nout_d; i++) dst[i] += out[i];
+  return 0;
+}
+
+// Series translations: local terms to local terms, and multipole terms to
+// multipole terms (to_local: to local terms).  The reference's Kylin
+// operators refuse sources of higher degree than the destination
+// (aran/aransphericalseriesd-kkylin.c:53,226,387).
+void ser_translate(int flavour, bool to_local, const char *fn,
+                   const AranSphericalSeriesd *src, const VsgVector3d *xs,
+                   AranSphericalSeriesd *dst, const VsgVector3d *xd) {
+  if (!src || !dst || !xs || !xd) {
+    critical("%s: invalid argument", fn);
+    return;
+  }
+  if (flavour == aran::kFlavKylin &&
+      (src->posdeg > dst->posdeg ||
+       (!to_local && src->negdeg > dst->negdeg) ||
+       (to_local && src->negdeg > 0 && src->negdeg - 1 > dst->posdeg))) {
+    critical("%s: source degrees above the destination's", fn);
+    return;
+  }
+  int rc = xlate(aran::kFlavL2L, flavour, src->posdeg, ser_pos(src),
+                 dst->posdeg, ser_pos(dst), *xs, *xd);
+  if (rc == 0 && src->negdeg > 0) {
+    if (to_local)
+      rc = xlate(aran::kFlavM2L, flavour, src->negdeg, ser_neg(src),
+                 dst->posdeg, ser_pos(dst), *xs, *xd);
+    else if (dst->negdeg > 0)
+      rc = xlate(aran::kFlavM2M, flavour, src->negdeg, ser_neg(src),
+                 dst->negdeg, ser_neg(dst), *xs, *xd);
+  }
+  if (rc != 0) critical("%s: %s", fn, aran::last_error());
+}
+
+// Value and gradient of a series at x relative to its centre: the local
+// terms by the L2P kernel, the multipole terms by the M2P kernel.
+int ser_eval(const AranSphericalSeriesd *a, const VsgVector3d &x, double *pot,
+             double *grad) {
+  const VsgVector3d zero = {0., 0., 0.};
+  double pl = 0., gl[3] = {0., 0., 0.}, pm = 0., gm[3] = {0., 0., 0.};
+  const int lp = std::max(1, (int)a->posdeg);
+  if (lp > aran::kMaxOrder || a->negdeg > aran::kMaxOrder) {
+    aran::set_error("expansion degrees above 30");
+    return ARAN_FMM3D_EINVAL;
+  }
+  std::vector<double> L(2 * (size_t)aran::tri(lp + 1), 0.);
+  std::copy(ser_pos(a), ser_pos(a) + 2 * aran::tri(a->posdeg + 1), L.begin());
+  int rc = 0;
+  if (!all_zero(L.data(), L.size()))
+    rc = leaf_op_one(kOpL2P, lp, zero, x, 0., nullptr, L.data(), &pl, gl);
+  if (rc == 0 && a->negdeg > 0)
+    rc = leaf_op_one(kOpM2P, a->negdeg, zero, x, 0., ser_neg(a), nullptr, &pm,
+                     gm);
+  if (pot) *pot = pl + pm;
+  if (grad)
+    for (int k = 0; k < 3; k++) grad[k] = gl[k] + gm[k];
+  return rc;
+}
+
+// The engine's FMM order of a development (the solver's nodes): its
+// multipole degree count; its local degree must be the same number.
 bool dev_ok(const AranDevelopment3d *d, const char *fn) {
-  if (!d || d->posdeg > d->negdeg || d->negdeg < 1 ||
-      d->negdeg > aran::kMaxOrder) {
-    critical("%s: development must have posdeg <= negdeg, 1 <= negdeg <= 15",
-             fn);
+  if (!d || !d->multipole || !d->local || d->multipole->negdeg < 1 ||
+      d->multipole->negdeg > aran::kMaxOrder ||
+      d->local->posdeg != d->multipole->negdeg) {
+    critical("%s: the solver's development must be aran_development3d_new "
+             "(posdeg, p) with posdeg <= p and 1 <= p <= 30", fn);
+    return false;
+  }
+  return true;
+}
+
+bool dev_any(const AranDevelopment3d *d, const char *fn) {
+  if (!d || !d->multipole || !d->local) {
+    critical("%s: invalid development", fn);
     return false;
   }
   return true;
@@ -688,190 +985,286 @@ bool dev_ok(const AranDevelopment3d *d, const char *fn) {
 
 }  // namespace
 
+// ================================================================ series
+
+extern "C" AranSphericalSeriesd *aran_spherical_seriesd_new(guint8 posdeg,
+                                                            guint8 negdeg) {
+  void *mem = calloc(1, 16 + 16 * ser_count(posdeg, negdeg));
+  AranSphericalSeriesd *a = static_cast<AranSphericalSeriesd *>(mem);
+  a->posdeg = posdeg;
+  a->negdeg = negdeg;
+  return a;
+}
+
+extern "C" void aran_spherical_seriesd_free(AranSphericalSeriesd *ass) {
+  free(ass);
+}
+
+// Copies the common degrees (aran/aransphericalseriesd.c:158-182).
+extern "C" void aran_spherical_seriesd_copy(const AranSphericalSeriesd *src,
+                                            AranSphericalSeriesd *dst) {
+  if (!src || !dst) return;
+  if (src->posdeg > dst->posdeg || src->negdeg > dst->negdeg)
+    critical("aran_spherical_seriesd_copy: could lose precision");
+  aran_spherical_seriesd_set_zero(dst);
+  const int pd = std::min(src->posdeg, dst->posdeg);
+  const int nd = std::min(src->negdeg, dst->negdeg);
+  memcpy(ser_pos(dst), ser_pos(src), sizeof(double) * 2 * aran::tri(pd + 1));
+  memcpy(ser_neg(dst), ser_neg(src), sizeof(double) * 2 * aran::tri(nd));
+}
+
+extern "C" AranSphericalSeriesd *
+aran_spherical_seriesd_clone(const AranSphericalSeriesd *src) {
+  if (!src) return nullptr;
+  AranSphericalSeriesd *a =
+      aran_spherical_seriesd_new(src->posdeg, src->negdeg);
+  memcpy(ser_pos(a), ser_pos(src),
+         16 * ser_count(src->posdeg, src->negdeg));
+  return a;
+}
+
+extern "C" gcomplex128 *aran_spherical_seriesd_get_term(
+    AranSphericalSeriesd *ass, gint i, gint j) {
+  if (!ass || j < 0 || j > (i < 0 ? -i - 1 : i)) return nullptr;
+  if (i >= 0) {
+    if (i > ass->posdeg) return nullptr;
+    return reinterpret_cast<gcomplex128 *>(ser_pos(ass)) + aran::tri(i) + j;
+  }
+  const int l = -i - 1;
+  if (l >= ass->negdeg) return nullptr;
+  return reinterpret_cast<gcomplex128 *>(ser_neg(ass)) + aran::tri(l) + j;
+}
+
+extern "C" guint8 aran_spherical_seriesd_get_posdeg(
+    const AranSphericalSeriesd *ass) {
+  return ass ? ass->posdeg : 0;
+}
+
+extern "C" guint8 aran_spherical_seriesd_get_negdeg(
+    const AranSphericalSeriesd *ass) {
+  return ass ? ass->negdeg : 0;
+}
+
+extern "C" void aran_spherical_seriesd_set_zero(AranSphericalSeriesd *ass) {
+  if (ass) memset(ser_pos(ass), 0, 16 * ser_count(ass->posdeg, ass->negdeg));
+}
+
+// "[i,j]:(re,im), " for i = -negdeg..posdeg (aran/aransphericalseriesd.c:
+// 272-298)
+extern "C" void aran_spherical_seriesd_write(const AranSphericalSeriesd *ass,
+                                             FILE *file) {
+  if (!ass || !file) return;
+  fprintf(file, "[");
+  for (int i = -ass->negdeg; i <= ass->posdeg; i++) {
+    const int maxj = i < 0 ? -i - 1 : i;
+    for (int j = 0; j <= maxj; j++) {
+      const double *c =
+          i < 0 ? ser_neg(ass) + 2 * (aran::tri(-i - 1) + j)
+                : ser_pos(ass) + 2 * (aran::tri(i) + j);
+      fprintf(file, "[%d,%d]:(%e,%e), ", i, j, c[0], c[1]);
+    }
+  }
+  fprintf(file, "]");
+}
+
+extern "C" gcomplex128 aran_spherical_seriesd_evaluate(
+    const AranSphericalSeriesd *ass, const VsgVector3d *x) {
+  double v = 0.;
+  if (ass && x && ser_eval(ass, *x, &v, nullptr) != 0)
+    critical("aran_spherical_seriesd_evaluate: %s", aran::last_error());
+  return v;
+}
+
+extern "C" void aran_spherical_seriesd_gradient_evaluate(
+    const AranSphericalSeriesd *ass, const VsgVector3d *x, VsgVector3d *grad) {
+  double g[3] = {0., 0., 0.};
+  if (ass && x && ser_eval(ass, *x, nullptr, g) != 0)
+    critical("aran_spherical_seriesd_gradient_evaluate: %s",
+             aran::last_error());
+  if (grad) {
+    grad->x = g[0];
+    grad->y = g[1];
+    grad->z = g[2];
+  }
+}
+
+#define ARAN_SERIES_XLATE(name, flavour, to_local)                           \
+  extern "C" void name(const AranSphericalSeriesd *src,                      \
+                       const VsgVector3d *xsrc, AranSphericalSeriesd *dst,   \
+                       const VsgVector3d *xdst) {                            \
+    ser_translate(flavour, to_local, #name, src, xsrc, dst, xdst);           \
+  }
+ARAN_SERIES_XLATE(aran_spherical_seriesd_translate, aran::kFlavNormal, false)
+ARAN_SERIES_XLATE(aran_spherical_seriesd_to_local, aran::kFlavNormal, true)
+ARAN_SERIES_XLATE(aran_spherical_seriesd_translate_kkylin, aran::kFlavKylin,
+                  false)
+ARAN_SERIES_XLATE(aran_spherical_seriesd_to_local_kkylin, aran::kFlavKylin,
+                  true)
+ARAN_SERIES_XLATE(aran_spherical_seriesd_translate_rotate, aran::kFlavRotate,
+                  false)
+ARAN_SERIES_XLATE(aran_spherical_seriesd_to_local_rotate, aran::kFlavRotate,
+                  true)
+#undef ARAN_SERIES_XLATE
+
+// ================================================================ development
+
 extern "C" GType aran_development3d_get_type(void) { return kDevelopmentType; }
 
+// (aran/arandevelopment3d.c:64-73)
 extern "C" AranDevelopment3d *aran_development3d_new(guint8 posdeg,
                                                      guint8 negdeg) {
   AranDevelopment3d *d =
       static_cast<AranDevelopment3d *>(calloc(1, sizeof(AranDevelopment3d)));
-  d->posdeg = posdeg;
-  d->negdeg = negdeg;
-  int ld = std::max(posdeg, negdeg);
-  d->multipole = static_cast<double *>(
-      calloc(2 * (size_t)std::max(1, aran::tri(negdeg)), sizeof(double)));
-  d->local =
-      static_cast<double *>(calloc(2 * (size_t)aran::tri(ld + 1), sizeof(double)));
+  d->multipole = aran_spherical_seriesd_new(posdeg, negdeg);
+  d->local = aran_spherical_seriesd_new(std::max(posdeg, negdeg), 0);
   return d;
 }
 
 extern "C" void aran_development3d_free(AranDevelopment3d *ad) {
   if (!ad) return;
-  free(ad->multipole);
-  free(ad->local);
+  aran_spherical_seriesd_free(ad->multipole);
+  aran_spherical_seriesd_free(ad->local);
   free(ad);
 }
 
 extern "C" void aran_development3d_copy(const AranDevelopment3d *src,
                                         AranDevelopment3d *dst) {
-  if (!src || !dst || src->posdeg != dst->posdeg || src->negdeg != dst->negdeg) {
-    critical("aran_development3d_copy: degrees must coincide");
+  if (!src || !dst) {
+    critical("aran_development3d_copy: invalid argument");
     return;
   }
-  memcpy(dst->multipole, src->multipole,
-         sizeof(double) * 2 * aran::tri(src->negdeg));
-  memcpy(dst->local, src->local,
-         sizeof(double) * 2 * aran::tri(dev_local_degree(src) + 1));
+  aran_spherical_seriesd_copy(src->multipole, dst->multipole);
+  aran_spherical_seriesd_copy(src->local, dst->local);
 }
 
 extern "C" AranDevelopment3d *aran_development3d_clone(AranDevelopment3d *src) {
   if (!src) return nullptr;
-  AranDevelopment3d *d = aran_development3d_new(src->posdeg, src->negdeg);
+  AranDevelopment3d *d = aran_development3d_new(src->multipole->posdeg,
+                                                src->multipole->negdeg);
   aran_development3d_copy(src, d);
   return d;
 }
 
 extern "C" void aran_development3d_set_zero(AranDevelopment3d *ad) {
   if (!ad) return;
-  memset(ad->multipole, 0, sizeof(double) * 2 * aran::tri(ad->negdeg));
-  memset(ad->local, 0, sizeof(double) * 2 * aran::tri(dev_local_degree(ad) + 1));
+  aran_spherical_seriesd_set_zero(ad->multipole);
+  aran_spherical_seriesd_set_zero(ad->local);
 }
 
-// Text dump in the layout of aran_spherical_seriesd_write
-// (aran/aransphericalseriesd.c:272-298).
 extern "C" void aran_development3d_write(AranDevelopment3d *ad, FILE *file) {
   if (!ad || !file) return;
-  fprintf(file, "{m:[");
-  for (int l = 0; l < ad->negdeg; l++)
-    for (int m = 0; m <= l; m++) {
-      const double *c = ad->multipole + 2 * (aran::tri(l) + m);
-      fprintf(file, "[%d,%d]:(%e,%e), ", -l - 1, m, c[0], c[1]);
-    }
-  fprintf(file, "], l:[");
-  for (int l = 0; l <= dev_local_degree(ad); l++)
-    for (int m = 0; m <= l; m++) {
-      const double *c = ad->local + 2 * (aran::tri(l) + m);
-      fprintf(file, "[%d,%d]:(%e,%e), ", l, m, c[0], c[1]);
-    }
-  fprintf(file, "]}");
+  fprintf(file, "{m:");
+  aran_spherical_seriesd_write(ad->multipole, file);
+  fprintf(file, ", l:");
+  aran_spherical_seriesd_write(ad->local, file);
+  fprintf(file, "}");
 }
 
+// q (4 pi/(2l+1)) r^l conj(Y_l^m) into the multipole terms
+// (aran/arandevelopment3d.c:169-216)
 extern "C" void aran_development3d_p2m(const VsgVector3d *position,
                                        const gdouble charge,
                                        const VsgPRTree3dNodeInfo *dst_node,
                                        AranDevelopment3d *dst) {
-  if (!position || !dst_node || !dev_ok(dst, "aran_development3d_p2m")) return;
-  if (leaf_op_one(true, dst->negdeg, dst_node->center, *position, charge,
-                  dst->multipole, nullptr, nullptr, nullptr) != 0)
+  if (!position || !dst_node || !dev_any(dst, "aran_development3d_p2m")) return;
+  const int nm = dst->multipole->negdeg;
+  if (nm == 0) return;
+  if (leaf_op_one(kOpP2M, nm, dst_node->center, *position, charge,
+                  ser_neg(dst->multipole), nullptr, nullptr, nullptr) != 0)
     critical("aran_development3d_p2m: %s", aran::last_error());
 }
 
-static void translate_api(int kind, const char *fn,
-                          const VsgPRTree3dNodeInfo *src_node,
-                          AranDevelopment3d *src,
-                          const VsgPRTree3dNodeInfo *dst_node,
-                          AranDevelopment3d *dst) {
-  if (!src_node || !dst_node || !dev_ok(src, fn) || !dev_ok(dst, fn)) return;
-  if (src->negdeg != dst->negdeg) {
-    critical("%s: source and destination orders differ", fn);
+// q (4 pi/(2l+1)) conj(Y_l^m) / r^(l+1) into the local terms
+// (aran/arandevelopment3d.c:227-277)
+extern "C" void aran_development3d_p2l(const VsgVector3d *position,
+                                       const gdouble charge,
+                                       const VsgPRTree3dNodeInfo *dst_node,
+                                       AranDevelopment3d *dst) {
+  if (!position || !dst_node || !dev_any(dst, "aran_development3d_p2l")) return;
+  const int pd = dst->local->posdeg, lp = std::max(1, pd);
+  std::vector<double> L(2 * (size_t)aran::tri(lp + 1), 0.);
+  if (leaf_op_one(kOpP2L, lp, dst_node->center, *position, charge, nullptr,
+                  L.data(), nullptr, nullptr) != 0) {
+    critical("aran_development3d_p2l: %s", aran::last_error());
     return;
   }
-  OneDir d = one_dir(src_node->center, dst_node->center);
-  const double *in = (kind == 2) ? src->local : src->multipole;
-  double *out = (kind == 0) ? dst->multipole : dst->local;
-  if (translate_one(kind, src->negdeg, in, d, out) != 0)
-    critical("%s: %s", fn, aran::last_error());
+  double *out = ser_pos(dst->local);
+  for (int i = 0; i < 2 * aran::tri(pd + 1); i++) out[i] += L[i];
 }
 
-extern "C" void aran_development3d_m2m_rotate(const VsgPRTree3dNodeInfo *src_node,
-                                              AranDevelopment3d *src,
-                                              const VsgPRTree3dNodeInfo *dst_node,
-                                              AranDevelopment3d *dst) {
-  translate_api(0, "aran_development3d_m2m_rotate", src_node, src, dst_node,
-                dst);
+#define ARAN_DEV_XLATE(name, flavour, part_src, part_dst, to_local)          \
+  extern "C" void name(const VsgPRTree3dNodeInfo *src_node,                  \
+                       AranDevelopment3d *src,                               \
+                       const VsgPRTree3dNodeInfo *dst_node,                  \
+                       AranDevelopment3d *dst) {                             \
+    if (!src_node || !dst_node || !dev_any(src, #name) || !dev_any(dst, #name)) \
+      return;                                                                \
+    ser_translate(flavour, to_local, #name, src->part_src, &src_node->center, \
+                  dst->part_dst, &dst_node->center);                         \
+  }
+// (aran/arandevelopment3d.c:288-331, 343-390, 402-451)
+ARAN_DEV_XLATE(aran_development3d_m2m, aran::kFlavNormal, multipole,
+               multipole, false)
+ARAN_DEV_XLATE(aran_development3d_m2l, aran::kFlavNormal, multipole, local,
+               true)
+ARAN_DEV_XLATE(aran_development3d_l2l, aran::kFlavNormal, local, local, false)
+ARAN_DEV_XLATE(aran_development3d_m2m_kkylin, aran::kFlavKylin, multipole,
+               multipole, false)
+ARAN_DEV_XLATE(aran_development3d_m2l_kkylin, aran::kFlavKylin, multipole,
+               local, true)
+ARAN_DEV_XLATE(aran_development3d_l2l_kkylin, aran::kFlavKylin, local, local,
+               false)
+ARAN_DEV_XLATE(aran_development3d_m2m_rotate, aran::kFlavRotate, multipole,
+               multipole, false)
+ARAN_DEV_XLATE(aran_development3d_m2l_rotate, aran::kFlavRotate, multipole,
+               local, true)
+ARAN_DEV_XLATE(aran_development3d_l2l_rotate, aran::kFlavRotate, local, local,
+               false)
+#undef ARAN_DEV_XLATE
+
+static VsgVector3d rel(const VsgVector3d &p, const VsgVector3d &c) {
+  VsgVector3d r = {p.x - c.x, p.y - c.y, p.z - c.z};
+  return r;
 }
 
-extern "C" void aran_development3d_m2l_rotate(const VsgPRTree3dNodeInfo *src_node,
-                                              AranDevelopment3d *src,
-                                              const VsgPRTree3dNodeInfo *dst_node,
-                                              AranDevelopment3d *dst) {
-  translate_api(1, "aran_development3d_m2l_rotate", src_node, src, dst_node,
-                dst);
-}
-
-extern "C" void aran_development3d_l2l_rotate(const VsgPRTree3dNodeInfo *src_node,
-                                              AranDevelopment3d *src,
-                                              const VsgPRTree3dNodeInfo *dst_node,
-                                              AranDevelopment3d *dst) {
-  translate_api(2, "aran_development3d_l2l_rotate", src_node, src, dst_node,
-                dst);
-}
-
+// (aran/arandevelopment3d.c:463-568): the series at pos - centre
 extern "C" gcomplex128 aran_development3d_local_evaluate(
     const VsgPRTree3dNodeInfo *devel_node, AranDevelopment3d *devel,
     const VsgVector3d *pos) {
-  double pot = 0.;
-  if (devel_node && pos && dev_ok(devel, "aran_development3d_local_evaluate"))
-    if (leaf_op_one(false, devel->negdeg, devel_node->center, *pos, 0., nullptr,
-                    devel->local, &pot, nullptr) != 0)
-      critical("aran_development3d_local_evaluate: %s", aran::last_error());
-  return pot;
+  if (!devel_node || !pos || !dev_any(devel, "aran_development3d_local_evaluate"))
+    return 0.;
+  const VsgVector3d x = rel(*pos, devel_node->center);
+  return aran_spherical_seriesd_evaluate(devel->local, &x);
 }
 
 extern "C" void aran_development3d_local_gradient_evaluate(
     const VsgPRTree3dNodeInfo *devel_node, AranDevelopment3d *devel,
     const VsgVector3d *pos, VsgVector3d *grad) {
-  double g[3] = {0., 0., 0.};
-  if (devel_node && pos && grad &&
-      dev_ok(devel, "aran_development3d_local_gradient_evaluate"))
-    if (leaf_op_one(false, devel->negdeg, devel_node->center, *pos, 0., nullptr,
-                    devel->local, nullptr, g) != 0)
-      critical("aran_development3d_local_gradient_evaluate: %s",
-               aran::last_error());
-  if (grad) {
-    grad->x = g[0];
-    grad->y = g[1];
-    grad->z = g[2];
-  }
-}
-
-extern "C" void aran_development3d_p2l(const VsgVector3d *position,
-                                       const gdouble charge,
-                                       const VsgPRTree3dNodeInfo *dst_node,
-                                       AranDevelopment3d *dst) {
-  if (!position || !dst_node || !dev_ok(dst, "aran_development3d_p2l")) return;
-  if (leaf_op_one(kOpP2L, dev_local_degree(dst), dst_node->center, *position,
-                  charge, nullptr, dst->local, nullptr, nullptr) != 0)
-    critical("aran_development3d_p2l: %s", aran::last_error());
+  if (!devel_node || !pos || !grad ||
+      !dev_any(devel, "aran_development3d_local_gradient_evaluate"))
+    return;
+  const VsgVector3d x = rel(*pos, devel_node->center);
+  aran_spherical_seriesd_gradient_evaluate(devel->local, &x, grad);
 }
 
 extern "C" gcomplex128 aran_development3d_multipole_evaluate(
     const VsgPRTree3dNodeInfo *devel_node, AranDevelopment3d *devel,
     const VsgVector3d *pos) {
-  double pot = 0.;
-  if (devel_node && pos &&
-      dev_ok(devel, "aran_development3d_multipole_evaluate"))
-    if (leaf_op_one(kOpM2P, devel->negdeg, devel_node->center, *pos, 0.,
-                    devel->multipole, nullptr, &pot, nullptr) != 0)
-      critical("aran_development3d_multipole_evaluate: %s",
-               aran::last_error());
-  return pot;
+  if (!devel_node || !pos ||
+      !dev_any(devel, "aran_development3d_multipole_evaluate"))
+    return 0.;
+  const VsgVector3d x = rel(*pos, devel_node->center);
+  return aran_spherical_seriesd_evaluate(devel->multipole, &x);
 }
 
 extern "C" void aran_development3d_multipole_gradient_evaluate(
     const VsgPRTree3dNodeInfo *devel_node, AranDevelopment3d *devel,
     const VsgVector3d *pos, VsgVector3d *grad) {
-  double g[3] = {0., 0., 0.};
-  if (devel_node && pos && grad &&
-      dev_ok(devel, "aran_development3d_multipole_gradient_evaluate"))
-    if (leaf_op_one(kOpM2P, devel->negdeg, devel_node->center, *pos, 0.,
-                    devel->multipole, nullptr, nullptr, g) != 0)
-      critical("aran_development3d_multipole_gradient_evaluate: %s",
-               aran::last_error());
-  if (grad) {
-    grad->x = g[0];
-    grad->y = g[1];
-    grad->z = g[2];
-  }
+  if (!devel_node || !pos || !grad ||
+      !dev_any(devel, "aran_development3d_multipole_gradient_evaluate"))
+    return;
+  const VsgVector3d x = rel(*pos, devel_node->center);
+  aran_spherical_seriesd_gradient_evaluate(devel->multipole, &x, grad);
 }
 
 extern "C" gcomplex128 aran_development3d_m2p(const VsgPRTree3dNodeInfo *n,
@@ -938,6 +1331,11 @@ struct _AranSolver3d {
   guint depth = 0;
   gint status = 0;
   int precision = ARAN_FMM3D_PLAIN;
+  // near/far leaf predicate (aran_solver3d_set_nf_isleaf)
+  VsgPRTree3dNFIsleafFunc nf_isleaf = nullptr;
+  gpointer nf_isleaf_data = nullptr;
+  bool hilbert = false;  // children order of aran_solver3d_traverse
+  bool solved = false;   // expansions of the current particles on the device
   AranFmm3d *ctx = nullptr;
   DevBuf dx, dy, dz, dq, dpot, dgrad;
   HostBuf hx, hy, hz, hq, hpot, hgrad;
@@ -1077,6 +1475,40 @@ extern "C" guint aran_solver3d_point_count(const AranSolver3d *solver) {
   return (guint)(solver->soa ? solver->soa_x.size() : solver->points.size());
 }
 
+// The tree's bounds grow to take a particle that falls outside them and
+// keep that size when it is removed again, as the reference's tree does
+// (the Plummer fill inserts and removes corner points for that,
+// tests/newtonpot3.c:528-542): the root cube doubles about its centre until
+// it holds the particle (the growth rule of the engine's root box).
+static void grow_bounds(VsgPRTree3d *t, double x, double y, double z) {
+  double c[3], h = 0.;
+  const double lo[3] = {t->lb.x, t->lb.y, t->lb.z},
+               hi[3] = {t->ub.x, t->ub.y, t->ub.z}, v[3] = {x, y, z};
+  bool inside = true;
+  for (int k = 0; k < 3; k++)
+    if (v[k] < lo[k] || v[k] > hi[k]) inside = false;
+  if (inside || !(std::isfinite(x) && std::isfinite(y) && std::isfinite(z)))
+    return;
+  for (int k = 0; k < 3; k++) {
+    c[k] = 0.5 * (lo[k] + hi[k]);
+    h = std::max(h, 0.5 * (hi[k] - lo[k]));
+  }
+  if (!(h > 0.)) h = 1.;
+  for (;;) {
+    bool in = true;
+    for (int k = 0; k < 3; k++)
+      if (v[k] < c[k] - h || v[k] > c[k] + h) in = false;
+    if (in) break;
+    h *= 2.;
+  }
+  t->lb.x = c[0] - h;
+  t->lb.y = c[1] - h;
+  t->lb.z = c[2] - h;
+  t->ub.x = c[0] + h;
+  t->ub.y = c[1] + h;
+  t->ub.z = c[2] + h;
+}
+
 extern "C" void aran_solver3d_insert_point(AranSolver3d *solver,
                                            VsgPoint3 point) {
   if (!solver || !point) return;
@@ -1084,7 +1516,34 @@ extern "C" void aran_solver3d_insert_point(AranSolver3d *solver,
     critical("aran_solver3d_insert_point: solver holds SoA particles");
     return;
   }
+  const VsgVector3d *v = static_cast<const VsgVector3d *>(point);
+  grow_bounds(solver->prtree, v->x, v->y, v->z);
   solver->points.push_back(point);
+  solver->solved = false;
+}
+
+// Without a distributed tree every insertion is local
+// (aran/aransolver3d.c:980-1000).
+extern "C" void aran_solver3d_insert_point_local(AranSolver3d *solver,
+                                                 VsgPoint3 point) {
+  aran_solver3d_insert_point(solver, point);
+}
+
+extern "C" void aran_solver3d_set_nf_isleaf(AranSolver3d *solver,
+                                            VsgPRTree3dNFIsleafFunc isleaf,
+                                            gpointer user_data) {
+  if (!solver) return;
+  solver->nf_isleaf = isleaf;
+  solver->nf_isleaf_data = user_data;
+  solver->solved = false;
+}
+
+extern "C" void aran_solver3d_set_children_order_hilbert(AranSolver3d *solver) {
+  if (solver) solver->hilbert = true;
+}
+
+extern "C" void aran_solver3d_set_children_order_default(AranSolver3d *solver) {
+  if (solver) solver->hilbert = false;
 }
 
 extern "C" gboolean aran_solver3d_remove_point(AranSolver3d *solver,
@@ -1093,6 +1552,7 @@ extern "C" gboolean aran_solver3d_remove_point(AranSolver3d *solver,
   auto it = std::find(solver->points.begin(), solver->points.end(), point);
   if (it == solver->points.end()) return FALSE;
   solver->points.erase(it);
+  solver->solved = false;
   return TRUE;
 }
 
@@ -1157,6 +1617,8 @@ extern "C" gint aran_solver3d_insert_points_soa(AranSolver3d *solver,
     return ARAN_FMM3D_EINVAL;
   }
   solver->soa = true;
+  solver->solved = false;
+  for (long i = 0; i < n; i++) grow_bounds(solver->prtree, x[i], y[i], z[i]);
   solver->soa_x.insert(solver->soa_x.end(), x, x + n);
   solver->soa_y.insert(solver->soa_y.end(), y, y + n);
   solver->soa_z.insert(solver->soa_z.end(), z, z + n);
@@ -1356,10 +1818,28 @@ template <class A, class B> static bool same_fn(A a, B b) {
   return reinterpret_cast<void *>(a) == reinterpret_cast<void *>(b);
 }
 
-extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
-  if (!solver) return;
-  solver->status = 0;
-  // recognised operator set
+namespace {
+
+// Flavour of a registered translation: 0 rotate, 1 normal, 2 Kylin, -1 not a
+// library operator.
+template <class F, class A, class B, class C>
+int flavour_of(F f, A rotate, B normal, C kylin) {
+  if (same_fn(f, rotate)) return aran::kFlavRotate;
+  if (same_fn(f, normal)) return aran::kFlavNormal;
+  if (same_fn(f, kylin)) return aran::kFlavKylin;
+  return -1;
+}
+
+struct SolveOps {
+  int outputs = 0;  // 1 potential, 2 field, 3 both
+  int flav[3] = {0, 0, 0};
+  long semifar = -1;
+  int order = 0;
+};
+
+// The recognised operator set (see aransolver3d.h); false with a critical
+// message otherwise.
+bool solver_ops(AranSolver3d *solver, SolveOps *o) {
   int pot_p2p = -1, pot_l2p = -1;
   if (solver->p2p == aran_newton3d_p2p_pot) pot_p2p = 1;
   if (solver->p2p == aran_newton3d_p2p_field) pot_p2p = 2;
@@ -1367,26 +1847,35 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
   if (solver->l2p == aran_newton3d_l2p_pot) pot_l2p = 1;
   if (solver->l2p == aran_newton3d_l2p_field) pot_l2p = 2;
   if (solver->l2p == aran_newton3d_l2p_potfield) pot_l2p = 3;
-  bool ops_ok = pot_p2p > 0 && pot_p2p == pot_l2p &&
-                solver->p2m == aran_newton3d_p2m &&
-                same_fn(solver->m2m, aran_development3d_m2m_rotate) &&
-                same_fn(solver->m2l, aran_development3d_m2l_rotate) &&
-                same_fn(solver->l2l, aran_development3d_l2l_rotate);
+  o->flav[0] = flavour_of(
+      solver->m2m, aran_development3d_m2m_rotate, aran_development3d_m2m,
+      aran_development3d_m2m_kkylin);
+  o->flav[1] = flavour_of(
+      solver->m2l, aran_development3d_m2l_rotate, aran_development3d_m2l,
+      aran_development3d_m2l_kkylin);
+  o->flav[2] = flavour_of(
+      solver->l2l, aran_development3d_l2l_rotate, aran_development3d_l2l,
+      aran_development3d_l2l_kkylin);
+  const bool ops_ok = pot_p2p > 0 && pot_p2p == pot_l2p &&
+                      solver->p2m == aran_newton3d_p2m && o->flav[0] >= 0 &&
+                      o->flav[1] >= 0 && o->flav[2] >= 0;
   if (!ops_ok) {
     critical("aran_solver3d_solve: unsupported callbacks; the GPU engine runs "
              "the library's Newton callbacks (aran_newton3d_*) with the "
-             "aran_development3d_*_rotate operators");
-    solver->status = ARAN_FMM3D_EINVAL;
-    return;
+             "aran_development3d_{m2m,m2l,l2l}[_rotate|_kkylin] operators");
+    return false;
   }
-  if (!dev_ok(solver->devel, "aran_solver3d_solve")) {
-    solver->status = ARAN_FMM3D_EINVAL;
-    return;
+  if ((o->flav[0] | o->flav[1] | o->flav[2]) != 0 &&
+      solver->precision != ARAN_FMM3D_PLAIN) {
+    critical("aran_solver3d_solve: the normal and Kylin operators run in the "
+             "plain FP64 form only");
+    return false;
   }
-  const int order = solver->devel->negdeg;
+  if (!dev_ok(solver->devel, "aran_solver3d_solve")) return false;
+  o->order = solver->devel->multipole->negdeg;
+  o->outputs = pot_p2p;
   // semifar pairs when both P2L and M2P are set (aran/aransolver3d.c:
   // 1152-1161): the library's P2L with the M2P of the same outputs as P2P
-  long semifar = -1;
   if (solver->p2l && solver->m2p) {
     int kind = -1;
     if (solver->m2p == aran_newton3d_m2p_pot) kind = 1;
@@ -1396,46 +1885,44 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
       critical("aran_solver3d_solve: unsupported semifar callbacks; the GPU "
                "engine runs aran_newton3d_p2l with the aran_newton3d_m2p_* "
                "callback of the same outputs as the P2P callback");
-      solver->status = ARAN_FMM3D_EINVAL;
-      return;
+      return false;
     }
     if (solver->semifar_threshold == 0) {
       // automatic threshold (the reference derives it from profiled
       // operator times, aran/aransolver3d.c:1090-1127; here from the
       // operators' algorithmic costs)
       solver->semifar_threshold =
-          (guint)aran_fmm3d_semifar_threshold_auto(order);
+          (guint)aran_fmm3d_semifar_threshold_auto(o->order);
     }
     if (solver->semifar_threshold != 0xffffffffu)
-      semifar = (long)solver->semifar_threshold;
+      o->semifar = (long)solver->semifar_threshold;
   }
-  const bool want_pot = (pot_p2p & 1) != 0, want_field = (pot_p2p & 2) != 0;
-  const long n = (long)(solver->soa ? solver->soa_x.size() : solver->points.size());
+  return true;
+}
+
+// Particle positions and charges on the device (gathered from the caller's
+// structs through the declared layout, or the SoA arrays); *n their count.
+int solver_upload(AranSolver3d *solver, long *n_out) {
+  const long n =
+      (long)(solver->soa ? solver->soa_x.size() : solver->points.size());
+  *n_out = n;
   if (!solver->ctx) {
     solver->ctx = aran_fmm3d_new(nullptr);
-    if (!solver->ctx) {
-      solver->status = ARAN_FMM3D_EHIP;
-      return;
-    }
+    if (!solver->ctx) return ARAN_FMM3D_EHIP;
   }
-  aran_fmm3d_set_precision(solver->ctx, solver->precision);
   hipStream_t s = solver->ctx->ws.stream;
   hipError_t e = hipSuccess;
-  auto fail = [&](hipError_t err) {
-    critical("aran_solver3d_solve: %s", hipGetErrorString(err));
-    solver->status = code_of_err(err);
-  };
-  size_t nb = sizeof(double) * std::max<long>(n, 1);
+  const size_t nb = sizeof(double) * std::max<long>(n, 1);
   HostBuf *hb[] = {&solver->hx, &solver->hy, &solver->hz, &solver->hq,
                    &solver->hpot};
   for (HostBuf *b : hb)
-    if ((e = b->reserve(nb)) != hipSuccess) return fail(e);
-  if ((e = solver->hgrad.reserve(3 * nb)) != hipSuccess) return fail(e);
+    if ((e = b->reserve(nb)) != hipSuccess) return code_of_err(e);
+  if ((e = solver->hgrad.reserve(3 * nb)) != hipSuccess) return code_of_err(e);
   DevBuf *db[] = {&solver->dx, &solver->dy, &solver->dz, &solver->dq,
                   &solver->dpot};
   for (DevBuf *b : db)
-    if ((e = b->reserve(nb)) != hipSuccess) return fail(e);
-  if ((e = solver->dgrad.reserve(3 * nb)) != hipSuccess) return fail(e);
+    if ((e = b->reserve(nb)) != hipSuccess) return code_of_err(e);
+  if ((e = solver->dgrad.reserve(3 * nb)) != hipSuccess) return code_of_err(e);
   double *hx = solver->hx.as<double>(), *hy = solver->hy.as<double>(),
          *hz = solver->hz.as<double>(), *hq = solver->hq.as<double>();
   if (solver->soa) {
@@ -1457,27 +1944,314 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
       }
     });
   }
-  const DevBuf *dsts[4] = {&solver->dx, &solver->dy, &solver->dz, &solver->dq};
+  const DevBuf *dsts[4] = {&solver->dx, &solver->dy, &solver->dz,
+                           &solver->dq};
   double *srcs[4] = {hx, hy, hz, hq};
   for (int k = 0; k < 4 && n > 0; k++)
     if ((e = hipMemcpyAsync(dsts[k]->ptr, srcs[k], sizeof(double) * n,
                             hipMemcpyHostToDevice, s)) != hipSuccess)
-      return fail(e);
-  const double lb[3] = {solver->prtree->lb.x, solver->prtree->lb.y,
-                        solver->prtree->lb.z};
-  const double ub[3] = {solver->prtree->ub.x, solver->prtree->ub.y,
-                        solver->prtree->ub.z};
-  aran_fmm3d_set_semifar(solver->ctx, semifar);
-  int rc = aran_fmm3d_solve(solver->ctx, n, solver->dx.as<double>(),
-                            solver->dy.as<double>(), solver->dz.as<double>(),
-                            solver->dq.as<double>(), lb, ub, order,
-                            (int)solver->prtree->maxbox,
-                            solver->dpot.as<double>(), solver->dgrad.as<double>());
+      return code_of_err(e);
+  return 0;
+}
+
+// The engine's problem for the solver's uploaded particles.
+aran::Problem solver_problem(AranSolver3d *solver, long n, int order) {
+  aran::Problem pb;
+  pb.n = n;
+  pb.order = order;
+  pb.leafcap = (int)solver->prtree->maxbox;
+  pb.lb[0] = solver->prtree->lb.x;
+  pb.lb[1] = solver->prtree->lb.y;
+  pb.lb[2] = solver->prtree->lb.z;
+  pb.ub[0] = solver->prtree->ub.x;
+  pb.ub[1] = solver->prtree->ub.y;
+  pb.ub[2] = solver->prtree->ub.z;
+  pb.x = solver->dx.as<double>();
+  pb.y = solver->dy.as<double>();
+  pb.z = solver->dz.as<double>();
+  pb.q = solver->dq.as<double>();
+  return pb;
+}
+
+// Host view of the engine's last tree: node records and LibVsg node infos
+// (bounds, centre, particle count, leaf flag, depth, father, key); the
+// particle lists are not kept on the host (point_list is NULL).
+struct HostTree {
+  long nn = 0;
+  std::vector<int> nodes;  // 10 ints per node (aran_fmm3d_export_tree)
+  std::vector<double> centers;
+  std::vector<VsgPRTree3dNodeInfo> info;
+};
+
+int host_tree(AranFmm3d *ctx, HostTree *t) {
+  long st[15];
+  aran_fmm3d_get_stats(ctx, st);
+  t->nn = st[8];
+  t->nodes.assign(10 * (size_t)t->nn, 0);
+  t->centers.assign(3 * (size_t)t->nn, 0.);
+  double lo[3], width = 1.;
+  int rc = aran_fmm3d_export_tree(ctx, nullptr, t->nodes.data(),
+                                  t->centers.data(), lo, &width);
+  if (rc != 0) return rc;
+  t->info.assign(t->nn, VsgPRTree3dNodeInfo());
+  for (long i = 0; i < t->nn; i++) {
+    const int *r = &t->nodes[10 * i];
+    VsgPRTree3dNodeInfo &f = t->info[i];
+    memset(&f, 0, sizeof f);
+    const double h = 0.5 * ldexp(width, -r[0]);
+    f.center.x = t->centers[3 * i];
+    f.center.y = t->centers[3 * i + 1];
+    f.center.z = t->centers[3 * i + 2];
+    f.lbound.x = f.center.x - h;
+    f.lbound.y = f.center.y - h;
+    f.lbound.z = f.center.z - h;
+    f.ubound.x = f.center.x + h;
+    f.ubound.y = f.center.y + h;
+    f.ubound.z = f.center.z + h;
+    f.point_count = (guint)(r[5] - r[4]);
+    f.point_list = nullptr;
+    f.isleaf = r[7] ? TRUE : FALSE;
+    f.depth = (guint8)r[0];
+    f.father_info = r[6] >= 0 ? &t->info[r[6]] : nullptr;
+    f.user_data = nullptr;
+    f.id.x = (guint64)r[1];
+    f.id.y = (guint64)r[2];
+    f.id.z = (guint64)r[3];
+    f.id.depth = (guint8)r[0];
+  }
+  return 0;
+}
+
+// Nodes the near/far leaf predicate turns into leaves: the first node on
+// each root-to-leaf path (breadth-first node order, parents first) for
+// which it returns TRUE, as (depth, Morton key), sorted.
+int solver_stop_list(AranSolver3d *solver, long n, int order,
+                     std::vector<unsigned long long> *stop) {
+  aran::Problem pb = solver_problem(solver, n, order);
+  pb.tree_only = true;
+  AranFmm3d *ctx = solver->ctx;
+  hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
+  ctx->n_last = (e == hipSuccess) ? n : 0;
+  ctx->order_last = 0;
+  if (e != hipSuccess) return code_of_err(e);
+  HostTree t;
+  int rc = host_tree(ctx, &t);
+  if (rc != 0) return rc;
+  std::vector<char> cut(t.nn, 0);
+  std::vector<std::pair<unsigned long long, unsigned long long>> v;
+  for (long i = 0; i < t.nn; i++) {
+    const int *r = &t.nodes[10 * i];
+    if (r[6] >= 0 && cut[r[6]]) {
+      cut[i] = 1;
+      continue;
+    }
+    if (r[7]) continue;  // a leaf already
+    if (solver->nf_isleaf(&t.info[i], solver->nf_isleaf_data)) {
+      cut[i] = 1;
+      v.emplace_back((unsigned long long)r[0],
+                     (unsigned long long)aran::morton_key(r[1], r[2], r[3]));
+    }
+  }
+  std::sort(v.begin(), v.end());
+  stop->clear();
+  for (auto &x : v) {
+    stop->push_back(x.first);
+    stop->push_back(x.second);
+  }
+  return 0;
+}
+
+// A Hilbert index of integer coordinates (b bits each) by J. Skilling's
+// transpose algorithm ("Programming the Hilbert curve", AIP Conf. Proc. 707,
+// 2004): the order in which aran_solver3d_traverse visits siblings after
+// aran_solver3d_set_children_order_hilbert.
+uint64_t hilbert_index(unsigned x, unsigned y, unsigned z, int b) {
+  if (b <= 0) return 0;
+  unsigned X[3] = {x, y, z};
+  const unsigned M = 1u << (b - 1);
+  for (unsigned Q = M; Q > 1; Q >>= 1) {
+    const unsigned P = Q - 1;
+    for (int i = 0; i < 3; i++) {
+      if (X[i] & Q) {
+        X[0] ^= P;
+      } else {
+        const unsigned t = (X[0] ^ X[i]) & P;
+        X[0] ^= t;
+        X[i] ^= t;
+      }
+    }
+  }
+  for (int i = 1; i < 3; i++) X[i] ^= X[i - 1];
+  unsigned t = 0;
+  for (unsigned Q = M; Q > 1; Q >>= 1)
+    if (X[2] & Q) t ^= Q - 1;
+  for (int i = 0; i < 3; i++) X[i] ^= t;
+  uint64_t h = 0;
+  for (int bit = b - 1; bit >= 0; bit--)
+    for (int i = 0; i < 3; i++) h = (h << 1) | ((X[i] >> bit) & 1u);
+  return h;
+}
+
+}  // namespace
+
+// Visits the nodes of the octree of the inserted particles (the last
+// solve's, or one built now) with func (aran/aransolver3d.c:1204-1217).
+// After a solve of the same particles each node's user_data is an
+// AranDevelopment3d holding its multipole and local; otherwise NULL.
+extern "C" void aran_solver3d_traverse(AranSolver3d *solver,
+                                       GTraverseType order,
+                                       VsgPRTree3dFunc func,
+                                       gpointer user_data) {
+  if (!solver || !func) return;
+  long n = 0;
+  if (!solver->solved) {
+    if (!dev_ok(solver->devel, "aran_solver3d_traverse")) return;
+    int rc = solver_upload(solver, &n);
+    if (rc == 0) {
+      aran::Problem pb =
+          solver_problem(solver, n, solver->devel->multipole->negdeg);
+      pb.tree_only = true;
+      AranFmm3d *ctx = solver->ctx;
+      hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
+      ctx->n_last = (e == hipSuccess) ? n : 0;
+      ctx->order_last = 0;
+      rc = code_of_err(e);
+    }
+    if (rc != 0) {
+      critical("aran_solver3d_traverse: %s", aran::last_error());
+      return;
+    }
+  }
+  HostTree t;
+  if (solver->ctx->n_last <= 0 || host_tree(solver->ctx, &t) != 0) {
+    if (solver->ctx->n_last > 0)
+      critical("aran_solver3d_traverse: %s", aran::last_error());
+    return;
+  }
+  // node developments from the device expansions of the last solve
+  std::vector<AranDevelopment3d *> devs;
+  if (solver->solved) {
+    const int p = solver->ctx->order_last;
+    const int mc = aran::tri(p), lc = aran::tri(p + 1);
+    std::vector<double> M(2 * (size_t)t.nn * mc), L(2 * (size_t)t.nn * lc);
+    if (aran_fmm3d_export_expansions(solver->ctx, M.data(), L.data()) != 0) {
+      critical("aran_solver3d_traverse: %s", aran::last_error());
+      return;
+    }
+    devs.resize(t.nn);
+    for (long i = 0; i < t.nn; i++) {
+      devs[i] = aran_development3d_clone(solver->devel);
+      aran_development3d_set_zero(devs[i]);
+      memcpy(ser_neg(devs[i]->multipole), &M[2 * (size_t)i * mc],
+             sizeof(double) * 2 * mc);
+      memcpy(ser_pos(devs[i]->local), &L[2 * (size_t)i * lc],
+             sizeof(double) * 2 * lc);
+      t.info[i].user_data = devs[i];
+    }
+  }
+  // children of each node, in octant order or along the Hilbert curve
+  auto children = [&](long v) {
+    std::vector<long> c;
+    const int *r = &t.nodes[10 * v];
+    for (int k = 0; k < r[9]; k++) c.push_back(r[8] + k);
+    if (solver->hilbert)
+      std::stable_sort(c.begin(), c.end(), [&](long a, long b) {
+        const int *ra = &t.nodes[10 * a], *rb = &t.nodes[10 * b];
+        return hilbert_index(ra[1], ra[2], ra[3], ra[0]) <
+               hilbert_index(rb[1], rb[2], rb[3], rb[0]);
+      });
+    return c;
+  };
+  if (order == G_LEVEL_ORDER) {
+    for (long i = 0; i < t.nn; i++) func(&t.info[i], user_data);
+  } else {
+    // depth-first from the root (node 0) without recursion
+    struct Frame {
+      long node;
+      std::vector<long> kids;
+      size_t next;
+    };
+    std::vector<Frame> stack;
+    auto push = [&](long v) {
+      if (order == G_PRE_ORDER) func(&t.info[v], user_data);
+      stack.push_back(Frame{v, children(v), 0});
+    };
+    if (t.nn > 0) push(0);
+    while (!stack.empty()) {
+      Frame &f = stack.back();
+      if (order == G_IN_ORDER && f.next == 1)  // after the first child
+        func(&t.info[f.node], user_data);
+      if (f.next < f.kids.size()) {
+        const long c = f.kids[f.next++];
+        push(c);
+        continue;
+      }
+      const long v = f.node;
+      const bool leafish = f.kids.empty();
+      stack.pop_back();
+      if (order == G_POST_ORDER || (order == G_IN_ORDER && leafish))
+        func(&t.info[v], user_data);
+    }
+  }
+  for (AranDevelopment3d *d : devs) aran_development3d_free(d);
+}
+
+extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
+  if (!solver) return;
+  solver->status = 0;
+  solver->solved = false;
+  SolveOps ops;
+  if (!solver_ops(solver, &ops)) {
+    solver->status = ARAN_FMM3D_EINVAL;
+    return;
+  }
+  const bool want_pot = (ops.outputs & 1) != 0,
+             want_field = (ops.outputs & 2) != 0;
+  long n = 0;
+  int rc = solver_upload(solver, &n);
   if (rc != 0) {
     critical("aran_solver3d_solve: %s", aran::last_error());
     solver->status = rc;
     return;
   }
+  std::vector<unsigned long long> stop;
+  if (solver->nf_isleaf && n > 0) {
+    rc = solver_stop_list(solver, n, ops.order, &stop);
+    if (rc != 0) {
+      critical("aran_solver3d_solve: %s", aran::last_error());
+      solver->status = rc;
+      return;
+    }
+  }
+  AranFmm3d *ctx = solver->ctx;
+  aran_fmm3d_set_precision(ctx, solver->precision);
+  aran_fmm3d_set_semifar(ctx, ops.semifar);
+  aran_fmm3d_set_translations(ctx, ops.flav[0], ops.flav[1], ops.flav[2]);
+  hipStream_t s = ctx->ws.stream;
+  aran::Problem pb = solver_problem(solver, n, ops.order);
+  pb.pot = solver->dpot.as<double>();
+  pb.grad = solver->dgrad.as<double>();
+  pb.timings = ctx->timings;
+  pb.precision = ctx->precision;
+  pb.semifar = ctx->semifar;
+  pb.autotune = ctx->autotune;
+  pb.flav_m2m = ctx->flav[0];
+  pb.flav_m2l = ctx->flav[1];
+  pb.flav_l2l = ctx->flav[2];
+  pb.stop = stop.empty() ? nullptr : stop.data();
+  pb.nstop = (int)(stop.size() / 2);
+  hipError_t e = aran::solve(ctx->ws, pb, &ctx->st, &ctx->ti);
+  ctx->n_last = (e == hipSuccess) ? n : 0;
+  ctx->order_last = (e == hipSuccess) ? ops.order : 0;
+  if (e != hipSuccess) {
+    critical("aran_solver3d_solve: %s", aran::last_error());
+    solver->status = code_of_err(e);
+    return;
+  }
+  auto fail = [&](hipError_t err) {
+    critical("aran_solver3d_solve: %s", hipGetErrorString(err));
+    solver->status = code_of_err(err);
+  };
   double *hp = solver->hpot.as<double>(), *hg = solver->hgrad.as<double>();
   if (n > 0) {
     if ((e = hipMemcpyAsync(hp, solver->dpot.ptr, sizeof(double) * n,
@@ -1500,8 +2274,10 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
     parallel_ranges(n, [=](long a, long b) {
       for (long i = a; i < b; i++) {
         char *p = static_cast<char *>(pts[i]);
-        if (want_pot && L.pot >= 0)
+        if (want_pot && L.pot >= 0) {
+          // a gcomplex128 potential takes the value in its real part
           *reinterpret_cast<double *>(p + L.pot) += hp[i];
+        }
         if (want_field && L.field >= 0) {
           double *f = reinterpret_cast<double *>(p + L.field);
           f[0] += hg[3 * i + 0];
@@ -1512,9 +2288,10 @@ extern "C" void aran_solver3d_solve(AranSolver3d *solver) {
     });
   }
   long st[15];
-  aran_fmm3d_get_stats(solver->ctx, st);
+  aran_fmm3d_get_stats(ctx, st);
   for (int k = 0; k < 7; k++) solver->stats[k] += st[k];
   solver->stats[7] += st[12];  // p2l
   solver->stats[8] += st[13];  // m2p
   solver->depth = (guint)st[7];
+  solver->solved = true;
 }

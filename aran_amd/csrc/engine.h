@@ -21,7 +21,12 @@
 namespace aran {
 
 constexpr int kMaxDepth = 21;   // 3 x 21 key bits
-constexpr int kMaxOrder = 15;   // local degrees 0..15
+// Orders: the plain FP64 form and the single-operator entry points take
+// local degrees 0..kMaxOrder; the mixed and MFMA forms, and the kernels
+// templated on the order (generic kernels take the orders above), go to
+// kMaxOrderFast.
+constexpr int kMaxOrder = 30;
+constexpr int kMaxOrderFast = 15;
 constexpr int kNumCodes = 343;  // translation offsets in [-3,3]^3
 
 inline __host__ __device__ int code_of(int dx, int dy, int dz) {
@@ -56,15 +61,21 @@ struct OpTables {
   const double *pgain;   // [tri(p)]    4 pi / (2l+1) N_l^m (l+m)!
   const double *igain;   // [tri(p+1)]  N_l^m / (l-m)!
   const double *xgain;   // [tri(p+1)]  4 pi / (2l+1) N_l^m / (l-m)!
+  const double *alpha;   // [tri(kAlphaDeg+1)]  1/sqrt((l-m)!(l+m)!)
+  const double *beta;    // [kAlphaDeg+1]       sqrt(4 pi / (2l+1))
   int p, rb;
 };
+// degree of the alpha / beta tables (every order's translations in any
+// flavour read them up to degree 2 kMaxOrder + 1)
+constexpr int kAlphaDeg = 2 * kMaxOrder + 2;
 
 struct Stats {
   long zero = 0, p2p = 0, p2m = 0, m2m = 0, m2l = 0, l2l = 0, l2p = 0;
   long p2l = 0, m2p = 0, semifar = 0;  // semifar: (leaf, node) pairs
   int depth = 0;
   long nnodes = 0, nleaves = 0;
-  int m2l_split = 0, m2l_tiles = 0;  // mixed form: tiled far-field launch
+  // tiled far-field launch: offset ranges, schedule, tiles
+  int m2l_split = 0, m2l_sched = 0, m2l_tiles = 0;
   // phase times in ms (only with Problem::timings)
   float t_tree = 0, t_lists = 0, t_up = 0, t_m2l = 0, t_down = 0, t_near = 0,
         t_total = 0;
@@ -84,7 +95,7 @@ struct Tables {
   int order = -1;
   int ntheta = 0;
   DevBuf rot, eiphi, dir, tm2l, tm2m, tl2l, ynorm, ogain, pgain, igain, xgain,
-      dscr, thetas;
+      dscr, thetas, alpha, beta;
   // tiled far field (mixed form): FP32 copies and octant offset lists
   DevBuf rot32, ephi32, vfac32, oct_codes, code_to_k;
   // MFMA far field: dense BF16 operators of every offset code (built on
@@ -117,12 +128,12 @@ struct Workspace {
   // of its P2L (x_*); s_part: per-leaf P2L/M2P particle counts
   DevBuf w_cnt, w_ofs, w_node, x_cnt, x_ofs, x_leaf, s_part;
   long semifar_nodes = -1;  // nodes covered by the semifar lists, -1: none
-  // autotuned far-field splits, by (form, order, size bucket)
-  std::map<long, int> tuned_split;
+  // autotuned far-field schedules, by (form, order, size bucket)
+  std::map<long, int> tuned_sched;
   DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
       tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma;
-  DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos;
+  DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos, stop;
   long node_cap = 0;
   int depth_hint = 0;  // depth of the last tree (levels of the first batch)
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
@@ -170,8 +181,17 @@ struct Problem {
   // semifar pairs (P2L / M2P) for nodes holding more than this many
   // particles; < 0: none (every leaf / deeper node pair is P2P)
   long semifar = -1;
-  // time the far-field split candidates on first use (see tune_split)
+  // time the far-field schedules on first use (see tune_schedule)
   bool autotune = false;
+  // translation flavours of M2M, M2L, L2L (0 rotate, 1 normal, 2 Kylin;
+  // the last two in the plain form only)
+  int flav_m2m = 0, flav_m2l = 0, flav_l2l = 0;
+  // build the tree and its leaf / internal lists only (no expansions)
+  bool tree_only = false;
+  // nodes that do not split (one system only): (depth, Morton key) pairs
+  // sorted by depth then key (host array)
+  const unsigned long long *stop = nullptr;  // [2 nstop]: depth, key
+  int nstop = 0;
 };
 
 struct TreeInfo {
@@ -233,8 +253,8 @@ hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
 // dl[2 d + 3].
 hipError_t launch_split_count(const NodeArrays &nd, int *dl, int d,
                               long max_count, const uint64_t *keys,
-                              int leafcap, int *bounds, int *cnt,
-                              hipStream_t s);
+                              int leafcap, const ulonglong2 *stop, int nstop,
+                              int *bounds, int *cnt, hipStream_t s);
 hipError_t launch_split_write(const NodeArrays &nd, const int *dl, int d,
                               long max_count, const int *bounds,
                               const int *cnt, const int *ofs, hipStream_t s);

@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "engine.h"
+#include "flavours.h"
 #include "m2l_mfma.h"
 #include "translate.h"
 #include "m2l_tile.h"
@@ -85,7 +86,7 @@ void Tables::release() {
   DevBuf *all[] = {&rot,    &eiphi,  &dir,       &tm2l,    &tm2m,  &tl2l,
                    &ynorm,  &ogain,  &pgain,     &igain,   &xgain, &dscr,
                    &thetas, &rot32,  &ephi32,    &vfac32,  &oct_codes,
-                   &code_to_k, &mfma_ops};
+                   &code_to_k, &mfma_ops, &alpha, &beta};
   for (DevBuf *b : all) b->release();
   order = -1;
   mfma_order = -1;
@@ -105,6 +106,8 @@ OpTables Tables::view() const {
   t.pgain = pgain.as<double>();
   t.igain = igain.as<double>();
   t.xgain = xgain.as<double>();
+  t.alpha = alpha.as<double>();
+  t.beta = beta.as<double>();
   t.p = order;
   t.rb = rot_block(order);
   return t;
@@ -143,7 +146,7 @@ Workspace::~Workspace() {
                    &small, &pleaf, &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
                    &mult_bf, &loc_mfma, &n_sys, &sys_box, &sys_off, &sys_mm,
-                   &skey, &spos, &internal_list,
+                   &skey, &spos, &stop, &internal_list,
                    &trans_rows, &packed,
                    &lvl_table, &child_map};
   for (DevBuf *b : all) b->release();
@@ -454,6 +457,17 @@ hipError_t ensure_tables(Workspace &ws, int p) {
                     hipMemcpyHostToDevice, s));
   CK(hipMemcpyAsync(T.xgain.ptr, xg.data(), sizeof(double) * xg.size(),
                     hipMemcpyHostToDevice, s));
+  // alpha / beta to degree kAlphaDeg (the normal and Kylin translations)
+  {
+    std::vector<double> ak = alpha_table(kAlphaDeg), bk(kAlphaDeg + 1);
+    for (int l = 0; l <= kAlphaDeg; l++) bk[l] = beta_l(l);
+    CK(T.alpha.reserve(sizeof(double) * ak.size()));
+    CK(T.beta.reserve(sizeof(double) * bk.size()));
+    CK(hipMemcpyAsync(T.alpha.ptr, ak.data(), sizeof(double) * ak.size(),
+                      hipMemcpyHostToDevice, s));
+    CK(hipMemcpyAsync(T.beta.ptr, bk.data(), sizeof(double) * bk.size(),
+                      hipMemcpyHostToDevice, s));
+  }
   CK(launch_wigner_tables(T.thetas.as<double>(), nth, p, T.dscr.as<double>(),
                           T.rot.as<double>(), s));
   // tiled far field: octant offset lists and FP32 tables
@@ -466,13 +480,16 @@ hipError_t ensure_tables(Workspace &ws, int p) {
   CK(hipMemcpyAsync(T.code_to_k.ptr, code_to_k.data(),
                     sizeof(short) * code_to_k.size(), hipMemcpyHostToDevice, s));
   const long nrot = (long)nth * 4 * rb;
-  CK(T.rot32.reserve(sizeof(float) * nrot));
-  CK(T.ephi32.reserve(sizeof(float) * 2 * eiphi.size()));
-  CK(T.vfac32.reserve(sizeof(float) * (size_t)kNumCodes *
-                      std::max(1, vfac_size(p))));
-  CK(launch_m2l_tables(1, p, T.dir.as<DirEntry>(), T.tm2l.as<double>(),
-                       T.rot.as<double>(), nrot, T.eiphi.as<double2>(),
-                       T.vfac32.ptr, T.rot32.ptr, T.ephi32.ptr, s));
+  // FP32 tables of the tiled far field (mixed form: orders <= 15)
+  if (p <= kMaxOrderFast) {
+    CK(T.rot32.reserve(sizeof(float) * nrot));
+    CK(T.ephi32.reserve(sizeof(float) * 2 * eiphi.size()));
+    CK(T.vfac32.reserve(sizeof(float) * (size_t)kNumCodes *
+                        std::max(1, vfac_size(p))));
+    CK(launch_m2l_tables(1, p, T.dir.as<DirEntry>(), T.tm2l.as<double>(),
+                         T.rot.as<double>(), nrot, T.eiphi.as<double2>(),
+                         T.vfac32.ptr, T.rot32.ptr, T.ephi32.ptr, s));
+  }
   T.dir_host = dirs;
   CK(hipStreamSynchronize(s));
   T.ntheta = nth;
@@ -529,17 +546,15 @@ hipError_t build_far_lists(Workspace &ws, long nn, int *nfar) {
   return hipSuccess;
 }
 
-// ---- far-field split autotuner
+// ---- far-field schedule autotuner
 //
-// The offset range count (nsplit) of the tiled far-field kernels trades
-// parallelism against partial-sum traffic; the best value depends on the
-// form, the order and the number of tiles or workgroups.  With
-// Problem::autotune the first solve of each (form, order, size bucket) times
-// every candidate on the device (HIP events on the engine stream) and keeps
-// the fastest for later solves of that bucket.
-constexpr int kTuneMaxSplit = 16;
-static const int kTuneCandidates[] = {1, 2, 3, 4, 5, 6, 8, 10, 12, 16};
-
+// The tiled far-field kernels can walk their work in several orders over the
+// XCDs (which tiles and offset ranges share an XCD's L2 at a time); every
+// order computes the same partial sums over the same fixed offset ranges, so
+// the choice changes the speed, never the result.  With Problem::autotune
+// the first solve of each (form, order, size bucket) times every schedule on
+// the device (HIP events on the engine stream, far field plus its slice
+// reduction) and keeps the fastest for later solves of that bucket.
 static long tune_key(int precision, int p, int units) {
   int bucket = 0;  // units rounded down to a power of two
   while ((2 << bucket) <= units) bucket++;
@@ -547,10 +562,11 @@ static long tune_key(int precision, int p, int units) {
 }
 
 template <class Run>
-static hipError_t tune_split(Workspace &ws, long key, Run run, int *nsplit) {
-  auto it = ws.tuned_split.find(key);
-  if (it != ws.tuned_split.end()) {
-    *nsplit = it->second;
+static hipError_t tune_schedule(Workspace &ws, long key, int nchoices, Run run,
+                                int *choice) {
+  auto it = ws.tuned_sched.find(key);
+  if (it != ws.tuned_sched.end()) {
+    *choice = it->second;
     return hipSuccess;
   }
   hipStream_t s = ws.stream;
@@ -558,9 +574,9 @@ static hipError_t tune_split(Workspace &ws, long key, Run run, int *nsplit) {
   CK(hipEventCreate(&e0));
   CK(hipEventCreate(&e1));
   float best_ms = -1.f;
-  int best = *nsplit;
+  int best = *choice;
   hipError_t err = run(best);  // warm the caches once
-  for (int c : kTuneCandidates) {
+  for (int c = 0; c < nchoices; c++) {
     if (err != hipSuccess) break;
     float ms = 0.f;
     if ((err = hipEventRecord(e0, s)) != hipSuccess) break;
@@ -576,8 +592,8 @@ static hipError_t tune_split(Workspace &ws, long key, Run run, int *nsplit) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (err != hipSuccess) return err;
-  ws.tuned_split[key] = best;
-  *nsplit = best;
+  ws.tuned_sched[key] = best;
+  *choice = best;
   return hipSuccess;
 }
 
@@ -605,7 +621,23 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   const long n = pb.n;
   const int p = pb.order;
   if (p < 1 || p > kMaxOrder) {
-    set_error("order must be in [1, 15]");
+    set_error("order must be in [1, 30]");
+    return hipErrorInvalidValue;
+  }
+  if (pb.precision != 0 && p > kMaxOrderFast) {
+    set_error("orders above 15 run in the plain FP64 form only");
+    return hipErrorInvalidValue;
+  }
+  const int flav[3] = {pb.flav_m2m, pb.flav_m2l, pb.flav_l2l};
+  for (int f : flav)
+    if (f < kFlavRotate || f > kFlavKylin ||
+        (f != kFlavRotate && pb.precision != 0)) {
+      set_error("the normal and Kylin translation flavours run in the plain "
+                "FP64 form only");
+      return hipErrorInvalidValue;
+    }
+  if (pb.nstop > 0 && (pb.nsys > 1 || !pb.stop)) {
+    set_error("a stop list needs one system");
     return hipErrorInvalidValue;
   }
   if (pb.leafcap < 1 || n < 0 || n > (1L << 30)) {
@@ -733,6 +765,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64 + nsys);
   if (ws.node_cap < cap0) CK(reserve_nodes(ws, cap0, 0));
   NodeArrays nd = ws.nodes();
+  const ulonglong2 *d_stop = nullptr;
+  if (pb.nstop > 0) {
+    CK(ws.stop.reserve(sizeof(ulonglong2) * pb.nstop));
+    CK(stage_copy(ws, ws.stop.ptr, pb.stop, sizeof(ulonglong2) * pb.nstop, s));
+    d_stop = ws.stop.as<ulonglong2>();
+  }
   CK(launch_roots(nd, nsys, ws.sys_off.as<long>(), s));
   // Levels in batches of kLevelBatch without a host round trip: the level
   // table lives on the device and grids are sized by upper bounds (a level
@@ -776,8 +814,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
       }
       for (int d = d0; d <= d1; d++) {
         const long ubd = ubs[d - d0];
-        CK(launch_split_count(nd, d_lvl, d, ubd, keys, pb.leafcap,
-                              ws.split_bounds.as<int>(),
+        CK(launch_split_count(nd, d_lvl, d, ubd, keys, pb.leafcap, d_stop,
+                              pb.nstop, ws.split_bounds.as<int>(),
                               ws.split_cnt.as<int>(), s));
         CK(scan_device_count(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(),
                              d_lvl + 2 * d + 1, d_lvl + 2 * d + 3, s));
@@ -878,6 +916,26 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                        ws.trans_rows.as<TransRow>(), s));
   const TransRow *trows = ws.trans_rows.as<TransRow>();
   if (pb.timings) CK(hipEventRecord(ws.ev[1], s));
+  if (pb.tree_only) {
+    CK(hipStreamSynchronize(s));
+    S.depth = depth;
+    S.nnodes = nn;
+    S.nleaves = nleaves;
+    if (ti) {
+      for (int k = 0; k < 3; k++) ti->lo[k] = box[k];
+      ti->width = box[3];
+      ti->nsys = nsys;
+      ti->depth = depth;
+      ti->nnodes = nn;
+      ti->nleaves = nleaves;
+      ti->nfar = ti->nnear = 0;
+      ti->level_start = lstart;
+      ti->level_count = lcount;
+    }
+    ws.far_csr_nodes = -1;
+    ws.semifar_nodes = -1;
+    return hipSuccess;
+  }
 
   // ---- colleagues, level by level
   CK(ws.colleagues.reserve(sizeof(int) * 27 * (size_t)nn));
@@ -951,10 +1009,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.node_slot.reserve(sizeof(int) * nn));
     CK(stage_copy(ws, ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles, s));
     // MFMA far field: workgroups of consecutive chunks of one (level,
-    // octant) group (they share every offset code), in the order level,
-    // chunk block, octant (neighbouring workgroups are neighbours in space)
+    // octant) group (they share every offset code).  Two orders of the same
+    // workgroups: level, chunk block, octant (schedule 0: neighbouring
+    // workgroups are neighbours in space) and octant, level, chunk block
+    // (schedule 1: neighbouring workgroups share their operators).
     if (pb.precision >= 2) {
-      std::vector<int> wgt;
+      std::vector<int> wgt, wgt1;
       const int tpw = mfma_wg_tiles(p);
       for (int d = 2; d <= depth; d++) {
         int maxch = 0;
@@ -968,7 +1028,15 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
               wgt.push_back(j < nch ? tile_of[gofs[d * 8 + o] + j] : -1);
           }
       }
+      for (int o = 0; o < 8; o++)
+        for (int d = 2; d <= depth; d++) {
+          const int nch = (h_cnt[d * 8 + o] + 63) / 64;
+          for (int jb = 0; jb < nch; jb += tpw)
+            for (int j = jb; j < jb + tpw; j++)
+              wgt1.push_back(j < nch ? tile_of[gofs[d * 8 + o] + j] : -1);
+        }
       nwg = (int)(wgt.size() / tpw);
+      wgt.insert(wgt.end(), wgt1.begin(), wgt1.end());
       CK(ws.tile_perm.reserve(sizeof(int) * std::max<size_t>(1, wgt.size())));
       CK(stage_copy(ws, ws.tile_perm.ptr, wgt.data(), sizeof(int) * wgt.size(),
                     s));
@@ -1055,52 +1123,63 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   OpTables tb = ws.tab.view();
   CK(launch_p2m(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
                 ws.zs.as<double>(), ws.qs.as<double>(), M, s));
-  for (int d = depth - 1; d >= 0; d--)
-    CK(launch_m2m_level(tb, trows + ibase[d], icount[d], M, s));
+  for (int d = depth - 1; d >= 0; d--) {
+    if (pb.flav_m2m != kFlavRotate)
+      CK(launch_flavour_level(tb, kFlavM2M, pb.flav_m2m, nd, trows + ibase[d],
+                              icount[d], M, s));
+    else
+      CK(launch_m2m_level(tb, trows + ibase[d], icount[d], M, s));
+  }
 
   // ---- far field (the M2L timer brackets the M2L kernel alone)
   if (pb.precision == 0) {
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
-    CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
-                  ws.v_code.as<short>(), M, L, s));
+    if (pb.flav_m2l != kFlavRotate)
+      CK(launch_flavour_m2l(tb, pb.flav_m2l, nd, nn, ws.v_ofs.as<int>(),
+                            ws.v_src.as<int>(), M, L, s));
+    else
+      CK(launch_m2l(tb, nd, nn, ws.v_ofs.as<int>(), ws.v_src.as<int>(),
+                    ws.v_code.as<short>(), M, L, s));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
   } else if (pb.precision >= 2) {
     const MfmaDims D = mfma_dims(p);
     const int first = depth >= 2 ? lstart[2] : nn;
-    auto run = [&](int ns) {
+    const int nsplit = m2l_mfma_split(p);
+    const size_t wgt_len = (size_t)nwg * mfma_wg_tiles(p);
+    auto run = [&](int sched) {
       return launch_m2l_mfma(pb.precision == 2 ? 3 : 6, p, nwg,
-                             ws.tile_perm.as<int>(), ntiles,
+                             ws.tile_perm.as<int>() + sched * wgt_len, ntiles,
                              ws.tile_src.as<int>(), ws.tile_oct.as<int>(),
                              ws.tab.oct_codes.as<short>(), ws.tab.mfma_ops.ptr,
-                             ws.mult_bf.ptr, nn, ns, ws.loc_mfma.as<float>(),
-                             s);
+                             ws.mult_bf.ptr, nn, nsplit, sched,
+                             ws.loc_mfma.as<float>(), s);
     };
     // the tuner times the far field with its slice reduction
-    auto run_all = [&](int ns) {
-      hipError_t e = run(ns);
+    auto run_all = [&](int sched) {
+      hipError_t e = run(sched);
       if (e != hipSuccess) return e;
       return launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
-                                 ws.loc_mfma.as<float>(), ns, L, s);
+                                 ws.loc_mfma.as<float>(), nsplit, L, s);
     };
     CK(ws.mult_bf.reserve(16 * (size_t)(nn + 1) * mfma_mult_units(p)));
     const size_t slice = sizeof(float) * std::max(1, ntiles) * 64 * D.ro;
     CK(launch_mfma_pack_mult(nd, nn, p, M, ws.mult_bf.ptr, s));
-    int nsplit = m2l_mfma_split(nwg, p);
-    if (pb.autotune && ntiles > 0) {
-      CK(ws.loc_mfma.reserve(slice * kTuneMaxSplit));
-      CK(tune_split(ws, tune_key(pb.precision, p, nwg), run_all, &nsplit));
-    }
     CK(ws.loc_mfma.reserve(slice * nsplit));
+    int sched = mfma_default_schedule();
+    if (pb.autotune && ntiles > 0)
+      CK(tune_schedule(ws, tune_key(pb.precision, p, nwg), kMfmaSchedules,
+                       run_all, &sched));
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
-    if (ntiles > 0) CK(run(nsplit));
+    if (ntiles > 0) CK(run(sched));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     CK(launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
                            ws.loc_mfma.as<float>(), nsplit, L, s));
     S.m2l_split = nsplit;
+    S.m2l_sched = sched;
     S.m2l_tiles = ntiles;
   } else {
     CK(ws.mult32.reserve(sizeof(float2) * (size_t)nn * mc));
-    int nsplit = m2l_choose_split(ntiles, p);
+    const int nsplit = m2l_choose_split(p);
     const size_t slice = sizeof(float2) * std::max(1, ntiles) * 64 * lc;
     CK(launch_scale_mult(1, nd, nn, p, M, ws.mult32.ptr, s));
     M2LTileArgs ta;
@@ -1113,30 +1192,30 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     ta.vfac = ws.tab.vfac32.ptr;
     ta.tile_major = m2l_tile_major();
     const int first = depth >= 2 ? lstart[2] : nn;
-    auto run = [&](int ns) {
+    ta.nsplit = nsplit;
+    auto run = [&](int major) {
       M2LTileArgs a = ta;
-      a.nsplit = ns;
+      a.tile_major = major;
       return launch_m2l_tile(1, p, ntiles, a, ws.mult32.ptr, ws.loc32.ptr, s);
     };
-    auto run_all = [&](int ns) {
-      hipError_t e = run(ns);
+    auto run_all = [&](int major) {
+      hipError_t e = run(major);
       if (e != hipSuccess) return e;
       return launch_unscale_local(1, nd, p, first, ntiles,
-                                  ws.tile_tgt.as<int>(), ws.loc32.ptr, ns, L,
-                                  s);
+                                  ws.tile_tgt.as<int>(), ws.loc32.ptr, nsplit,
+                                  L, s);
     };
-    if (pb.autotune && ntiles > 0) {
-      CK(ws.loc32.reserve(slice * kTuneMaxSplit));
-      CK(tune_split(ws, tune_key(1, p, ntiles), run_all, &nsplit));
-    }
     CK(ws.loc32.reserve(slice * nsplit));
-    ta.nsplit = nsplit;
+    int major = ta.tile_major;
+    if (pb.autotune && ntiles > 0)
+      CK(tune_schedule(ws, tune_key(1, p, ntiles), 2, run_all, &major));
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
-    if (ntiles > 0) CK(run(nsplit));
+    if (ntiles > 0) CK(run(major));
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     CK(launch_unscale_local(1, nd, p, first, ntiles, ws.tile_tgt.as<int>(),
                             ws.loc32.ptr, nsplit, L, s));
     S.m2l_split = nsplit;
+    S.m2l_sched = major;
     S.m2l_tiles = ntiles;
   }
   // ---- semifar P2L into the locals (before they move down)
@@ -1147,8 +1226,13 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // ---- downward pass
   // locals of levels 0 and 1 receive nothing without semifar pairs (far
   // interactions start at level 2), so their translations would add zeros
-  for (int d = semifar ? 1 : 3; d <= depth; d++)
-    CK(launch_l2l_level(tb, trows + ibase[d - 1], icount[d - 1], L, s));
+  for (int d = semifar ? 1 : 3; d <= depth; d++) {
+    if (pb.flav_l2l != kFlavRotate)
+      CK(launch_flavour_level(tb, kFlavL2L, pb.flav_l2l, nd,
+                              trows + ibase[d - 1], icount[d - 1], L, s));
+    else
+      CK(launch_l2l_level(tb, trows + ibase[d - 1], icount[d - 1], L, s));
+  }
   if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
   // ---- join the near field, then M2P and L2P add to its results
   CK(hipStreamWaitEvent(s, ws.ev_join, 0));

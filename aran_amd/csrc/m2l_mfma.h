@@ -56,12 +56,13 @@ hipError_t launch_mfma_pack_mult(const NodeArrays &nd, int nnodes, int p,
 // The tiled far field: nprod 3 or 6; wg_tiles[nwg][8 / (2 / col blocks)]
 // lists the tiles of each workgroup (one (level, octant) group, Morton
 // order, -1 padding); Lsm receives [split][tile][ro][64] FP32 partial
-// scaled locals.
+// scaled locals; sched picks the workgroup order (schedule 1 expects an
+// octant-major wg_tiles list).
 hipError_t launch_m2l_mfma(int nprod, int p, int nwg, const int *wg_tiles,
                            int ntiles, const int *tile_src,
                            const int *tile_oct, const short *oct_codes,
                            const void *ops, const void *mb, int zero_node,
-                           int nsplit, float *Lsm, hipStream_t s);
+                           int nsplit, int sched, float *Lsm, hipStream_t s);
 // Tiles per workgroup of the MFMA far field at order p.
 constexpr __host__ __device__ int mfma_wg_tiles(int p) {
   return 8 / (2 / mfma_col_blocks(p));
@@ -72,6 +73,14 @@ hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
                                int ntiles, const int *tile_tgt,
                                const float *Lsm, int nsplit, double2 *L,
                                hipStream_t s);
-int m2l_mfma_split(int nwg, int p);
+// Offset ranges of the MFMA far field (fixed per order: a target's sums do
+// not depend on the rest of the solve).
+constexpr int kFarRangesMfma = 6;
+int m2l_mfma_split(int p);
+// Workgroup schedules of the MFMA far field (same sums, different order of
+// work over the XCDs): 0 space-major, 1 offset-range-major.
+constexpr int kMfmaSchedules = 2;
+// Schedule used without the autotuner (ARAN_MFMA_SCHED overrides it).
+int mfma_default_schedule();
 
 }  // namespace aran

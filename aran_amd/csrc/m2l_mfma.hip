@@ -67,7 +67,8 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
     const int *__restrict__ wg_tiles, int nwg, const int *__restrict__ tile_src,
     const int *__restrict__ tile_oct, const short *__restrict__ oct_codes,
     const uint4 *__restrict__ ops, const uint4 *__restrict__ mb,
-    int zero_node, float *__restrict__ Lsm, int ntiles, int nsplit) {
+    int zero_node, float *__restrict__ Lsm, int ntiles, int nsplit,
+    int sched) {
   constexpr MfmaDims D = mfma_dims(P);
   constexpr int KS = D.ks, RB = D.rb, RO = D.ro;
   constexpr int CB = mfma_col_blocks(P);
@@ -83,13 +84,19 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
   constexpr int GU = G * CHUNK;
   constexpr int PER = (GU + NT - 1) / NT;
   __shared__ uint4 ring[2][GU];
-  // workgroups are dealt round-robin to the 8 XCDs: give each XCD one
-  // contiguous range of tile groups, with all their offset ranges side by
-  // side, so that the sources a group gathers stay in that XCD's L2
+  // Workgroups are dealt round-robin to the 8 XCDs; each XCD takes one
+  // contiguous range of the logical order.  Schedule 0: tile groups in
+  // space order with all their offset ranges side by side (the sources a
+  // group gathers stay in that XCD's L2).  Schedule 1: offset range major
+  // over an octant-major group list (the XCD's workgroups share the
+  // operators of one octant's range in L2).  Both give the same sums: a
+  // target's partial sum over one fixed offset range does not depend on
+  // which workgroup computes it or when.
   const int per_xcd = gridDim.x / 8;
   const int logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
   if (logical >= nwg * nsplit) return;  // whole workgroup
-  const int split = logical % nsplit, wgi = logical / nsplit;
+  const int split = sched ? logical / nwg : logical % nsplit;
+  const int wgi = sched ? logical % nwg : logical / nsplit;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
   const int tid = threadIdx.x, lane = tid % 64;
   const int g = lane / 32, c32 = lane % 32;
@@ -286,7 +293,7 @@ static hipError_t mfma_range(int nprod, int p, int nwg, const int *wgt,
                              int ntiles, const int *tsrc, const int *toct,
                              const short *codes, const uint4 *ops,
                              const uint4 *mb, int zero_node, int nsplit,
-                             float *Lsm, hipStream_t s) {
+                             int sched, float *Lsm, hipStream_t s) {
   if constexpr (P > ARAN_MFMA_PMAX) {
     return hipErrorInvalidValue;
   } else {
@@ -296,15 +303,15 @@ static hipError_t mfma_range(int nprod, int p, int nwg, const int *wgt,
       if (nprod == 6)
         k_m2l_mfma<P, 6><<<nblocks, 64 * kMfmaWaves, 0, s>>>(
             wgt, nwg, tsrc, toct, codes, ops, mb, zero_node, Lsm, ntiles,
-            nsplit);
+            nsplit, sched);
       else
         k_m2l_mfma<P, 3><<<nblocks, 64 * kMfmaWaves, 0, s>>>(
             wgt, nwg, tsrc, toct, codes, ops, mb, zero_node, Lsm, ntiles,
-            nsplit);
+            nsplit, sched);
       return hipGetLastError();
     }
     return mfma_range<P + 1>(nprod, p, nwg, wgt, ntiles, tsrc, toct, codes,
-                             ops, mb, zero_node, nsplit, Lsm, s);
+                             ops, mb, zero_node, nsplit, sched, Lsm, s);
   }
 }
 
@@ -314,12 +321,12 @@ static hipError_t mfma_range(int nprod, int p, int nwg, const int *wgt,
 hipError_t ARAN_MCAT(launch_m2l_mfma_, ARAN_MFMA_PMIN)(
     int nprod, int p, int nwg, const int *wgt, int ntiles, const int *tsrc,
     const int *toct, const short *codes, const void *ops, const void *mb,
-    int zero_node, int nsplit, float *Lsm, hipStream_t s) {
+    int zero_node, int nsplit, int sched, float *Lsm, hipStream_t s) {
   if (p < ARAN_MFMA_PMIN || p > ARAN_MFMA_PMAX) return hipErrorInvalidValue;
   return mfma_range<ARAN_MFMA_PMIN>(
       nprod, p, nwg, wgt, ntiles, tsrc, toct, codes,
       static_cast<const uint4 *>(ops), static_cast<const uint4 *>(mb),
-      zero_node, nsplit, Lsm, s);
+      zero_node, nsplit, sched, Lsm, s);
 }
 
 }  // namespace aran

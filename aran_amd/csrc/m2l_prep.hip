@@ -38,8 +38,8 @@ hipError_t launch_m2l_tile_14(int p, int ntiles, const M2LTileArgs &a,
                                  int ntiles, const int *tsrc,                \
                                  const int *toct, const short *codes,        \
                                  const void *ops, const void *mb,            \
-                                 int zero_node, int nsplit, float *Lsm,      \
-                                 hipStream_t s);
+                                 int zero_node, int nsplit, int sched,       \
+                                 float *Lsm, hipStream_t s);
 ARAN_MFMA_DECL(1)
 ARAN_MFMA_DECL(7)
 ARAN_MFMA_DECL(10)
@@ -50,14 +50,14 @@ hipError_t launch_m2l_mfma(int nprod, int p, int nwg, const int *wg_tiles,
                            int ntiles, const int *tsrc, const int *toct,
                            const short *codes, const void *ops,
                            const void *mb, int zero_node, int nsplit,
-                           float *Lsm, hipStream_t s) {
+                           int sched, float *Lsm, hipStream_t s) {
   if (nwg <= 0 || ntiles <= 0) return hipSuccess;
   if (nprod != 3 && nprod != 6) return hipErrorInvalidValue;
   auto f = p <= 6 ? launch_m2l_mfma_1 : p <= 9 ? launch_m2l_mfma_7
            : p <= 11 ? launch_m2l_mfma_10 : p <= 13 ? launch_m2l_mfma_12
                                                     : launch_m2l_mfma_14;
   return f(nprod, p, nwg, wg_tiles, ntiles, tsrc, toct, codes, ops, mb,
-           zero_node, nsplit, Lsm, s);
+           zero_node, nsplit, sched, Lsm, s);
 }
 
 hipError_t launch_m2l_tile(int precision, int p, int ntiles,
@@ -295,9 +295,9 @@ __global__ __launch_bounds__(256) void k_unscale_local(
     const typename V2<T>::type *__restrict__ Ls, int nsplit,
     double2 *__restrict__ L) {
   __shared__ double2 sl[64 * (kUnscaleRows + 1)];
-  __shared__ double spw[64][kMaxOrder + 2];  // 1 / w^(l+1)
+  __shared__ double spw[64][kMaxOrderFast + 2];  // 1 / w^(l+1)
   __shared__ int snode[64];
-  __shared__ unsigned char sdeg[(kMaxOrder + 2) * (kMaxOrder + 3) / 2];
+  __shared__ unsigned char sdeg[(kMaxOrderFast + 2) * (kMaxOrderFast + 3) / 2];
   const int lc = tri_p(p + 1);
   const int row = kUnscaleRows + 1;  // odd row stride in LDS
   const int tile = blockIdx.x;
@@ -353,28 +353,18 @@ int m2l_tile_major() {
   return 1;
 }
 
-int m2l_choose_split(int ntiles, int p) {
+// Offset ranges per tile of the FP32 far field: a fixed number per order,
+// never derived from the size of the solve, so that a target's partial sums
+// (and the order they are added in) are the same whatever else is solved
+// with it (a batch of systems, a served mix).  ARAN_M2L_SPLIT overrides it
+// (1..189) for experiments; results then follow that split.
+int m2l_choose_split(int p) {
+  (void)p;
   if (const char *e = getenv("ARAN_M2L_SPLIT")) {
     int v = atoi(e);
     if (v >= 1 && v <= kOctOffsets) return v;
   }
-  // A CU holds 4 * m2l_tile_waves(p) / kTileWaves workgroups; the duration
-  // is modelled as the number of workgroup rounds times the offsets per
-  // workgroup (plus a fixed cost of two offsets for the output rows and the
-  // slice reduction).
-  const int ngroups = (ntiles + kTileWaves - 1) / kTileWaves;
-  const long slots = 256L * std::max(1, 4 * m2l_tile_waves(p) / kTileWaves);
-  int best = 1;
-  long best_cost = -1;
-  for (int ns = 1; ns <= 24; ns++) {
-    long rounds = ((long)ngroups * ns + slots - 1) / slots;
-    long cost = rounds * ((kOctOffsets + ns - 1) / ns + 2);
-    if (best_cost < 0 || cost < best_cost) {
-      best_cost = cost;
-      best = ns;
-    }
-  }
-  return best;
+  return kFarRangesTile;
 }
 
 hipError_t launch_scale_mult(int precision, const NodeArrays &nd, int nnodes,
@@ -546,10 +536,10 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
     NodeArrays nd, int p, int ntiles, const int *__restrict__ tile_tgt,
     const float *__restrict__ Lsm, int nsplit, double2 *__restrict__ L) {
   __shared__ double2 sl[64 * (kUnscaleRows + 1)];
-  __shared__ double spw[64][kMaxOrder + 2];  // 1 / w^(l+1)
+  __shared__ double spw[64][kMaxOrderFast + 2];  // 1 / w^(l+1)
   __shared__ int snode[64];
-  __shared__ short srim[(kMaxOrder + 2) * (kMaxOrder + 3) / 2];
-  __shared__ unsigned char sdeg[(kMaxOrder + 2) * (kMaxOrder + 3) / 2];
+  __shared__ short srim[(kMaxOrderFast + 2) * (kMaxOrderFast + 3) / 2];
+  __shared__ unsigned char sdeg[(kMaxOrderFast + 2) * (kMaxOrderFast + 3) / 2];
   const MfmaDims D = mfma_dims(p);
   const int lc = D.lc;
   const int row = kUnscaleRows + 1;  // odd row stride in LDS
@@ -615,27 +605,24 @@ hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
   return hipGetLastError();
 }
 
-// Offset ranges per workgroup of the MFMA far field: one workgroup of 8
-// waves fills a CU (two waves per SIMD), so the duration is modelled as
-// rounds of 256 workgroups times the offsets per range (plus two offsets'
-// worth for the output rows and the slice reduction).
-int m2l_mfma_split(int nwg, int p) {
+// Offset ranges per tile of the MFMA far field: fixed per order, as for the
+// FP32 far field (results do not depend on the batch a system is solved
+// in); ARAN_M2L_SPLIT overrides it.
+int m2l_mfma_split(int p) {
   (void)p;
   if (const char *e = getenv("ARAN_M2L_SPLIT")) {
     int v = atoi(e);
     if (v >= 1 && v <= kOctOffsets) return v;
   }
-  int best = 1;
-  long best_cost = -1;
-  for (int ns = 1; ns <= 24; ns++) {
-    const long rounds = ((long)std::max(1, nwg) * ns + 255) / 256;
-    const long cost = rounds * ((kOctOffsets + ns - 1) / ns + 2);
-    if (best_cost < 0 || cost < best_cost) {
-      best_cost = cost;
-      best = ns;
-    }
+  return kFarRangesMfma;
+}
+
+int mfma_default_schedule() {
+  if (const char *e = getenv("ARAN_MFMA_SCHED")) {
+    const int v = atoi(e);
+    if (v >= 0 && v < kMfmaSchedules) return v;
   }
-  return best;
+  return 0;
 }
 
 // ------------------------------------------------------------ tables

@@ -80,8 +80,10 @@ hipError_t launch_unscale_local(int precision, const NodeArrays &nd, int p,
                                 int first, int ntiles, const int *tile_tgt,
                                 const void *Ls, int nsplit, double2 *L,
                                 hipStream_t s);
-// Offset ranges per tile for a launch over ntiles tiles of order p.
-int m2l_choose_split(int ntiles, int p);
+// Offset ranges per tile of the FP32 far field at order p (fixed per order,
+// so a target's sums do not depend on the rest of the solve).
+constexpr int kFarRangesTile = 8;
+int m2l_choose_split(int p);
 // Workgroup order of that launch (M2LTileArgs::tile_major).
 int m2l_tile_major();
 hipError_t launch_m2l_tables(int precision, int p, const DirEntry *dir,

@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <algorithm>
 #include <type_traits>
 
 #include "engine.h"
@@ -79,7 +80,7 @@ __device__ double wigner_seed(int a, int b, double c, double s) {
   return ((a + j) & 1) ? -v : v;
 }
 
-// One block per theta; (2L+1)^2 <= blockDim threads, one per (a, b).
+// One block per theta; threads over the (2L+1)^2 pairs (a, b).
 __global__ void k_wigner_tables(const double *__restrict__ thetas, int L,
                                 double *__restrict__ dscr,
                                 double *__restrict__ rot) {
@@ -89,8 +90,7 @@ __global__ void k_wigner_tables(const double *__restrict__ thetas, int L,
   double *dth = dscr + (size_t)th * (L + 1) * sq;
   const double beta = thetas[th];
   const double cb = cos(beta), c = cos(0.5 * beta), s = sin(0.5 * beta);
-  int t = threadIdx.x;
-  if (t < sq) {
+  for (int t = threadIdx.x; t < sq; t += blockDim.x) {
     int a = t / w - L, b = t % w - L;
     int j = max(abs(a), abs(b));
     double dm2 = 0., dm1 = wigner_seed(a, b, c, s);
@@ -133,16 +133,18 @@ __global__ void k_wigner_tables(const double *__restrict__ thetas, int L,
 hipError_t launch_wigner_tables(const double *thetas, int ntheta, int L,
                                 double *dscr, double *rot, hipStream_t s) {
   int w = 2 * L + 1;
-  int nt = ((w * w + 63) / 64) * 64;
+  int nt = std::min(1024, ((w * w + 63) / 64) * 64);
   k_wigner_tables<<<ntheta, nt, 0, s>>>(thetas, L, dscr, rot);
   return hipGetLastError();
 }
 
 // ------------------------------------------------------------ shared pieces
 
-constexpr int kMaxMc = kMaxOrder * (kMaxOrder + 1) / 2;        // 120
-constexpr int kMaxLc = (kMaxOrder + 1) * (kMaxOrder + 2) / 2;  // 136
+constexpr int kMaxMc = kMaxOrder * (kMaxOrder + 1) / 2;        // 465
+constexpr int kMaxLc = (kMaxOrder + 1) * (kMaxOrder + 2) / 2;  // 496
 constexpr int kOpThreads = 64;
+// coefficients per thread of a 64-thread block over a local
+constexpr int kLcSlots = (kMaxLc + kOpThreads - 1) / kOpThreads;
 
 
 
@@ -195,13 +197,15 @@ __device__ inline double2 rotate_inv_one(const double *__restrict__ R, int rb,
 //   (d/dx - i d/dy) O_l^m = -O_{l-1}^{m-1}.
 // With g_l^m = N_l^m (l+m)!, r^l Y_l^m = g_l^m O_l^m.
 
-// 1/n for 0 < n < 256, each correctly rounded (compile-time constants).
+// 1/n for 0 < n < 1024 (n = l^2 - m^2 <= kMaxOrder^2), each correctly
+// rounded (compile-time constants).
 struct InvTable {
-  double v[256];
+  double v[1024];
 };
+static_assert(kMaxOrder * kMaxOrder < 1024, "reciprocal table too short");
 constexpr InvTable make_inv_table() {
   InvTable t{};
-  for (int i = 1; i < 256; i++) t.v[i] = 1.0 / i;
+  for (int i = 1; i < 1024; i++) t.v[i] = 1.0 / i;
   return t;
 }
 __constant__ InvTable kInv = make_inv_table();
@@ -313,8 +317,8 @@ __global__ __launch_bounds__(kOpThreads) void k_m2l(
   const int A = blockIdx.x;
   const int p = tb.p, mc = tri(p), lc = tri(p + 1), rb = tb.rb;
   const double w = node_width(nd, A);
-  double2 acc[3] = {make_double2(0., 0.), make_double2(0., 0.),
-                    make_double2(0., 0.)};
+  double2 acc[kLcSlots];
+  for (int slot = 0; slot < kLcSlots; slot++) acc[slot] = make_double2(0., 0.);
   const int e0 = vofs[A], e1 = vofs[A + 1];
   for (int e = e0; e < e1; e++) {
     const int B = vsrc[e];
@@ -355,7 +359,7 @@ __global__ __launch_bounds__(kOpThreads) void k_m2l(
       Z[k] = make_double2(re, im);
     }
     __syncthreads();
-    for (int slot = 0; slot < 3; slot++) {
+    for (int slot = 0; slot < kLcSlots; slot++) {
       int k = threadIdx.x + slot * kOpThreads;
       if (k < lc) {
         double2 v = rotate_inv_one(R, rb, ph, Z, k);
@@ -365,7 +369,7 @@ __global__ __launch_bounds__(kOpThreads) void k_m2l(
     }
     __syncthreads();
   }
-  for (int slot = 0; slot < 3; slot++) {
+  for (int slot = 0; slot < kLcSlots; slot++) {
     int k = threadIdx.x + slot * kOpThreads;
     if (k < lc) L[(size_t)A * lc + k] = acc[slot];
   }
@@ -655,8 +659,8 @@ __global__ __launch_bounds__(64) void k_p2l(
   const int lane = threadIdx.x;
   const double cx = nd.cx[D], cy = nd.cy[D], cz = nd.cz[D];
   const int chunk = min(64, kP2mSlots / lc);
-  double2 acc[3] = {make_double2(0., 0.), make_double2(0., 0.),
-                    make_double2(0., 0.)};
+  double2 acc[kLcSlots];
+  for (int slot = 0; slot < kLcSlots; slot++) acc[slot] = make_double2(0., 0.);
   for (int x = x0; x < x1; x++) {
     const int J = xleaf[x];
     const int b = nd.pbegin[J], e = nd.pend[J];
@@ -693,7 +697,7 @@ __global__ __launch_bounds__(64) void k_p2l(
         }
       }
       __syncthreads();
-      for (int slot = 0; slot < 3; slot++) {
+      for (int slot = 0; slot < kLcSlots; slot++) {
         const int k = lane + 64 * slot;
         if (k >= lc) break;
         double2 a = acc[slot];
@@ -706,7 +710,7 @@ __global__ __launch_bounds__(64) void k_p2l(
       }
     }
   }
-  for (int slot = 0; slot < 3; slot++) {
+  for (int slot = 0; slot < kLcSlots; slot++) {
     const int k = lane + 64 * slot;
     if (k < lc) {
       const double g = tb.xgain[k];
@@ -728,6 +732,58 @@ __global__ void k_particle_leaf(NodeArrays nd, const int *__restrict__ leaves,
   for (int j = nd.pbegin[A] + sub; j < nd.pend[A]; j += 8) pleaf[j] = A;
 }
 
+// P2M above the templated orders: one wave per leaf, lane m runs column m
+// of the degree recurrence for the leaf's particles in order and adds
+// q O_l^m into the LDS entries of its own column (no two lanes share one).
+__global__ __launch_bounds__(64) void k_p2m_generic(
+    OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
+    const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ zs, const double *__restrict__ qs,
+    double2 *__restrict__ M) {
+  __shared__ double2 T[kMaxMc];
+  const int p = tb.p, mc = tri(p);
+  const int lane = threadIdx.x;
+  const int A = leaves[blockIdx.x];
+  const int b = nd.pbegin[A], e = nd.pend[A];
+  const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
+  for (int k = lane; k < mc; k += 64) T[k] = make_double2(0., 0.);
+  __syncthreads();
+  if (lane < p) {
+    const int m = lane;
+    for (int i = b; i < e; i++) {
+      const double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
+      const double r2 = vx * vx + vy * vy + vz * vz;
+      double ox = qs[i], oy = 0.;  // q O_m^m
+      for (int k = 1; k <= m; k++) {
+        const double f = -0.5 * kInv.v[k];
+        const double ax = f * vx, ay = f * vy;
+        const double nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
+        ox = nx;
+        oy = ny;
+      }
+      T[tri(m) + m].x += ox;
+      T[tri(m) + m].y += oy;
+      double px = ox, py = oy, qx = 0., qy = 0.;
+      for (int l = m + 1; l < p; l++) {
+        const double a = (2. * l - 1.) * vz;
+        const double g = kInv.v[l * l - m * m];
+        const double nx = (a * px - r2 * qx) * g, ny = (a * py - r2 * qy) * g;
+        qx = px;
+        qy = py;
+        px = nx;
+        py = ny;
+        T[tri(l) + m].x += nx;
+        T[tri(l) + m].y += ny;
+      }
+    }
+  }
+  __syncthreads();
+  for (int k = lane; k < mc; k += 64) {
+    const double g = tb.pgain[k];
+    M[(size_t)A * mc + k] = make_double2(g * T[k].x, -g * T[k].y);
+  }
+}
+
 // ------------------------------------------------------------ launchers
 
 template <int P>
@@ -736,8 +792,9 @@ static hipError_t p2m_dispatch(const OpTables &tb, const NodeArrays &nd,
                                const double *xs, const double *ys,
                                const double *zs, const double *qs, double2 *M,
                                hipStream_t s) {
-  if constexpr (P > kMaxOrder) {
-    return hipErrorInvalidValue;
+  if constexpr (P > kMaxOrderFast) {
+    k_p2m_generic<<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
+    return hipGetLastError();
   } else {
     if (tb.p != P)
       return p2m_dispatch<P + 1>(tb, nd, leaves, nleaves, xs, ys, zs, qs, M,

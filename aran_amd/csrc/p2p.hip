@@ -31,7 +31,8 @@ template <> __device__ inline double rsqrt_t<double>(double x) {
 }
 // v_rsq_f32 directly: rsqrtf adds a rescaling for denormal arguments, which
 // a squared distance here never is (two particles that close have equal
-// FP32 coordinates relative to the leaf centre, r2 == 0)
+// FP32 coordinates relative to the leaf centre, r2 == 0: k_near then sums
+// that target again in FP64)
 template <> __device__ inline float rsqrt_t<float>(float x) {
   return __builtin_amdgcn_rsqf(x);
 }
@@ -61,6 +62,29 @@ __device__ __forceinline__ void near_one(const typename Src4<T>::type &s,
   gx -= qr3 * dx;
   gy -= qr3 * dy;
   gz -= qr3 * dz;
+}
+
+// Near-field sum of target i over its near ranges e0..e1 in FP64 from
+// global memory (the recomputation of a target whose FP32 sum overflowed).
+__device__ __noinline__ void near_fp64_one(
+    int i, int e0, int e1, const int *__restrict__ ubeg,
+    const int *__restrict__ uend, const double *__restrict__ xs,
+    const double *__restrict__ ys, const double *__restrict__ zs,
+    const double *__restrict__ qs, double &p, double &gx, double &gy,
+    double &gz) {
+  p = gx = gy = gz = 0.;
+  const double xi = xs[i], yi = ys[i], zi = zs[i];
+  for (int e = e0; e < e1; e++)
+    for (int j = ubeg[e]; j < uend[e]; j++) {
+      if (j == i) continue;
+      const double dx = xi - xs[j], dy = yi - ys[j], dz = zi - zs[j];
+      const double ir = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+      const double qr = qs[j] * ir, qr3 = qr * (ir * ir);
+      p += qr;
+      gx -= qr3 * dx;
+      gy -= qr3 * dy;
+      gz -= qr3 * dz;
+    }
 }
 
 template <typename T>
@@ -201,6 +225,15 @@ __global__ __launch_bounds__(kNearThreads) __attribute__((
         s1 += red[1][k];
         s2 += red[2][k];
         s3 += red[3][k];
+      }
+      if constexpr (std::is_same<T, float>::value) {
+        // Two distinct particles whose FP32 leaf-relative coordinates
+        // coincide give r2 == 0 and an infinite FP32 term (and a NaN
+        // gradient).  Such a target is summed again in FP64, as the plain
+        // form sums it; no other target pays for the check.
+        if (!(isfinite(s0) && isfinite(s1) && isfinite(s2) && isfinite(s3)))
+          near_fp64_one(i, e0, e1, ubeg, uend, xs, ys, zs, qs, s0, s1, s2,
+                        s3);
       }
       pot[i] = s0;
       grad[3 * (size_t)i + 0] = s1;

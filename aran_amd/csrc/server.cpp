@@ -44,6 +44,7 @@ struct Request {
   double bytes = 0.;
   Clock::time_point t_submit, t_start, t_done;
   bool done = false;
+  bool claimed = false;  // a waiter holds this request
   int status = 0;
 };
 
@@ -281,7 +282,7 @@ extern "C" long aran_server3d_submit(AranServer3d *S, long n, const double *x,
                                      const double *ub, int order, int leafcap,
                                      int want_grad) {
   if (!S || n < 0 || (n > 0 && (!x || !y || !z || !q)) || !lb || !ub ||
-      order < 1 || order > aran::kMaxOrder || leafcap < 1) {
+      order < 1 || order > aran::kMaxOrderFast || leafcap < 1) {
     aran::set_error("aran_server3d_submit: invalid argument");
     return ARAN_FMM3D_EINVAL;
   }
@@ -333,9 +334,17 @@ extern "C" int aran_server3d_wait(AranServer3d *S, long id, double *pot,
       return ARAN_FMM3D_EINVAL;
     }
     Request *rp = it->second.get();
+    if (rp->claimed) {
+      aran::set_error("aran_server3d_wait: request already being waited for");
+      return ARAN_FMM3D_EINVAL;
+    }
+    // claimed under the lock: a second waiter is refused above, so the
+    // entry stays in the map until this waiter removes it
+    rp->claimed = true;
     S->cv_done.wait(lock, [&] { return rp->done; });
-    r = std::move(it->second);
-    S->all.erase(it);
+    auto again = S->all.find(id);
+    r = std::move(again->second);
+    S->all.erase(again);
   }
   if (r->status != 0) return r->status;
   if (pot) memcpy(pot, r->pot.data(), sizeof(double) * r->n);
@@ -359,7 +368,7 @@ extern "C" int aran_server3d_prepare(AranServer3d *S, long n_max,
     return ARAN_FMM3D_EINVAL;
   }
   for (int k = 0; k < norders; k++)
-    if (orders[k] < 1 || orders[k] > aran::kMaxOrder) {
+    if (orders[k] < 1 || orders[k] > aran::kMaxOrderFast) {
       aran::set_error("aran_server3d_prepare: invalid order");
       return ARAN_FMM3D_EINVAL;
     }

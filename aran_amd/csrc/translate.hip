@@ -324,12 +324,164 @@ __global__ __launch_bounds__(512) void k_l2l_level(
     });
 }
 
+// ---- orders above the templated kernels: one workgroup per parent, the
+// children one after another in octant order (the sum order of the
+// templated kernels), tables from global memory.
+
+constexpr int kGenThreads = 256;
+constexpr int kGenLc = tri_c(kMaxOrder + 1);
+
+// X[k] = forward rotation of U over degrees <= deg (threads over k).
+__device__ inline void gen_rot_fwd(const double *__restrict__ R, int rb,
+                                   int deg, const double2 *U, double2 *X) {
+  for (int k = threadIdx.x; k < tri_c(deg + 1); k += kGenThreads) {
+    const int l = kLm.l[k], mp = kLm.m[k];
+    const double *ep = R + rofs_c(l) + mp * (l + 1);
+    const double *em = ep + rb;
+    const double2 *u = U + tri_c(l);
+    double re = 0., im = 0.;
+    for (int m = 0; m <= l; m++) {
+      re += ep[m] * u[m].x;
+      im += em[m] * u[m].y;
+    }
+    X[k] = make_double2(re, im);
+  }
+}
+
+// Inverse rotation and phase back of coefficient k of Z.
+__device__ inline double2 gen_rot_inv(const double *__restrict__ R, int rb,
+                                      const double2 *__restrict__ ph, int k,
+                                      const double2 *Z) {
+  const int l = kLm.l[k], mp = kLm.m[k];
+  const double *fp = R + 2 * rb + rofs_c(l) + mp * (l + 1);
+  const double *fm = fp + rb;
+  const double2 *z = Z + tri_c(l);
+  double re = 0., im = 0.;
+  for (int m = 0; m <= l; m++) {
+    re += fp[m] * z[m].x;
+    im += fm[m] * z[m].y;
+  }
+  const double2 e = ph[mp];
+  return make_double2(e.x * re + e.y * im, e.x * im - e.y * re);
+}
+
+__global__ __launch_bounds__(kGenThreads) void k_m2m_generic(
+    OpTables tb, const TransRow *__restrict__ rows, double2 *__restrict__ M) {
+  __shared__ double2 U[kGenLc], X[kGenLc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  constexpr int NS = (kGenLc + kGenThreads - 1) / kGenThreads;
+  const int p = tb.p, mc = tri_c(p), rb = tb.rb;
+  const TransRow &row = rows[blockIdx.x];
+  const double half = row.width * 0.25;  // half a child width
+  double2 acc[NS];
+  for (int j = 0; j < NS; j++) acc[j] = make_double2(0., 0.);
+  for (int o = 0; o < 8; o++) {
+    const int C = row.child[o];
+    if (C < 0) continue;  // uniform over the workgroup
+    const int code =
+        code_of(1 - 2 * (o & 1), 1 - 2 * ((o >> 1) & 1), 1 - 2 * (o >> 2));
+    const DirEntry de = tb.dir[code];
+    const double2 *ph = tb.eiphi + code * (p + 1);
+    const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+    __syncthreads();
+    for (int k = threadIdx.x; k < mc; k += kGenThreads)
+      U[k] = cmul2(ph[kLm.m[k]], M[(size_t)C * mc + k]);
+    if (threadIdx.x < 2 * p + 2) {
+      const double f = -de.c * de.norm * half;
+      double v = 1.;
+      for (int i = 0; i < (int)threadIdx.x; i++) v *= f;
+      rp[threadIdx.x] = v;  // ((-c) r)^k
+    }
+    __syncthreads();
+    gen_rot_fwd(R, rb, p - 1, U, X);
+    __syncthreads();
+    for (int k = threadIdx.x; k < mc; k += kGenThreads) {
+      const int l = kLm.l[k], m = kLm.m[k];
+      const double *T = tb.tm2m + ((size_t)l * p + m) * p;
+      double re = 0., im = 0.;
+      for (int n = m; n <= l; n++) {
+        const double f = T[n] * rp[l - n];
+        re += f * X[tri_c(n) + m].x;
+        im += f * X[tri_c(n) + m].y;
+      }
+      U[k] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int j = 0; j < NS; j++) {
+      const int k = threadIdx.x + j * kGenThreads;
+      if (k < mc) {
+        const double2 v = gen_rot_inv(R, rb, ph, k, U);
+        acc[j].x += v.x;
+        acc[j].y += v.y;
+      }
+    }
+  }
+  for (int j = 0; j < NS; j++) {
+    const int k = threadIdx.x + j * kGenThreads;
+    if (k < mc) M[(size_t)row.node * mc + k] = acc[j];
+  }
+}
+
+__global__ __launch_bounds__(kGenThreads) void k_l2l_generic(
+    OpTables tb, const TransRow *__restrict__ rows, double2 *__restrict__ L) {
+  __shared__ double2 L0[kGenLc], U[kGenLc], X[kGenLc];
+  __shared__ double rp[2 * kMaxOrder + 2];
+  const int p = tb.p, lc = tri_c(p + 1), rb = tb.rb;
+  const TransRow &row = rows[blockIdx.x];
+  const double half = row.width * 0.25;  // half a child width
+  for (int k = threadIdx.x; k < lc; k += kGenThreads)
+    L0[k] = L[(size_t)row.node * lc + k];
+  for (int o = 0; o < 8; o++) {
+    const int C = row.child[o];
+    if (C < 0) continue;  // uniform over the workgroup
+    const int code = code_of(2 * (o & 1) - 1, 2 * ((o >> 1) & 1) - 1,
+                             2 * (o >> 2) - 1);
+    const DirEntry de = tb.dir[code];
+    const double2 *ph = tb.eiphi + code * (p + 1);
+    const double *R = tb.rot + (size_t)de.theta * 4 * rb;
+    __syncthreads();
+    for (int k = threadIdx.x; k < lc; k += kGenThreads)
+      U[k] = cmul2(ph[kLm.m[k]], L0[k]);
+    if (threadIdx.x < 2 * p + 2) {
+      const double f = de.c * de.norm * half;
+      double v = 1.;
+      for (int i = 0; i < (int)threadIdx.x; i++) v *= f;
+      rp[threadIdx.x] = v;  // (c r)^k
+    }
+    __syncthreads();
+    gen_rot_fwd(R, rb, p, U, X);
+    __syncthreads();
+    for (int k = threadIdx.x; k < lc; k += kGenThreads) {
+      const int l = kLm.l[k], m = kLm.m[k];
+      const double *T = tb.tl2l + ((size_t)l * (p + 1) + m) * (p + 1);
+      double re = 0., im = 0.;
+      for (int n = l; n <= p; n++) {
+        const double f = T[n] * rp[n - l];
+        re += f * X[tri_c(n) + m].x;
+        im += f * X[tri_c(n) + m].y;
+      }
+      U[k] = make_double2(re, im);
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < lc; k += kGenThreads) {
+      const double2 v = gen_rot_inv(R, rb, ph, k, U);
+      double2 *dst = L + (size_t)C * lc + k;
+      const double2 d = *dst;
+      *dst = make_double2(d.x + v.x, d.y + v.y);
+    }
+  }
+}
+
 template <int P>
 static hipError_t level_dispatch(int kind, const OpTables &tb,
                                  const TransRow *rows, int nrows, double2 *X,
                                  hipStream_t s) {
-  if constexpr (P > kMaxOrder) {
-    return hipErrorInvalidValue;
+  if constexpr (P > kMaxOrderFast) {
+    if (kind == 0)
+      k_m2m_generic<<<nrows, kGenThreads, 0, s>>>(tb, rows, X);
+    else
+      k_l2l_generic<<<nrows, kGenThreads, 0, s>>>(tb, rows, X);
+    return hipGetLastError();
   } else {
     if (tb.p != P)
       return level_dispatch<P + 1>(kind, tb, rows, nrows, X, s);

@@ -670,8 +670,24 @@ __device__ inline int digit_lower_bound(const uint64_t *__restrict__ keys,
 // splits when it holds more than leafcap particles below the maximum depth,
 // and cnt counts its non-empty octants.  Also sets the next level's first
 // node.
+// Whether (depth, key) is in the sorted stop list.
+__device__ inline bool stopped(const ulonglong2 *__restrict__ stop, int nstop,
+                               unsigned long long dep, uint64_t key) {
+  int lo = 0, hi = nstop;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    const ulonglong2 v = stop[mid];
+    if (v.x < dep || (v.x == dep && v.y < key))
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo < nstop && stop[lo].x == dep && stop[lo].y == key;
+}
+
 __global__ void k_split_count(NodeArrays nd, int *__restrict__ dl, int d,
                               const uint64_t *__restrict__ keys, int leafcap,
+                              const ulonglong2 *__restrict__ stop, int nstop,
                               int *__restrict__ bounds, int *__restrict__ cnt) {
   const int lvl_begin = dl[2 * d], lvl_count = dl[2 * d + 1];
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -680,7 +696,8 @@ __global__ void k_split_count(NodeArrays nd, int *__restrict__ dl, int d,
   const int i = (int)(t / 8), o = (int)(t % 8);
   const int node = lvl_begin + i;
   const int b = nd.pbegin[node], e = nd.pend[node], dep = nd.depth[node];
-  const bool split = e - b > leafcap && dep < kMaxDepth;
+  const bool split = e - b > leafcap && dep < kMaxDepth &&
+                     !(nstop > 0 && stopped(stop, nstop, dep, nd.key[node]));
   int x = b;
   if (split && o > 0)
     x = digit_lower_bound(keys, b, e, 3 * (kMaxDepth - dep - 1), o);
@@ -749,11 +766,11 @@ __global__ void k_split_write(NodeArrays nd, const int *__restrict__ dl,
 
 hipError_t launch_split_count(const NodeArrays &nd, int *dl, int d,
                               long max_count, const uint64_t *keys,
-                              int leafcap, int *bounds, int *cnt,
-                              hipStream_t s) {
+                              int leafcap, const ulonglong2 *stop, int nstop,
+                              int *bounds, int *cnt, hipStream_t s) {
   const long t = 8 * std::max(1L, max_count);
   k_split_count<<<(t + 255) / 256, 256, 0, s>>>(nd, dl, d, keys, leafcap,
-                                                bounds, cnt);
+                                                stop, nstop, bounds, cnt);
   return hipGetLastError();
 }
 

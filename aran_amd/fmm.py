@@ -65,9 +65,28 @@ class Fmm3d:
     def semifar_threshold_auto(order):
         return int(lib().aran_fmm3d_semifar_threshold_auto(int(order)))
 
+    def set_translations(self, m2m="rotate", m2l="rotate", l2l="rotate"):
+        """Translation flavours of M2M, M2L, L2L: "rotate" (the operators of
+        record), "normal" (O(p^4) formulas) or "kylin" (plain form only)."""
+        code = {"rotate": 0, "normal": 1, "kylin": 2}
+        check(lib().aran_fmm3d_set_translations(
+            self._ctx, code[m2m], code[m2l], code[l2l]), "set_translations")
+
+    def export_expansions(self, order):
+        """Multipoles and locals of the last solve (node order of
+        export_tree) as complex128 arrays [nodes][terms]."""
+        nn = self.stats()["nodes"]
+        mc, lc = order * (order + 1) // 2, (order + 1) * (order + 2) // 2
+        M = np.zeros((nn, mc), dtype=np.complex128)
+        L = np.zeros((nn, lc), dtype=np.complex128)
+        check(lib().aran_fmm3d_export_expansions(
+            self._ctx, M.ctypes.data, L.ctypes.data), "export_expansions")
+        return M, L
+
     def set_autotune(self, on=True):
-        """Time the far-field split candidates on first use of each (form,
-        order, size bucket) and keep the fastest."""
+        """Time the far-field workgroup schedules on first use of each (form,
+        order, size bucket) and keep the fastest (results do not depend on
+        the schedule)."""
         check(lib().aran_fmm3d_set_autotune(self._ctx, int(on)),
               "set_autotune")
 

@@ -37,9 +37,16 @@ class Complex128(C.Structure):
 
 
 class AranDevelopment3d(C.Structure):
-    _fields_ = [("posdeg", C.c_uint8), ("negdeg", C.c_uint8),
-                ("multipole", C.POINTER(C.c_double)),
-                ("local", C.POINTER(C.c_double))]
+    """struct _AranDevelopment3d (aran/arandevelopment3d.h:36-42): two
+    opaque AranSphericalSeriesd pointers"""
+    _fields_ = [("multipole", C.c_void_p), ("local", C.c_void_p)]
+
+
+# VsgPRTree3dFunc / VsgPRTree3dNFIsleafFunc callbacks (include/vsg/vsgd.h)
+PRTREE_FUNC = C.CFUNCTYPE(None, C.POINTER(VsgPRTree3dNodeInfo), C.c_void_p)
+NF_ISLEAF_FUNC = C.CFUNCTYPE(C.c_int, C.POINTER(VsgPRTree3dNodeInfo),
+                             C.c_void_p)
+G_IN_ORDER, G_PRE_ORDER, G_POST_ORDER, G_LEVEL_ORDER = 0, 1, 2, 3
 
 
 class PointAccumPot(C.Structure):
@@ -80,8 +87,51 @@ def lib():
     L.aran_development3d_set_zero.argtypes = [dev_p]
     L.aran_development3d_p2m.argtypes = [v3_p, d, ni_p, dev_p]
     for f in ("m2m", "m2l", "l2l"):
-        getattr(L, "aran_development3d_%s_rotate" % f).argtypes = [
-            ni_p, dev_p, ni_p, dev_p]
+        for suffix in ("_rotate", "", "_kkylin"):
+            getattr(L, "aran_development3d_%s%s" % (f, suffix)).argtypes = [
+                ni_p, dev_p, ni_p, dev_p]
+    L.aran_development3d_clone.restype = dev_p
+    L.aran_development3d_clone.argtypes = [dev_p]
+    L.aran_development3d_copy.argtypes = [dev_p, dev_p]
+    L.aran_development3d_write.argtypes = [dev_p, vp]
+    ser = vp
+    L.aran_spherical_seriesd_new.restype = ser
+    L.aran_spherical_seriesd_new.argtypes = [C.c_uint8, C.c_uint8]
+    L.aran_spherical_seriesd_free.argtypes = [ser]
+    L.aran_spherical_seriesd_clone.restype = ser
+    L.aran_spherical_seriesd_clone.argtypes = [ser]
+    L.aran_spherical_seriesd_copy.argtypes = [ser, ser]
+    L.aran_spherical_seriesd_get_term.restype = C.POINTER(C.c_double)
+    L.aran_spherical_seriesd_get_term.argtypes = [ser, C.c_int, C.c_int]
+    L.aran_spherical_seriesd_get_posdeg.restype = C.c_uint8
+    L.aran_spherical_seriesd_get_posdeg.argtypes = [ser]
+    L.aran_spherical_seriesd_get_negdeg.restype = C.c_uint8
+    L.aran_spherical_seriesd_get_negdeg.argtypes = [ser]
+    L.aran_spherical_seriesd_set_zero.argtypes = [ser]
+    L.aran_spherical_seriesd_write.argtypes = [ser, vp]
+    L.aran_spherical_seriesd_evaluate.restype = Complex128
+    L.aran_spherical_seriesd_evaluate.argtypes = [ser, v3_p]
+    L.aran_spherical_seriesd_gradient_evaluate.argtypes = [ser, v3_p, v3_p]
+    for f in ("translate", "to_local"):
+        for suffix in ("", "_kkylin", "_rotate"):
+            getattr(L, "aran_spherical_seriesd_%s%s" % (f, suffix)).argtypes = [
+                ser, v3_p, ser, v3_p]
+    L.aran_solver3d_traverse.argtypes = [vp, C.c_int, PRTREE_FUNC, vp]
+    L.aran_solver3d_set_nf_isleaf.argtypes = [vp, NF_ISLEAF_FUNC, vp]
+    L.aran_solver3d_set_children_order_hilbert.argtypes = [vp]
+    L.aran_solver3d_set_children_order_default.argtypes = [vp]
+    L.aran_solver3d_insert_point_local.argtypes = [vp, vp]
+    L.vsg_vector3d_norm.restype = d
+    L.vsg_vector3d_norm.argtypes = [v3_p]
+    L.vsg_vector3d_dotp.restype = d
+    L.vsg_vector3d_dotp.argtypes = [v3_p, v3_p]
+    for f in ("add", "sub"):
+        getattr(L, "vsg_vector3d_" + f).argtypes = [v3_p, v3_p, v3_p]
+    L.vsg_vector3d_scalp.argtypes = [v3_p, d, v3_p]
+    L.vsg_vector3d_set.argtypes = [v3_p, d, d, d]
+    L.vsg_vector3d_to_spherical_internal.argtypes = [v3_p] + [
+        C.POINTER(C.c_double)] * 5
+    L.vsg_vector3d_from_spherical_internal.argtypes = [v3_p] + [d] * 5
     L.aran_development3d_local_evaluate.restype = Complex128
     L.aran_development3d_local_evaluate.argtypes = [ni_p, dev_p, v3_p]
     L.aran_development3d_local_gradient_evaluate.argtypes = [ni_p, dev_p, v3_p,
@@ -126,6 +176,33 @@ def lib():
     L.aran_fma_read_points.argtypes = [vp, C.c_long] + [vp] * 4
     _declared = True
     return L
+
+
+def series_terms(series, positive):
+    """The local (positive) or multipole (negative) terms of a series as a
+    numpy complex128 view, m >= 0 packed by degree."""
+    import numpy as np
+    L = lib()
+    if positive:
+        n = L.aran_spherical_seriesd_get_posdeg(series) + 1
+        ptr = L.aran_spherical_seriesd_get_term(series, 0, 0)
+    else:
+        n = L.aran_spherical_seriesd_get_negdeg(series)
+        if n == 0:
+            return np.zeros(0, dtype=np.complex128)
+        ptr = L.aran_spherical_seriesd_get_term(series, -1, 0)
+    cnt = n * (n + 1) // 2
+    return np.ctypeslib.as_array(ptr, shape=(2 * cnt,)).view(np.complex128)
+
+
+def multipole(dev):
+    """Multipole terms (degrees < negdeg) of an AranDevelopment3d pointer."""
+    return series_terms(dev.contents.multipole, False)
+
+
+def local(dev):
+    """Local terms (degrees <= posdeg) of an AranDevelopment3d pointer."""
+    return series_terms(dev.contents.local, True)
 
 
 def fn(name):

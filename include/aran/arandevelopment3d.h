@@ -2,12 +2,13 @@
  * arandevelopment3d.h -- FMM node data and its operators
  * (reference: aran/arandevelopment3d.h:30-150).
  *
- * Coefficients are complex FP64 stored as (re, im) pairs, m >= 0 only
- * (c_l^{-m} = (-1)^m conj(c_l^m)), index l(l+1)/2 + m:
- *   multipole : degrees l = 0 .. negdeg-1
- *   local     : degrees l = 0 .. max(posdeg, negdeg)
- * aran_development3d_new (0, p) is the "order p" development of the
- * reference's tests (tests/newtonpot3.c:608-610).
+ * An AranDevelopment3d holds two spherical series, as the reference's:
+ *   multipole : aran_spherical_seriesd_new (posdeg, negdeg)
+ *   local     : aran_spherical_seriesd_new (max (posdeg, negdeg), 0)
+ * (aran/arandevelopment3d.c:64-73).  aran_development3d_new (0, p) is the
+ * "order p" development of the reference's tests (tests/newtonpot3.c:
+ * 608-610): multipole degrees 0..p-1, local degrees 0..p.  Degrees go up to
+ * 30.
  *
  * The operator entry points below run one operator as a GPU launch on the
  * current device; inside aran_solver3d_solve() they are recognised by
@@ -19,6 +20,7 @@
 #include <stdio.h>
 
 #include <aran/arantypes.h>
+#include <aran/aransphericalseriesd.h>
 #include <vsg/vsgd.h>
 
 #ifdef __cplusplus
@@ -29,12 +31,11 @@ extern "C" {
 
 typedef struct _AranDevelopment3d AranDevelopment3d;
 
+/* the reference's layout (aran/arandevelopment3d.h:36-42) */
 struct _AranDevelopment3d
 {
-  guint8 posdeg;
-  guint8 negdeg;
-  gdouble *multipole; /* 2 * negdeg(negdeg+1)/2 doubles */
-  gdouble *local;     /* 2 * (d+1)(d+2)/2 doubles, d = max(posdeg, negdeg) */
+  AranSphericalSeriesd *multipole;
+  AranSphericalSeriesd *local;
 };
 
 GType aran_development3d_get_type (void);                 /* :59 */
@@ -53,8 +54,33 @@ void aran_development3d_p2m (const VsgVector3d *position,
                              const VsgPRTree3dNodeInfo *dst_node,
                              AranDevelopment3d *dst);
 
-/* rotate / z-translate / rotate-back translations
- * (aran/arandevelopment3d.c:402-451) */
+/* translations in the three flavours: O(p^4) "normal" formulas
+ * (aran/arandevelopment3d.c:288-331), K. Kylin's recurrences (:343-390) and
+ * rotate / z-translate / rotate-back, the operators of record (:402-451) */
+void aran_development3d_m2m (const VsgPRTree3dNodeInfo *src_node,
+                             AranDevelopment3d *src,
+                             const VsgPRTree3dNodeInfo *dst_node,
+                             AranDevelopment3d *dst);
+void aran_development3d_m2l (const VsgPRTree3dNodeInfo *src_node,
+                             AranDevelopment3d *src,
+                             const VsgPRTree3dNodeInfo *dst_node,
+                             AranDevelopment3d *dst);
+void aran_development3d_l2l (const VsgPRTree3dNodeInfo *src_node,
+                             AranDevelopment3d *src,
+                             const VsgPRTree3dNodeInfo *dst_node,
+                             AranDevelopment3d *dst);
+void aran_development3d_m2m_kkylin (const VsgPRTree3dNodeInfo *src_node,
+                                    AranDevelopment3d *src,
+                                    const VsgPRTree3dNodeInfo *dst_node,
+                                    AranDevelopment3d *dst);
+void aran_development3d_m2l_kkylin (const VsgPRTree3dNodeInfo *src_node,
+                                    AranDevelopment3d *src,
+                                    const VsgPRTree3dNodeInfo *dst_node,
+                                    AranDevelopment3d *dst);
+void aran_development3d_l2l_kkylin (const VsgPRTree3dNodeInfo *src_node,
+                                    AranDevelopment3d *src,
+                                    const VsgPRTree3dNodeInfo *dst_node,
+                                    AranDevelopment3d *dst);
 void aran_development3d_m2m_rotate (const VsgPRTree3dNodeInfo *src_node,
                                     AranDevelopment3d *src,
                                     const VsgPRTree3dNodeInfo *dst_node,

@@ -29,7 +29,8 @@ void aran_fmm3d_free (AranFmm3d *ctx);
 
 /* One FMM solve: potential phi_i = sum_{j != i} q_j / |x_i - x_j| and its
  * gradient, written (not accumulated) in the input order.  order p:
- * multipole degrees 0..p-1, local degrees 0..p (1 <= p <= 15); leafcap: leaf
+ * multipole degrees 0..p-1, local degrees 0..p (1 <= p <= 30 in the plain
+ * form, <= 15 in the others); leafcap: leaf
  * capacity of the octree; lb/ub: host arrays of 3 doubles giving the root
  * box before growth.  pot / grad (n x 3, row-major) may be NULL. */
 int aran_fmm3d_solve (AranFmm3d *ctx, long n, const double *x,
@@ -65,11 +66,31 @@ int aran_fmm3d_solve_batch (AranFmm3d *ctx, int nsys, const long *offsets,
 int aran_fmm3d_set_precision (AranFmm3d *ctx, int precision);
 
 /* Far-field autotuner: when on, the first solve of each (form, order,
- * problem-size bucket) times every candidate split of the tiled far-field
- * kernel over its offset ranges on the device and later solves of that
- * bucket use the fastest (kept in the context).  Off by default (the split
- * then comes from a cost model). */
+ * problem-size bucket) times every workgroup schedule of the tiled
+ * far-field kernel (the order in which its tiles and fixed offset ranges
+ * are dealt to the XCDs) on the device, and later solves of that bucket use
+ * the fastest (kept in the context).  Every schedule computes the same sums
+ * in the same order, so results do not depend on it; nor on the rest of a
+ * batch (the offset ranges are fixed per order).  Off by default. */
 int aran_fmm3d_set_autotune (AranFmm3d *ctx, int on);
+
+/* Translation flavours of the next solves (plain form only): the operators
+ * of record rotate / z-translate / rotate back (ARAN_FMM3D_ROTATE, the
+ * default), the O(p^4) "normal" formulas (aran/aransphericalseriesd-
+ * translate.c:366-834) or K. Kylin's recurrences (aran/aransphericalseriesd-
+ * kkylin.c), for M2M, M2L and L2L each. */
+#define ARAN_FMM3D_ROTATE 0
+#define ARAN_FMM3D_NORMAL 1
+#define ARAN_FMM3D_KYLIN 2
+int aran_fmm3d_set_translations (AranFmm3d *ctx, int m2m, int m2l, int l2l);
+
+/* Expansions of the last solve (host arrays, complex FP64 interleaved):
+ * multipoles[nnodes * p(p+1)/2], locals[nnodes * (p+1)(p+2)/2] in the node
+ * order of export_tree; either may be NULL.  The plain form keeps them in
+ * FP64 on the device; after a mixed or MFMA solve the locals of levels >= 2
+ * are those of the far field's FP32 accumulation. */
+int aran_fmm3d_export_expansions (const AranFmm3d *ctx, double *multipoles,
+                                  double *locals);
 
 /* Counters of the last solve: zero, p2p, p2m, m2m, m2l, l2l, l2p (the
  * reference's semantics), then depth, nodes, leaves, far-list entries,

@@ -9,9 +9,10 @@
  * are the library's own:
  *   p2p : aran_newton3d_p2p_pot / _field / _potfield
  *   p2m : aran_newton3d_p2m
- *   m2m : aran_development3d_m2m_rotate
- *   m2l : aran_development3d_m2l_rotate
- *   l2l : aran_development3d_l2l_rotate
+ *   m2m : aran_development3d_m2m_rotate, _m2m or _m2m_kkylin
+ *   m2l : aran_development3d_m2l_rotate, _m2l or _m2l_kkylin
+ *   l2l : aran_development3d_l2l_rotate, _l2l or _l2l_kkylin
+ *         (the normal and Kylin flavours in the plain FP64 form only)
  *   l2p : aran_newton3d_l2p_pot / _field / _potfield
  * optionally with the semifar pair (aran_solver3d_set_functions_full)
  *   p2l : aran_newton3d_p2l
@@ -112,6 +113,9 @@ void aran_solver3d_get_bounds (AranSolver3d *solver, VsgVector3d *lbound,
 guint aran_solver3d_depth (const AranSolver3d *solver);
 guint aran_solver3d_point_count (const AranSolver3d *solver);
 void aran_solver3d_insert_point (AranSolver3d *solver, VsgPoint3 point);
+/* (aran/aransolver3d.h:161) one process per tree: same as insert_point */
+void aran_solver3d_insert_point_local (AranSolver3d *solver,
+                                       VsgPoint3 point);
 gboolean aran_solver3d_remove_point (AranSolver3d *solver, VsgPoint3 point);
 VsgPoint3 aran_solver3d_find_point (AranSolver3d *solver, VsgPoint3 selector);
 void aran_solver3d_foreach_point (AranSolver3d *solver, GFunc func,
@@ -119,6 +123,35 @@ void aran_solver3d_foreach_point (AranSolver3d *solver, GFunc func,
 
 /* the solve (aran/aransolver3d.c:1135-1202) */
 void aran_solver3d_solve (AranSolver3d *solver);
+
+/* (aran/aransolver3d.h:180) Visits the octree of the inserted particles
+ * (the last solve's, or one built for the call) in the given order
+ * (G_LEVEL_ORDER: breadth first).  Node infos carry bounds, centre, particle
+ * count, leaf flag, depth, father and key; point_list is NULL.  After a
+ * solve of the same particles user_data is the node's AranDevelopment3d
+ * (multipole and local of that solve, freed after the call). */
+void aran_solver3d_traverse (AranSolver3d *solver, GTraverseType order,
+                             VsgPRTree3dFunc func, gpointer user_data);
+
+/* (aran/aransolver3d.h:185-187) the order of siblings in
+ * aran_solver3d_traverse: octant order (default) or along a 3-D Hilbert
+ * curve (J. Skilling's index of the children's integer coordinates).  A
+ * solve's results do not depend on it. */
+void aran_solver3d_set_children_order_hilbert (AranSolver3d *solver);
+void aran_solver3d_set_children_order_default (AranSolver3d *solver);
+
+/* (aran/aransolver3d.h:212) Near/far leaf predicate: a node for which
+ * isleaf returns TRUE acts as a leaf (its particles interact by P2P with
+ * the near leaves, its subtree takes no far pairs), as in the reference's
+ * traversal.  The solve builds the octree, asks the predicate of every
+ * non-leaf node from the root down (node infos as in traverse, user_data
+ * NULL), and solves on the octree where those nodes are leaves; the
+ * expansions of a leaf made this way are its P2M (the reference's M2M of
+ * its subtree, equal up to rounding: M2M and L2L are exact on truncated
+ * expansions).  NULL removes the predicate. */
+void aran_solver3d_set_nf_isleaf (AranSolver3d *solver,
+                                  VsgPRTree3dNFIsleafFunc isleaf,
+                                  gpointer user_data);
 
 /* ---- extensions (no counterpart in the reference) ---- */
 

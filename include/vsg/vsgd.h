@@ -11,6 +11,8 @@
 #ifndef ARAN_VSGD_COMPAT_H
 #define ARAN_VSGD_COMPAT_H
 
+#include <stdio.h>
+
 #include <aran/arantypes.h>
 
 #ifdef __cplusplus
@@ -79,6 +81,42 @@ VsgPRTree3d *vsg_prtree3d_new_full (const VsgVector3d *lbound,
                                     VsgRegion3dLocFunc region_locfunc,
                                     guint maxbox);
 void vsg_prtree3d_free (VsgPRTree3d *prtree);
+
+/* Writes the recorded bounds and leaf capacity (the octree itself exists
+ * on the GPU during a solve only). */
+void vsg_prtree3d_write (VsgPRTree3d *prtree, FILE *file);
+
+/* Vector helpers the reference's harnesses call (tests/newtonpot3.c,
+ * tests/newtonfield3.c), written from those call sites: dst may alias an
+ * operand. */
+#define VSG_V3D_ZERO ((VsgVector3d) {0., 0., 0.})
+void vsg_vector3d_set (VsgVector3d *vec, gdouble x, gdouble y, gdouble z);
+void vsg_vector3d_copy (const VsgVector3d *src, VsgVector3d *dst);
+void vsg_vector3d_add (const VsgVector3d *src, const VsgVector3d *other,
+                       VsgVector3d *dst);                 /* src + other */
+void vsg_vector3d_sub (const VsgVector3d *src, const VsgVector3d *other,
+                       VsgVector3d *dst);                 /* src - other */
+void vsg_vector3d_scalp (const VsgVector3d *src, gdouble scal,
+                         VsgVector3d *dst);               /* scal * src */
+gdouble vsg_vector3d_dotp (const VsgVector3d *one, const VsgVector3d *other);
+gdouble vsg_vector3d_norm (const VsgVector3d *vec);
+/* "[x, y, z]" */
+void vsg_vector3d_write (const VsgVector3d *vec, FILE *file);
+/* r, cos theta, sin theta (theta from +z), cos phi, sin phi (phi from +x);
+ * on the z axis cos phi = 1, at the origin cos theta = 1. */
+void vsg_vector3d_to_spherical_internal (const VsgVector3d *vec, gdouble *r,
+                                         gdouble *cost, gdouble *sint,
+                                         gdouble *cosp, gdouble *sinp);
+void vsg_vector3d_from_spherical_internal (VsgVector3d *vec, gdouble r,
+                                           gdouble cost, gdouble sint,
+                                           gdouble cosp, gdouble sinp);
+
+/* Message tracing of LibVsg's MPI layer: nothing to trace on one GPU. */
+void vsg_packed_msg_trace_set_active (gboolean active);
+
+/* Whether a node is shared between processes of a distributed tree: never
+ * here (one process per tree). */
+#define VSG_PRTREE3D_NODE_INFO_IS_SHARED(info) ((void) (info), FALSE)
 
 #ifdef __cplusplus
 }

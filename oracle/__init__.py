@@ -71,6 +71,9 @@ def lib():
         L.or_m2l_rotate.argtypes = [i, _dp, _dp, i, _dp, _dp]
         L.or_m2m_rotate.argtypes = [i, _dp, _dp, i, _dp, _dp]
         L.or_l2l_rotate.argtypes = [i, _dp, _dp, i, _dp, _dp]
+        L.or_m2l_flavour.argtypes = [i, i, _dp, _dp, i, _dp, _dp]
+        L.or_m2m_flavour.argtypes = [i, i, _dp, _dp, i, _dp, _dp]
+        L.or_l2l_flavour.argtypes = [i, i, _dp, _dp, i, _dp, _dp]
         L.or_local_evaluate.restype = d
         L.or_local_evaluate.argtypes = [i, _dp, _dp, _dp]
         L.or_local_gradient.argtypes = [i, _dp, _dp, _dp, _dp]
@@ -166,27 +169,41 @@ def p2m(nm, pos, q, center, M=None):
     return M
 
 
-def m2l(nm, M, xs, pl, xd, L=None):
+# flavour: 0 rotate / z-translate / rotate back (the operator of record),
+# 1 the O(p^4) "normal" formulas, 2 K. Kylin's recurrences
+def m2l(nm, M, xs, pl, xd, L=None, flavour=0):
     if L is None:
         L = np.zeros(tri(pl + 1), dtype=np.complex128)
-    lib().or_m2l_rotate(nm, _c(M.view(np.float64)), _c(xs), pl,
-                        L.view(np.float64), _c(xd))
+    if flavour == 0:
+        lib().or_m2l_rotate(nm, _c(M.view(np.float64)), _c(xs), pl,
+                            L.view(np.float64), _c(xd))
+    else:
+        lib().or_m2l_flavour(flavour, nm, _c(M.view(np.float64)), _c(xs), pl,
+                             L.view(np.float64), _c(xd))
     return L
 
 
-def m2m(nm, M, xs, nmd, xd, Md=None):
+def m2m(nm, M, xs, nmd, xd, Md=None, flavour=0):
     if Md is None:
         Md = np.zeros(tri(nmd), dtype=np.complex128)
-    lib().or_m2m_rotate(nm, _c(M.view(np.float64)), _c(xs), nmd,
-                        Md.view(np.float64), _c(xd))
+    if flavour == 0:
+        lib().or_m2m_rotate(nm, _c(M.view(np.float64)), _c(xs), nmd,
+                            Md.view(np.float64), _c(xd))
+    else:
+        lib().or_m2m_flavour(flavour, nm, _c(M.view(np.float64)), _c(xs), nmd,
+                             Md.view(np.float64), _c(xd))
     return Md
 
 
-def l2l(pl, Ls, xs, pld, xd, Ld=None):
+def l2l(pl, Ls, xs, pld, xd, Ld=None, flavour=0):
     if Ld is None:
         Ld = np.zeros(tri(pld + 1), dtype=np.complex128)
-    lib().or_l2l_rotate(pl, _c(Ls.view(np.float64)), _c(xs), pld,
-                        Ld.view(np.float64), _c(xd))
+    if flavour == 0:
+        lib().or_l2l_rotate(pl, _c(Ls.view(np.float64)), _c(xs), pld,
+                            Ld.view(np.float64), _c(xd))
+    else:
+        lib().or_l2l_flavour(flavour, pl, _c(Ls.view(np.float64)), _c(xs),
+                             pld, Ld.view(np.float64), _c(xd))
     return Ld
 
 

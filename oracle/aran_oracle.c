@@ -587,6 +587,445 @@ void or_l2l_rotate (int pl, const double *Lsrc, const double *xsrc, int pldst,
   free (trans);
 }
 
+/* ---------------------------------------- alternate translation flavours
+ *
+ * The O(p^4) "normal" translations (aran/aransphericalseriesd-translate.c:
+ * 366-443, 507-579, 678-834) and K. Kylin's partial-sum recurrences
+ * (aran/aransphericalseriesd-kkylin.c:34-579).  They map the same truncated
+ * expansions as the rotate flavour, by other arithmetic.  The translation
+ * vector is xdst - xsrc in spherical coordinates (or_to_spherical); the
+ * multipole translations take the opposite vector (r, -cost, sint, -cosp,
+ * -sinp), as the reference's translate wrappers pass it
+ * (translate.c:784-793, kkylin.c:540-549). */
+
+/* Y_l^m for any sign of m from the m >= 0 table (_sph_sym). */
+static inline cplx harm_any (const cplx *Y, int l, int m)
+{
+  return (m >= 0) ? Y[TRI (l, m)] : sph_sym (Y[TRI (l, -m)], -m);
+}
+
+/* coefficient (l, m) of a series for any sign of m */
+static inline cplx coef_any (const cplx *S, int l, int m)
+{
+  return (m >= 0) ? S[TRI (l, m)] : sph_sym (S[TRI (l, -m)], -m);
+}
+
+/* aran_local_translate (translate.c:366-443) */
+static void l2l_normal (int pls, const cplx *src, int pld, cplx *dst,
+                        double r, double cost, double sint, double cosp,
+                        double sinp)
+{
+  int d = (pls > pld) ? pls : pld, l, m, n, o;
+  cplx *Y = malloc (sizeof (cplx) * TRI (d + 1, 0));
+  double rpow[OR_MAXDEG + 2], pw = 1.;
+  or_harmonics (pls, cost, sint, cosp, sinp, (double *) Y);
+  for (l = 0; l <= pls; l++)
+    {
+      rpow[l] = pw;
+      pw *= r;
+    }
+  for (l = 0; l <= pld; l++)
+    for (m = 0; m <= l; m++)
+      for (n = l; n <= pls; n++)
+        {
+          double normaliz = or_beta (n - l) * (or_beta (l) / or_beta (n));
+          cplx sum = 0.;
+          for (o = l + m - n; o <= m + n - l; o++)
+            {
+              double factor = or_alpha (n - l, abs (o - m)) * or_alpha (l, m)
+                              / or_alpha (n, abs (o));
+              sum += harm_any (Y, n - l, o - m) * coef_any (src, n, o)
+                     * factor;
+            }
+          dst[TRI (l, m)] += sum * (normaliz * rpow[n - l]);
+        }
+  free (Y);
+}
+
+/* aran_multipole_translate (translate.c:507-579); (cost, cosp, sinp) are
+ * those of the opposite vector */
+static void m2m_normal (int nms, const cplx *src, int nmd, cplx *dst,
+                        double r, double cost, double sint, double cosp,
+                        double sinp)
+{
+  int d = ((nms > nmd) ? nms : nmd) - 1, l, m, n, o;
+  cplx *Y = malloc (sizeof (cplx) * TRI (d + 2, 0));
+  double rpow[OR_MAXDEG + 2], pw = 1.;
+  or_harmonics (nmd - 1, cost, sint, cosp, sinp, (double *) Y);
+  for (l = 0; l < nmd; l++)
+    {
+      rpow[l] = pw;
+      pw *= r;
+    }
+  for (l = 0; l < nmd; l++)
+    for (m = 0; m <= l; m++)
+      {
+        int nmax = (l < nms - 1) ? l : nms - 1;
+        for (n = 0; n <= nmax; n++)
+          {
+            double normaliz = or_beta (l - n) * (or_beta (l) / or_beta (n));
+            int olo = (-n > m + n - l) ? -n : m + n - l;
+            int ohi = (n < l + m - n) ? n : l + m - n;
+            cplx sum = 0.;
+            for (o = olo; o <= ohi; o++)
+              {
+                double factor = or_alpha (l - n, abs (m - o))
+                                * or_alpha (n, abs (o)) / or_alpha (l, m);
+                sum += conj (harm_any (Y, l - n, m - o)) * coef_any (src, n, o)
+                       * factor;
+              }
+            dst[TRI (l, m)] += sum * normaliz * rpow[l - n];
+          }
+      }
+  free (Y);
+}
+
+/* aran_multipole_to_local (translate.c:678-768), with the vertical M2L of
+ * the reference when |sin theta| < 1e-5 (translate.c:695-701, 609-676:
+ * r^-(k) times cos theta for even k, phi unused) */
+static void m2l_vertical_cost (int nm, const cplx *src, int pl, cplx *dst,
+                               double r, double cost)
+{
+  m2l_vertical (nm, src, pl, dst, r, cost);
+}
+
+static void m2l_normal (int nm, const cplx *src, int pl, cplx *dst, double r,
+                        double cost, double sint, double cosp, double sinp)
+{
+  int d = pl + nm, l, m, n, o;
+  cplx *Y;
+  double rpow[2 * OR_MAXDEG + 4], pw = 1., invr = 1. / r, sign = 1.;
+  if (fabs (sint) < 1.e-5)
+    {
+      m2l_vertical_cost (nm, src, pl, dst, r, cost);
+      return;
+    }
+  Y = malloc (sizeof (cplx) * TRI (d + 1, 0));
+  or_harmonics (d, cost, sint, cosp, sinp, (double *) Y);
+  for (l = 0; l <= d; l++)
+    {
+      rpow[l] = pw;
+      pw *= invr;
+    }
+  for (l = 0; l <= pl; l++)
+    {
+      for (m = 0; m <= l; m++)
+        for (n = 0; n < nm; n++)
+          {
+            double normaliz = or_beta (l + n) * (or_beta (l) / or_beta (n));
+            const cplx *S = src + TRI (n, 0);
+            const cplx *h = Y + TRI (l + n, 0);
+            cplx sum = or_alpha (l, m) * or_alpha (n, 0)
+                       / or_alpha (l + n, m) * h[m] * S[0];
+            for (o = 1; o <= n; o++)
+              {
+                double factor = or_alpha (l, m) * or_alpha (n, o);
+                sum += h[m + o] * S[o] * factor / or_alpha (l + n, m + o);
+                sum += harm_any (Y, l + n, m - o) * sph_sym (S[o], o) * factor
+                       / or_alpha (l + n, abs (m - o));
+              }
+            dst[TRI (l, m)] += conj (sum) * sign * normaliz * rpow[l + n + 1];
+          }
+      sign = -sign;
+    }
+  free (Y);
+}
+
+/* aran_multipole_translate_kkylin (kkylin.c:34-205); (cost, cosp, sinp)
+ * are those of the opposite vector; degrees of src and dst equal */
+static void m2m_kylin (int nms, const cplx *src, int nmd, cplx *dst,
+                       double r, double cost, double sint, double cosp,
+                       double sinp)
+{
+  int d = ((nms > nmd) ? nms : nmd) - 1, l, m, n, o;
+  int w = d + 1;
+  cplx *Y = malloc (sizeof (cplx) * TRI (d + 2, 0));
+  cplx *b1 = calloc ((size_t) w * w, sizeof (cplx));
+  cplx *b2 = calloc ((size_t) w * w, sizeof (cplx));
+  cplx *part = b1, *prev = b2, *t, expp = cosp + I * sinp, fse;
+  double rpow[OR_MAXDEG + 2], pw = 1., fact;
+#define PT(a, mm, nn) (a)[(mm) * w + (nn)]
+  or_harmonics (nmd - 1, cost, sint, cosp, sinp, (double *) Y);
+  rpow[0] = pw;
+  pw *= r;
+  PT (part, 0, 0) = conj (Y[0]) * src[0];
+  dst[0] += PT (part, 0, 0) * or_beta (0);
+  for (l = 1; l < nmd; l++)
+    {
+      rpow[l] = pw;
+      pw *= r;
+      t = part;
+      part = prev;
+      prev = t;
+      for (n = 0; n <= l - 1; n++)
+        {
+          fact = or_beta (l - n) / or_beta (n);
+          PT (part, 0, n) = 0.;
+          for (o = (-n > n - l ? -n : n - l); o <= (n < l - n ? n : l - n);
+               o++)
+            PT (part, 0, n) += (rpow[l - n] * or_alpha (n, abs (o))
+                                * or_alpha (l - n, abs (o)))
+                               * conj (harm_any (Y, l - n, -o))
+                               * coef_any (src, n, o);
+          PT (part, 0, n) *= fact;
+          if (l > 1)
+            {
+              PT (part, l - 1, n) = 0.;
+              for (o = (-n > n - 1 ? -n : n - 1);
+                   o <= (n < l + l - 1 - n ? n : l + l - 1 - n); o++)
+                PT (part, l - 1, n) +=
+                    (rpow[l - n] * or_alpha (n, abs (o))
+                     * or_alpha (l - n, abs (l - o - 1)))
+                    * conj (harm_any (Y, l - n, l - o - 1))
+                    * coef_any (src, n, o);
+              PT (part, l - 1, n) *= fact;
+            }
+          PT (part, l, n) = 0.;
+          for (o = (-n > n ? -n : n); o <= (n < l + l - n ? n : l + l - n);
+               o++)
+            PT (part, l, n) += (rpow[l - n] * or_alpha (n, abs (o))
+                                * or_alpha (l - n, abs (l - o)))
+                               * conj (harm_any (Y, l - n, l - o))
+                               * coef_any (src, n, o);
+          PT (part, l, n) *= fact;
+          fact = 0.5 * r / (l - n);
+          fse = fact * sint * expp;
+          fact *= 2. * cost;
+          for (m = 1; m < l - 1; m++)
+            PT (part, m, n) = fse * PT (prev, m + 1, n)
+                              - conj (fse) * PT (prev, m - 1, n)
+                              + fact * PT (prev, m, n);
+        }
+      fact = or_beta (0) / or_beta (l);
+      for (m = 0; m <= l; m++)
+        PT (part, m, l) = (or_alpha (l, m) * fact) * conj (Y[0])
+                          * src[TRI (l, m)];
+      for (m = 0; m <= l; m++)
+        {
+          cplx sum = 0.;
+          for (n = 0; n <= l; n++)
+            sum += PT (part, m, n);
+          dst[TRI (l, m)] += sum * (or_beta (l) / or_alpha (l, m));
+        }
+    }
+#undef PT
+  free (Y);
+  free (b1);
+  free (b2);
+}
+
+/* aran_local_translate_kkylin (kkylin.c:207-364); src degree <= dst */
+static void l2l_kylin (int pls, const cplx *src, int pld, cplx *dst,
+                       double r, double cost, double sint, double cosp,
+                       double sinp)
+{
+  int d = (pls > pld) ? pls : pld, l, m, n, o;
+  int w = d + 1;
+  cplx *Y = malloc (sizeof (cplx) * TRI (d + 1, 0));
+  cplx *b1 = calloc ((size_t) w * w, sizeof (cplx));
+  cplx *b2 = calloc ((size_t) w * w, sizeof (cplx));
+  cplx *part = b1, *prev = b2, *t, expp = cosp + I * sinp, fse, y00;
+  double rpow[OR_MAXDEG + 2], pw = 1., fact;
+#define PT(a, mm, nn) (a)[(mm) * w + (nn)]
+  or_harmonics (pls, cost, sint, cosp, sinp, (double *) Y);
+  y00 = Y[0];
+  for (l = 0; l <= pls; l++)
+    {
+      rpow[l] = pw;
+      pw *= r;
+    }
+  for (l = pld; l >= 0; l--)
+    {
+      if (l <= pls)
+        {
+          fact = or_beta (0) / or_beta (l);
+          for (m = 0; m <= l; m++)
+            PT (part, m, l) = fact / or_alpha (l, m)
+                              * conj (y00 * src[TRI (l, m)]);
+        }
+      for (n = l + 1; n <= pls; n++)
+        {
+          fact = or_beta (n - l) / or_beta (n);
+          PT (part, 0, n) = 0.;
+          for (o = (-n > l - n ? -n : l - n); o <= (n < n - l ? n : n - l);
+               o++)
+            PT (part, 0, n) += (rpow[n - l] * or_alpha (n - l, abs (o))
+                                / or_alpha (n, abs (o)))
+                               * conj (harm_any (Y, n - l, o)
+                                       * coef_any (src, n, o));
+          PT (part, 0, n) *= fact;
+          if (l > 0)
+            {
+              PT (part, l, n) = 0.;
+              for (o = (-n > l + l - n ? -n : l + l - n); o <= n; o++)
+                PT (part, l, n) += (rpow[n - l]
+                                    * or_alpha (n - l, abs (o - l))
+                                    / or_alpha (n, abs (o)))
+                                   * conj (harm_any (Y, n - l, o - l)
+                                           * coef_any (src, n, o));
+              PT (part, l, n) *= fact;
+            }
+          fact = 0.5 * r / (n - l);
+          fse = fact * sint * expp;
+          fact *= 2. * cost;
+          for (m = 1; m < l; m++)
+            PT (part, m, n) = fse * PT (prev, m - 1, n)
+                              - conj (fse) * PT (prev, m + 1, n)
+                              + fact * PT (prev, m, n);
+        }
+      for (m = 0; m <= l; m++)
+        {
+          cplx sum = 0.;
+          for (n = l; n <= pls; n++)
+            sum += PT (part, m, n);
+          dst[TRI (l, m)] += conj (sum) * (or_beta (l) * or_alpha (l, m));
+        }
+      t = part;
+      part = prev;
+      prev = t;
+    }
+#undef PT
+  free (Y);
+  free (b1);
+  free (b2);
+}
+
+/* aran_multipole_to_local_kkylin (kkylin.c:364-501) */
+static void m2l_kylin (int nm, const cplx *src, int pl, cplx *dst, double r,
+                       double cost, double sint, double cosp, double sinp)
+{
+  int d = pl + nm, l, m, n, o;
+  int w = d + 1;
+  cplx *Y, *b1, *b2, *part, *prev, *t, expp = cosp + I * sinp;
+  cplx tce, e2, tl;
+  double rpow[2 * OR_MAXDEG + 4], pw = 1., invr = 1. / r, sign = 1., fact;
+  if (fabs (sint) < 1.e-5)
+    {
+      m2l_vertical_cost (nm, src, pl, dst, r, cost);
+      return;
+    }
+  Y = malloc (sizeof (cplx) * TRI (d + 1, 0));
+  b1 = calloc ((size_t) w * w, sizeof (cplx));
+  b2 = calloc ((size_t) w * w, sizeof (cplx));
+  part = b1;
+  prev = b2;
+#define PT(a, mm, nn) (a)[(mm) * w + (nn)]
+  or_harmonics (d, cost, sint, cosp, sinp, (double *) Y);
+  for (l = 0; l <= d; l++)
+    {
+      rpow[l] = pw;
+      pw *= invr;
+    }
+  for (l = 0; l <= pl; l++)
+    {
+      for (n = 0; n < nm; n++)
+        {
+          fact = rpow[l + n + 1] * or_beta (l + n) / or_beta (n);
+          PT (part, 0, n) = 0.;
+          for (o = -n; o <= n; o++)
+            PT (part, 0, n) += (or_alpha (n, abs (o))
+                                / or_alpha (l + n, abs (o)))
+                               * harm_any (Y, l + n, o) * coef_any (src, n, o);
+          PT (part, 0, n) *= fact;
+          if (l > 0)
+            {
+              PT (part, 1, n) = 0.;
+              for (o = -n; o <= n; o++)
+                PT (part, 1, n) += (or_alpha (n, abs (o))
+                                    / or_alpha (l + n, abs (o + 1)))
+                                   * harm_any (Y, l + n, o + 1)
+                                   * coef_any (src, n, o);
+              PT (part, 1, n) *= fact;
+            }
+          tce = 2. * cost / sint * expp;
+          e2 = expp * expp;
+          tl = 2. * (l + n) * invr / sint * expp;
+          for (m = 2; m <= l; m++)
+            PT (part, m, n) = tce * PT (part, m - 1, n)
+                              + e2 * PT (part, m - 2, n)
+                              - tl * PT (prev, m - 1, n);
+        }
+      fact = or_beta (l) * sign;
+      for (m = 0; m <= l; m++)
+        {
+          cplx sum = 0.;
+          for (n = 0; n < nm; n++)
+            sum += PT (part, m, n);
+          dst[TRI (l, m)] += conj (sum) * (or_alpha (l, m) * fact);
+        }
+      t = part;
+      part = prev;
+      prev = t;
+      sign = -sign;
+    }
+#undef PT
+  free (Y);
+  free (b1);
+  free (b2);
+}
+
+/* Public entry points: one part of a series, translated by xdst - xsrc and
+ * added into dst (flavour 1: normal, 2: Kylin). */
+static void flav_vector (const double *xsrc, const double *xdst, int opposite,
+                         double *r, double *cost, double *sint, double *cosp,
+                         double *sinp)
+{
+  double v[3];
+  int k;
+  for (k = 0; k < 3; k++)
+    v[k] = xdst[k] - xsrc[k];
+  or_to_spherical (v, r, cost, sint, cosp, sinp);
+  if (opposite)
+    {
+      *cost = -*cost;
+      *cosp = -*cosp;
+      *sinp = -*sinp;
+    }
+}
+
+void or_m2m_flavour (int flavour, int nm, const double *Msrc,
+                     const double *xsrc, int nmdst, double *Mdst,
+                     const double *xdst)
+{
+  double r, ct, st, cp, sp;
+  if (nm <= 0)
+    return;
+  flav_vector (xsrc, xdst, 1, &r, &ct, &st, &cp, &sp);
+  if (flavour == 2)
+    m2m_kylin (nm, (const cplx *) Msrc, nmdst, (cplx *) Mdst, r, ct, st, cp,
+               sp);
+  else
+    m2m_normal (nm, (const cplx *) Msrc, nmdst, (cplx *) Mdst, r, ct, st, cp,
+                sp);
+}
+
+void or_l2l_flavour (int flavour, int pl, const double *Lsrc,
+                     const double *xsrc, int pldst, double *Ldst,
+                     const double *xdst)
+{
+  double r, ct, st, cp, sp;
+  flav_vector (xsrc, xdst, 0, &r, &ct, &st, &cp, &sp);
+  if (flavour == 2)
+    l2l_kylin (pl, (const cplx *) Lsrc, pldst, (cplx *) Ldst, r, ct, st, cp,
+               sp);
+  else
+    l2l_normal (pl, (const cplx *) Lsrc, pldst, (cplx *) Ldst, r, ct, st, cp,
+                sp);
+}
+
+void or_m2l_flavour (int flavour, int nm, const double *M, const double *xsrc,
+                     int pl, double *L, const double *xdst)
+{
+  double r, ct, st, cp, sp;
+  if (nm <= 0)
+    return;
+  flav_vector (xsrc, xdst, 0, &r, &ct, &st, &cp, &sp);
+  if (flavour == 2)
+    m2l_kylin (nm, (const cplx *) M, pl, (cplx *) L, r, ct, st, cp, sp);
+  else
+    m2l_normal (nm, (const cplx *) M, pl, (cplx *) L, r, ct, st, cp, sp);
+}
+
 /* ------------------------------------------------- particle <-> series */
 
 /* aran_development3d_p2m (aran/arandevelopment3d.c:169-216) */

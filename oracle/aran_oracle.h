@@ -56,6 +56,17 @@ void or_m2m_rotate (int nm, const double *Msrc, const double *xsrc,
                     int nmdst, double *Mdst, const double *xdst);
 void or_l2l_rotate (int pl, const double *Lsrc, const double *xsrc,
                     int pldst, double *Ldst, const double *xdst);
+/* alternate flavours of the same translations (flavour 1: the O(p^4)
+ * "normal" formulas, 2: K. Kylin's recurrences); same arguments as the
+ * rotate flavour above */
+void or_m2m_flavour (int flavour, int nm, const double *Msrc,
+                     const double *xsrc, int nmdst, double *Mdst,
+                     const double *xdst);
+void or_l2l_flavour (int flavour, int pl, const double *Lsrc,
+                     const double *xsrc, int pldst, double *Ldst,
+                     const double *xdst);
+void or_m2l_flavour (int flavour, int nm, const double *M, const double *xsrc,
+                     int pl, double *L, const double *xdst);
 double or_local_evaluate (int pl, const double *L, const double *center,
                           const double *pos);
 void or_local_gradient (int pl, const double *L, const double *center,

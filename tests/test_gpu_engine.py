@@ -69,12 +69,12 @@ def test_single_operators_match_oracle(torch_cuda, p):
         L.aran_development3d_p2m(C.byref(S.VsgVector3d(*pos)), q,
                                  C.byref(ni_s), d_src)
         O.p2m(p, pos, q, src_c, Mref)
-    M = _arr(d_src.contents.multipole, 2 * mc).view(np.complex128)
+    M = S.multipole(d_src)
     assert np.max(np.abs(M - Mref)) <= 1e-13 * np.max(np.abs(Mref))
     # M2L to a far centre
     L.aran_development3d_m2l_rotate(C.byref(ni_s), d_src, C.byref(ni_d), d_dst)
     Lref = O.m2l(p, Mref, src_c, p, dst_c)
-    Lg = _arr(d_dst.contents.local, 2 * lc).view(np.complex128)
+    Lg = S.local(d_dst)
     assert np.max(np.abs(Lg - Lref)) <= 1e-12 * np.max(np.abs(Lref))
     # M2M to a parent centre
     par_c = src_c + np.array([0.25, -0.25, 0.25])
@@ -82,7 +82,7 @@ def test_single_operators_match_oracle(torch_cuda, p):
     d_par = _new_dev(L, p)
     L.aran_development3d_m2m_rotate(C.byref(ni_s), d_src, C.byref(ni_p), d_par)
     Mpref = O.m2m(p, Mref, src_c, p, par_c)
-    Mp = _arr(d_par.contents.multipole, 2 * mc).view(np.complex128)
+    Mp = S.multipole(d_par)
     assert np.max(np.abs(Mp - Mpref)) <= 1e-12 * np.max(np.abs(Mpref))
     # L2L to a child centre (z below: exercises the z flip)
     ch_c = dst_c + np.array([-0.1, 0.15, -0.2])
@@ -90,7 +90,7 @@ def test_single_operators_match_oracle(torch_cuda, p):
     d_ch = _new_dev(L, p)
     L.aran_development3d_l2l_rotate(C.byref(ni_d), d_dst, C.byref(ni_c), d_ch)
     Lcref = O.l2l(p, Lref, dst_c, p, ch_c)
-    Lc = _arr(d_ch.contents.local, 2 * lc).view(np.complex128)
+    Lc = S.local(d_ch)
     assert np.max(np.abs(Lc - Lcref)) <= 1e-12 * np.max(np.abs(Lcref))
     # L2P potential and gradient
     for k in range(4):
@@ -112,7 +112,8 @@ def test_single_operators_match_oracle(torch_cuda, p):
 def test_m2l_kat_against_newton_potential(torch_cuda):
     """tests/m2l3.c:211-340 on the GPU operator: multipole of a unit charge at
     (0.1,0.2,0.3), M2L to 7 vectors, local checked against 1/|x-p| at
-    1e-7 (order 15 here, the engine maximum; the reference uses 26)."""
+    1e-7, at order 15 (the reference's order 26 runs in
+    test_gpu_reference_gates.py, in every flavour)."""
     from aran_amd import solver as S
     L = S.lib()
     p = 15
@@ -272,9 +273,16 @@ def test_edge_cases(torch_cuda):
     pot, grad = f.solve(xs, ys, zs, qs, (0, 0, 0), (1, 1, 1), 8, 1)
     r = np.hypot(0.8, 0.1)
     assert np.allclose(pot.cpu().numpy(), [2.0 / r, 1.0 / r], rtol=1e-14)
-    # invalid order and non-finite input are rejected
+    # invalid order (above 30; above 15 outside the plain form) and
+    # non-finite input are rejected
+    with pytest.raises(RuntimeError):
+        f.solve(xs, ys, zs, qs, (0, 0, 0), (1, 1, 1), 31, 1)
+    with pytest.raises(RuntimeError):
+        f.solve(xs, ys, zs, qs, (0, 0, 0), (1, 1, 1), 0, 1)
+    f.set_precision("mixed")
     with pytest.raises(RuntimeError):
         f.solve(xs, ys, zs, qs, (0, 0, 0), (1, 1, 1), 16, 1)
+    f.set_precision("plain")
     bad = torch.tensor([0.1, float("nan")], dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError):
         f.solve(bad, ys, zs, qs, (0, 0, 0), (1, 1, 1), 8, 1)

@@ -463,3 +463,130 @@ def test_planted_mistake_is_caught():
     x = t + np.array([0.3, -0.4, 0.5])
     ref = 1 / np.linalg.norm(x - src)
     assert abs(O.local_eval(15, L, t, x) - ref) > 1e-7 * ref
+
+
+# ------------------------------------------- counters, root box, flavours
+
+@pytest.mark.parametrize("n,q,seed", [(60, 3, 0), (200, 1, 2), (150, 8, 1)])
+def test_counters_against_brute_force(n, q, seed):
+    """The solve's operator counters (the reference's semantics,
+    aran/aransolver3d.c:254-396) equal counts taken by brute force from the
+    tree and the near / far pairs: p2p counts n_a n_b per near leaf pair and
+    n (n + 1) / 2 per reflexive one, m2l one per ordered far pair."""
+    rng = np.random.default_rng(seed)
+    x, y, z = rng.random((3, n)) ** np.array([[1.0], [2.0], [3.0]])
+    cover, nodes, perm = _pair_accounting(x, y, z, q)
+    (fs, fd), (na, nb) = O.traverse(nodes)
+    cnt = nodes["pend"] - nodes["pbegin"]
+    p2p = sum(int(cnt[a]) * (int(cnt[a]) + 1) // 2 if a == b
+              else int(cnt[a]) * int(cnt[b]) for a, b in zip(na, nb))
+    _, _, st = O.fmm_solve(x, y, z, np.ones(n), np.zeros(3), np.ones(3), 3,
+                           q)
+    nn = len(nodes)
+    assert list(st) == [nn, p2p, n, nn - 1, len(fs), nn - 1, n]
+
+
+def test_root_box_growth_against_direct_sum():
+    """Particles outside the given bounds: the root cube doubles about the
+    bounds' centre until it holds them (or_root_box), and the solve on it
+    agrees with the direct sum as inside particles do."""
+    rng = np.random.default_rng(7)
+    n = 600
+    x, y, z = rng.normal(size=(3, n)) * 3.0  # bounds below are [-1, 1]^3
+    lb, ub = -np.ones(3), np.ones(3)
+    lo, w = O.root_box(lb, ub, x, y, z)
+    half = w / 2
+    assert np.allclose(lo + half, 0.0)
+    assert np.log2(half) == int(np.log2(half)) and half >= 1
+    for a in (x, y, z):
+        assert np.all(a >= lo[0]) and np.all(a <= lo[0] + w)
+    assert np.max(np.abs(np.concatenate([x, y, z]))) > half / 2
+    q = rng.uniform(0.1, 1.0, n)
+    pot, grad, _ = O.fmm_solve(x, y, z, q, lb, ub, 10, 8)
+    # the same solve given the grown cube as bounds: identical
+    pot2, grad2, _ = O.fmm_solve(x, y, z, q, lo, lo + w, 10, 8)
+    assert np.array_equal(pot, pot2) and np.array_equal(grad, grad2)
+    # and the FMM error of order 10 against the direct sum
+    t = np.arange(n)
+    dp, dg = O.direct(x, y, z, q, t)
+    assert np.max(np.abs(pot - dp) / np.abs(dp)) <= 1e-5
+    assert np.sqrt(np.sum((pot - dp) ** 2) / np.sum(dp ** 2)) <= 1e-6
+    assert np.max(np.linalg.norm(grad - dg, axis=1) /
+                  np.linalg.norm(dg, axis=1)) <= 1e-4
+
+
+@pytest.mark.parametrize("flavour", [1, 2])
+def test_flavour_m2l_kat_seven_vectors(flavour):
+    """tests/m2l3.c:211-340 with the normal (1) and Kylin (2) M2L
+    (aransphericalseriesd-translate.c:678-768, -kkylin.c:364-501): the
+    local vs 1/|x-p| on the 10x10x10 grid at 1e-7, and the three flavours
+    agree coefficient by coefficient (two of the vectors are vertical, where
+    both alternate flavours take the vertical M2L)."""
+    src = np.array([0.1, 0.2, 0.3])
+    deg = 25
+    M = O.p2m(deg + 1, src, 1.0, np.zeros(3))
+    for t in [(5, 5, 5), (5, 5, -5), (0, 0, -5), (0, 0, 5), (5, 0, 5),
+              (0, 5, 5), (-5, -5, 5)]:
+        t = np.array(t, dtype=float)
+        L = O.m2l(deg + 1, M, np.zeros(3), deg, t, flavour=flavour)
+        Lr = O.m2l(deg + 1, M, np.zeros(3), deg, t)
+        assert np.max(np.abs(L - Lr)) <= 1e-13 * np.max(np.abs(Lr))
+        for x in _grid(t, 1.0):
+            ref = 1.0 / np.linalg.norm(x - src)
+            res = O.local_eval(deg, L, t, x)
+            assert abs(res - ref) <= 1e-7 * abs(ref), (t, x)
+
+
+@pytest.mark.parametrize("flavour", [1, 2])
+def test_flavour_m2m_and_l2l_kat(flavour):
+    """tests/multipole3.c:270-329 (M2M of an order-30 multipole to
+    (1,1.1,1.2), checked on r in [5,10] at 1e-7) and tests/taylor3.c:
+    155-259 (L2L of a degree-29 local from (5,0,0) to (4.5,0,0), checked on
+    the r <= 0.5 grid at 1e-9) in the normal (1) and Kylin (2) flavours,
+    and against the rotate flavour (1e-10 of the largest coefficient)."""
+    src = np.array([0.1, 0.2, 0.3])
+    M = O.p2m(30, src, 1.0, np.zeros(3))
+    c2 = np.array([1.0, 1.1, 1.2])
+    M2 = O.m2m(30, M, np.zeros(3), 30, c2, flavour=flavour)
+    M2r = O.m2m(30, M, np.zeros(3), 30, c2)
+    # order 30: the algorithms' rounding differs at 1e-11 of the largest
+    # coefficient (degree-29 terms of 1e4 from cancelling sums)
+    assert np.max(np.abs(M2 - M2r)) <= 1e-10 * np.max(np.abs(M2r))
+    rng = np.random.default_rng(2)
+    for _ in range(100):
+        v = rng.normal(size=3)
+        x = c2 + v / np.linalg.norm(v) * rng.uniform(5, 10)
+        ref = 1 / np.linalg.norm(x - src)
+        assert abs(O.multipole_eval(30, M2, c2, x) - ref) <= 1e-7 * ref
+    p = np.array([3.0, -1.0, -1.0])
+    c = np.array([5.0, 0.0, 0.0])
+    tr = np.array([4.5, 0.0, 0.0])
+    Mp = O.p2m(30, p, 1.0, p)
+    Lc = O.m2l(30, Mp, p, 29, c)
+    L2 = O.l2l(29, Lc, c, 29, tr, flavour=flavour)
+    L2r = O.l2l(29, Lc, c, 29, tr)
+    assert np.max(np.abs(L2 - L2r)) <= 1e-10 * np.max(np.abs(L2r))
+    for x in _grid(tr, 0.5):
+        d = x - p
+        ref = 1 / np.linalg.norm(d)
+        assert abs(O.local_eval(29, L2, tr, x) - ref) <= 1e-9 * ref
+        g = O.local_grad(29, L2, tr, x)
+        gref = -d / np.linalg.norm(d) ** 3
+        assert np.linalg.norm(g - gref) <= 1e-9 * np.linalg.norm(gref)
+
+
+def test_flavour_planted_mistake_is_caught():
+    """A sign slip in one Kylin partial sum (planted in a copy of the
+    oracle's output, as the kernels' tests compare them) is far outside the
+    cross-flavour bar used above."""
+    src = np.array([0.1, 0.2, 0.3])
+    M = O.p2m(12, src, 1.0, np.zeros(3))
+    t = np.array([5.0, 5.0, 5.0])
+    L = O.m2l(12, M, np.zeros(3), 12, t, flavour=2)
+    Lr = O.m2l(12, M, np.zeros(3), 12, t)
+    bad = L.copy()
+    k = O.tri(7) + int(np.argmax(np.abs(L[O.tri(7):O.tri(8)])))
+    bad[k] = -bad[k]
+    # (degree-7 terms are 4e-8 of the largest at |t| = 8.7)
+    assert np.max(np.abs(bad - Lr)) > 1e-9 * np.max(np.abs(Lr))
+    assert np.max(np.abs(L - Lr)) <= 1e-13 * np.max(np.abs(Lr))
