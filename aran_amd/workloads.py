@@ -100,3 +100,54 @@ def make(kind, n, seed):
     if kind == "signed":
         return signed_cube(n, seed)
     raise ValueError(kind)
+
+
+def batch_systems(nsys, seed):
+    """Independent systems of the batched configuration: sizes
+    round(10^(3 + 2u)) (log-uniform in [1e3, 1e5]), each a uniform cube or
+    a Plummer sphere with probability 1/2, masses in (0, 1].  Returns the
+    concatenated (x, y, z, m), the offsets and the per-system root bounds."""
+    u = uniform(seed, 20, 2 * nsys).reshape(nsys, 2)
+    xs, ys, zs, ms, lbs, ubs, offs = [], [], [], [], [], [], [0]
+    for k in range(nsys):
+        n = int(round(10 ** (3 + 2 * u[k, 0])))
+        if u[k, 1] < 0.5:
+            x, y, z, _ = uniform_cube(n, seed * 1000 + k)
+            lbs.append((0, 0, 0))
+            ubs.append((1, 1, 1))
+        else:
+            x, y, z, _ = plummer(n, seed * 1000 + k)
+            lbs.append((-1, -1, -1))
+            ubs.append((1, 1, 1))
+        m = 1.0 - uniform(seed * 1000 + k, 9, n)
+        for a, b in ((xs, x), (ys, y), (zs, z), (ms, m)):
+            a.append(b)
+        offs.append(offs[-1] + n)
+    cat = [np.concatenate(a) for a in (xs, ys, zs, ms)]
+    return cat, offs, np.array(lbs, float), np.array(ubs, float)
+
+
+def serving_requests(n_requests, seed, nmax=2_000_000, orders=(6, 8, 12)):
+    """Requests of the serving configuration: N = round(10^(3 + u log10
+    2000)) (log-uniform in [1e3, 2e6]), p drawn from `orders`, half
+    potential-only, uniform cube or Plummer (the first N points of seeded
+    2e6-point pools, so each request is an i.i.d. sample), masses in (0, 1],
+    leaf 64.  Returns a list of (x, y, z, m, lb, ub, order, leafcap,
+    want_grad) and the Poisson inter-arrival gaps for rate 1 (divide by the
+    rate)."""
+    pools = {"cube": uniform_cube(nmax, seed), "plummer": plummer(nmax, seed)}
+    mass = 1.0 - uniform(seed, 9, nmax)
+    u = uniform(seed, 10, 5 * n_requests).reshape(n_requests, 5)
+    reqs = []
+    for k in range(n_requests):
+        n = min(nmax, int(round(10 ** (3 + u[k, 0] * np.log10(2000)))))
+        order = orders[min(len(orders) - 1, int(u[k, 1] * len(orders)))]
+        kind = "cube" if u[k, 2] < 0.5 else "plummer"
+        want_grad = bool(u[k, 3] >= 0.5)
+        x, y, z, _ = pools[kind]
+        lb, ub = ((0, 0, 0), (1, 1, 1)) if kind == "cube" else \
+            ((-1, -1, -1), (1, 1, 1))
+        reqs.append((x[:n], y[:n], z[:n], mass[:n], lb, ub, order, 64,
+                     want_grad))
+    gaps = -np.log(1.0 - u[:, 4])
+    return reqs, gaps

@@ -202,31 +202,6 @@ def error_vs_direct(f, dev, pot, grad, n, seed):
             "targets": 4096}
 
 
-def batch_systems(nsys, seed):
-    """nsys independent systems, sizes round(10^(3 + 2u)) (log-uniform in
-    [1e3, 1e5]), each a uniform cube or a Plummer sphere with probability
-    1/2, masses in (0, 1]."""
-    from aran_amd import workloads as W
-    u = W.uniform(seed, 20, 2 * nsys).reshape(nsys, 2)
-    xs, ys, zs, ms, lbs, ubs, offs = [], [], [], [], [], [], [0]
-    for k in range(nsys):
-        n = int(round(10 ** (3 + 2 * u[k, 0])))
-        if u[k, 1] < 0.5:
-            x, y, z, _ = W.uniform_cube(n, seed * 1000 + k)
-            lbs.append((0, 0, 0))
-            ubs.append((1, 1, 1))
-        else:
-            x, y, z, _ = W.plummer(n, seed * 1000 + k)
-            lbs.append((-1, -1, -1))
-            ubs.append((1, 1, 1))
-        m = 1.0 - W.uniform(seed * 1000 + k, 9, n)
-        for a, b in ((xs, x), (ys, y), (zs, z), (ms, m)):
-            a.append(b)
-        offs.append(offs[-1] + n)
-    cat = [np.concatenate(a) for a in (xs, ys, zs, ms)]
-    return cat, offs, np.array(lbs, float), np.array(ubs, float)
-
-
 def batch_session(nsys, order, leafcap, steps, warmup, seed,
                   precision="mixed"):
     """One batched solve of nsys independent systems (shared launches, one
@@ -236,7 +211,7 @@ def batch_session(nsys, order, leafcap, steps, warmup, seed,
 
     from aran_amd import workloads as W
     from aran_amd.fmm import Fmm3d
-    cat, offs, lbs, ubs = batch_systems(nsys, seed)
+    cat, offs, lbs, ubs = W.batch_systems(nsys, seed)
     dev = [torch.from_numpy(a).cuda() for a in cat]
     n = offs[-1]
     f = Fmm3d()
@@ -323,21 +298,8 @@ def serve_session(n_requests, rate, seed, precision="mixed"):
     from aran_amd import workloads as W
     from aran_amd.server import Server
     nmax = 2_000_000
-    pools = {"cube": W.uniform_cube(nmax, seed), "plummer": W.plummer(nmax, seed)}
-    mass = 1.0 - W.uniform(seed, 9, nmax)
-    u = W.uniform(seed, 10, 5 * n_requests).reshape(n_requests, 5)
-    reqs = []
-    for k in range(n_requests):
-        n = int(round(10 ** (3 + u[k, 0] * np.log10(2000))))
-        order = (6, 8, 12)[min(2, int(u[k, 1] * 3))]
-        kind = "cube" if u[k, 2] < 0.5 else "plummer"
-        want_grad = u[k, 3] >= 0.5
-        x, y, z, _ = pools[kind]
-        lb, ub = ((0, 0, 0), (1, 1, 1)) if kind == "cube" else \
-            ((-1, -1, -1), (1, 1, 1))
-        reqs.append((x[:n], y[:n], z[:n], mass[:n], lb, ub, order, 64,
-                     want_grad))
-    gaps = -np.log(1.0 - u[:, 4]) / rate
+    reqs, gaps = W.serving_requests(n_requests, seed, nmax)
+    gaps = gaps / rate
     srv = Server(budget_bytes=200e9, nworkers=2, precision=precision,
                  max_batch=32)
     # staging, workspaces and tables for the largest request and every
