@@ -51,6 +51,9 @@ def _declare(L):
                                     vp, vp]
     L.aran_fmm3d_set_translations.restype = ip
     L.aran_fmm3d_set_translations.argtypes = [vp, ip, ip, ip]
+    L.aran_fmm3d_solve_noalloc.restype = ip
+    L.aran_fmm3d_solve_noalloc.argtypes = [vp, C.c_long, vp, vp, vp, vp, dp,
+                                           dp, ip, ip, vp, vp]
     L.aran_fmm3d_export_expansions.restype = ip
     L.aran_fmm3d_export_expansions.argtypes = [vp, vp, vp]
 

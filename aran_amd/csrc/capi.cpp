@@ -254,18 +254,115 @@ struct _AranFmm3d {
   int flav[3] = {0, 0, 0};  // M2M, M2L, L2L translation flavours
   long n_last = 0;
   int order_last = 0;
+  // workspace of aran_fmm3d_solve_noalloc: its device buffers come from the
+  // device scratch below, its pinned staging is one fixed block
+  aran::Workspace na;
 };
+
+// ---- the no-allocation solve's device scratch: one array per device (the
+// loader gives every device its own copy), used through the address
+// hipGetSymbolAddress returns on that device (the array's name in host code
+// is not a device address).  Every aran_fmm3d_solve_noalloc call on a device
+// shares it: calls that overlap on one device race.
+static __device__ __align__(256) char scratch[ARAN_FMM3D_NOALLOC_SCRATCH];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch[kMaxDevices] = {};
+constexpr size_t kNoAllocStage = 4u << 20;  // pinned staging per context
+}  // namespace
 
 extern "C" AranFmm3d *aran_fmm3d_new(void *hip_stream) {
   AranFmm3d *c = new AranFmm3d;
   hipError_t e =
       aran::workspace_init(c->ws, static_cast<hipStream_t>(hip_stream));
+  // set-up of the no-allocation solve: the scratch address on this device
+  // (once per device), a workspace on the same stream and its staging
+  if (e == hipSuccess) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    const int dev = c->ws.device;
+    if (dev < 0 || dev >= kMaxDevices) {
+      aran::set_error("device index beyond the scratch table");
+      e = hipErrorInvalidDevice;
+    } else if (!g_scratch[dev]) {
+      void *ptr = nullptr;
+      e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+      if (e == hipSuccess) g_scratch[dev] = ptr;
+      else aran::set_error(std::string("hipGetSymbolAddress: ") +
+                           hipGetErrorString(e));
+    }
+  }
+  if (e == hipSuccess) e = aran::workspace_init(c->na, c->ws.stream);
+  if (e == hipSuccess) {
+    e = hipHostMalloc(reinterpret_cast<void **>(&c->na.host_stage),
+                      kNoAllocStage);
+    if (e == hipSuccess) {
+      c->na.host_stage_bytes = kNoAllocStage;
+      c->na.fixed_stage = true;
+    }
+  }
   if (e != hipSuccess) {
     critical("aran_fmm3d_new: %s", aran::last_error());
     delete c;
     return nullptr;
   }
   return c;
+}
+
+// aran_fmm3d_solve on the device scratch: no hipMalloc, hipFree or other
+// device or pinned allocation (the tables of the order are rebuilt in the
+// scratch every call).
+extern "C" int aran_fmm3d_solve_noalloc(AranFmm3d *ctx, long n,
+                                        const double *x, const double *y,
+                                        const double *z, const double *q,
+                                        const double *lb, const double *ub,
+                                        int order, int leafcap, double *pot,
+                                        double *grad) {
+  if (!ctx || n < 0 || (n > 0 && (!x || !y || !z || !q)) || !lb || !ub) {
+    aran::set_error("aran_fmm3d_solve_noalloc: invalid argument");
+    return ARAN_FMM3D_EINVAL;
+  }
+  if (n > ARAN_FMM3D_NOALLOC_MAX_N || order < 1 ||
+      order > ARAN_FMM3D_NOALLOC_MAX_ORDER ||
+      (ctx->precision != ARAN_FMM3D_PLAIN &&
+       ctx->precision != ARAN_FMM3D_MIXED) ||
+      ctx->flav[0] || ctx->flav[1] || ctx->flav[2]) {
+    aran::set_error("aran_fmm3d_solve_noalloc: beyond its limits (n <= "
+                    "65536, order <= 10, plain or mixed form, rotate "
+                    "translations)");
+    return ARAN_FMM3D_EINVAL;
+  }
+  void *base = g_scratch[ctx->ws.device];
+  if (!base) {
+    aran::set_error("aran_fmm3d_solve_noalloc: no scratch on this device");
+    return ARAN_FMM3D_EINVAL;
+  }
+  aran::DevArena arena;
+  arena.base = static_cast<char *>(base);
+  arena.size = ARAN_FMM3D_NOALLOC_SCRATCH;
+  ctx->na.forget_arena_buffers();
+  aran::Problem pb;
+  pb.n = n;
+  pb.order = order;
+  pb.leafcap = leafcap;
+  for (int k = 0; k < 3; k++) {
+    pb.lb[k] = lb[k];
+    pb.ub[k] = ub[k];
+  }
+  pb.x = x;
+  pb.y = y;
+  pb.z = z;
+  pb.q = q;
+  pb.pot = pot;
+  pb.grad = grad;
+  pb.precision = ctx->precision;
+  pb.semifar = ctx->semifar;
+  aran::Stats st;
+  aran::set_arena(&arena);
+  hipError_t e = aran::solve(ctx->na, pb, &st, nullptr);
+  aran::set_arena(nullptr);
+  return code_of_err(e);
 }
 
 extern "C" void aran_fmm3d_free(AranFmm3d *ctx) { delete ctx; }

@@ -82,9 +82,19 @@ struct Stats {
 };
 
 // Grow-only device buffer.
+// While an arena is set on the calling thread (set_arena), buffers are
+// carved from it (bump allocation, never freed) instead of hipMalloc: the
+// no-allocation solve runs the engine over a fixed device scratch that way.
+struct DevArena {
+  char *base = nullptr;
+  size_t size = 0, used = 0;
+};
+void set_arena(DevArena *arena);
+
 struct DevBuf {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool in_arena = false;  // ptr points into an arena (not to be freed)
   hipError_t reserve(size_t nbytes);
   hipError_t grow_keep(size_t nbytes, size_t keep, hipStream_t s);
   void release();
@@ -145,6 +155,9 @@ struct Workspace {
   char *host_stage = nullptr;
   size_t host_stage_bytes = 0, host_stage_used = 0;
   std::vector<char *> host_stage_old;
+  // the staging block is fixed (allocated once; a solve that needs more
+  // fails instead of allocating)
+  bool fixed_stage = false;
   Tables tab;
   std::vector<Tables> cached_tabs;  // other orders, least recently used first
   hipStream_t stream = nullptr;
@@ -157,6 +170,11 @@ struct Workspace {
   bool events_ready = false;
   int device = -1;
   NodeArrays nodes() const;
+  // every device buffer of the workspace and of its tables
+  std::vector<DevBuf *> buffers();
+  // drops every buffer pointer without freeing it (the arena they came from
+  // starts over) and forgets the tables
+  void forget_arena_buffers();
   ~Workspace();
 };
 

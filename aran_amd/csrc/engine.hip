@@ -30,17 +30,51 @@ static thread_local std::string g_err;
 const char *last_error() { return g_err.c_str(); }
 void set_error(const std::string &msg) { g_err = msg; }
 
+// On failure: the expression, the HIP error and the detail a callee set
+// (a solve clears the detail when it starts).
 #define CK(expr)                                                        \
   do {                                                                  \
     hipError_t _e = (expr);                                             \
     if (_e != hipSuccess) {                                             \
-      set_error(std::string(#expr) + ": " + hipGetErrorString(_e));     \
+      const std::string _d = g_err;                                     \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(_e) +     \
+                (_d.empty() ? std::string() : " (" + _d + ")"));        \
       return _e;                                                        \
     }                                                                   \
   } while (0)
 
+static thread_local DevArena *g_arena = nullptr;
+void set_arena(DevArena *arena) { g_arena = arena; }
+
+// A region of the current arena (256-byte aligned).
+static hipError_t arena_take(size_t nbytes, void **out) {
+  const size_t off = (g_arena->used + 255) & ~(size_t)255;
+  const size_t want = std::max<size_t>(nbytes, 256);
+  if (off > g_arena->size || want > g_arena->size - off) {
+    set_error("the problem needs more working memory than the no-allocation "
+              "scratch holds");
+    return hipErrorOutOfMemory;
+  }
+  *out = g_arena->base + off;
+  g_arena->used = off + want;
+  return hipSuccess;
+}
+
 hipError_t DevBuf::reserve(size_t nbytes) {
   if (nbytes <= bytes && ptr) return hipSuccess;
+  if (g_arena) {
+    if (ptr && !in_arena) {
+      set_error("internal error: heap buffer in no-allocation mode");
+      return hipErrorUnknown;
+    }
+    void *np = nullptr;
+    hipError_t e = arena_take(nbytes, &np);
+    if (e != hipSuccess) return e;
+    ptr = np;
+    bytes = std::max<size_t>(nbytes, 256);
+    in_arena = true;
+    return hipSuccess;
+  }
   if (ptr) {
     hipError_t e = hipFree(ptr);
     ptr = nullptr;
@@ -61,7 +95,17 @@ hipError_t DevBuf::grow_keep(size_t nbytes, size_t keep, hipStream_t s) {
   if (nbytes <= bytes && ptr) return hipSuccess;
   void *np = nullptr;
   size_t want = std::max<size_t>(nbytes + nbytes / 2, 256);
-  hipError_t e = hipMalloc(&np, want);
+  hipError_t e;
+  if (g_arena) {
+    if (ptr && !in_arena) {
+      set_error("internal error: heap buffer in no-allocation mode");
+      return hipErrorUnknown;
+    }
+    want = std::max<size_t>(nbytes, 256);
+    e = arena_take(want, &np);
+  } else {
+    e = hipMalloc(&np, want);
+  }
   if (e != hipSuccess) return e;
   if (ptr && keep) {
     e = hipMemcpyAsync(np, ptr, std::min(keep, bytes), hipMemcpyDeviceToDevice,
@@ -70,16 +114,18 @@ hipError_t DevBuf::grow_keep(size_t nbytes, size_t keep, hipStream_t s) {
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
   }
-  if (ptr) (void)hipFree(ptr);
+  if (ptr && !in_arena) (void)hipFree(ptr);
   ptr = np;
   bytes = want;
+  in_arena = g_arena != nullptr;
   return hipSuccess;
 }
 
 void DevBuf::release() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr && !in_arena) (void)hipFree(ptr);
   ptr = nullptr;
   bytes = 0;
+  in_arena = false;
 }
 
 void Tables::release() {
@@ -134,8 +180,8 @@ NodeArrays Workspace::nodes() const {
   return a;
 }
 
-Workspace::~Workspace() {
-  DevBuf *all[] = {&keys_a, &keys_b, &idx_a, &idx_b, &xs, &ys, &zs, &qs,
+std::vector<DevBuf *> Workspace::buffers() {
+  std::vector<DevBuf *> all = {&keys_a, &keys_b, &idx_a, &idx_b, &xs, &ys, &zs, &qs,
                    &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
                    &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
                    &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz,
@@ -149,6 +195,35 @@ Workspace::~Workspace() {
                    &skey, &spos, &stop, &internal_list,
                    &trans_rows, &packed,
                    &lvl_table, &child_map};
+  Tables *ts[1] = {&tab};
+  for (Tables *t : ts) {
+    DevBuf *tb[] = {&t->rot,    &t->eiphi,  &t->dir,    &t->tm2l,   &t->tm2m,
+                    &t->tl2l,   &t->ynorm,  &t->ogain,  &t->pgain,  &t->igain,
+                    &t->xgain,  &t->dscr,   &t->thetas, &t->rot32,  &t->ephi32,
+                    &t->vfac32, &t->oct_codes, &t->code_to_k, &t->mfma_ops,
+                    &t->alpha,  &t->beta};
+    all.insert(all.end(), tb, tb + sizeof tb / sizeof tb[0]);
+  }
+  return all;
+}
+
+void Workspace::forget_arena_buffers() {
+  for (DevBuf *b : buffers()) {
+    b->ptr = nullptr;
+    b->bytes = 0;
+    b->in_arena = false;
+  }
+  // capacities recorded beside the buffers go with them
+  node_cap = 0;
+  far_csr_nodes = -1;
+  semifar_nodes = -1;
+  tab.order = -1;
+  tab.mfma_order = -1;
+  cached_tabs.clear();
+}
+
+Workspace::~Workspace() {
+  std::vector<DevBuf *> all = buffers();
   for (DevBuf *b : all) b->release();
   tab.release();
   for (Tables &t : cached_tabs) t.release();
@@ -189,6 +264,11 @@ hipError_t workspace_init(Workspace &ws, hipStream_t stream) {
 // Pinned host memory for one copy of this solve (see Workspace::host_stage).
 static hipError_t stage_alloc(Workspace &ws, size_t bytes, void **out) {
   bytes = (bytes + 255) & ~(size_t)255;
+  if (ws.host_stage_used + bytes > ws.host_stage_bytes && ws.fixed_stage) {
+    set_error("the problem needs more pinned staging than the no-allocation "
+              "context holds");
+    return hipErrorOutOfMemory;
+  }
   if (ws.host_stage_used + bytes > ws.host_stage_bytes) {
     if (ws.host_stage) ws.host_stage_old.push_back(ws.host_stage);
     const size_t want =
@@ -618,6 +698,7 @@ hipError_t ensure_mfma_tables(Workspace &ws, int order) {
 }
 
 hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
+  set_error("");
   const long n = pb.n;
   const int p = pb.order;
   if (p < 1 || p > kMaxOrder) {
@@ -763,7 +844,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
 
   // ---- levels (level 0: one root per system)
   long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64 + nsys);
-  if (ws.node_cap < cap0) CK(reserve_nodes(ws, cap0, 0));
+  if (ws.node_cap < cap0 || !ws.n_depth.ptr) CK(reserve_nodes(ws, cap0, 0));
   NodeArrays nd = ws.nodes();
   const ulonglong2 *d_stop = nullptr;
   if (pb.nstop > 0) {

@@ -143,6 +143,30 @@ int aran_fmm3d_direct (AranFmm3d *ctx, long n, const double *x,
                        long nt, const long *targets, double *pot,
                        double *grad);
 
+/* aran_fmm3d_solve without allocating: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same inputs,
+ * outputs and results as aran_fmm3d_solve on the same context settings, but
+ * every working buffer (tree, lists, expansions, tables of the order) comes
+ * from one device scratch array of ARAN_FMM3D_NOALLOC_SCRATCH bytes per
+ * device, set up by aran_fmm3d_new, and host staging from a pinned block
+ * allocated there too: the call makes no hipMalloc, hipFree or other device
+ * or pinned allocation.  Limits: n <= ARAN_FMM3D_NOALLOC_MAX_N, order <=
+ * ARAN_FMM3D_NOALLOC_MAX_ORDER, the plain or mixed form, rotate
+ * translations; a problem whose working memory exceeds the scratch (small
+ * leaf capacities) is refused with ARAN_FMM3D_ENOMEM.  The scratch is
+ * shared by every call on a device and keeps the last call's contents: calls
+ * that overlap on one device race.  Statistics and exports of the context
+ * keep describing its last aran_fmm3d_solve. */
+#define ARAN_FMM3D_NOALLOC_SCRATCH (128L << 20)
+#define ARAN_FMM3D_NOALLOC_MAX_N 65536
+#define ARAN_FMM3D_NOALLOC_MAX_ORDER 10
+int aran_fmm3d_solve_noalloc (AranFmm3d *ctx, long n, const double *x,
+                              const double *y, const double *z,
+                              const double *q, const double *lb,
+                              const double *ub, int order, int leafcap,
+                              double *pot, double *grad);
+
 /* Stream the context launches on (hipStream_t). */
 void *aran_fmm3d_stream (const AranFmm3d *ctx);
 
