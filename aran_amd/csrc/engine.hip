@@ -1244,7 +1244,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     };
     CK(ws.mult_bf.reserve(16 * (size_t)(nn + 1) * mfma_mult_units(p)));
     const size_t slice = sizeof(float) * std::max(1, ntiles) * 64 * D.ro;
-    CK(launch_mfma_pack_mult(nd, nn, p, M, ws.mult_bf.ptr, s));
+    CK(launch_mfma_pack_mult(nd, nn, p, pb.precision == 2 ? 2 : 3, M,
+                             ws.mult_bf.ptr, s));
     CK(ws.loc_mfma.reserve(slice * nsplit));
     int sched = mfma_default_schedule();
     if (pb.autotune && ntiles > 0)

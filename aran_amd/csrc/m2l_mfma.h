@@ -49,10 +49,12 @@ constexpr __host__ __device__ long mfma_mult_units(int p) {
 hipError_t launch_mfma_pack_ops(int p, const double *O, void *ops,
                                 hipStream_t s);
 // Multipoles of every node, scaled by w^-l, to the B-operand layout
-// [node][ks][2][part][8 x bf16], followed by one all-zero node (index
-// nnodes) that stands in for a missing source.
+// [node][ks][2][part][8 x bf16] with the nparts parts the form reads (2 for
+// three products, 3 for six), followed by one all-zero node (index nnodes)
+// that stands in for a missing source.
 hipError_t launch_mfma_pack_mult(const NodeArrays &nd, int nnodes, int p,
-                                 const double2 *M, void *mb, hipStream_t s);
+                                 int nparts, const double2 *M, void *mb,
+                                 hipStream_t s);
 // The tiled far field: nprod 3 or 6; wg_tiles[nwg][8 / (2 / col blocks)]
 // lists the tiles of each workgroup (one (level, octant) group, Morton
 // order, -1 padding); Lsm receives [split][tile][ro][64] FP32 partial

@@ -178,7 +178,7 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
 #pragma unroll
     for (int cb = 0; cb < CB; cb++) {
       const int node = (ok && srcv[cb] >= 0) ? srcv[cb] : zero_node;
-      const uint4 *pb = mb + ((size_t)node * KS + s) * 6 + g * 3;
+      const uint4 *pb = mb + ((size_t)node * KS + s) * (2 * NPART) + g * NPART;
 #pragma unroll
       for (int q = 0; q < NPART; q++) bb[cb][q] = as_bf16x8(pb[q]);
     }

@@ -462,8 +462,9 @@ __global__ void k_mfma_pack_ops(int p, const double *__restrict__ O,
 }
 
 // One thread per (node, ks, half): 8 scaled multipole entries 16 ks +
-// 8 half + j of the real input vector.
-__global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
+// 8 half + j of the real input vector, as the first nparts BF16 parts (the
+// parts the form's products read, so a source's bytes are all used).
+__global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p, int nparts,
                                  const double2 *__restrict__ M,
                                  uint4 *__restrict__ mb) {
   // issue priority over the near field's waves (second stream), which
@@ -477,9 +478,7 @@ __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
   const int s = rest / 2, g = rest % 2;
   if (node == nnodes) {  // the all-zero node that stands in for no source
     const uint4 z = make_uint4(0, 0, 0, 0);
-    mb[t * 3] = z;
-    mb[t * 3 + 1] = z;
-    mb[t * 3 + 2] = z;
+    for (int q = 0; q < nparts; q++) mb[t * nparts + q] = z;
     return;
   }
   const double iw = 1. / node_width(nd, node);
@@ -507,7 +506,9 @@ __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p,
     }
     v[j] = x;
   }
-  pack8(v, mb + t * 3);
+  uint4 parts[3];
+  pack8(v, parts);
+  for (int q = 0; q < nparts; q++) mb[t * nparts + q] = parts[q];
 }
 
 hipError_t launch_mfma_pack_ops(int p, const double *O, void *ops,
@@ -520,10 +521,12 @@ hipError_t launch_mfma_pack_ops(int p, const double *O, void *ops,
 }
 
 hipError_t launch_mfma_pack_mult(const NodeArrays &nd, int nnodes, int p,
-                                 const double2 *M, void *mb, hipStream_t s) {
+                                 int nparts, const double2 *M, void *mb,
+                                 hipStream_t s) {
+  if (nparts < 2 || nparts > 3) return hipErrorInvalidValue;
   const long total = (long)(nnodes + 1) * mfma_dims(p).ks * 2;
   k_mfma_pack_mult<<<(total + 255) / 256, 256, 0, s>>>(
-      nd, nnodes, p, M, static_cast<uint4 *>(mb));
+      nd, nnodes, p, nparts, M, static_cast<uint4 *>(mb));
   return hipGetLastError();
 }
 
