@@ -21,6 +21,7 @@ SOURCES = [("tree.hip", "tree", []), ("operators.hip", "operators", []),
            ("m2l_prep.hip", "m2l_prep", []), ("p2p.hip", "p2p", []),
            ("translate.hip", "translate", []),
            ("flavours.hip", "flavours", []),
+           ("m2l_f64.hip", "m2l_f64", []),
            ("engine.hip", "engine", []), ("capi.cpp", "capi", []),
            ("server.cpp", "server", [])] + [
     ("m2l_tile.hip", "m2l_tile_%d" % a,

@@ -358,6 +358,9 @@ extern "C" int aran_fmm3d_solve_noalloc(AranFmm3d *ctx, long n,
   pb.grad = grad;
   pb.precision = ctx->precision;
   pb.semifar = ctx->semifar;
+  // the plain form's dense FP64 operator tables would not fit the scratch:
+  // its far field runs on the per-pair rotate kernel here
+  pb.dense_far = false;
   aran::Stats st;
   aran::set_arena(&arena);
   hipError_t e = aran::solve(ctx->na, pb, &st, nullptr);
@@ -456,6 +459,14 @@ extern "C" int aran_fmm3d_get_stats(const AranFmm3d *ctx, long *o) {
                 s.l2l,    s.l2p,     (long)s.depth, s.nnodes,     s.nleaves,
                 ctx->ti.nfar, ctx->ti.nnear, s.p2l, s.m2p,        s.semifar};
   memcpy(o, v, sizeof v);
+  return 0;
+}
+
+extern "C" int aran_fmm3d_get_far_plan(const AranFmm3d *ctx, int *o) {
+  if (!ctx || !o) return ARAN_FMM3D_EINVAL;
+  o[0] = (int)ctx->st.m2l_tiles;
+  o[1] = (int)ctx->st.m2l_split;
+  o[2] = (int)ctx->st.m2l_sched;
   return 0;
 }
 

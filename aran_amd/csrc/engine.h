@@ -112,6 +112,9 @@ struct Tables {
   // first use for this order)
   DevBuf mfma_ops;
   int mfma_order = -1;
+  // plain form's far field: dense FP64 operators in the MFMA layout
+  DevBuf f64_ops;
+  int f64_order = -1;
   std::vector<DirEntry> dir_host;
   OpTables view() const;
   void release();
@@ -142,7 +145,7 @@ struct Workspace {
   std::map<long, int> tuned_sched;
   DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
-      tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma;
+      tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma, mult64, loc64;
   DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos, stop;
   long node_cap = 0;
   int depth_hint = 0;  // depth of the last tree (levels of the first batch)
@@ -210,6 +213,9 @@ struct Problem {
   // sorted by depth then key (host array)
   const unsigned long long *stop = nullptr;  // [2 nstop]: depth, key
   int nstop = 0;
+  // plain form, rotate flavour, orders <= 15: the far field as dense FP64
+  // products on MFMA (false: the per-pair rotate kernel)
+  bool dense_far = true;
 };
 
 struct TreeInfo {

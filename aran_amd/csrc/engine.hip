@@ -20,6 +20,7 @@
 
 #include "engine.h"
 #include "flavours.h"
+#include "m2l_f64.h"
 #include "m2l_mfma.h"
 #include "translate.h"
 #include "m2l_tile.h"
@@ -132,10 +133,11 @@ void Tables::release() {
   DevBuf *all[] = {&rot,    &eiphi,  &dir,       &tm2l,    &tm2m,  &tl2l,
                    &ynorm,  &ogain,  &pgain,     &igain,   &xgain, &dscr,
                    &thetas, &rot32,  &ephi32,    &vfac32,  &oct_codes,
-                   &code_to_k, &mfma_ops, &alpha, &beta};
+                   &code_to_k, &mfma_ops, &f64_ops, &alpha, &beta};
   for (DevBuf *b : all) b->release();
   order = -1;
   mfma_order = -1;
+  f64_order = -1;
   dir_host.clear();
 }
 
@@ -191,7 +193,8 @@ std::vector<DevBuf *> Workspace::buffers() {
                    &w_node, &x_cnt, &x_ofs, &x_leaf, &s_part, &mult, &loc,
                    &small, &pleaf, &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
-                   &mult_bf, &loc_mfma, &n_sys, &sys_box, &sys_off, &sys_mm,
+                   &mult_bf, &loc_mfma, &mult64, &loc64, &n_sys, &sys_box,
+                   &sys_off, &sys_mm,
                    &skey, &spos, &stop, &internal_list,
                    &trans_rows, &packed,
                    &lvl_table, &child_map};
@@ -201,7 +204,7 @@ std::vector<DevBuf *> Workspace::buffers() {
                     &t->tl2l,   &t->ynorm,  &t->ogain,  &t->pgain,  &t->igain,
                     &t->xgain,  &t->dscr,   &t->thetas, &t->rot32,  &t->ephi32,
                     &t->vfac32, &t->oct_codes, &t->code_to_k, &t->mfma_ops,
-                    &t->alpha,  &t->beta};
+                    &t->f64_ops, &t->alpha,  &t->beta};
     all.insert(all.end(), tb, tb + sizeof tb / sizeof tb[0]);
   }
   return all;
@@ -219,6 +222,7 @@ void Workspace::forget_arena_buffers() {
   semifar_nodes = -1;
   tab.order = -1;
   tab.mfma_order = -1;
+  tab.f64_order = -1;
   cached_tabs.clear();
 }
 
@@ -697,6 +701,28 @@ hipError_t ensure_mfma_tables(Workspace &ws, int order) {
   return hipSuccess;
 }
 
+hipError_t ensure_f64_tables(Workspace &ws, int order) {
+  CK(ensure_tables(ws, order));
+  Tables &T = ws.tab;
+  if (T.f64_order == order) return hipSuccess;
+  hipStream_t s = ws.stream;
+  const MfmaDims D = mfma_dims(order);
+  DevBuf tmp;
+  hipError_t e = tmp.reserve(sizeof(double) * (size_t)kNumCodes * D.ro * D.ki);
+  if (e == hipSuccess) e = launch_m2l_dense(T.view(), tmp.as<double>(), s);
+  if (e == hipSuccess)
+    e = T.f64_ops.reserve(sizeof(double) * (size_t)kNumCodes *
+                          f64_ops_doubles(order));
+  if (e == hipSuccess)
+    e = launch_f64_pack_ops(order, tmp.as<double>(), T.f64_ops.as<double>(),
+                            s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  tmp.release();
+  if (e != hipSuccess) return e;
+  T.f64_order = order;
+  return hipSuccess;
+}
+
 hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   set_error("");
   const long n = pb.n;
@@ -733,6 +759,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(workspace_init(ws, ws.stream));
   CK(ensure_tables(ws, p));
   if (pb.precision >= 2) CK(ensure_mfma_tables(ws, p));
+  // plain form: dense FP64 far field on MFMA where it applies
+  const bool dense64 = pb.precision == 0 && pb.dense_far &&
+                       pb.flav_m2l == kFlavRotate && m2l_f64_enabled(p);
+  if (dense64) CK(ensure_f64_tables(ws, p));
   hipStream_t s = ws.stream;
   Stats local_st;
   Stats &S = st ? *st : local_st;
@@ -950,10 +980,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                            ws.lstart.as<int>(), depth + 1, nn, d_lvl, s));
   CK(hipMemcpyAsync(h_lvl, d_lvl, sizeof(int) * (depth + 2),
                     hipMemcpyDeviceToHost, s));
-  // Far-field tiles (all but the plain form): 64 targets of one (level,
-  // octant) group each.  The group sizes are counted here, so that they
-  // come back with the leaf counts.
-  const bool tiled = pb.precision != 0 && depth >= 2;
+  // Far-field tiles (all forms but the plain one on its per-pair kernel):
+  // 64 targets of one (level, octant) group each.  The group sizes are
+  // counted here, so that they come back with the leaf counts.
+  const bool tiled = (pb.precision != 0 || dense64) && depth >= 2;
   const int ngroups = (kMaxDepth + 1) * 8;
   const int tfirst = tiled ? lstart[2] : 0;
   const int tcount = tiled ? (int)nn - lstart[2] : 0;
@@ -1027,7 +1057,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // the colleagues and builds this only when the lists are exported)
   int nfar = 0;
   ws.far_csr_nodes = -1;
-  if (pb.precision == 0) CK(build_far_lists(ws, nn, &nfar));
+  if (pb.precision == 0 && !tiled) CK(build_far_lists(ws, nn, &nfar));
   // ---- near list (and, with a semifar threshold, the semifar lists)
   const bool semifar = pb.semifar >= 0;
   CK(ws.u_cnt.reserve(sizeof(int) * nleaves));
@@ -1094,9 +1124,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     // workgroups: level, chunk block, octant (schedule 0: neighbouring
     // workgroups are neighbours in space) and octant, level, chunk block
     // (schedule 1: neighbouring workgroups share their operators).
-    if (pb.precision >= 2) {
+    if (pb.precision >= 2 || pb.precision == 0) {
       std::vector<int> wgt, wgt1;
-      const int tpw = mfma_wg_tiles(p);
+      const int tpw = pb.precision == 0 ? f64_wg_tiles(p) : mfma_wg_tiles(p);
       for (int d = 2; d <= depth; d++) {
         int maxch = 0;
         for (int o = 0; o < 8; o++)
@@ -1213,7 +1243,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   }
 
   // ---- far field (the M2L timer brackets the M2L kernel alone)
-  if (pb.precision == 0) {
+  if (pb.precision == 0 && !tiled) {
     if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
     if (pb.flav_m2l != kFlavRotate)
       CK(launch_flavour_m2l(tb, pb.flav_m2l, nd, nn, ws.v_ofs.as<int>(),
@@ -1256,6 +1286,43 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
     CK(launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
                            ws.loc_mfma.as<float>(), nsplit, L, s));
+    S.m2l_split = nsplit;
+    S.m2l_sched = sched;
+    S.m2l_tiles = ntiles;
+  } else if (pb.precision == 0) {
+    // plain form, dense FP64 products on MFMA (tiled levels >= 2)
+    const int first = lstart[2];
+    const int nsplit = m2l_f64_split(p);
+    const size_t wgt_len = (size_t)nwg * f64_wg_tiles(p);
+    auto run = [&](int sched) {
+      return launch_m2l_f64(p, nwg, ws.tile_perm.as<int>() + sched * wgt_len,
+                            ntiles, ws.tile_src.as<int>(),
+                            ws.tile_oct.as<int>(),
+                            ws.tab.oct_codes.as<short>(),
+                            ws.tab.f64_ops.as<double>(), ws.mult64.as<double>(),
+                            nn, nsplit, sched, ws.loc64.as<double>(), s);
+    };
+    auto run_all = [&](int sched) {
+      hipError_t e = run(sched);
+      if (e != hipSuccess) return e;
+      return launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
+                                 ws.loc64.as<double>(), nsplit, L, s);
+    };
+    CK(ws.mult64.reserve(sizeof(double) * (size_t)(nn + 1) *
+                         f64_mult_doubles(p)));
+    CK(launch_f64_pack_mult(nd, nn, p, M, ws.mult64.as<double>(), s));
+    const size_t slice = sizeof(double) * std::max(1, ntiles) * 64 *
+                         (size_t)mfma_dims(p).ro;
+    CK(ws.loc64.reserve(slice * nsplit));
+    int sched = mfma_default_schedule();
+    if (pb.autotune && ntiles > 0)
+      CK(tune_schedule(ws, tune_key(0, p, nwg), kMfmaSchedules, run_all,
+                       &sched));
+    if (pb.timings) CK(hipEventRecord(ws.ev[3], s));
+    if (ntiles > 0) CK(run(sched));
+    if (pb.timings) CK(hipEventRecord(ws.ev[4], s));
+    CK(launch_unscale_mfma(nd, p, first, ntiles, ws.tile_tgt.as<int>(),
+                           ws.loc64.as<double>(), nsplit, L, s));
     S.m2l_split = nsplit;
     S.m2l_sched = sched;
     S.m2l_tiles = ntiles;
@@ -1342,10 +1409,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (semifar)
     CK(hipMemcpyAsync(hsp, d_spart, sizeof(long long), hipMemcpyDeviceToHost,
                       s));
-  if (pb.precision != 0 && depth >= 2)
+  if (tiled)
     CK(hipMemcpyAsync(hfar, d_far, sizeof(int), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
-  if (pb.precision != 0) nfar = depth >= 2 ? *hfar : 0;
+  if (pb.precision != 0 || tiled) nfar = tiled ? *hfar : 0;
   S.zero = nn;
   S.p2p = (long)(*hp / 2);
   S.p2m = n;

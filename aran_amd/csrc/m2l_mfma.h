@@ -75,6 +75,11 @@ hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
                                int ntiles, const int *tile_tgt,
                                const float *Lsm, int nsplit, double2 *L,
                                hipStream_t s);
+// The same from FP64 slices (the plain form's dense far field).
+hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
+                               int ntiles, const int *tile_tgt,
+                               const double *Lsm, int nsplit, double2 *L,
+                               hipStream_t s);
 // Offset ranges of the MFMA far field (fixed per order: a target's sums do
 // not depend on the rest of the solve).
 constexpr int kFarRangesMfma = 6;

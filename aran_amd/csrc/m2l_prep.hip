@@ -535,9 +535,10 @@ hipError_t launch_mfma_pack_mult(const NodeArrays &nd, int nnodes, int p,
 // in slice order, transpose through LDS and write L[node][k] =
 // (Re row k, Im row of (l, m >= 1)) / w^(l+1), kUnscaleRows coefficients at
 // a time.
+template <class T>
 __global__ __launch_bounds__(256) void k_unscale_mfma(
     NodeArrays nd, int p, int ntiles, const int *__restrict__ tile_tgt,
-    const float *__restrict__ Lsm, int nsplit, double2 *__restrict__ L) {
+    const T *__restrict__ Lsm, int nsplit, double2 *__restrict__ L) {
   __shared__ double2 sl[64 * (kUnscaleRows + 1)];
   __shared__ double spw[64][kMaxOrderFast + 2];  // 1 / w^(l+1)
   __shared__ int snode[64];
@@ -548,7 +549,7 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
   const int row = kUnscaleRows + 1;  // odd row stride in LDS
   const int tile = blockIdx.x;
   const long slice = (long)ntiles * D.ro * 64;
-  const float *src = Lsm + (long)tile * D.ro * 64;
+  const T *src = Lsm + (long)tile * D.ro * 64;
   if (threadIdx.x < 64) {
     const int node = tile_tgt[tile * 64 + threadIdx.x];
     snode[threadIdx.x] = node;
@@ -595,17 +596,32 @@ __global__ __launch_bounds__(256) void k_unscale_mfma(
   }
 }
 
-hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
-                               int ntiles, const int *tile_tgt,
-                               const float *Lsm, int nsplit, double2 *L,
-                               hipStream_t s) {
+template <class T>
+static hipError_t unscale_slices(const NodeArrays &nd, int p, int first,
+                                 int ntiles, const int *tile_tgt, const T *Lsm,
+                                 int nsplit, double2 *L, hipStream_t s) {
   const int lc = tri_p(p + 1);
   const long nz = (long)first * lc;
   if (nz > 0) k_zero_double2<<<(nz + 255) / 256, 256, 0, s>>>(L, nz);
   if (ntiles <= 0) return hipGetLastError();
   const dim3 grid(ntiles, (lc + kUnscaleRows - 1) / kUnscaleRows);
-  k_unscale_mfma<<<grid, 256, 0, s>>>(nd, p, ntiles, tile_tgt, Lsm, nsplit, L);
+  k_unscale_mfma<T><<<grid, 256, 0, s>>>(nd, p, ntiles, tile_tgt, Lsm, nsplit,
+                                         L);
   return hipGetLastError();
+}
+
+hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
+                               int ntiles, const int *tile_tgt,
+                               const float *Lsm, int nsplit, double2 *L,
+                               hipStream_t s) {
+  return unscale_slices(nd, p, first, ntiles, tile_tgt, Lsm, nsplit, L, s);
+}
+
+hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
+                               int ntiles, const int *tile_tgt,
+                               const double *Lsm, int nsplit, double2 *L,
+                               hipStream_t s) {
+  return unscale_slices(nd, p, first, ntiles, tile_tgt, Lsm, nsplit, L, s);
 }
 
 // Offset ranges per tile of the MFMA far field: fixed per order, as for the

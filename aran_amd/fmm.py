@@ -137,6 +137,13 @@ class Fmm3d:
         self.n = n
         return pot, grad
 
+    def far_plan(self):
+        """Far field of the last solve: tiles (0: the per-pair kernel ran),
+        offset ranges per tile and workgroup schedule."""
+        v = (C.c_int * 3)()
+        check(lib().aran_fmm3d_get_far_plan(self._ctx, v), "get_far_plan")
+        return {"tiles": v[0], "ranges": v[1], "schedule": v[2]}
+
     def stats(self):
         v = (C.c_long * len(STAT_NAMES))()
         check(lib().aran_fmm3d_get_stats(self._ctx, v), "get_stats")

@@ -98,6 +98,12 @@ int aran_fmm3d_export_expansions (const AranFmm3d *ctx, double *multipoles,
  * reference's counters) and the number of semifar (leaf, node) pairs. */
 int aran_fmm3d_get_stats (const AranFmm3d *ctx, long *out15);
 
+/* Far-field plan of the last solve: out3 = {tiles of 64 targets (0: the
+ * per-pair kernel ran), offset ranges per tile, workgroup schedule}.  No
+ * counterpart in the reference (its far field has no tiles); for callers
+ * that check which far-field kernel ran. */
+int aran_fmm3d_get_far_plan (const AranFmm3d *ctx, int *out3);
+
 /* Semifar pairs of the next solves (aran/aransolver3d.c:357-396): when the
  * near/far recursion pairs a leaf with a deeper node whose box does not
  * touch the leaf's and which holds more than `threshold` particles, the

@@ -2,7 +2,11 @@
 aran_fmm3d_solve on the same context settings, through the C-ABI: bitwise
 equal potentials and gradients on two calls in a row, the second with a
 smaller input (the scratch keeps the first call's contents), in the plain
-and mixed forms; inputs beyond its limits are refused with an error."""
+and mixed forms; inputs beyond its limits are refused with an error.  The
+no-allocation solve runs the plain form's far field on the per-pair rotate
+kernel (the dense FP64 operator tables would not fit its scratch), so the
+allocating solve it is compared with is set to that kernel too
+(ARAN_PLAIN_FAR=rotate)."""
 import ctypes as C
 
 import numpy as np
@@ -25,9 +29,10 @@ def _noalloc(f, dev, lb, ub, p, q, pot, grad):
 
 
 @pytest.mark.parametrize("form", ["plain", "mixed"])
-def test_noalloc_equals_allocating_solve(torch_cuda, form):
+def test_noalloc_equals_allocating_solve(torch_cuda, form, monkeypatch):
     from aran_amd.fmm import Fmm3d
     torch = torch_cuda
+    monkeypatch.setenv("ARAN_PLAIN_FAR", "rotate")
     f = Fmm3d()
     f.set_precision(form)
     for n, p, q, kind in ((20_000, 10, 32, "plummer"), (5000, 8, 20,
@@ -46,10 +51,11 @@ def test_noalloc_equals_allocating_solve(torch_cuda, form):
         assert torch.equal(grad, ref_g), (form, n)
 
 
-def test_noalloc_refuses_beyond_limits(torch_cuda):
+def test_noalloc_refuses_beyond_limits(torch_cuda, monkeypatch):
     from aran_amd._lib import lib
     from aran_amd.fmm import Fmm3d
     torch = torch_cuda
+    monkeypatch.setenv("ARAN_PLAIN_FAR", "rotate")
     f = Fmm3d()
     n = 70_000
     x, y, z, m = W.uniform_cube(n, 5)
