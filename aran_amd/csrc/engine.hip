@@ -1277,7 +1277,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(launch_mfma_pack_mult(nd, nn, p, pb.precision == 2 ? 2 : 3, M,
                              ws.mult_bf.ptr, s));
     CK(ws.loc_mfma.reserve(slice * nsplit));
-    int sched = mfma_default_schedule();
+    int sched = mfma_default_schedule(0);
     if (pb.autotune && ntiles > 0)
       CK(tune_schedule(ws, tune_key(pb.precision, p, nwg), kMfmaSchedules,
                        run_all, &sched));
@@ -1314,7 +1314,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     const size_t slice = sizeof(double) * std::max(1, ntiles) * 64 *
                          (size_t)mfma_dims(p).ro;
     CK(ws.loc64.reserve(slice * nsplit));
-    int sched = mfma_default_schedule();
+    int sched = mfma_default_schedule(1);
     if (pb.autotune && ntiles > 0)
       CK(tune_schedule(ws, tune_key(0, p, nwg), kMfmaSchedules, run_all,
                        &sched));

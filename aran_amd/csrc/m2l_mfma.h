@@ -81,13 +81,18 @@ hipError_t launch_unscale_mfma(const NodeArrays &nd, int p, int first,
                                const double *Lsm, int nsplit, double2 *L,
                                hipStream_t s);
 // Offset ranges of the MFMA far field (fixed per order: a target's sums do
-// not depend on the rest of the solve).
-constexpr int kFarRangesMfma = 6;
+// not depend on the rest of the solve).  A workgroup takes one range of 8
+// tiles and holds its CU, so the count sets how evenly the work fills the
+// 256 CUs; 5 measured fastest at N = 2^20 (Plummer, p = 10: 690 workgroups,
+// 2.7 rounds, against 3.2 for 6) for both BF16 forms.
+constexpr int kFarRangesMfma = 5;
 int m2l_mfma_split(int p);
 // Workgroup schedules of the MFMA far field (same sums, different order of
 // work over the XCDs): 0 space-major, 1 offset-range-major.
 constexpr int kMfmaSchedules = 2;
-// Schedule used without the autotuner (ARAN_MFMA_SCHED overrides it).
-int mfma_default_schedule();
+// Schedule used without the autotuner: `fallback` unless ARAN_MFMA_SCHED
+// overrides it (BF16 forms: 0, measured best; FP64: 1, its operators are
+// four times the BF16 bytes and gain most from sharing an XCD's L2).
+int mfma_default_schedule(int fallback);
 
 }  // namespace aran

@@ -636,12 +636,12 @@ int m2l_mfma_split(int p) {
   return kFarRangesMfma;
 }
 
-int mfma_default_schedule() {
+int mfma_default_schedule(int fallback) {
   if (const char *e = getenv("ARAN_MFMA_SCHED")) {
     const int v = atoi(e);
     if (v >= 0 && v < kMfmaSchedules) return v;
   }
-  return 0;
+  return fallback;
 }
 
 // ------------------------------------------------------------ tables
