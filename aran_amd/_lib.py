@@ -38,6 +38,8 @@ def _declare(L):
     L.aran_fmm3d_export_tree.argtypes = [vp, vp, vp, vp, vp, vp]
     L.aran_fmm3d_export_lists.restype = ip
     L.aran_fmm3d_export_lists.argtypes = [vp, vp, vp, vp, vp, vp, vp, vp]
+    L.aran_fmm3d_get_kernel_timings.restype = ip
+    L.aran_fmm3d_get_kernel_timings.argtypes = [vp, C.POINTER(C.c_float)]
     L.aran_fmm3d_get_far_plan.restype = ip
     L.aran_fmm3d_get_far_plan.argtypes = [vp, C.POINTER(C.c_int)]
     L.aran_fmm3d_set_autotune.restype = ip

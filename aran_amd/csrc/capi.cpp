@@ -462,6 +462,15 @@ extern "C" int aran_fmm3d_get_stats(const AranFmm3d *ctx, long *o) {
   return 0;
 }
 
+extern "C" int aran_fmm3d_get_kernel_timings(const AranFmm3d *ctx,
+                                             float *o) {
+  if (!ctx || !o) return ARAN_FMM3D_EINVAL;
+  const aran::Stats &s = ctx->st;
+  float v[4] = {s.t_p2m, s.t_m2m, s.t_l2l, s.t_l2p};
+  memcpy(o, v, sizeof v);
+  return 0;
+}
+
 extern "C" int aran_fmm3d_get_far_plan(const AranFmm3d *ctx, int *o) {
   if (!ctx || !o) return ARAN_FMM3D_EINVAL;
   o[0] = (int)ctx->st.m2l_tiles;

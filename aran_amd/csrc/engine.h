@@ -79,6 +79,9 @@ struct Stats {
   // phase times in ms (only with Problem::timings)
   float t_tree = 0, t_lists = 0, t_up = 0, t_m2l = 0, t_down = 0, t_near = 0,
         t_total = 0;
+  // kernel groups inside the phases: P2M, the M2M levels, the L2L levels,
+  // M2P + L2P (after the near field has joined)
+  float t_p2m = 0, t_m2m = 0, t_l2l = 0, t_l2p = 0;
 };
 
 // Grow-only device buffer.
@@ -170,6 +173,7 @@ struct Workspace {
   hipStream_t aux = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_near = nullptr;
   hipEvent_t ev[8] = {};
+  hipEvent_t evk[8] = {};  // kernel-group timers (Stats::t_p2m ...)
   bool events_ready = false;
   int device = -1;
   NodeArrays nodes() const;

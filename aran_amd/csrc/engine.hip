@@ -236,6 +236,7 @@ Workspace::~Workspace() {
   for (char *b : host_stage_old) (void)hipHostFree(b);
   if (events_ready) {
     for (auto &e : ev) (void)hipEventDestroy(e);
+    for (auto &e : evk) (void)hipEventDestroy(e);
     (void)hipEventDestroy(ev_fork);
     (void)hipEventDestroy(ev_join);
     (void)hipEventDestroy(ev_near);
@@ -255,6 +256,7 @@ hipError_t workspace_init(Workspace &ws, hipStream_t stream) {
   }
   if (!ws.events_ready) {
     for (auto &e : ws.ev) CK(hipEventCreate(&e));
+    for (auto &e : ws.evk) CK(hipEventCreate(&e));
     CK(hipEventCreateWithFlags(&ws.ev_fork, hipEventDisableTiming));
     CK(hipEventCreateWithFlags(&ws.ev_join, hipEventDisableTiming));
     CK(hipEventCreate(&ws.ev_near));
@@ -1232,8 +1234,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   double2 *M = ws.mult.as<double2>();
   double2 *L = ws.loc.as<double2>();
   OpTables tb = ws.tab.view();
+  if (pb.timings) CK(hipEventRecord(ws.evk[0], s));
   CK(launch_p2m(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
                 ws.zs.as<double>(), ws.qs.as<double>(), M, s));
+  if (pb.timings) CK(hipEventRecord(ws.evk[1], s));
   for (int d = depth - 1; d >= 0; d--) {
     if (pb.flav_m2m != kFlavRotate)
       CK(launch_flavour_level(tb, kFlavM2M, pb.flav_m2m, nd, trows + ibase[d],
@@ -1241,6 +1245,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     else
       CK(launch_m2m_level(tb, trows + ibase[d], icount[d], M, s));
   }
+  if (pb.timings) CK(hipEventRecord(ws.evk[2], s));
 
   // ---- far field (the M2L timer brackets the M2L kernel alone)
   if (pb.precision == 0 && !tiled) {
@@ -1375,6 +1380,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // ---- downward pass
   // locals of levels 0 and 1 receive nothing without semifar pairs (far
   // interactions start at level 2), so their translations would add zeros
+  if (pb.timings) CK(hipEventRecord(ws.evk[3], s));
   for (int d = semifar ? 1 : 3; d <= depth; d++) {
     if (pb.flav_l2l != kFlavRotate)
       CK(launch_flavour_level(tb, kFlavL2L, pb.flav_l2l, nd,
@@ -1385,6 +1391,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (pb.timings) CK(hipEventRecord(ws.ev[5], s));
   // ---- join the near field, then M2P and L2P add to its results
   CK(hipStreamWaitEvent(s, ws.ev_join, 0));
+  if (pb.timings) CK(hipEventRecord(ws.evk[4], s));
   if (semifar && nsemi > 0)
     CK(launch_m2p(tb, nd, leaves, nleaves, ws.w_ofs.as<int>(),
                   ws.w_node.as<int>(), ws.xs.as<double>(), ws.ys.as<double>(),
@@ -1394,6 +1401,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                 ws.ys.as<double>(),
                 ws.zs.as<double>(), L, ws.pot_s.as<double>(),
                 ws.grad_s.as<double>(), s));
+  if (pb.timings) CK(hipEventRecord(ws.evk[5], s));
   CK(launch_scatter(ws.idx_a.as<int>(), n, ws.pot_s.as<double>(),
                     ws.grad_s.as<double>(), pb.pot, pb.grad, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[6], s));
@@ -1437,6 +1445,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     // and down, so the phases do not add up to the total)
     CK(hipEventElapsedTime(&S.t_near, ws.ev[7], ws.ev_near));
     CK(hipEventElapsedTime(&S.t_total, ws.ev[0], ws.ev[6]));
+    CK(hipEventElapsedTime(&S.t_p2m, ws.evk[0], ws.evk[1]));
+    CK(hipEventElapsedTime(&S.t_m2m, ws.evk[1], ws.evk[2]));
+    CK(hipEventElapsedTime(&S.t_l2l, ws.evk[3], ws.ev[5]));
+    CK(hipEventElapsedTime(&S.t_l2p, ws.evk[4], ws.evk[5]));
   }
   if (ti) {
     for (int k = 0; k < 3; k++) ti->lo[k] = box[k];

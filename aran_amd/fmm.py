@@ -14,6 +14,7 @@ STAT_NAMES = ("zero", "p2p", "p2m", "m2m", "m2l", "l2l", "l2p", "depth",
               "nodes", "leaves", "far_entries", "near_entries", "p2l", "m2p",
               "semifar_pairs")
 TIMER_NAMES = ("tree", "lists", "up", "m2l", "down", "near", "total")
+KERNEL_TIMER_NAMES = ("p2m", "m2m", "l2l", "l2p")
 
 
 def _ptr(t):
@@ -150,9 +151,17 @@ class Fmm3d:
         return dict(zip(STAT_NAMES, list(v)))
 
     def timings(self):
+        """Phase times of the last solve in ms (tree, lists, up, m2l, down,
+        near, total) and the kernel groups inside them (p2m, m2m, l2l,
+        l2p)."""
         v = (C.c_float * 7)()
         check(lib().aran_fmm3d_get_timings(self._ctx, v), "get_timings")
-        return dict(zip(TIMER_NAMES, list(v)))
+        k = (C.c_float * 4)()
+        check(lib().aran_fmm3d_get_kernel_timings(self._ctx, k),
+              "get_kernel_timings")
+        out = dict(zip(TIMER_NAMES, list(v)))
+        out.update(zip(KERNEL_TIMER_NAMES, list(k)))
+        return out
 
     def export_tree(self):
         st = self.stats()

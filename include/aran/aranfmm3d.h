@@ -127,6 +127,10 @@ int aran_fmm3d_export_semifar (const AranFmm3d *ctx, int *w_ofs, int *w_node,
  * tree, lists, up, m2l, down, near, total in ms. */
 int aran_fmm3d_set_timings (AranFmm3d *ctx, int on);
 int aran_fmm3d_get_timings (const AranFmm3d *ctx, float *out7);
+/* Kernel groups inside those phases, in ms: P2M, the M2M levels, the L2L
+ * levels, M2P + L2P.  No counterpart in the reference; for per-operator
+ * rates. */
+int aran_fmm3d_get_kernel_timings (const AranFmm3d *ctx, float *out4);
 
 /* Tree of the last solve, copied to host arrays sized from get_stats:
  * perm[n] (sorted position -> input index), nodes[nnodes*10] with
