@@ -8,8 +8,10 @@
 // footprint of everything in flight stays under the budget, and solves them
 // together as independent systems.  With two workers one batch's copies
 // and kernels overlap another's.  A system's result does not depend on the
-// batch it ran in (the engine gives every system its own octree and
-// accumulates in a fixed order), so served results equal standalone ones.
+// batch it ran in: the engine gives every system its own octree, and every
+// form sums a target's far field over offset ranges fixed per order, each
+// in a fixed order, then adds the ranges in range order.  Served results
+// therefore equal standalone solves bit for bit in every form.
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <string.h>
@@ -35,18 +37,53 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 
+// A request's host array (left uninitialised on allocation: every element
+// is written by a copy before it is read).
+struct HostArr {
+  std::unique_ptr<double[]> p;
+  void alloc(long k) { p.reset(k > 0 ? new double[k] : nullptr); }
+  void release() { p.reset(); }
+  double *data() const { return p.get(); }
+};
+
 struct Request {
   long id = 0;
   long n = 0;
   int order = 8, leafcap = 32, want_grad = 1;
   double lb[3], ub[3];
-  std::vector<double> x, y, z, q, pot, grad;
+  HostArr x, y, z, q, pot, grad;
   double bytes = 0.;
   Clock::time_point t_submit, t_start, t_done;
   bool done = false;
   bool claimed = false;  // a waiter holds this request
   int status = 0;
 };
+
+// Host copies of request data run on several threads when they are large
+// (one thread moves well under the host's memory bandwidth, and the copies
+// in and out of the pinned staging otherwise bound the served rate).
+static void par_copy(void *dst, const void *src, size_t bytes) {
+  const size_t kChunk = size_t(4) << 20;
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const size_t nt = std::min<size_t>(std::min(8u, hw), bytes / kChunk);
+  if (nt < 2) {
+    if (bytes) memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = (bytes / nt + 63) / 64 * 64;
+  std::vector<std::thread> th;
+  for (size_t t = 1; t < nt; t++) {
+    const size_t b = t * part;
+    if (b >= bytes) break;
+    const size_t e = std::min(bytes, b + part);
+    th.emplace_back([=] {
+      memcpy(static_cast<char *>(dst) + b, static_cast<const char *>(src) + b,
+             e - b);
+    });
+  }
+  memcpy(dst, src, std::min(bytes, part));
+  for (auto &t : th) t.join();
+}
 
 struct PinnedBuf {
   void *ptr = nullptr;
@@ -173,10 +210,11 @@ static void worker_loop(AranServer3d *S, Worker *W) {
     if (rc == 0 && n > 0) {
       for (int k = 0; k < nsys; k++) {
         const Request *r = batch[k];
-        memcpy(hin + off[k], r->x.data(), sizeof(double) * r->n);
-        memcpy(hin + n + off[k], r->y.data(), sizeof(double) * r->n);
-        memcpy(hin + 2 * n + off[k], r->z.data(), sizeof(double) * r->n);
-        memcpy(hin + 3 * n + off[k], r->q.data(), sizeof(double) * r->n);
+        const size_t nb = sizeof(double) * r->n;
+        par_copy(hin + off[k], r->x.data(), nb);
+        par_copy(hin + n + off[k], r->y.data(), nb);
+        par_copy(hin + 2 * n + off[k], r->z.data(), nb);
+        par_copy(hin + 3 * n + off[k], r->q.data(), nb);
       }
       if (hipMemcpyAsync(din, hin, 4 * sizeof(double) * n,
                          hipMemcpyHostToDevice, st) != hipSuccess)
@@ -200,18 +238,18 @@ static void worker_loop(AranServer3d *S, Worker *W) {
         Request *r = batch[k];
         r->status = rc;
         if (rc == 0) {
-          r->pot.assign(hout + off[k], hout + off[k] + r->n);
-          if (r->want_grad)
-            r->grad.assign(hout + n + 3 * off[k], hout + n + 3 * off[k + 1]);
+          r->pot.alloc(r->n);
+          par_copy(r->pot.data(), hout + off[k], sizeof(double) * r->n);
+          if (r->want_grad) {
+            r->grad.alloc(3 * r->n);
+            par_copy(r->grad.data(), hout + n + 3 * off[k],
+                     sizeof(double) * 3 * r->n);
+          }
         }
-        r->x.clear();
-        r->x.shrink_to_fit();
-        r->y.clear();
-        r->y.shrink_to_fit();
-        r->z.clear();
-        r->z.shrink_to_fit();
-        r->q.clear();
-        r->q.shrink_to_fit();
+        r->x.release();
+        r->y.release();
+        r->z.release();
+        r->q.release();
         r->t_start = t0;
         r->t_done = t1;
         r->done = true;
@@ -300,10 +338,15 @@ extern "C" long aran_server3d_submit(AranServer3d *S, long n, const double *x,
     r->lb[a] = lb[a];
     r->ub[a] = ub[a];
   }
-  r->x.assign(x, x + n);
-  r->y.assign(y, y + n);
-  r->z.assign(z, z + n);
-  r->q.assign(q, q + n);
+  r->x.alloc(n);
+  r->y.alloc(n);
+  r->z.alloc(n);
+  r->q.alloc(n);
+  const size_t nb = sizeof(double) * n;
+  par_copy(r->x.data(), x, nb);
+  par_copy(r->y.data(), y, nb);
+  par_copy(r->z.data(), z, nb);
+  par_copy(r->q.data(), q, nb);
   r->bytes = bytes;
   r->t_submit = Clock::now();
   long id;
@@ -347,9 +390,9 @@ extern "C" int aran_server3d_wait(AranServer3d *S, long id, double *pot,
     S->all.erase(again);
   }
   if (r->status != 0) return r->status;
-  if (pot) memcpy(pot, r->pot.data(), sizeof(double) * r->n);
+  if (pot) par_copy(pot, r->pot.data(), sizeof(double) * r->n);
   if (grad && r->want_grad)
-    memcpy(grad, r->grad.data(), sizeof(double) * 3 * r->n);
+    par_copy(grad, r->grad.data(), sizeof(double) * 3 * r->n);
   if (times3) {
     using sec = std::chrono::duration<double>;
     times3[0] = sec(r->t_done - r->t_submit).count();       // latency
