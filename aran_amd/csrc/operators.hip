@@ -245,15 +245,36 @@ __global__ __launch_bounds__(64) void k_p2m(
   const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
   const int s = lane / P, m = lane % P;
   double ar[P], ai[P];
+  // factors 1 / (l^2 - m^2) of the degree recurrence: m is fixed per lane,
+  // so they are read once, not per particle (a per-lane table read inside
+  // the recurrence is a dependent memory round trip per degree)
+  double gfac[P];
   static_loop<0, P>([&](auto lc) {
-    ar[decltype(lc)::value] = 0.;
-    ai[decltype(lc)::value] = 0.;
+    constexpr int l = decltype(lc)::value;
+    ar[l] = 0.;
+    ai[l] = 0.;
+    gfac[l] = kInv.v[l > m ? l * l - m * m : 1];
   });
   if (s < S) {
-    for (int i = b + s; i < e; i += S) {
-      const double vx = xs[i] - cx, vy = ys[i] - cy, vz = zs[i] - cz;
+    // the next particle's coordinates are loaded while this one is summed
+    int i = b + s;
+    double nx_ = 0., ny_ = 0., nz_ = 0., nq_ = 0.;
+    if (i < e) {
+      nx_ = xs[i];
+      ny_ = ys[i];
+      nz_ = zs[i];
+      nq_ = qs[i];
+    }
+    for (; i < e; i += S) {
+      const double vx = nx_ - cx, vy = ny_ - cy, vz = nz_ - cz, qi = nq_;
+      if (i + S < e) {
+        nx_ = xs[i + S];
+        ny_ = ys[i + S];
+        nz_ = zs[i + S];
+        nq_ = qs[i + S];
+      }
       const double r2 = vx * vx + vy * vy + vz * vz;
-      double ox = qs[i], oy = 0.;  // q O_m^m
+      double ox = qi, oy = 0.;  // q O_m^m
       static_loop<1, P>([&](auto kc) {
         constexpr int k = decltype(kc)::value;
         if (k <= m) {
@@ -274,7 +295,7 @@ __global__ __launch_bounds__(64) void k_p2m(
           ai[l] += oy;
         } else if (l > m) {
           const double a = (2. * l - 1.) * vz;
-          const double g = kInv.v[l * l - m * m];
+          const double g = gfac[l];
           const double nx = (a * px - r2 * qx) * g,
                        ny = (a * py - r2 * qy) * g;
           qx = px;
