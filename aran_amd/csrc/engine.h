@@ -323,21 +323,26 @@ hipError_t launch_xlist(const NodeArrays &nd, int nnodes, const int *col,
 // ---- operators.hip
 hipError_t launch_wigner_tables(const double *thetas, int ntheta, int L,
                                 double *dscr, double *rot, hipStream_t s);
+// low: the FP32 recurrence on width-scaled coordinates (the reduced-
+// precision forms, for leaves below the top two levels); FP64 otherwise
 hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
                       const int *leaves, int nleaves, const double *xs,
                       const double *ys, const double *zs, const double *qs,
-                      double2 *M, hipStream_t s);
+                      double2 *M, hipStream_t s, bool low = false);
 hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
                       const int *vofs, const int *vsrc, const short *vcode,
                       const double2 *M, double2 *L, hipStream_t s);
 // pleaf[j] = leaf node of sorted particle j (every particle of the leaves)
 hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
                                 int nleaves, int *pleaf, hipStream_t s);
-// L2P of sorted particles [0, n), adding into pot / grad
+// L2P of sorted particles [0, n), adding into pot / grad.  sel 0: every
+// particle, FP64; 1: the particles of leaves at depth >= 2, FP32 recurrence
+// on width-scaled coordinates (the reduced-precision forms); 2: those of
+// leaves at depth < 2, FP64
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
-                      double *pot, double *grad, hipStream_t s);
+                      double *pot, double *grad, hipStream_t s, int sel = 0);
 
 // M2P: for every leaf, the multipoles of its semifar nodes evaluated at its
 // sorted particles, added into pot / grad (potential and gradient).

@@ -1235,8 +1235,24 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   double2 *L = ws.loc.as<double2>();
   OpTables tb = ws.tab.view();
   if (pb.timings) CK(hipEventRecord(ws.evk[0], s));
-  CK(launch_p2m(tb, nd, leaves, nleaves, ws.xs.as<double>(), ws.ys.as<double>(),
-                ws.zs.as<double>(), ws.qs.as<double>(), M, s));
+  // reduced-precision forms: FP32 P2M and L2P for the leaves below the top
+  // two levels (the leaf list is in node order, so the leaves of levels 0
+  // and 1 come first), FP64 for those; the plain form FP64 throughout
+  int top = nleaves;
+  if (pb.precision != 0) {
+    top = 0;
+    for (int d = 0; d < 2 && d <= depth; d++) top += lcount[d] - icount[d];
+  }
+  {
+    if (top > 0)
+      CK(launch_p2m(tb, nd, leaves, top, ws.xs.as<double>(),
+                    ws.ys.as<double>(), ws.zs.as<double>(), ws.qs.as<double>(),
+                    M, s, false));
+    if (top < nleaves)
+      CK(launch_p2m(tb, nd, leaves + top, nleaves - top, ws.xs.as<double>(),
+                    ws.ys.as<double>(), ws.zs.as<double>(), ws.qs.as<double>(),
+                    M, s, true));
+  }
   if (pb.timings) CK(hipEventRecord(ws.evk[1], s));
   for (int d = depth - 1; d >= 0; d--) {
     if (pb.flav_m2m != kFlavRotate)
@@ -1397,10 +1413,19 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                   ws.w_node.as<int>(), ws.xs.as<double>(), ws.ys.as<double>(),
                   ws.zs.as<double>(), M, ws.pot_s.as<double>(),
                   ws.grad_s.as<double>(), s));
-  CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
-                ws.ys.as<double>(),
-                ws.zs.as<double>(), L, ws.pot_s.as<double>(),
-                ws.grad_s.as<double>(), s));
+  if (pb.precision == 0) {
+    CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
+                  ws.ys.as<double>(), ws.zs.as<double>(), L,
+                  ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 0));
+  } else {
+    CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
+                  ws.ys.as<double>(), ws.zs.as<double>(), L,
+                  ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 1));
+    if (top > 0)
+      CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
+                    ws.ys.as<double>(), ws.zs.as<double>(), L,
+                    ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 2));
+  }
   if (pb.timings) CK(hipEventRecord(ws.evk[5], s));
   CK(launch_scatter(ws.idx_a.as<int>(), n, ws.pot_s.as<double>(),
                     ws.grad_s.as<double>(), pb.pot, pb.grad, s));

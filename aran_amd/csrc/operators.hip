@@ -228,7 +228,104 @@ __device__ __forceinline__ void static_loop(F &&f) {
 // access sits in the recurrence (the reciprocals come from a constant
 // table), and the LDS is small enough that several leaves share a CU with
 // the near field.
-template <int P>
+//
+// R = double: the plain form.  R = float (the reduced-precision forms, for
+// leaves below the top two levels): the recurrence runs on coordinates
+// scaled by the leaf's width, u = (x - c) / w (|u| <= sqrt(3)/2, so the
+// harmonics stay in FP32 range at every order), and each slot's sums are
+// scaled back by w^l in FP64; FP32 issues no faster than FP64 on this VALU,
+// but the lane needs half the registers and the near field beside it gets
+// the issue slots (measured 0.22 -> 0.12 ms at N = 2^20, p = 10).
+template <int P, class R>
+__device__ __forceinline__ void p2m_slots(
+    int b, int e, int s, int m, double cx, double cy, double cz, double w,
+    const double *__restrict__ xs, const double *__restrict__ ys,
+    const double *__restrict__ zs, const double *__restrict__ qs,
+    double2 *__restrict__ Ts) {
+  constexpr bool kScaled = !std::is_same<R, double>::value;
+  constexpr int S = 64 / P;
+  const double iw = kScaled ? 1. / w : 1.;
+  R ar[P], ai[P];
+  // factors 1 / (l^2 - m^2) of the degree recurrence: m is fixed per lane,
+  // so they are read once, not per particle (a per-lane table read inside
+  // the recurrence is a dependent memory round trip per degree)
+  R gfac[P];
+  static_loop<0, P>([&](auto lc) {
+    constexpr int l = decltype(lc)::value;
+    ar[l] = R(0);
+    ai[l] = R(0);
+    gfac[l] = (R)kInv.v[l > m ? l * l - m * m : 1];
+  });
+  // the next particle's coordinates are loaded while this one is summed
+  int i = b + s;
+  double nx_ = 0., ny_ = 0., nz_ = 0., nq_ = 0.;
+  if (i < e) {
+    nx_ = xs[i];
+    ny_ = ys[i];
+    nz_ = zs[i];
+    nq_ = qs[i];
+  }
+  for (; i < e; i += S) {
+    R vx, vy, vz;
+    if constexpr (kScaled) {
+      vx = (R)((nx_ - cx) * iw);
+      vy = (R)((ny_ - cy) * iw);
+      vz = (R)((nz_ - cz) * iw);
+    } else {
+      vx = nx_ - cx;
+      vy = ny_ - cy;
+      vz = nz_ - cz;
+    }
+    const R qi = (R)nq_;
+    if (i + S < e) {
+      nx_ = xs[i + S];
+      ny_ = ys[i + S];
+      nz_ = zs[i + S];
+      nq_ = qs[i + S];
+    }
+    const R r2 = vx * vx + vy * vy + vz * vz;
+    R ox = qi, oy = R(0);  // q O_m^m
+    static_loop<1, P>([&](auto kc) {
+      constexpr int k = decltype(kc)::value;
+      if (k <= m) {
+        const R f = (R)(-0.5 * kInv.v[k]);
+        const R ax = f * vx, ay = f * vy;
+        const R nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
+        ox = nx;
+        oy = ny;
+      }
+    });
+    R px = R(0), py = R(0), qx = R(0), qy = R(0);  // degrees l-1, l-2
+    static_loop<0, P>([&](auto lc) {
+      constexpr int l = decltype(lc)::value;
+      if (l == m) {
+        px = ox;
+        py = oy;
+        ar[l] += ox;
+        ai[l] += oy;
+      } else if (l > m) {
+        const R a = (R)(2. * l - 1.) * vz;
+        const R g = gfac[l];
+        const R nx = (a * px - r2 * qx) * g, ny = (a * py - r2 * qy) * g;
+        qx = px;
+        qy = py;
+        px = nx;
+        py = ny;
+        ar[l] += nx;
+        ai[l] += ny;
+      }
+    });
+  }
+  double wl = 1.;  // w^l of the scaled form
+  static_loop<0, P>([&](auto lc) {
+    constexpr int l = decltype(lc)::value;
+    if (l >= m)
+      Ts[tri(l) + m] = make_double2((double)ar[l] * wl, (double)ai[l] * wl);
+    if (kScaled) wl *= w;
+  });
+}
+
+template <int P, class R>
 __global__ __launch_bounds__(64) void k_p2m(
     OpTables tb, NodeArrays nd, const int *__restrict__ leaves,
     const double *__restrict__ xs, const double *__restrict__ ys,
@@ -244,74 +341,12 @@ __global__ __launch_bounds__(64) void k_p2m(
   const int b = nd.pbegin[A], e = nd.pend[A];
   const double cx = nd.cx[A], cy = nd.cy[A], cz = nd.cz[A];
   const int s = lane / P, m = lane % P;
-  double ar[P], ai[P];
-  // factors 1 / (l^2 - m^2) of the degree recurrence: m is fixed per lane,
-  // so they are read once, not per particle (a per-lane table read inside
-  // the recurrence is a dependent memory round trip per degree)
-  double gfac[P];
-  static_loop<0, P>([&](auto lc) {
-    constexpr int l = decltype(lc)::value;
-    ar[l] = 0.;
-    ai[l] = 0.;
-    gfac[l] = kInv.v[l > m ? l * l - m * m : 1];
-  });
-  if (s < S) {
-    // the next particle's coordinates are loaded while this one is summed
-    int i = b + s;
-    double nx_ = 0., ny_ = 0., nz_ = 0., nq_ = 0.;
-    if (i < e) {
-      nx_ = xs[i];
-      ny_ = ys[i];
-      nz_ = zs[i];
-      nq_ = qs[i];
-    }
-    for (; i < e; i += S) {
-      const double vx = nx_ - cx, vy = ny_ - cy, vz = nz_ - cz, qi = nq_;
-      if (i + S < e) {
-        nx_ = xs[i + S];
-        ny_ = ys[i + S];
-        nz_ = zs[i + S];
-        nq_ = qs[i + S];
-      }
-      const double r2 = vx * vx + vy * vy + vz * vz;
-      double ox = qi, oy = 0.;  // q O_m^m
-      static_loop<1, P>([&](auto kc) {
-        constexpr int k = decltype(kc)::value;
-        if (k <= m) {
-          const double f = -0.5 * kInv.v[k];
-          const double ax = f * vx, ay = f * vy;
-          const double nx = ox * ax - oy * ay, ny = ox * ay + oy * ax;
-          ox = nx;
-          oy = ny;
-        }
-      });
-      double px = 0., py = 0., qx = 0., qy = 0.;  // degrees l-1, l-2
-      static_loop<0, P>([&](auto lc) {
-        constexpr int l = decltype(lc)::value;
-        if (l == m) {
-          px = ox;
-          py = oy;
-          ar[l] += ox;
-          ai[l] += oy;
-        } else if (l > m) {
-          const double a = (2. * l - 1.) * vz;
-          const double g = gfac[l];
-          const double nx = (a * px - r2 * qx) * g,
-                       ny = (a * py - r2 * qy) * g;
-          qx = px;
-          qy = py;
-          px = nx;
-          py = ny;
-          ar[l] += nx;
-          ai[l] += ny;
-        }
-      });
-    }
-    static_loop<0, P>([&](auto lc) {
-      constexpr int l = decltype(lc)::value;
-      if (l >= m) T[s][tri(l) + m] = make_double2(ar[l], ai[l]);
-    });
-  }
+  // the width only for the scaled form (the single-operator entry points
+  // describe their one leaf by its centre: no system box)
+  if (s < S)
+    p2m_slots<P, R>(b, e, s, m, cx, cy, cz,
+                    std::is_same<R, float>::value ? node_width(nd, A) : 1.,
+                    xs, ys, zs, qs, T[s]);
   __syncthreads();
   for (int k = lane; k < MC; k += 64) {
     double2 acc = make_double2(0., 0.);
@@ -496,73 +531,143 @@ hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s) {
 //   (d/dx + i d/dy) phi = sum_l [c_l^0 O_{l-1}^1
 //       + sum_{m>0} (c_l^m O_{l-1}^{m+1} - conj(c_l^m O_{l-1}^{m-1}))],
 // accumulated per harmonic O_j^m as it is generated (column by column).
+//
+// A block takes 256 consecutive sorted particles; the c rows of their
+// leaves (a leaf's particles are consecutive) are staged in LDS, at most
+// `rows` leaves per round, so the recurrence reads its coefficients from
+// LDS instead of waiting on a global load per degree step.  R = float (the
+// reduced-precision forms, particles of leaves below the top two levels):
+// the recurrence runs on coordinates scaled by the leaf width,
+// u = (x - c) / w, on coefficients c_l^m w^l (staged in FP32), and the
+// gradient takes the factor 1 / w at the end.  sel: 0 every particle, 1 the
+// particles of leaves at depth >= 2, 2 those of leaves at depth < 2 (the
+// choice is per leaf, so a system's result does not depend on its batch).
+template <class R>
 __global__ __launch_bounds__(256) void k_l2p(
     OpTables tb, NodeArrays nd, const int *__restrict__ pleaf, int n,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double2 *__restrict__ L,
-    double *__restrict__ pot, double *__restrict__ grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+    double *__restrict__ pot, double *__restrict__ grad, int rows, int sel) {
+  using R2 = typename std::conditional<std::is_same<R, float>::value, float2,
+                                       double2>::type;
+  constexpr bool kScaled = std::is_same<R, float>::value;
+  extern __shared__ double2 l2p_shared[];
+  R2 *srow = reinterpret_cast<R2 *>(l2p_shared);
+  __shared__ int sleaf[256], wbase[4];
+  __shared__ double sw[256];
+  const int tid = threadIdx.x, lane = tid % 64, wv = tid / 64;
+  const int i = blockIdx.x * 256 + tid;
   const int p = tb.p, lc = tri(p + 1);
-  const int A = pleaf[i];
-  const double2 *__restrict__ La = L + (size_t)A * lc;
   const double *__restrict__ g = tb.ogain;
-  const double vx = xs[i] - nd.cx[A], vy = ys[i] - nd.cy[A],
-               vz = zs[i] - nd.cz[A];
-  const double r2 = vx * vx + vy * vy + vz * vz;
-  double phi = 0., dz = 0., gx = 0., gy = 0.;
-  double mx = 1., my = 0.;  // O_m^m
-  for (int m = 0; m <= p; m++) {
-    if (m > 0) {
-      const double f = -0.5 * kInv.v[m];
-      const double ax = f * vx, ay = f * vy;
-      const double nx = mx * ax - my * ay, ny = mx * ay + my * ax;
-      mx = nx;
-      my = ny;
-    }
-    const double w = m ? 2. : 1.;
-    double ox = mx, oy = my, qx = 0., qy = 0.;  // O_j^m, O_{j-1}^m
-    double2 c = La[tri(m) + m];
-    double gc = g[tri(m) + m];
-    c.x *= gc;
-    c.y *= gc;
-    for (int j = m; j <= p; j++) {
-      if (j > m) {
-        const double a = (2. * j - 1.) * vz, s = kInv.v[j * j - m * m];
-        const double nx = (a * ox - r2 * qx) * s, ny = (a * oy - r2 * qy) * s;
-        qx = ox;
-        qy = oy;
-        ox = nx;
-        oy = ny;
-      }
-      phi += w * (c.x * ox - c.y * oy);
-      if (j < p) {
-        const int k1 = tri(j + 1) + m;
-        double2 c1 = La[k1];  // c_{j+1}^m
-        const double g1 = g[k1];
-        c1.x *= g1;
-        c1.y *= g1;
-        dz += w * (c1.x * ox - c1.y * oy);
-        if (m > 0) {  // c_{j+1}^{m-1} O_j^m
-          double2 c2 = La[k1 - 1];
-          const double g2 = g[k1 - 1];
-          gx += g2 * (c2.x * ox - c2.y * oy);
-          gy += g2 * (c2.x * oy + c2.y * ox);
-        }
-        {  // - conj(c_{j+1}^{m+1} O_j^m)
-          double2 c3 = La[k1 + 1];
-          const double g3 = g[k1 + 1];
-          gx -= g3 * (c3.x * ox - c3.y * oy);
-          gy += g3 * (c3.x * oy + c3.y * ox);
-        }
-        c = c1;
-      }
-    }
+  int A = i < n ? pleaf[i] : -1;
+  if (A >= 0 && sel != 0 && ((nd.depth[A] >= 2) != (sel == 1))) A = -1;
+  sleaf[tid] = A;
+  __syncthreads();
+  // slot of the particle's leaf among the block's distinct leaves
+  const bool start = A >= 0 && (tid == 0 || sleaf[tid - 1] != A);
+  const unsigned long long bal = __ballot(start);
+  const int before = __popcll(bal & ((2ull << lane) - 1ull));  // <= lane
+  if (lane == 63) wbase[wv] = __popcll(bal);
+  __syncthreads();
+  int slot = before - 1, nslots = 0;
+  for (int k = 0; k < 4; k++) {
+    if (k < wv) slot += wbase[k];
+    nslots += wbase[k];
   }
-  pot[i] += phi;
-  grad[3 * (size_t)i + 0] += gx;
-  grad[3 * (size_t)i + 1] += gy;
-  grad[3 * (size_t)i + 2] += dz;
+  __syncthreads();  // sleaf is rewritten below
+  if (start) sleaf[slot] = A;
+  __syncthreads();
+  double vx = 0., vy = 0., vz = 0.;
+  if (A >= 0) {
+    vx = xs[i] - nd.cx[A];
+    vy = ys[i] - nd.cy[A];
+    vz = zs[i] - nd.cz[A];
+  }
+  for (int r0 = 0; r0 < nslots; r0 += rows) {
+    const int nr = min(rows, nslots - r0);
+    if (r0 > 0) __syncthreads();  // the previous round's rows are read
+    // leaf widths, for the scaled form only (the single-operator entry
+    // points describe their one leaf by its centre: no system box)
+    if (kScaled)
+      for (int t = tid; t < nr; t += 256)
+        sw[t] = node_width(nd, sleaf[r0 + t]);
+    __syncthreads();
+    for (int e = tid; e < nr * lc; e += 256) {
+      const int rr = e / lc, k = e % lc;
+      double2 c = L[(size_t)sleaf[r0 + rr] * lc + k];
+      double f = g[k];
+      if (kScaled) {
+        int l = 0;
+        while (tri(l + 1) <= k) l++;
+        double wl = 1.;
+        for (int t = 0; t < l; t++) wl *= sw[rr];
+        f *= wl;
+      }
+      srow[(size_t)rr * lc + k] = R2{(R)(c.x * f), (R)(c.y * f)};
+    }
+    __syncthreads();
+    if (A < 0 || slot < r0 || slot >= r0 + nr) continue;
+    const R2 *__restrict__ La = srow + (size_t)(slot - r0) * lc;
+    R ux, uy, uz;
+    if (kScaled) {
+      const double iw = 1. / sw[slot - r0];
+      ux = (R)(vx * iw);
+      uy = (R)(vy * iw);
+      uz = (R)(vz * iw);
+    } else {
+      ux = vx;
+      uy = vy;
+      uz = vz;
+    }
+    const R r2 = ux * ux + uy * uy + uz * uz;
+    R phi = R(0), dz = R(0), gx = R(0), gy = R(0);
+    R mx = R(1), my = R(0);  // O_m^m
+    for (int m = 0; m <= p; m++) {
+      if (m > 0) {
+        const R f = (R)(-0.5 * kInv.v[m]);
+        const R ax = f * ux, ay = f * uy;
+        const R nx = mx * ax - my * ay, ny = mx * ay + my * ax;
+        mx = nx;
+        my = ny;
+      }
+      const R w = m ? R(2) : R(1);
+      R ox = mx, oy = my, qx = R(0), qy = R(0);  // O_j^m, O_{j-1}^m
+      R2 c = La[tri(m) + m];
+      for (int j = m; j <= p; j++) {
+        if (j > m) {
+          const R a = (R)(2 * j - 1) * uz, sj = (R)kInv.v[j * j - m * m];
+          const R nx = (a * ox - r2 * qx) * sj, ny = (a * oy - r2 * qy) * sj;
+          qx = ox;
+          qy = oy;
+          ox = nx;
+          oy = ny;
+        }
+        phi += w * (c.x * ox - c.y * oy);
+        if (j < p) {
+          const int k1 = tri(j + 1) + m;
+          const R2 c1 = La[k1];  // c_{j+1}^m
+          dz += w * (c1.x * ox - c1.y * oy);
+          if (m > 0) {  // c_{j+1}^{m-1} O_j^m
+            const R2 c2 = La[k1 - 1];
+            gx += c2.x * ox - c2.y * oy;
+            gy += c2.x * oy + c2.y * ox;
+          }
+          {  // - conj(c_{j+1}^{m+1} O_j^m)
+            const R2 c3 = La[k1 + 1];
+            gx -= c3.x * ox - c3.y * oy;
+            gy += c3.x * oy + c3.y * ox;
+          }
+          c = c1;
+        }
+      }
+    }
+    double s = 1.;
+    if (kScaled) s = 1. / sw[slot - r0];
+    pot[i] += (double)phi;
+    grad[3 * (size_t)i + 0] += (double)gx * s;
+    grad[3 * (size_t)i + 1] += (double)gy * s;
+    grad[3 * (size_t)i + 2] += (double)dz * s;
+  }
 }
 
 // ------------------------------------------------------------ M2P / P2L
@@ -812,15 +917,20 @@ static hipError_t p2m_dispatch(const OpTables &tb, const NodeArrays &nd,
                                const int *leaves, int nleaves,
                                const double *xs, const double *ys,
                                const double *zs, const double *qs, double2 *M,
-                               hipStream_t s) {
+                               int low, hipStream_t s) {
   if constexpr (P > kMaxOrderFast) {
     k_p2m_generic<<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
     return hipGetLastError();
   } else {
     if (tb.p != P)
       return p2m_dispatch<P + 1>(tb, nd, leaves, nleaves, xs, ys, zs, qs, M,
-                                 s);
-    k_p2m<P><<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs, M);
+                                 low, s);
+    if (low)
+      k_p2m<P, float><<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs,
+                                             M);
+    else
+      k_p2m<P, double><<<nleaves, 64, 0, s>>>(tb, nd, leaves, xs, ys, zs, qs,
+                                              M);
     return hipGetLastError();
   }
 }
@@ -828,9 +938,10 @@ static hipError_t p2m_dispatch(const OpTables &tb, const NodeArrays &nd,
 hipError_t launch_p2m(const OpTables &tb, const NodeArrays &nd,
                       const int *leaves, int nleaves, const double *xs,
                       const double *ys, const double *zs, const double *qs,
-                      double2 *M, hipStream_t s) {
+                      double2 *M, hipStream_t s, bool low) {
   if (nleaves == 0) return hipSuccess;
-  return p2m_dispatch<1>(tb, nd, leaves, nleaves, xs, ys, zs, qs, M, s);
+  return p2m_dispatch<1>(tb, nd, leaves, nleaves, xs, ys, zs, qs, M,
+                         low ? 1 : 0, s);
 }
 
 hipError_t launch_m2l(const OpTables &tb, const NodeArrays &nd, int nnodes,
@@ -872,10 +983,19 @@ hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
-                      double *pot, double *grad, hipStream_t s) {
+                      double *pot, double *grad, hipStream_t s, int sel) {
   if (n == 0) return hipSuccess;
-  k_l2p<<<(n + 255) / 256, 256, 0, s>>>(tb, nd, pleaf, n, xs, ys, zs, L, pot,
-                                        grad);
+  if (sel < 0 || sel > 2) return hipErrorInvalidValue;
+  // staged rows per round: up to 16 leaves within 32 KiB of LDS
+  const int lc = tri(tb.p + 1);
+  const size_t row = (sel == 1 ? sizeof(float2) : sizeof(double2)) * lc;
+  const int rows = (int)std::max<size_t>(1, std::min<size_t>(16, 32768 / row));
+  if (sel == 1)
+    k_l2p<float><<<(n + 255) / 256, 256, rows * row, s>>>(
+        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel);
+  else
+    k_l2p<double><<<(n + 255) / 256, 256, rows * row, s>>>(
+        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel);
   return hipGetLastError();
 }
 
