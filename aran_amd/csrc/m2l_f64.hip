@@ -295,46 +295,57 @@ hipError_t launch_f64_pack_ops(int p, const double *O, double *ops,
   return hipGetLastError();
 }
 
-// One thread per (node, quarter, position): the scaled real input entry
-// k = quarter * kp + position of the node's multipole (zero past ki and for
-// the all-zero node nnodes).
+// One thread per (node, quarter, chunk of 8 positions): the scaled real
+// input entries k = quarter * kp + position of the node's multipole (zero
+// past ki and for the all-zero node nnodes).  The entries of a chunk are
+// consecutive in k, so the degree l and the scale w^-l advance
+// incrementally.
 __global__ void k_f64_pack_mult(NodeArrays nd, int nnodes, int p,
                                 const double2 *__restrict__ M,
                                 double *__restrict__ mb) {
   // issue priority over the near field's waves (second stream)
   __builtin_amdgcn_s_setprio(2);
   const F64Dims D = f64_dims(p);
+  const int nch = (D.kp + 7) / 8;
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  const long per_node = 4L * D.kp;
-  if (t >= (long)(nnodes + 1) * per_node) return;
-  const int node = (int)(t / per_node);
-  const int k = (int)(t % per_node);  // = quarter * kp + position
-  double x = 0.;
-  if (node < nnodes && k < D.ki) {
-    int l = 0, idx;
-    bool im = false;
-    if (k < D.mc) {
-      while (tri(l + 1) <= k) l++;
-      idx = k;
-    } else {
-      const int jj = k - D.mc;  // jj = tri(l-1) + m - 1
-      l = 1;
-      while (tri(l) <= jj) l++;
-      idx = tri(l) + (jj - tri(l - 1)) + 1;
-      im = true;
-    }
-    const double iw = 1. / node_width(nd, node);
-    double f = 1.;
-    for (int i = 0; i < l; i++) f *= iw;
-    const double2 c = M[(size_t)node * D.mc + idx];
-    x = (im ? c.y : c.x) * f;
+  if (t >= (long)(nnodes + 1) * 4 * nch) return;
+  const int node = (int)(t / (4 * nch));
+  const int q = (int)(t / nch % 4), c = (int)(t % nch);
+  double *dst = mb + ((size_t)node * 4 + q) * D.kp + 8 * c;
+  const int npos = min(8, D.kp - 8 * c);
+  if (node == nnodes) {
+    for (int j = 0; j < npos; j++) dst[j] = 0.;
+    return;
   }
-  mb[t] = x;
+  const double iw = 1. / node_width(nd, node);
+  const double2 *Mn = M + (size_t)node * D.mc;
+  int l = -1, sec = -1;  // sec 0: Re part (k < mc), 1: Im part
+  double f = 1.;
+  for (int j = 0; j < npos; j++) {
+    const int k = q * D.kp + 8 * c + j;
+    double x = 0.;
+    if (k < D.ki) {
+      const int ks = k < D.mc ? 0 : 1;
+      const int kk = ks ? k - D.mc : k;  // Im part: kk = tri(l-1) + m - 1
+      if (ks != sec) {
+        sec = ks;
+        l = ks;
+        f = ks ? iw : 1.;
+      }
+      while (ks ? tri(l) <= kk : tri(l + 1) <= kk) {
+        l++;
+        f *= iw;
+      }
+      const int idx = ks ? tri(l) + (kk - tri(l - 1)) + 1 : kk;
+      x = (ks ? Mn[idx].y : Mn[idx].x) * f;
+    }
+    dst[j] = x;
+  }
 }
 
 hipError_t launch_f64_pack_mult(const NodeArrays &nd, int nnodes, int p,
                                 const double2 *M, double *mb, hipStream_t s) {
-  const long total = (long)(nnodes + 1) * f64_mult_doubles(p);
+  const long total = (long)(nnodes + 1) * 4 * ((f64_dims(p).kp + 7) / 8);
   k_f64_pack_mult<<<(total + 255) / 256, 256, 0, s>>>(nd, nnodes, p, M, mb);
   return hipGetLastError();
 }

@@ -483,32 +483,37 @@ __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p, int nparts,
   }
   const double iw = 1. / node_width(nd, node);
   const double2 *Mn = M + (size_t)node * D.mc;
+  // the 8 entries are consecutive in k, so the degree l and the scale
+  // w^-l advance incrementally (one power per section, not one per entry)
   double v[8];
+  int l = -1, sec = -1;  // sec 0: Re part (k < mc), 1: Im part
+  double f = 1.;
   for (int j = 0; j < 8; j++) {
     const int k = 16 * s + 8 * g + j;
     double x = 0.;
-    int l = 0, idx = -1;
-    bool im = false;
-    if (k < D.mc) {
-      while (tri(l + 1) <= k) l++;
-      idx = k;
-    } else if (k < D.ki) {
-      const int jj = k - D.mc;  // jj = tri(l-1) + m - 1
-      l = 1;
-      while (tri(l) <= jj) l++;
-      idx = tri(l) + (jj - tri(l - 1)) + 1;
-      im = true;
-    }
-    if (idx >= 0) {
-      double f = 1.;
-      for (int i = 0; i < l; i++) f *= iw;
-      x = (im ? Mn[idx].y : Mn[idx].x) * f;
+    if (k < D.ki) {
+      const int ks = k < D.mc ? 0 : 1;
+      const int kk = ks ? k - D.mc : k;  // Im part: kk = tri(l-1) + m - 1
+      if (ks != sec) {                   // (re)start the section
+        sec = ks;
+        l = ks;
+        f = ks ? iw : 1.;
+      }
+      while (ks ? tri(l) <= kk : tri(l + 1) <= kk) {
+        l++;
+        f *= iw;
+      }
+      const int idx = ks ? tri(l) + (kk - tri(l - 1)) + 1 : kk;
+      x = (ks ? Mn[idx].y : Mn[idx].x) * f;
     }
     v[j] = x;
   }
   uint4 parts[3];
   pack8(v, parts);
-  for (int q = 0; q < nparts; q++) mb[t * nparts + q] = parts[q];
+  // constant indices keep parts[] in registers
+  mb[t * nparts] = parts[0];
+  mb[t * nparts + 1] = parts[1];
+  if (nparts == 3) mb[t * nparts + 2] = parts[2];
 }
 
 hipError_t launch_mfma_pack_ops(int p, const double *O, void *ops,

@@ -50,6 +50,8 @@ TRAFFIC_SOURCES = {
     "bf16": ["aran_amd/csrc/m2l_mfma.hip", "aran_amd/csrc/m2l_prep.hip",
              "aran_amd/csrc/m2l_mfma.h"],
     "f64": ["aran_amd/csrc/m2l_f64.hip", "aran_amd/csrc/m2l_f64.h"],
+    "tile": ["aran_amd/csrc/m2l_tile.hip", "aran_amd/csrc/m2l_tile.h",
+             "aran_amd/csrc/m2l_prep.hip"],
 }
 
 
@@ -555,6 +557,9 @@ def main():
                     help="also run one elementwise kernel that reads and "
                          "writes 512 MiB (the reference point for HBM byte "
                          "counters)")
+    ap.add_argument("--no-config1", action="store_true",
+                    help="skip the Config-1 GPU line (profiler runs of the "
+                         "headline workload alone)")
     ap.add_argument("--batch-systems", type=int, default=512,
                     help="independent systems of the batched solve (0: skip)")
     ap.add_argument("--sweep", action="store_true",
@@ -619,17 +624,19 @@ def main():
 
     # Config 1 on the GPU (plain FP64, N = 10,000 uniform cube, p = 8, leaf
     # 20), beside its CPU line
-    x1, y1, z1, m1 = W.uniform_cube(10_000, 0xA2A0 + 1)
-    d1 = [torch.from_numpy(a).cuda() for a in (x1, y1, z1, m1)]
-    p1 = torch.empty(10_000, dtype=torch.float64, device="cuda")
-    g1 = torch.empty((10_000, 3), dtype=torch.float64, device="cuda")
-    r1 = run_form(f, d1, p1, g1, (0, 0, 0), (1, 1, 1), 8, 20, "plain",
-                  max(5, args.steps), 2)
-    config1 = {"workload": "N=10000 uniform cube, p=8, leaf 20, "
-                           "potential+gradient, plain FP64",
-               "value": r1["value"], "ms_per_step": r1["ms_per_step"],
-               "error": None if args.no_check else
-               error_vs_direct(f, d1, p1, g1, 10_000, 0xA2A0 + 1)}
+    config1 = None
+    if not args.no_config1:
+        x1, y1, z1, m1 = W.uniform_cube(10_000, 0xA2A0 + 1)
+        d1 = [torch.from_numpy(a).cuda() for a in (x1, y1, z1, m1)]
+        p1 = torch.empty(10_000, dtype=torch.float64, device="cuda")
+        g1 = torch.empty((10_000, 3), dtype=torch.float64, device="cuda")
+        r1 = run_form(f, d1, p1, g1, (0, 0, 0), (1, 1, 1), 8, 20, "plain",
+                      max(5, args.steps), 2)
+        config1 = {"workload": "N=10000 uniform cube, p=8, leaf 20, "
+                               "potential+gradient, plain FP64",
+                   "value": r1["value"], "ms_per_step": r1["ms_per_step"],
+                   "error": None if args.no_check else
+                   error_vs_direct(f, d1, p1, g1, 10_000, 0xA2A0 + 1)}
 
     hostinc = host_inclusive(f, [torch.from_numpy(a) for a in (x, y, z, m)],
                              lb, ub, p, q, args.precision,
