@@ -288,6 +288,31 @@ def test_edge_cases(torch_cuda):
         f.solve(bad, ys, zs, qs, (0, 0, 0), (1, 1, 1), 8, 1)
 
 
+@pytest.mark.parametrize("form", ["mixed", "bf16x3"])
+def test_fp32_near_field_coincident_coordinates(torch_cuda, form):
+    """Two distinct particles 1e-12 apart share their FP32 leaf-relative
+    coordinates, so the FP32 near field sees r2 = 0 for the pair; the
+    targets whose FP32 sums are not finite are summed again in FP64, so the
+    result stays finite and equal to the FP64 direct sum (the pair's term,
+    1e12, dominates their potentials)."""
+    torch = torch_cuda
+    f = _fmm(torch)
+    f.set_precision(form)
+    rng = np.random.default_rng(11)
+    n = 300
+    x, y, z = rng.random((3, n)) * 0.9 + 0.05
+    q = 1.0 - rng.random(n)
+    x[1], y[1], z[1] = x[0] + 1e-12, y[0], z[0]
+    dev = [torch.from_numpy(a).cuda() for a in (x, y, z, q)]
+    pot, grad = f.solve(*dev, (0, 0, 0), (1, 1, 1), 6, 16)
+    pot, grad = pot.cpu().numpy(), grad.cpu().numpy()
+    assert np.all(np.isfinite(pot)) and np.all(np.isfinite(grad))
+    dp, dg = f.direct(*dev, torch.arange(n))
+    dp = dp.cpu().numpy()
+    assert np.max(np.abs(pot[:2] - dp[:2]) / np.abs(dp[:2])) <= 1e-9
+    assert np.linalg.norm(pot - dp) / np.linalg.norm(dp) <= 1e-6
+
+
 def test_solver_api_newtonpot3_style(torch_cuda):
     """AranSolver3d used as tests/newtonpot3.c / newtonfield3.c use it, with
     the library callbacks and the declared particle layout; results match
