@@ -51,6 +51,11 @@ struct Request {
   long n = 0;
   int order = 8, leafcap = 32, want_grad = 1;
   double lb[3], ub[3];
+  // inputs x, y, z, q and outputs pot, grad: in a block of the server's
+  // pinned pool ([x | y | z | q | pot | grad], 8 n doubles) when it has
+  // room, else in pageable arrays staged through the worker's buffers
+  double *pin = nullptr;
+  size_t pin_off = 0, pin_bytes = 0;
   HostArr x, y, z, q, pot, grad;
   double bytes = 0.;
   Clock::time_point t_submit, t_start, t_done;
@@ -103,6 +108,57 @@ struct PinnedBuf {
   }
 };
 
+// First-fit allocator over one pinned host region (offsets and sizes in
+// 256-byte units), under the server's lock.  Request data copied once into
+// it at submission goes to the GPU and comes back without staging copies.
+struct PinnedPool {
+  char *base = nullptr;
+  size_t size = 0;
+  std::map<size_t, size_t> free_;  // offset -> bytes, coalesced
+  bool init(size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    if (hipHostMalloc(reinterpret_cast<void **>(&base), bytes) != hipSuccess) {
+      base = nullptr;
+      return false;
+    }
+    size = bytes;
+    free_.clear();
+    free_[0] = bytes;
+    return true;
+  }
+  char *alloc(size_t bytes, size_t *off) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    for (auto it = free_.begin(); it != free_.end(); ++it)
+      if (it->second >= bytes) {
+        *off = it->first;
+        const size_t rest = it->second - bytes, at = it->first + bytes;
+        free_.erase(it);
+        if (rest) free_[at] = rest;
+        return base + *off;
+      }
+    return nullptr;
+  }
+  void release(size_t off, size_t bytes) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  ~PinnedPool() {
+    if (base) (void)hipHostFree(base);
+  }
+};
+
 struct Worker {
   AranFmm3d *ctx = nullptr;
   aran::DevBuf din, dout;
@@ -126,6 +182,7 @@ struct _AranServer3d {
   long batches = 0, served = 0, max_batch_seen = 0;
   int running = 0;  // batches taken and not yet finished
   std::vector<std::unique_ptr<Worker>> workers;
+  PinnedPool pool;  // request blocks (set up by aran_server3d_prepare)
 };
 
 extern "C" double aran_server3d_footprint(long n, int order, int leafcap) {
@@ -208,17 +265,36 @@ static void worker_loop(AranServer3d *S, Worker *W) {
     double *hout = static_cast<double *>(W->hout.ptr);
     double *dout = W->dout.as<double>();
     if (rc == 0 && n > 0) {
-      for (int k = 0; k < nsys; k++) {
+      // pinned requests go to the device from their own block; the others
+      // through the worker's pinned staging
+      bool staged = false;
+      for (int k = 0; k < nsys && rc == 0; k++) {
         const Request *r = batch[k];
         const size_t nb = sizeof(double) * r->n;
-        par_copy(hin + off[k], r->x.data(), nb);
-        par_copy(hin + n + off[k], r->y.data(), nb);
-        par_copy(hin + 2 * n + off[k], r->z.data(), nb);
-        par_copy(hin + 3 * n + off[k], r->q.data(), nb);
+        for (int a = 0; a < 4 && rc == 0; a++) {
+          double *d = din + a * n + off[k];
+          if (r->pin) {
+            if (r->n > 0 &&
+                hipMemcpyAsync(d, r->pin + a * r->n, nb, hipMemcpyHostToDevice,
+                               st) != hipSuccess)
+              rc = ARAN_FMM3D_EHIP;
+          } else {
+            const HostArr *src[4] = {&r->x, &r->y, &r->z, &r->q};
+            par_copy(hin + a * n + off[k], src[a]->data(), nb);
+            staged = true;
+          }
+        }
       }
-      if (hipMemcpyAsync(din, hin, 4 * sizeof(double) * n,
-                         hipMemcpyHostToDevice, st) != hipSuccess)
-        rc = ARAN_FMM3D_EHIP;
+      if (rc == 0 && staged)
+        for (int k = 0; k < nsys && rc == 0; k++) {
+          const Request *r = batch[k];
+          if (r->pin || r->n == 0) continue;
+          for (int a = 0; a < 4 && rc == 0; a++)
+            if (hipMemcpyAsync(din + a * n + off[k], hin + a * n + off[k],
+                               sizeof(double) * r->n, hipMemcpyHostToDevice,
+                               st) != hipSuccess)
+              rc = ARAN_FMM3D_EHIP;
+        }
     }
     if (rc == 0)
       rc = aran_fmm3d_solve_batch(W->ctx, nsys, off.data(), din, din + n,
@@ -226,9 +302,22 @@ static void worker_loop(AranServer3d *S, Worker *W) {
                                   ubs.data(), batch[0]->order,
                                   batch[0]->leafcap, dout, dout + n);
     if (rc == 0 && n > 0) {
-      if (hipMemcpyAsync(hout, dout, 4 * sizeof(double) * n,
-                         hipMemcpyDeviceToHost, st) != hipSuccess ||
-          hipStreamSynchronize(st) != hipSuccess)
+      // results of pinned requests straight into their blocks, the rest
+      // into the worker's staging
+      for (int k = 0; k < nsys && rc == 0; k++) {
+        const Request *r = batch[k];
+        if (r->n == 0) continue;
+        double *hp = r->pin ? r->pin + 4 * r->n : hout + off[k];
+        double *hg = r->pin ? r->pin + 5 * r->n : hout + n + 3 * off[k];
+        if (hipMemcpyAsync(hp, dout + off[k], sizeof(double) * r->n,
+                           hipMemcpyDeviceToHost, st) != hipSuccess ||
+            (r->want_grad &&
+             hipMemcpyAsync(hg, dout + n + 3 * off[k],
+                            sizeof(double) * 3 * r->n, hipMemcpyDeviceToHost,
+                            st) != hipSuccess))
+          rc = ARAN_FMM3D_EHIP;
+      }
+      if (rc == 0 && hipStreamSynchronize(st) != hipSuccess)
         rc = ARAN_FMM3D_EHIP;
     }
     {
@@ -237,7 +326,7 @@ static void worker_loop(AranServer3d *S, Worker *W) {
       for (int k = 0; k < nsys; k++) {
         Request *r = batch[k];
         r->status = rc;
-        if (rc == 0) {
+        if (rc == 0 && !r->pin) {
           r->pot.alloc(r->n);
           par_copy(r->pot.data(), hout + off[k], sizeof(double) * r->n);
           if (r->want_grad) {
@@ -338,21 +427,35 @@ extern "C" long aran_server3d_submit(AranServer3d *S, long n, const double *x,
     r->lb[a] = lb[a];
     r->ub[a] = ub[a];
   }
-  r->x.alloc(n);
-  r->y.alloc(n);
-  r->z.alloc(n);
-  r->q.alloc(n);
   const size_t nb = sizeof(double) * n;
-  par_copy(r->x.data(), x, nb);
-  par_copy(r->y.data(), y, nb);
-  par_copy(r->z.data(), z, nb);
-  par_copy(r->q.data(), q, nb);
+  if (n > 0 && S->pool.base) {
+    std::lock_guard<std::mutex> lock(S->mu);
+    r->pin_bytes = 8 * nb;
+    r->pin = reinterpret_cast<double *>(S->pool.alloc(r->pin_bytes,
+                                                      &r->pin_off));
+  }
+  if (r->pin) {
+    par_copy(r->pin, x, nb);
+    par_copy(r->pin + n, y, nb);
+    par_copy(r->pin + 2 * n, z, nb);
+    par_copy(r->pin + 3 * n, q, nb);
+  } else {
+    r->x.alloc(n);
+    r->y.alloc(n);
+    r->z.alloc(n);
+    r->q.alloc(n);
+    par_copy(r->x.data(), x, nb);
+    par_copy(r->y.data(), y, nb);
+    par_copy(r->z.data(), z, nb);
+    par_copy(r->q.data(), q, nb);
+  }
   r->bytes = bytes;
   r->t_submit = Clock::now();
   long id;
   {
     std::lock_guard<std::mutex> lock(S->mu);
     if (S->stop) {
+      if (r->pin) S->pool.release(r->pin_off, r->pin_bytes);
       aran::set_error("aran_server3d_submit: server is stopping");
       return ARAN_FMM3D_EINVAL;
     }
@@ -389,10 +492,19 @@ extern "C" int aran_server3d_wait(AranServer3d *S, long id, double *pot,
     r = std::move(again->second);
     S->all.erase(again);
   }
-  if (r->status != 0) return r->status;
-  if (pot) par_copy(pot, r->pot.data(), sizeof(double) * r->n);
-  if (grad && r->want_grad)
-    par_copy(grad, r->grad.data(), sizeof(double) * 3 * r->n);
+  const int status = r->status;
+  if (status == 0) {
+    const double *rp = r->pin ? r->pin + 4 * r->n : r->pot.data();
+    const double *rg = r->pin ? r->pin + 5 * r->n : r->grad.data();
+    if (pot) par_copy(pot, rp, sizeof(double) * r->n);
+    if (grad && r->want_grad) par_copy(grad, rg, sizeof(double) * 3 * r->n);
+  }
+  if (r->pin) {
+    std::lock_guard<std::mutex> lock(S->mu);
+    S->pool.release(r->pin_off, r->pin_bytes);
+    r->pin = nullptr;
+  }
+  if (status != 0) return status;
   if (times3) {
     using sec = std::chrono::duration<double>;
     times3[0] = sec(r->t_done - r->t_submit).count();       // latency
@@ -425,7 +537,14 @@ extern "C" int aran_server3d_prepare(AranServer3d *S, long n_max,
   }
   (void)hipSetDevice(S->device);
   int rc = 0;
+  // the pinned pool for request data: room for 16 requests of n_max
+  // particles (8 doubles each: x, y, z, q in, potential and gradient out)
+  if (!S->pool.base && !S->pool.init(16 * 64 * (size_t)n_max)) {
+    aran::set_error("aran_server3d_prepare: pinned pool allocation failed");
+    rc = ARAN_FMM3D_ENOMEM;
+  }
   for (auto &w : S->workers) {
+    if (rc != 0) break;
     Worker *W = w.get();
     if (!reserve_io(W, n_max)) {
       aran::set_error("aran_server3d_prepare: staging allocation failed");

@@ -61,10 +61,15 @@ int aran_server3d_wait (AranServer3d *srv, long id, double *pot, double *grad,
  * reserves host (pinned) and device staging for batches of up to n_max
  * particles, sizes its solver workspace with a synthetic uniform system of
  * n_max particles, and builds its tables for each of the norders orders
- * (one solve per order at leaf capacity leafcap).  Without it the first
- * requests of each size class and order pay for those allocations and table
- * builds.  Returns ARAN_FMM3D_EINVAL when requests are queued or in flight;
- * blocks until done. */
+ * (one solve per order at leaf capacity leafcap).  It also sets up a pool
+ * of pinned host memory for request data (room for 16 requests of n_max
+ * particles): a request that fits is copied once into it at submission,
+ * goes to the GPU and comes back from it without staging copies, and its
+ * results are copied once into the caller's buffers at wait.  Without it
+ * the first requests of each size class and order pay for those
+ * allocations and table builds, and every request is staged.  Returns
+ * ARAN_FMM3D_EINVAL when requests are queued or in flight; blocks until
+ * done. */
 int aran_server3d_prepare (AranServer3d *srv, long n_max, const int *orders,
                            int norders, int leafcap);
 

@@ -8,7 +8,8 @@
   reported beside the plain form's on the same targets.
 * Config 5: a served mix (p in {6, 8, 12}, up to 2e6 particles, BF16x3):
   every request's result equals its standalone solve bit for bit, in two
-  sessions whose batches differ in composition.
+  sessions whose batches differ in composition (one prepared, so its
+  requests travel in pinned pool blocks, one not, so they are staged).
 * Config 3: signed charges (leaf 64, p in {6, 10, 14}): FP64 against the
   oracle at 1e-10, every form against the plain one and the direct sum.
 * A deep FP64 tree (Plummer N = 2^18, p = 10, leaf 64, depth >= 8) against
@@ -87,9 +88,12 @@ def test_config5_served_equal_standalone(torch_cuda):
     assert max(len(r[0]) for r in reqs) > 5e5
     assert {r[6] for r in reqs} == {6, 8, 12}
     results = []
+    nmax = max(len(r[0]) for r in reqs)
     for max_batch in (32, 2):
         srv = Server(budget_bytes=200e9, nworkers=2, precision="bf16x3",
                      max_batch=max_batch)
+        if max_batch == 32:  # pinned request blocks
+            srv.prepare(nmax, (6, 8, 12), 64)
         srv.pause(True)
         ids = [srv.submit(*r[:4], r[4], r[5], r[6], r[7], want_grad=r[8])
                for r in reqs]
