@@ -19,6 +19,7 @@
 // summed in slice order by k_unscale_mfma.
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
 #include <type_traits>
 
 #include "engine.h"
@@ -283,10 +284,310 @@ __global__ __launch_bounds__(64 * kMfmaWaves) __attribute__((
       }
 }
 
+
+// ------------------------------------------------------- M2L, one wave per SIMD
+//
+// The three-product form with 256 FP32 accumulators per wave (in AGPRs):
+// a wave holds every row block of a step and W1Cfg::CB 32-column blocks
+// (two 64-target tiles at p <= 10, one above), four waves per workgroup on
+// the same tiles of wg_tiles as k_m2l_mfma.  Each accumulator receives the
+// same sequence of MFMAs as in k_m2l_mfma, so the two kernels give the same
+// bits.
+//
+// Vector-memory loads retire in order: a wait for one load also waits for
+// every load issued before it, so a load with a short lead would cut the
+// lead of all older ones.  The loop therefore issues only two kinds of
+// global loads, both with the same lead: source fragments W1_NB - 1 steps
+// ahead of their products (a register ring), and operator units
+// W1Cfg::G + W1_L steps ahead (staging registers, stored into a two-group
+// LDS ring W1_L steps after their load).  The source node indices of the
+// workgroup's tiles over its offset range are copied into LDS once, as byte
+// offsets of the packed multipoles, so no index load sits in the loop.
+// Loads go through raw buffer descriptors with 32-bit offsets (no 64-bit
+// address arithmetic); a missing source carries an offset past the end of
+// the table, which the buffer unit answers with zeros.
+constexpr int kW1Waves = 4;
+constexpr int W1_NB = 4;  // source buffers (loads W1_NB - 1 steps ahead)
+constexpr int W1_L = 2;   // steps between an operator unit's load and store
+constexpr int kW1MinRanges = 5;
+constexpr int kW1Rows = 40;  // index rows per tile: ceil(189 / 5) rounded up
+static_assert((kOctOffsets + kW1MinRanges - 1) / kW1MinRanges < kW1Rows, "");
+constexpr unsigned kW1Missing = 0x80000000u;  // byte offset past every table
+
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+
+template <int P>
+struct W1Cfg {
+  static constexpr MfmaDims D = mfma_dims(P);
+  static constexpr int KS = D.ks, RB = D.rb, RO = D.ro;
+  static constexpr int CB = D.rb <= 4 ? 4 : 2;  // 32-column blocks per wave
+  static constexpr int TPW = CB / 2;             // tiles per wave
+  static constexpr int NPART = 2;
+  static constexpr int CHUNK = RB * NPART * 64;  // 16-byte units of a step
+  static constexpr int G = D.rb <= 4 ? 4 : 2;    // steps per operator group
+  static constexpr int NT = 64 * kW1Waves;
+  static constexpr int SPT = (CHUNK + NT - 1) / NT;  // units / thread / step
+  static constexpr unsigned kNodeBytes = KS * 2 * NPART * 16;
+  static_assert(W1_NB % G == 0 && W1_NB % W1_L == 0, "loop body");
+  static_assert(kW1Waves * TPW == 8 / (2 / mfma_col_blocks(P)),
+                "tiles per workgroup differ from the plan's");
+};
+
+template <int P>
+__global__ __launch_bounds__(64 * kW1Waves) __attribute__((
+    amdgpu_waves_per_eu(1, 1))) void k_m2l_mfma1w(
+    const int *__restrict__ wg_tiles, int nwg, const int *__restrict__ tile_src,
+    const int *__restrict__ tile_oct, const short *__restrict__ oct_codes,
+    const uint4 *__restrict__ ops, const uint4 *__restrict__ mb,
+    int mb_bytes, float *__restrict__ Lsm, int ntiles, int nsplit,
+    int sched) {
+  using C = W1Cfg<P>;
+  constexpr int KS = C::KS, RB = C::RB, RO = C::RO, CB = C::CB, TPW = C::TPW;
+  constexpr int NPART = C::NPART, CHUNK = C::CHUNK, G = C::G, NT = C::NT;
+  constexpr int SPT = C::SPT, NB = W1_NB, DL = W1_NB - 1;
+  __shared__ u32x4_t ring[2][G * CHUNK];
+  __shared__ int idx_tab[kW1Waves][TPW][kW1Rows * 64];
+  __shared__ short codes[kOctOffsets];
+  // workgroup order over the XCDs as in k_m2l_mfma
+  const int per_xcd = gridDim.x / 8;
+  const int logical = (blockIdx.x % 8) * per_xcd + blockIdx.x / 8;
+  if (logical >= nwg * nsplit) return;  // whole workgroup
+  const int split = sched ? logical / nwg : logical % nsplit;
+  const int wgi = sched ? logical % nwg : logical / nsplit;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+  const int tid = threadIdx.x, lane = tid % 64;
+  const int g = lane / 32, c32 = lane % 32;
+  const int *wt = wg_tiles + (size_t)wgi * (kW1Waves * TPW);
+  int tl[TPW];
+#pragma unroll
+  for (int h = 0; h < TPW; h++)
+    tl[h] = __builtin_amdgcn_readfirstlane(wt[TPW * wave + h]);
+  const int oct = __builtin_amdgcn_readfirstlane(tile_oct[wt[0]]);
+  const int chunk = (kOctOffsets + nsplit - 1) / nsplit;
+  const int k0 = split * chunk, k1 = min(kOctOffsets, k0 + chunk);
+  const int nslots = max(0, k1 - k0);
+  const int nsteps = nslots * KS;
+  for (int k = tid; k < kOctOffsets; k += NT)
+    codes[k] = oct_codes[oct * kOctOffsets + k];
+  // index tables of the wave's tiles: rows k0 .. k1 - 1 (64 sources each)
+  // as byte offsets; missing sources, missing tiles and the rows from
+  // nslots on (read by steps past the range) hold kW1Missing
+#pragma unroll
+  for (int h = 0; h < TPW; h++) {
+    const int *base = tile_src + ((size_t)max(tl[h], 0) * kOctOffsets + k0) * 64;
+    u32x4_t *dst = (u32x4_t *)&idx_tab[wave][h][0];
+    for (int e = lane; e < kW1Rows * 16; e += 64) {
+      u32x4_t v;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const int el = 4 * e + j;
+        const int nd = (tl[h] >= 0 && el < nslots * 64) ? base[el] : -1;
+        v[j] = nd >= 0 ? (unsigned)nd * C::kNodeBytes : kW1Missing;
+      }
+      dst[e] = v;
+    }
+  }
+  __syncthreads();  // codes and index tables
+  const __amdgpu_buffer_rsrc_t ops_rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)ops, (short)0, (int)((long)kNumCodes * KS * RB * 3 * 64 * 16),
+      0x00020000);
+  const __amdgpu_buffer_rsrc_t mb_rs = __builtin_amdgcn_make_buffer_rsrc(
+      (void *)mb, (short)0, mb_bytes, 0x00020000);
+  // operator units: this thread's part of a unit's offset, and the step's
+  // part (uniform, the instruction's scalar offset)
+  unsigned uoff[SPT];
+#pragma unroll
+  for (int i = 0; i < SPT; i++) {
+    const int u = min(tid + i * NT, CHUNK - 1);
+    const int rb = u / (NPART * 64), part = (u / 64) % NPART, ln = u % 64;
+    uoff[i] = (unsigned)(((rb * 3 + part) * 64 + ln) * 16);
+  }
+  auto stage_load = [&](int code, int s, u32x4_t (&r)[SPT])
+      __attribute__((always_inline)) {
+    const int cs = __builtin_amdgcn_readfirstlane((code * KS + s) *
+                                                  (RB * 3 * 64 * 16));
+#pragma unroll
+    for (int i = 0; i < SPT; i++)
+      r[i] = __builtin_amdgcn_raw_buffer_load_b128(ops_rs, (int)uoff[i], cs, 0);
+  };
+  auto stage_store = [&](int t, const u32x4_t (&r)[SPT])
+      __attribute__((always_inline)) {
+    u32x4_t *dst = ring[(t / G) % 2] + (t % G) * CHUNK;
+#pragma unroll
+    for (int i = 0; i < SPT; i++)
+      if (CHUNK % NT == 0 || tid + i * NT < CHUNK) dst[tid + i * NT] = r[i];
+  };
+  // group 0 in LDS; steps G .. G + L - 1 staged in registers (steps past
+  // the range reread the last step: valid units, never multiplied with a
+  // source)
+  u32x4_t stg[W1_L][SPT];
+  int cbuf[2];
+  if (nsteps > 0) {
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+      const int tc = min(j, nsteps - 1);
+      stage_load(codes[k0 + tc / KS], tc % KS, stg[0]);
+      stage_store(j, stg[0]);
+    }
+#pragma unroll
+    for (int j = 0; j < W1_L; j++) {
+      const int tc = min(G + j, nsteps - 1);
+      stage_load(codes[k0 + tc / KS], tc % KS, stg[j]);
+    }
+    cbuf[0] = codes[k0 + min(G + W1_L, nsteps - 1) / KS];
+  }
+  f32x16_t acc[RB][CB];
+#pragma unroll
+  for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+#pragma unroll
+      for (int r = 0; r < 16; r++) acc[rb][cb][r] = 0.f;
+  // byte offsets of the sources of index row `row` (read one step before
+  // the loads that use them)
+  auto read_idx = [&](int row, int (&iv)[CB]) __attribute__((always_inline)) {
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++)
+      iv[cb] = idx_tab[wave][cb / 2][row * 64 + 32 * (cb % 2) + c32];
+  };
+  // B fragments of k step s of the sources iv
+  const unsigned goff = (unsigned)(g * NPART * 16);
+  auto load_src = [&](int s, const int (&iv)[CB], bf16x8_t (&dst)[CB][NPART])
+      __attribute__((always_inline)) {
+    const unsigned so = (unsigned)(s * 2 * NPART * 16) + goff;
+#pragma unroll
+    for (int cb = 0; cb < CB; cb++) {
+      const unsigned ob = (unsigned)iv[cb] + so;
+#pragma unroll
+      for (int q = 0; q < NPART; q++)
+        dst[cb][q] = __builtin_bit_cast(
+            bf16x8_t, __builtin_amdgcn_raw_buffer_load_b128(
+                          mb_rs, (int)(ob + 16 * q), 0, 0));
+    }
+  };
+  bf16x8_t bq[NB][CB][NPART];
+  int ib[2][CB];
+  // sources of steps 0 .. DL - 1, in step order as the loop issues them
+  // (the loop's waits count the loads issued after each step's)
+#pragma unroll
+  for (int d = 0; d < DL; d++) {
+    read_idx(d < nsteps ? d / KS : nslots, ib[0]);
+    load_src(d % KS, ib[0], bq[d]);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+  read_idx(DL < nsteps ? DL / KS : nslots, ib[0]);
+  __syncthreads();  // operator group 0
+  auto read_a = [&](const u32x4_t *ra, int rb, bf16x8_t (&a)[NPART])
+      __attribute__((always_inline)) {
+#pragma unroll
+    for (int q = 0; q < NPART; q++)
+      a[q] = __builtin_bit_cast(bf16x8_t, ra[(rb * NPART + q) * 64 + lane]);
+  };
+  // streams as (row, k step): sources of step t + DL, operator units of
+  // step t + G + L
+  int rB = DL / KS, sB = DL % KS;
+  int rO = (G + W1_L) / KS, sO = (G + W1_L) % KS;
+  const int nbody = (nsteps + NB - 1) / NB;
+  for (int it = 0; it < nbody; it++) {
+    mfma_static_for<0, NB>([&](auto gc) {
+      constexpr int gg = decltype(gc)::value;
+      const int t = it * NB + gg;
+      bf16x8_t (&cur)[CB][NPART] = bq[gg % NB];
+      bf16x8_t (&nxt)[CB][NPART] = bq[(gg + DL) % NB];
+      // operator units of step t + G (loaded L steps ago) into the ring
+      // slot of the next group, units of step t + G + L into the same
+      // registers, the code of step t + G + L + 1
+      stage_store(t + G, stg[gg % W1_L]);
+      stage_load(cbuf[gg % 2], sO, stg[gg % W1_L]);
+      {
+        int s1 = sO + 1, r1 = rO;
+        if (s1 == KS) {
+          s1 = 0;
+          r1++;
+        }
+        rO = r1;
+        sO = s1;
+        cbuf[(gg + 1) % 2] = codes[k0 + min(r1, nslots - 1)];
+      }
+      // sources of step t + DL, indices of step t + DL + 1
+      {
+        load_src(sB, ib[gg % 2], nxt);
+        int s1 = sB + 1, r1 = rB;
+        if (s1 == KS) {
+          s1 = 0;
+          r1++;
+        }
+        rB = r1;
+        sB = s1;
+        read_idx(min(r1, nslots), ib[(gg + 1) % 2]);
+      }
+      // products of step t: row block by row block, each block's operator
+      // fragments read one block ahead; per accumulator the products in
+      // the order (m,h) (h,m) (h,h)
+      const u32x4_t *ra = ring[(t / G) % 2] + (t % G) * CHUNK;
+      bf16x8_t a[2][NPART];
+      read_a(ra, 0, a[0]);
+      mfma_static_for<0, RB>([&](auto rc) {
+        constexpr int rb = decltype(rc)::value;
+        if constexpr (rb + 1 < RB) read_a(ra, rb + 1, a[(rb + 1) % 2]);
+        constexpr int QA[3] = {1, 0, 0}, QB[3] = {0, 1, 0};
+#pragma unroll
+        for (int q = 0; q < 3; q++)
+#pragma unroll
+          for (int cb = 0; cb < CB; cb++)
+            acc[rb][cb] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(
+                a[rb % 2][QA[q]], cur[cb][QB[q]], acc[rb][cb], 0, 0, 0);
+      });
+      // issue order: index reads and the first two row blocks' fragments,
+      // source loads between the first products, each later row block's
+      // fragments one block ahead of its products
+      __builtin_amdgcn_sched_group_barrier(0x100, 2 + 2 * NPART, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, CB * NPART / 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 4, 0);
+      __builtin_amdgcn_sched_group_barrier(0x020, CB * NPART / 2, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 3 * CB - 8, 0);
+      mfma_static_for<1, RB>([&](auto rc) {
+        constexpr int rb = decltype(rc)::value;
+        if constexpr (rb + 1 < RB)
+          __builtin_amdgcn_sched_group_barrier(0x100, NPART, 0);
+        __builtin_amdgcn_sched_group_barrier(0x008, 3 * CB, 0);
+      });
+      if constexpr (gg % G == G - 1) __syncthreads();
+      // each step is scheduled on its own (loads stay in their step)
+      __builtin_amdgcn_sched_barrier(0);
+    });
+  }
+  // steps past the range multiplied zero sources: every accumulator holds
+  // the sum over the range
+#pragma unroll
+  for (int h = 0; h < TPW; h++) {
+    if (tl[h] < 0) continue;
+    float *out = Lsm + ((size_t)split * ntiles + tl[h]) * RO * 64;
+#pragma unroll
+    for (int rb = 0; rb < RB; rb++)
+#pragma unroll
+      for (int cc = 0; cc < 2; cc++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+          const int row = 32 * rb + 8 * (r / 4) + 4 * g + (r % 4);
+          if (row < RO)
+            out[(size_t)row * 64 + 32 * cc + c32] = acc[rb][2 * h + cc][r];
+        }
+  }
+}
+
 #ifndef ARAN_MFMA_PMIN
 #define ARAN_MFMA_PMIN 1
 #define ARAN_MFMA_PMAX 15
 #endif
+
+// ARAN_MFMA_KERNEL=8w selects k_m2l_mfma for the three-product form (the
+// same sums; for comparisons)
+static bool mfma_one_wave() {
+  const char *e = getenv("ARAN_MFMA_KERNEL");
+  return !(e && e[0] == '8');
+}
 
 template <int P>
 static hipError_t mfma_range(int nprod, int p, int nwg, const int *wgt,
@@ -300,6 +601,15 @@ static hipError_t mfma_range(int nprod, int p, int nwg, const int *wgt,
     if (p == P) {
       const long wgs = (long)nwg * nsplit;
       const int nblocks = (int)((wgs + 7) / 8 * 8);
+      // the one-wave kernel where its index tables and 32-bit offsets fit
+      const long mb_bytes = ((long)zero_node + 1) * W1Cfg<P>::kNodeBytes;
+      if (nprod == 3 && nsplit >= kW1MinRanges && mb_bytes < (long)kW1Missing &&
+          mfma_one_wave()) {
+        k_m2l_mfma1w<P><<<nblocks, 64 * kW1Waves, 0, s>>>(
+            wgt, nwg, tsrc, toct, codes, ops, mb, (int)mb_bytes, Lsm, ntiles,
+            nsplit, sched);
+        return hipGetLastError();
+      }
       if (nprod == 6)
         k_m2l_mfma<P, 6><<<nblocks, 64 * kMfmaWaves, 0, s>>>(
             wgt, nwg, tsrc, toct, codes, ops, mb, zero_node, Lsm, ntiles,

@@ -115,3 +115,54 @@ def test_mfma_batch_and_semifar(torch_cuda):
     f.set_semifar(None)
     a, b = res["bf16x6"], res["plain"]
     assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 5e-7
+
+
+@pytest.mark.parametrize("kind,n,p,q", [("plummer", 1 << 17, 10, 64),
+                                        ("uniform", 20_000, 8, 32),
+                                        ("plummer", 8000, 6, 16),
+                                        ("uniform", 6000, 12, 32),
+                                        ("plummer", 5000, 15, 32),
+                                        ("uniform", 3000, 2, 8),
+                                        ("plummer", 3000, 1, 16)])
+def test_one_wave_kernel_equals_eight_wave_kernel(torch_cuda, monkeypatch,
+                                                  kind, n, p, q):
+    """The three-product far field has two kernels that give every
+    accumulator the same MFMA sequence: k_m2l_mfma1w (one wave per SIMD,
+    256 accumulators; the default) and k_m2l_mfma (eight waves per
+    workgroup; ARAN_MFMA_KERNEL=8w).  Their solves agree bit for bit."""
+    torch = torch_cuda
+    from aran_amd.fmm import Fmm3d
+    (x, y, z, m), lb, ub = _case(kind, n, 17)
+    dev = _dev(torch, x, y, z, m)
+    f = Fmm3d()
+    f.set_precision("bf16x3")
+    monkeypatch.setenv("ARAN_MFMA_KERNEL", "8w")
+    p8, g8 = [t.cpu().numpy() for t in f.solve(*dev, lb, ub, p, q)]
+    monkeypatch.delenv("ARAN_MFMA_KERNEL")
+    p1, g1 = [t.cpu().numpy() for t in f.solve(*dev, lb, ub, p, q)]
+    assert f.far_plan()["tiles"] > 0
+    assert np.array_equal(p1, p8) and np.array_equal(g1, g8)
+
+
+def test_one_wave_kernel_batch_equals_eight_wave_kernel(torch_cuda,
+                                                         monkeypatch):
+    """The same for a batch of independent systems of mixed sizes (tiles of
+    several systems and levels in one launch, workgroups with missing
+    tiles)."""
+    torch = torch_cuda
+    from aran_amd.fmm import Fmm3d
+    systems = [_case("plummer", 4000, 3), _case("uniform", 6000, 4),
+               _case("plummer", 2500, 5), _case("uniform", 700, 6),
+               _case("plummer", 30_000, 7)]
+    offs = np.cumsum([0] + [len(s[0][0]) for s in systems])
+    cat = [np.concatenate([s[0][i] for s in systems]) for i in range(4)]
+    dev = _dev(torch, *cat)
+    lbs = [s[1] for s in systems]
+    ubs = [s[2] for s in systems]
+    f = Fmm3d()
+    f.set_precision("bf16x3")
+    monkeypatch.setenv("ARAN_MFMA_KERNEL", "8w")
+    a = [t.cpu().numpy() for t in f.solve_batch(offs, *dev, lbs, ubs, 8, 32)]
+    monkeypatch.delenv("ARAN_MFMA_KERNEL")
+    b = [t.cpu().numpy() for t in f.solve_batch(offs, *dev, lbs, ubs, 8, 32)]
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
