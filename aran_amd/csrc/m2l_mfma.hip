@@ -365,29 +365,68 @@ __global__ __launch_bounds__(64 * kW1Waves) __attribute__((
   const int oct = __builtin_amdgcn_readfirstlane(tile_oct[wt[0]]);
   const int chunk = (kOctOffsets + nsplit - 1) / nsplit;
   const int k0 = split * chunk, k1 = min(kOctOffsets, k0 + chunk);
-  const int nslots = max(0, k1 - k0);
-  const int nsteps = nslots * KS;
-  for (int k = tid; k < kOctOffsets; k += NT)
-    codes[k] = oct_codes[oct * kOctOffsets + k];
-  // index tables of the wave's tiles: rows k0 .. k1 - 1 (64 sources each)
-  // as byte offsets; missing sources, missing tiles and the rows from
-  // nslots on (read by steps past the range) hold kW1Missing
+  const int nrange = max(0, k1 - k0);
+  __shared__ unsigned live_bits[2];
+  if (tid < 2) live_bits[tid] = 0u;
+  __syncthreads();
+  // index tables of the wave's tiles: rows k0 .. k1 - 1 (64 sources each,
+  // one 16-byte load per lane and row quarter) as byte offsets; a missing
+  // source or tile holds kW1Missing.  Row j is live for the wave when one
+  // of its tiles has a source at slot k0 + j.
+  unsigned long long wlive = 0;
+  for (int e = lane; e < kW1Rows * 16; e += 64) {
+    const int row0 = (e - lane) / 16;  // rows row0 .. row0 + 3 of this pass
+    unsigned long long any = 0;
 #pragma unroll
-  for (int h = 0; h < TPW; h++) {
-    const int *base = tile_src + ((size_t)max(tl[h], 0) * kOctOffsets + k0) * 64;
-    u32x4_t *dst = (u32x4_t *)&idx_tab[wave][h][0];
-    for (int e = lane; e < kW1Rows * 16; e += 64) {
+    for (int h = 0; h < TPW; h++) {
+      const bool valid = tl[h] >= 0 && e < nrange * 16;
+      const int4 raw = valid ? ((const int4 *)(tile_src + ((size_t)tl[h] * kOctOffsets + k0) * 64))[e]
+                             : make_int4(-1, -1, -1, -1);
       u32x4_t v;
-#pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int el = 4 * e + j;
-        const int nd = (tl[h] >= 0 && el < nslots * 64) ? base[el] : -1;
-        v[j] = nd >= 0 ? (unsigned)nd * C::kNodeBytes : kW1Missing;
-      }
-      dst[e] = v;
+      v[0] = raw.x >= 0 ? (unsigned)raw.x * C::kNodeBytes : kW1Missing;
+      v[1] = raw.y >= 0 ? (unsigned)raw.y * C::kNodeBytes : kW1Missing;
+      v[2] = raw.z >= 0 ? (unsigned)raw.z * C::kNodeBytes : kW1Missing;
+      v[3] = raw.w >= 0 ? (unsigned)raw.w * C::kNodeBytes : kW1Missing;
+      ((u32x4_t *)&idx_tab[wave][h][0])[e] = v;
+      any |= __ballot(raw.x >= 0 || raw.y >= 0 || raw.z >= 0 || raw.w >= 0);
     }
+#pragma unroll
+    for (int r = 0; r < 4; r++)
+      if ((any >> (16 * r)) & 0xffffull) wlive |= 1ull << (row0 + r);
   }
-  __syncthreads();  // codes and index tables
+  if (lane == 0) {
+    atomicOr(&live_bits[0], (unsigned)wlive);
+    atomicOr(&live_bits[1], (unsigned)(wlive >> 32));
+  }
+  __syncthreads();  // index tables, liveness of every wave
+  // the workgroup runs only the slots live for one of its waves: rows and
+  // codes of those slots, compacted in slot order (a row moves down only)
+  const unsigned long long live =
+      ((unsigned long long)live_bits[1] << 32) | live_bits[0];
+  const int nslots = __builtin_popcountll(live);
+  const int nsteps = nslots * KS;
+  {
+    unsigned long long rest = live;
+    int pos = 0;
+    while (rest) {
+      const int j = __builtin_ctzll(rest);
+      rest &= rest - 1;
+      if (j != pos) {
+#pragma unroll
+        for (int h = 0; h < TPW; h++)
+          idx_tab[wave][h][pos * 64 + lane] = idx_tab[wave][h][j * 64 + lane];
+      }
+      pos++;
+    }
+    // row nslots: read by the steps past the range (zero sources)
+#pragma unroll
+    for (int h = 0; h < TPW; h++)
+      idx_tab[wave][h][nslots * 64 + lane] = (int)kW1Missing;
+    if (tid < nrange && ((live >> tid) & 1ull))
+      codes[__builtin_popcountll(live & ((1ull << tid) - 1))] =
+          oct_codes[oct * kOctOffsets + k0 + tid];
+  }
+  __syncthreads();  // compacted codes
   const __amdgpu_buffer_rsrc_t ops_rs = __builtin_amdgcn_make_buffer_rsrc(
       (void *)ops, (short)0, (int)((long)kNumCodes * KS * RB * 3 * 64 * 16),
       0x00020000);
@@ -426,15 +465,15 @@ __global__ __launch_bounds__(64 * kW1Waves) __attribute__((
 #pragma unroll
     for (int j = 0; j < G; j++) {
       const int tc = min(j, nsteps - 1);
-      stage_load(codes[k0 + tc / KS], tc % KS, stg[0]);
+      stage_load(codes[tc / KS], tc % KS, stg[0]);
       stage_store(j, stg[0]);
     }
 #pragma unroll
     for (int j = 0; j < W1_L; j++) {
       const int tc = min(G + j, nsteps - 1);
-      stage_load(codes[k0 + tc / KS], tc % KS, stg[j]);
+      stage_load(codes[tc / KS], tc % KS, stg[j]);
     }
-    cbuf[0] = codes[k0 + min(G + W1_L, nsteps - 1) / KS];
+    cbuf[0] = codes[min(G + W1_L, nsteps - 1) / KS];
   }
   f32x16_t acc[RB][CB];
 #pragma unroll
@@ -507,7 +546,7 @@ __global__ __launch_bounds__(64 * kW1Waves) __attribute__((
         }
         rO = r1;
         sO = s1;
-        cbuf[(gg + 1) % 2] = codes[k0 + min(r1, nslots - 1)];
+        cbuf[(gg + 1) % 2] = codes[min(r1, nslots - 1)];
       }
       // sources of step t + DL, indices of step t + DL + 1
       {
