@@ -373,26 +373,42 @@ __global__ __launch_bounds__(64 * kW1Waves) __attribute__((
   // one 16-byte load per lane and row quarter) as byte offsets; a missing
   // source or tile holds kW1Missing.  Row j is live for the wave when one
   // of its tiles has a source at slot k0 + j.
+  // the range's codes (one per thread, written compacted below), loaded
+  // beside the tables
+  const int my_code = tid < nrange ? oct_codes[oct * kOctOffsets + k0 + tid] : 0;
   unsigned long long wlive = 0;
-  for (int e = lane; e < kW1Rows * 16; e += 64) {
-    const int row0 = (e - lane) / 16;  // rows row0 .. row0 + 3 of this pass
-    unsigned long long any = 0;
+  {
+    // every load first (one round trip), then the stores
+    constexpr int NPASS = kW1Rows * 16 / 64;
+    int4 raw[NPASS][TPW];
 #pragma unroll
-    for (int h = 0; h < TPW; h++) {
-      const bool valid = tl[h] >= 0 && e < nrange * 16;
-      const int4 raw = valid ? ((const int4 *)(tile_src + ((size_t)tl[h] * kOctOffsets + k0) * 64))[e]
-                             : make_int4(-1, -1, -1, -1);
-      u32x4_t v;
-      v[0] = raw.x >= 0 ? (unsigned)raw.x * C::kNodeBytes : kW1Missing;
-      v[1] = raw.y >= 0 ? (unsigned)raw.y * C::kNodeBytes : kW1Missing;
-      v[2] = raw.z >= 0 ? (unsigned)raw.z * C::kNodeBytes : kW1Missing;
-      v[3] = raw.w >= 0 ? (unsigned)raw.w * C::kNodeBytes : kW1Missing;
-      ((u32x4_t *)&idx_tab[wave][h][0])[e] = v;
-      any |= __ballot(raw.x >= 0 || raw.y >= 0 || raw.z >= 0 || raw.w >= 0);
+    for (int i = 0; i < NPASS; i++)
+#pragma unroll
+      for (int h = 0; h < TPW; h++) {
+        const int e = lane + 64 * i;
+        raw[i][h] = (tl[h] >= 0 && e < nrange * 16)
+                        ? ((const int4 *)(tile_src + ((size_t)tl[h] * kOctOffsets + k0) * 64))[e]
+                        : make_int4(-1, -1, -1, -1);
+      }
+#pragma unroll
+    for (int i = 0; i < NPASS; i++) {
+      unsigned long long any = 0;
+#pragma unroll
+      for (int h = 0; h < TPW; h++) {
+        const int4 r4 = raw[i][h];
+        u32x4_t v;
+        v[0] = r4.x >= 0 ? (unsigned)r4.x * C::kNodeBytes : kW1Missing;
+        v[1] = r4.y >= 0 ? (unsigned)r4.y * C::kNodeBytes : kW1Missing;
+        v[2] = r4.z >= 0 ? (unsigned)r4.z * C::kNodeBytes : kW1Missing;
+        v[3] = r4.w >= 0 ? (unsigned)r4.w * C::kNodeBytes : kW1Missing;
+        ((u32x4_t *)&idx_tab[wave][h][0])[lane + 64 * i] = v;
+        any |= __ballot(r4.x >= 0 || r4.y >= 0 || r4.z >= 0 || r4.w >= 0);
+      }
+      // rows 4i .. 4i + 3 of this pass: 16 lanes each
+#pragma unroll
+      for (int r = 0; r < 4; r++)
+        if ((any >> (16 * r)) & 0xffffull) wlive |= 1ull << (4 * i + r);
     }
-#pragma unroll
-    for (int r = 0; r < 4; r++)
-      if ((any >> (16 * r)) & 0xffffull) wlive |= 1ull << (row0 + r);
   }
   if (lane == 0) {
     atomicOr(&live_bits[0], (unsigned)wlive);
@@ -423,8 +439,7 @@ __global__ __launch_bounds__(64 * kW1Waves) __attribute__((
     for (int h = 0; h < TPW; h++)
       idx_tab[wave][h][nslots * 64 + lane] = (int)kW1Missing;
     if (tid < nrange && ((live >> tid) & 1ull))
-      codes[__builtin_popcountll(live & ((1ull << tid) - 1))] =
-          oct_codes[oct * kOctOffsets + k0 + tid];
+      codes[__builtin_popcountll(live & ((1ull << tid) - 1))] = (short)my_code;
   }
   __syncthreads();  // compacted codes
   const __amdgpu_buffer_rsrc_t ops_rs = __builtin_amdgcn_make_buffer_rsrc(
@@ -462,17 +477,19 @@ __global__ __launch_bounds__(64 * kW1Waves) __attribute__((
   u32x4_t stg[W1_L][SPT];
   int cbuf[2];
   if (nsteps > 0) {
+    u32x4_t g0[G][SPT];
 #pragma unroll
     for (int j = 0; j < G; j++) {
       const int tc = min(j, nsteps - 1);
-      stage_load(codes[tc / KS], tc % KS, stg[0]);
-      stage_store(j, stg[0]);
+      stage_load(codes[tc / KS], tc % KS, g0[j]);
     }
 #pragma unroll
     for (int j = 0; j < W1_L; j++) {
       const int tc = min(G + j, nsteps - 1);
       stage_load(codes[tc / KS], tc % KS, stg[j]);
     }
+#pragma unroll
+    for (int j = 0; j < G; j++) stage_store(j, g0[j]);
     cbuf[0] = codes[min(G + W1_L, nsteps - 1) / KS];
   }
   f32x16_t acc[RB][CB];
