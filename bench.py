@@ -656,7 +656,10 @@ def main():
             "bf16x3": MI355X_BF16_TFLOPS, "bf16x6": MI355X_BF16_TFLOPS}[form]
     kname = {"plain": "k_m2l_f64 (FP64 MFMA)" if dense else "k_m2l (FP64)",
              "mixed": "k_m2l_tile (FP32)",
-             "bf16x3": "k_m2l_mfma (BF16 MFMA, 3 products)",
+             "bf16x3": "k_m2l_mfma1w (BF16 MFMA, 3 products, one wave "
+                       "per SIMD)" if not os.environ.get(
+                           "ARAN_MFMA_KERNEL", "").startswith("8") else
+                       "k_m2l_mfma (BF16 MFMA, 3 products)",
              "bf16x6": "k_m2l_mfma (BF16 MFMA, 6 products)"}[form]
     flops = st["m2l"] * m2l_flops(p)
     achieved = flops / (head["m2l_ms"] * 1e-3) / 1e12
