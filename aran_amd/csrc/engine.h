@@ -150,6 +150,9 @@ struct Workspace {
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
       tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma, mult64, loc64;
   DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos, stop;
+  // look-back counters (first 256 bytes) and status words (Lookback)
+  DevBuf lb_buf;
+  unsigned lb_epoch = 0;
   long node_cap = 0;
   int depth_hint = 0;  // depth of the last tree (levels of the first batch)
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
@@ -275,17 +278,33 @@ hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
                           hipStream_t s);
 hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
                         hipStream_t s);
+// Status words and counters of a launch that scans across its workgroups
+// by decoupled look-back (one word per chunk of work).  The words need no
+// clearing between launches (each launch has its own epoch); the counters
+// start at zero and the launch leaves them at zero.
+struct Lookback {
+  unsigned long long *status;
+  unsigned *ctr;  // [0] chunk tickets, [1] workgroups finished
+  int *err;       // set to 1 when a wait gave up (a wrong result)
+  unsigned epoch; // 1 .. 2^30 - 1
+};
+// Exclusive scan of n ints in one launch of scan_lookback_chunks(n)
+// workgroups (in and out may alias); writes the total to *total.
+hipError_t scan_lookback(const int *in, int *out, long n, int *total,
+                         const Lookback &lb, hipStream_t s);
+long scan_lookback_chunks(long n);
+// whether scan_small takes its one-workgroup launch for n entries
+bool scan_fits_single(long n);
 // Level table on the device: dl[2 d] first node of level d, dl[2 d + 1] its
-// node count; max_count bounds the count for the grid.  split_count also
-// sets dl[2 d + 2] (the next level's first node); the scan of cnt writes
-// dl[2 d + 3].
-hipError_t launch_split_count(const NodeArrays &nd, int *dl, int d,
+// node count; max_count bounds the count for the grid.  One launch splits
+// level d and sets dl[2 d + 2] (the next level's first node) and
+// dl[2 d + 3] (its count); it uses split_level_chunks(max_count) status
+// words.
+hipError_t launch_split_level(const NodeArrays &nd, int *dl, int d,
                               long max_count, const uint64_t *keys,
                               int leafcap, const ulonglong2 *stop, int nstop,
-                              int *bounds, int *cnt, hipStream_t s);
-hipError_t launch_split_write(const NodeArrays &nd, const int *dl, int d,
-                              long max_count, const int *bounds,
-                              const int *cnt, const int *ofs, hipStream_t s);
+                              const Lookback &lb, hipStream_t s);
+long split_level_chunks(long max_count);
 // Exclusive scan of (*n_dev) entries, one launch.
 hipError_t scan_device_count(const int *in, int *out, const int *n_dev,
                              int *total, hipStream_t s);

@@ -197,7 +197,7 @@ std::vector<DevBuf *> Workspace::buffers() {
                    &sys_off, &sys_mm,
                    &skey, &spos, &stop, &internal_list,
                    &trans_rows, &packed,
-                   &lvl_table, &child_map};
+                   &lvl_table, &child_map, &lb_buf};
   Tables *ts[1] = {&tab};
   for (Tables *t : ts) {
     DevBuf *tb[] = {&t->rot,    &t->eiphi,  &t->dir,    &t->tm2l,   &t->tm2m,
@@ -263,8 +263,18 @@ hipError_t workspace_init(Workspace &ws, hipStream_t stream) {
     ws.events_ready = true;
   }
   if (!ws.aux) CK(hipStreamCreateWithFlags(&ws.aux, hipStreamNonBlocking));
+  const void *old_small = ws.small.ptr;
   CK(ws.small.reserve(4096));
+  // (zero once: the look-back error word in it is sticky)
+  if (ws.small.ptr != old_small)
+    CK(launch_fill_int(ws.small.as<int>(), 1024, 0, ws.stream));
   return hipSuccess;
+}
+
+// Error word of the look-back launches outside the level build (set when a
+// wait gave up); read back with the pair count at the end of each solve.
+static int *lb_err_word(Workspace &ws) {
+  return reinterpret_cast<int *>(ws.small.as<char>() + 1016);
 }
 
 // Pinned host memory for one copy of this solve (see Workspace::host_stage).
@@ -298,6 +308,40 @@ static hipError_t stage_copy(Workspace &ws, void *dst, const void *src,
   CK(stage_alloc(ws, bytes, &h));
   memcpy(h, src, bytes);
   return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s);
+}
+
+// Look-back state of one launch over nchunks chunks.  The status words
+// are cleared when their buffer is (re)allocated or the epoch wraps; the
+// counters start at zero then and every launch leaves them at zero.
+static hipError_t lookback_take(Workspace &ws, long nchunks, int *err,
+                                Lookback *lb, hipStream_t s) {
+  const void *old = ws.lb_buf.ptr;
+  const size_t old_bytes = ws.lb_buf.bytes;
+  CK(ws.lb_buf.reserve(256 + sizeof(unsigned long long) * (size_t)nchunks));
+  bool clear = ws.lb_buf.ptr != old || ws.lb_buf.bytes != old_bytes;
+  if (++ws.lb_epoch >= (1u << 30)) {
+    ws.lb_epoch = 1;
+    clear = true;
+  }
+  if (clear)
+    CK(launch_fill_int(ws.lb_buf.as<int>(), (long)(ws.lb_buf.bytes / 4), 0, s));
+  lb->ctr = ws.lb_buf.as<unsigned>();
+  lb->status =
+      reinterpret_cast<unsigned long long *>(ws.lb_buf.as<char>() + 256);
+  lb->err = err;
+  lb->epoch = ws.lb_epoch;
+  return hipSuccess;
+}
+
+// Exclusive scan in one launch whatever n: one workgroup for short
+// arrays, look-back across workgroups for the others.
+static hipError_t scan_one_launch(Workspace &ws, const int *in, int *out,
+                                  long n, int *total, hipStream_t s) {
+  if (scan_fits_single(n))
+    return scan_small(in, out, n, total, ws.scan_tmp, s);
+  Lookback lb;
+  CK(lookback_take(ws, scan_lookback_chunks(n), lb_err_word(ws), &lb, s));
+  return scan_lookback(in, out, n, total, lb, s);
 }
 
 static void stage_reset(Workspace &ws) {
@@ -619,8 +663,8 @@ hipError_t build_far_lists(Workspace &ws, long nn, int *nfar) {
   CK(ws.v_ofs.reserve(sizeof(int) * (nn + 1)));
   CK(launch_vlist(nd, (int)nn, col, 0, ws.v_cnt.as<int>(), nullptr, nullptr,
                   nullptr, s));
-  CK(scan_small(ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
-                    ws.v_ofs.as<int>() + nn, ws.scan_tmp, s));
+  CK(scan_one_launch(ws, ws.v_cnt.as<int>(), ws.v_ofs.as<int>(), nn,
+                     ws.v_ofs.as<int>() + nn, s));
   int nf = 0;
   CK(read_int(ws, ws.v_ofs.as<int>() + nn, &nf));
   CK(ws.v_src.reserve(sizeof(int) * std::max(1, nf)));
@@ -895,9 +939,11 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   {
     constexpr int kLevelBatch = 4;
     const long max_split = n / ((long)pb.leafcap + 1);
-    CK(ws.lvl_table.reserve(sizeof(int) * 2 * (kMaxDepth + 3)));
+    // the table, then the look-back error word of the level launches
+    constexpr int kTab = 2 * (kMaxDepth + 3);
+    CK(ws.lvl_table.reserve(sizeof(int) * (kTab + 1)));
     int *d_lvl = ws.lvl_table.as<int>();
-    const int lvl0[2] = {0, nsys};
+    int lvl0[kTab + 1] = {0, nsys};
     CK(stage_copy(ws, d_lvl, lvl0, sizeof(lvl0), s));
     long ub = nsys;  // bound on the count of level d
     bool done = false;
@@ -911,44 +957,41 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
           kMaxDepth, d0 + (d0 == 0 ? first_batch : kLevelBatch) - 1);
       // bounds of the levels in this batch and of the nodes they add
       std::vector<long> ubs;
-      long b = ub, add = 0, bmax = 1;
+      long b = ub, add = 0, bmax = 1;  // bmax: the largest level bound
       for (int d = d0; d <= d1; d++) {
         ubs.push_back(b);
         bmax = std::max(bmax, b);
         b = std::min(8 * b, 8 * max_split);
         add += b;
       }
-      CK(ws.split_bounds.reserve(sizeof(int) * 9 * (size_t)bmax));
-      CK(ws.split_cnt.reserve(sizeof(int) * (size_t)bmax));
-      CK(ws.split_ofs.reserve(sizeof(int) * ((size_t)bmax + 1)));
       if (nnodes + add > ws.node_cap) {
         CK(reserve_nodes(ws, std::max(2 * ws.node_cap, nnodes + add), nnodes));
         nd = ws.nodes();
       }
       for (int d = d0; d <= d1; d++) {
         const long ubd = ubs[d - d0];
-        CK(launch_split_count(nd, d_lvl, d, ubd, keys, pb.leafcap, d_stop,
-                              pb.nstop, ws.split_bounds.as<int>(),
-                              ws.split_cnt.as<int>(), s));
-        CK(scan_device_count(ws.split_cnt.as<int>(), ws.split_ofs.as<int>(),
-                             d_lvl + 2 * d + 1, d_lvl + 2 * d + 3, s));
-        CK(launch_split_write(nd, d_lvl, d, ubd, ws.split_bounds.as<int>(),
-                              ws.split_cnt.as<int>(), ws.split_ofs.as<int>(),
-                              s));
+        Lookback lb;  // (sized for the batch: one allocation at most)
+        CK(lookback_take(ws, split_level_chunks(bmax), d_lvl + kTab, &lb, s));
+        CK(launch_split_level(nd, d_lvl, d, ubd, keys, pb.leafcap, d_stop,
+                              pb.nstop, lb, s));
       }
       int *h_tab = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                            3072);
-      const int nget = 2 * (d1 - d0 + 1);
-      CK(hipMemcpyAsync(h_tab, d_lvl + 2 * d0 + 2, sizeof(int) * nget,
+      CK(hipMemcpyAsync(h_tab, d_lvl, sizeof(int) * (kTab + 1),
                         hipMemcpyDeviceToHost, s));
       CK(hipStreamSynchronize(s));
+      if (h_tab[kTab] != 0) {
+        set_error("tree build: a level launch gave up waiting (internal "
+                  "error)");
+        return hipErrorUnknown;
+      }
       for (int d = d0; d <= d1; d++) {
-        const int cnt = h_tab[2 * (d - d0) + 1];
+        const int cnt = h_tab[2 * d + 3];
         if (cnt == 0) {
           done = true;
           break;
         }
-        lstart.push_back(h_tab[2 * (d - d0)]);
+        lstart.push_back(h_tab[2 * d + 2]);
         lcount.push_back(cnt);
         nnodes += cnt;
       }
@@ -973,8 +1016,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(stage_copy(ws, ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
                 s));
   CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
-  CK(scan_small(ws.flags.as<int>(), ws.split_ofs.as<int>(), nn, d_total,
-                    ws.scan_tmp, s));
+  CK(scan_one_launch(ws, ws.flags.as<int>(), ws.split_ofs.as<int>(), nn,
+                     d_total, s));
   int *d_lvl = reinterpret_cast<int *>(ws.small.as<char>() + 3584);
   int *h_lvl = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                        3584);
@@ -1006,8 +1049,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)tcount + 1)));
     trank = ws.tile_rank.as<int>();
     CK(launch_octant_flags(nd, tfirst, tcount, trank, s));
-    CK(scan_small(trank, trank, 8L * tcount, trank + 8L * tcount,
-                  ws.scan_tmp, s));
+    CK(scan_one_launch(ws, trank, trank, 8L * tcount, trank + 8L * tcount,
+                       s));
     CK(launch_group_counts(tfirst, tcount, depth, d_meta, trank, cnt, s));
     CK(hipMemcpyAsync(const_cast<int *>(h_cnt), cnt, sizeof(int) * ngroups,
                       hipMemcpyDeviceToHost, s));
@@ -1031,6 +1074,13 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (pb.timings) CK(hipEventRecord(ws.ev[1], s));
   if (pb.tree_only) {
     CK(hipStreamSynchronize(s));
+    int lb_err = 0;
+    CK(read_int(ws, lb_err_word(ws), &lb_err));
+    if (lb_err != 0) {
+      CK(launch_fill_int(lb_err_word(ws), 1, 0, s));
+      set_error("a scan launch gave up waiting (internal error)");
+      return hipErrorUnknown;
+    }
     S.depth = depth;
     S.nnodes = nn;
     S.nleaves = nleaves;
@@ -1082,13 +1132,13 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                             nullptr, s));
     CK(launch_xlist(nd, nn, col, pb.semifar, 0, ws.x_cnt.as<int>(), nullptr,
                     nullptr, s));
-    CK(scan_small(ws.w_cnt.as<int>(), ws.w_ofs.as<int>(), nleaves,
-                      ws.w_ofs.as<int>() + nleaves, ws.scan_tmp, s));
-    CK(scan_small(ws.x_cnt.as<int>(), ws.x_ofs.as<int>(), nn,
-                      ws.x_ofs.as<int>() + nn, ws.scan_tmp, s));
+    CK(scan_one_launch(ws, ws.w_cnt.as<int>(), ws.w_ofs.as<int>(), nleaves,
+                       ws.w_ofs.as<int>() + nleaves, s));
+    CK(scan_one_launch(ws, ws.x_cnt.as<int>(), ws.x_ofs.as<int>(), nn,
+                       ws.x_ofs.as<int>() + nn, s));
   }
-  CK(scan_small(ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
-                    ws.u_ofs.as<int>() + nleaves, ws.scan_tmp, s));
+  CK(scan_one_launch(ws, ws.u_cnt.as<int>(), ws.u_ofs.as<int>(), nleaves,
+                     ws.u_ofs.as<int>() + nleaves, s));
   // ---- tile tables from the group counts (host, while the lists run);
   // within a level the tiles take chunk j of every octant in turn, so that
   // consecutive tiles cover the same region
@@ -1435,7 +1485,11 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
       static_cast<char *>(ws.host_small) + 1024);
   int *hfar = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                       2048);
-  CK(hipMemcpyAsync(hp, d_pairs, sizeof(long long), hipMemcpyDeviceToHost, s));
+  // the look-back error word sits just before the pair count
+  int *herr = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
+                                      1016);
+  CK(hipMemcpyAsync(herr, lb_err_word(ws), 8 + sizeof(long long),
+                    hipMemcpyDeviceToHost, s));
   long long *hsp = reinterpret_cast<long long *>(
       static_cast<char *>(ws.host_small) + 2560);
   *hsp = 0;
@@ -1445,6 +1499,11 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   if (tiled)
     CK(hipMemcpyAsync(hfar, d_far, sizeof(int), hipMemcpyDeviceToHost, s));
   CK(hipStreamSynchronize(s));
+  if (*herr != 0) {
+    CK(launch_fill_int(lb_err_word(ws), 1, 0, s));
+    set_error("a scan launch gave up waiting (internal error)");
+    return hipErrorUnknown;
+  }
   if (pb.precision != 0 || tiled) nfar = tiled ? *hfar : 0;
   S.zero = nn;
   S.p2p = (long)(*hp / 2);

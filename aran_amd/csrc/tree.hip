@@ -190,6 +190,101 @@ hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
   return hipGetLastError();
 }
 
+// ------------------------------------------------------------ look-back
+//
+// A launch whose workgroups each own one chunk of a sequence and need the
+// sum of the totals of the chunks before theirs (a scan across workgroups)
+// gets it in the same launch: a workgroup publishes its chunk's total in the
+// chunk's status word, then reads the words of the chunks below, 64 at a
+// time, down to the nearest one that already holds its inclusive total
+// (decoupled look-back).  Chunks are handed out in ticket order, so a
+// workgroup only ever waits for chunks that running workgroups hold.
+//
+// Status word: bits 63..34 the launch's epoch, 33..32 the state (1: the
+// chunk's own total, 2: the total of chunks 0..c), 31..0 the value; a word
+// of another launch (another epoch, or the zero of the clear at allocation)
+// reads as not yet published, so the words need no clearing between
+// launches.  Every access to them is a single 64-bit atomic at agent scope.
+
+constexpr unsigned kLbAgg = 1u, kLbIncl = 2u;
+// polls before a wait gives up (it then sets *err and goes on with a wrong
+// prefix; the host turns the flag into an error)
+constexpr int kLbMaxPolls = 1 << 20;
+
+__device__ inline unsigned long long lb_word(unsigned epoch, unsigned st,
+                                             unsigned v) {
+  return ((unsigned long long)((epoch << 2) | st) << 32) | v;
+}
+
+// Ticket of the calling workgroup (every thread gets it).
+__device__ inline int lb_ticket(const Lookback &lb, int *sh) {
+  if (threadIdx.x == 0) *sh = (int)atomicAdd(&lb.ctr[0], 1u);
+  __syncthreads();
+  return *sh;
+}
+
+// Called once by every workgroup that took a ticket, after its last use of
+// the status words: the last of the nwork workgroups resets the counters
+// for the next launch.
+__device__ inline void lb_finish(const Lookback &lb, int nwork) {
+  if (threadIdx.x == 0) {
+    const unsigned done = atomicAdd(&lb.ctr[1], 1u);
+    if (done == (unsigned)nwork - 1) {
+      __hip_atomic_store(&lb.ctr[0], 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_store(&lb.ctr[1], 0u, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+}
+
+// Exclusive prefix of chunk c whose own total is 'total'; called by one
+// whole wave of the workgroup, every lane returns the prefix.
+__device__ inline unsigned lb_prefix_wave(const Lookback &lb, int c,
+                                          unsigned total) {
+  const int lane = threadIdx.x & 63;
+  if (lane == 0)
+    __hip_atomic_store(&lb.status[c],
+                       lb_word(lb.epoch, c == 0 ? kLbIncl : kLbAgg, total),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  unsigned pre = 0;
+  if (c == 0) return 0;
+  const unsigned tag_agg = (lb.epoch << 2) | kLbAgg;
+  const unsigned tag_incl = (lb.epoch << 2) | kLbIncl;
+  int top = c - 1, polls = 0;
+  while (true) {
+    const int k = top - lane;  // lane 0 reads the nearest chunk
+    unsigned long long w = 0;
+    if (k >= 0)
+      w = __hip_atomic_load(&lb.status[k], __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned tag = (unsigned)(w >> 32);
+    const bool incl = k < 0 || tag == tag_incl;  // below chunk 0: zero
+    const bool ready = incl || tag == tag_agg;
+    const unsigned long long im = __ballot(incl);
+    const int first = im ? __builtin_ctzll(im) : 64;
+    // the lanes up to the first inclusive word must all be published
+    const unsigned long long need = first >= 63 ? ~0ull : (2ull << first) - 1;
+    if (__ballot(!ready) & need) {
+      if (++polls > kLbMaxPolls) {
+        if (lane == 0) atomicOr(lb.err, 1);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    unsigned v = (lane <= first && k >= 0) ? (unsigned)w : 0u;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    pre += v;
+    if (first < 64) break;
+    top -= 64;
+  }
+  if (lane == 0)
+    __hip_atomic_store(&lb.status[c], lb_word(lb.epoch, kLbIncl, pre + total),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return pre;
+}
+
 // ------------------------------------------------------------ scan
 
 constexpr int kScanThreads = 256;
@@ -375,6 +470,97 @@ hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
                       hipStream_t s) {
   if (n > kScan1Max) return scan_exclusive(in, out, n, total, tmp, s);
   k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, (int)n, nullptr, total);
+  return hipGetLastError();
+}
+
+bool scan_fits_single(long n) { return n <= kScan1Max; }
+
+// ------------------------------------------------------------ look-back scan
+
+// Exclusive scan of n ints in one launch: workgroup c owns entries
+// [kLsTile c, kLsTile (c + 1)) (16 per thread, read with 16-byte loads when
+// whole), scans them and takes the total of the entries before its tile by
+// look-back.  Each thread reads its entries before it writes them, so in
+// and out may alias.  The workgroup of the last tile writes the total.
+constexpr int kLsThreads = 256;
+constexpr int kLsPer = 16;
+constexpr long kLsTile = (long)kLsThreads * kLsPer;
+
+__global__ __launch_bounds__(kLsThreads) void k_scan_lookback(
+    const int *in, int *out, long n, int *__restrict__ total, Lookback lb,
+    int nwork) {
+  __shared__ int sh_ticket;
+  __shared__ int ws[kLsThreads / 64];
+  __shared__ unsigned sh_pre;
+  const int c = lb_ticket(lb, &sh_ticket);
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const long b = c * kLsTile + (long)t * kLsPer;
+  const bool full = b + kLsPer <= n;
+  int v[kLsPer];
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < kLsPer; k += 4) {
+      const int4 q = *reinterpret_cast<const int4 *>(in + b + k);
+      v[k] = q.x;
+      v[k + 1] = q.y;
+      v[k + 2] = q.z;
+      v[k + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kLsPer; k++) v[k] = b + k < n ? in[b + k] : 0;
+  }
+  int sum = 0;
+#pragma unroll
+  for (int k = 0; k < kLsPer; k++) sum += v[k];
+  int x = sum;  // inclusive scan over the wave
+  for (int off = 1; off < 64; off <<= 1) {
+    const int y = __shfl_up(x, off);
+    if (lane >= off) x += y;
+  }
+  if (lane == 63) ws[w] = x;
+  __syncthreads();
+  int wpre = 0, tot = 0;
+  for (int k = 0; k < kLsThreads / 64; k++) {
+    if (k < w) wpre += ws[k];
+    tot += ws[k];
+  }
+  if (w == 0) {
+    const unsigned p = lb_prefix_wave(lb, c, (unsigned)tot);
+    if (lane == 0) sh_pre = p;
+  }
+  __syncthreads();
+  int run = (int)sh_pre + wpre + x - sum;
+  if (full) {
+#pragma unroll
+    for (int k = 0; k < kLsPer; k += 4) {
+      int4 q;
+      q.x = run;
+      q.y = q.x + v[k];
+      q.z = q.y + v[k + 1];
+      q.w = q.z + v[k + 2];
+      run = q.w + v[k + 3];
+      *reinterpret_cast<int4 *>(out + b + k) = q;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < kLsPer; k++) {
+      if (b + k < n) out[b + k] = run;
+      run += v[k];
+    }
+  }
+  if (c == nwork - 1 && t == 0) *total = (int)sh_pre + tot;
+  lb_finish(lb, nwork);
+}
+
+long scan_lookback_chunks(long n) {
+  return std::max(1L, (n + kLsTile - 1) / kLsTile);
+}
+
+hipError_t scan_lookback(const int *in, int *out, long n, int *total,
+                         const Lookback &lb, hipStream_t s) {
+  const long nb = scan_lookback_chunks(n);
+  k_scan_lookback<<<nb, kLsThreads, 0, s>>>(in, out, n, total, lb, (int)nb);
   return hipGetLastError();
 }
 
@@ -685,102 +871,120 @@ __device__ inline bool stopped(const ulonglong2 *__restrict__ stop, int nstop,
   return lo < nstop && stop[lo].x == dep && stop[lo].y == key;
 }
 
-__global__ void k_split_count(NodeArrays nd, int *__restrict__ dl, int d,
-                              const uint64_t *__restrict__ keys, int leafcap,
-                              const ulonglong2 *__restrict__ stop, int nstop,
-                              int *__restrict__ bounds, int *__restrict__ cnt) {
+// One launch per level: eight lanes per node of level d (32 nodes per
+// workgroup); lane o finds where octant digit o starts inside the node's
+// sorted key range (lane 0: the range start).  The node splits when it
+// holds more than leafcap particles below the maximum depth and is not in
+// the stop list; each non-empty octant of a split node becomes a child.
+// Children are numbered in (node, octant) order from the next level's first
+// node: the workgroup ranks its own children with ballots and takes the
+// count of the children of the nodes before its chunk by look-back.  The
+// chunk holding the level's last node writes the next level's first node
+// and count into the level table.
+constexpr int kLvNodes = 32;
+
+__global__ __launch_bounds__(256) void k_split_level(
+    NodeArrays nd, int *__restrict__ dl, int d,
+    const uint64_t *__restrict__ keys, int leafcap,
+    const ulonglong2 *__restrict__ stop, int nstop, Lookback lb) {
+  __shared__ int sh_ticket;
+  __shared__ unsigned sh_wcnt[4], sh_pre;
   const int lvl_begin = dl[2 * d], lvl_count = dl[2 * d + 1];
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t == 0) dl[2 * d + 2] = lvl_begin + lvl_count;
-  if (t >= 8L * lvl_count) return;  // whole groups of 8 (lvl_count * 8)
-  const int i = (int)(t / 8), o = (int)(t % 8);
+  // a level without nodes still has one chunk (it writes the zero count)
+  const int nwork = max(1, (lvl_count + kLvNodes - 1) / kLvNodes);
+  if ((int)gridDim.x < nwork) {  // the bound was wrong: report, touch nothing
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicOr(lb.err, 2);
+    return;
+  }
+  if ((int)blockIdx.x >= nwork) return;  // grid sized by an upper bound
+  const int c = lb_ticket(lb, &sh_ticket);
+  const int next_begin = lvl_begin + lvl_count;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int i = c * kLvNodes + (threadIdx.x >> 3), o = threadIdx.x & 7;
+  const bool valid = i < lvl_count;
   const int node = lvl_begin + i;
-  const int b = nd.pbegin[node], e = nd.pend[node], dep = nd.depth[node];
-  const bool split = e - b > leafcap && dep < kMaxDepth &&
-                     !(nstop > 0 && stopped(stop, nstop, dep, nd.key[node]));
+  int b = 0, e = 0, dep = 0;
+  bool split = false;
+  if (valid) {
+    b = nd.pbegin[node];
+    e = nd.pend[node];
+    dep = nd.depth[node];
+    split = e - b > leafcap && dep < kMaxDepth &&
+            !(nstop > 0 && stopped(stop, nstop, dep, nd.key[node]));
+  }
   int x = b;
   if (split && o > 0)
     x = digit_lower_bound(keys, b, e, 3 * (kMaxDepth - dep - 1), o);
   // next boundary: the lane above in the group of 8, the range end for o = 7
   int nx = __shfl_down(x, 1, 8);
   if (o == 7) nx = e;
-  int c = (split && nx > x) ? 1 : 0;
-  c += __shfl_xor(c, 1, 8);
-  c += __shfl_xor(c, 2, 8);
-  c += __shfl_xor(c, 4, 8);
-  if (split) {
-    bounds[9 * i + o] = x;
-    if (o == 7) bounds[9 * i + 8] = e;
+  const bool ex = split && nx > x;
+  const unsigned long long m = __ballot(ex);
+  const unsigned long long lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  if (lane == 0) sh_wcnt[wave] = (unsigned)__popcll(m);
+  __syncthreads();
+  unsigned wpre = 0, tot = 0;
+  for (int w = 0; w < 4; w++) {
+    if (w < wave) wpre += sh_wcnt[w];
+    tot += sh_wcnt[w];
   }
-  if (o == 0) cnt[i] = c;
+  if (wave == 0) {
+    const unsigned p = lb_prefix_wave(lb, c, tot);
+    if (lane == 0) sh_pre = p;
+  }
+  __syncthreads();
+  const unsigned pre = sh_pre;
+  // children of this block before this lane (octant o of node i)
+  const int rank = (int)(pre + wpre) + __popcll(m & lt);
+  const int nch = __popcll((m >> (lane & ~7)) & 0xffull);
+  if (valid && o == 0) {
+    nd.leaf[node] = nch == 0 ? 1 : 0;
+    nd.child[node] = nch == 0 ? -1 : next_begin + rank;
+    nd.nchild[node] = nch;
+  }
+  if (ex) {
+    const int ch = next_begin + rank;
+    const int dd = dep + 1;
+    const int sy = nd.sys[node];
+    const double *bx = nd.box + 4 * sy;
+    const double h = ldexp(bx[3], -dd);
+    const int ix = 2 * nd.ix[node] + (o & 1);
+    const int iy = 2 * nd.iy[node] + ((o >> 1) & 1);
+    const int iz = 2 * nd.iz[node] + ((o >> 2) & 1);
+    nd.depth[ch] = dd;
+    nd.ix[ch] = ix;
+    nd.iy[ch] = iy;
+    nd.iz[ch] = iz;
+    nd.key[ch] = (nd.key[node] << 3) | (uint64_t)o;
+    nd.pbegin[ch] = x;
+    nd.pend[ch] = nx;
+    nd.sys[ch] = sy;
+    nd.parent[ch] = node;
+    nd.child[ch] = -1;
+    nd.nchild[ch] = 0;
+    nd.leaf[ch] = 1;
+    nd.cx[ch] = fma(ix + 0.5, h, bx[0]);
+    nd.cy[ch] = fma(iy + 0.5, h, bx[1]);
+    nd.cz[ch] = fma(iz + 0.5, h, bx[2]);
+  }
+  if (c == nwork - 1 && threadIdx.x == 0) {
+    dl[2 * d + 2] = next_begin;
+    dl[2 * d + 3] = (int)(pre + tot);
+  }
+  lb_finish(lb, nwork);
 }
 
-__global__ void k_split_write(NodeArrays nd, const int *__restrict__ dl,
-                              int d, const int *__restrict__ bounds,
-                              const int *__restrict__ cnt,
-                              const int *__restrict__ ofs) {
-  const int lvl_begin = dl[2 * d], lvl_count = dl[2 * d + 1];
-  const int next_begin = lvl_begin + lvl_count;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= lvl_count) return;
-  int node = lvl_begin + i;
-  if (cnt[i] == 0) {
-    nd.leaf[node] = 1;
-    nd.child[node] = -1;
-    nd.nchild[node] = 0;
-    return;
-  }
-  int first = next_begin + ofs[i];
-  nd.leaf[node] = 0;
-  nd.child[node] = first;
-  nd.nchild[node] = cnt[i];
-  const int dd = nd.depth[node] + 1;
-  const int sy = nd.sys[node];
-  const double *b = nd.box + 4 * sy;
-  const double h = ldexp(b[3], -dd), lo0 = b[0], lo1 = b[1], lo2 = b[2];
-  int k = 0;
-  for (int o = 0; o < 8; o++) {
-    int bb = bounds[9 * i + o], e = bounds[9 * i + o + 1];
-    if (e <= bb) continue;
-    int c = first + k++;
-    int ix = 2 * nd.ix[node] + (o & 1);
-    int iy = 2 * nd.iy[node] + ((o >> 1) & 1);
-    int iz = 2 * nd.iz[node] + ((o >> 2) & 1);
-    nd.depth[c] = dd;
-    nd.ix[c] = ix;
-    nd.iy[c] = iy;
-    nd.iz[c] = iz;
-    nd.key[c] = (nd.key[node] << 3) | (uint64_t)o;
-    nd.pbegin[c] = bb;
-    nd.pend[c] = e;
-    nd.sys[c] = sy;
-    nd.parent[c] = node;
-    nd.child[c] = -1;
-    nd.nchild[c] = 0;
-    nd.leaf[c] = 1;
-    nd.cx[c] = fma(ix + 0.5, h, lo0);
-    nd.cy[c] = fma(iy + 0.5, h, lo1);
-    nd.cz[c] = fma(iz + 0.5, h, lo2);
-  }
-}
-
-hipError_t launch_split_count(const NodeArrays &nd, int *dl, int d,
+hipError_t launch_split_level(const NodeArrays &nd, int *dl, int d,
                               long max_count, const uint64_t *keys,
                               int leafcap, const ulonglong2 *stop, int nstop,
-                              int *bounds, int *cnt, hipStream_t s) {
-  const long t = 8 * std::max(1L, max_count);
-  k_split_count<<<(t + 255) / 256, 256, 0, s>>>(nd, dl, d, keys, leafcap,
-                                                stop, nstop, bounds, cnt);
+                              const Lookback &lb, hipStream_t s) {
+  const long nb = std::max(1L, (max_count + kLvNodes - 1) / kLvNodes);
+  k_split_level<<<nb, 256, 0, s>>>(nd, dl, d, keys, leafcap, stop, nstop, lb);
   return hipGetLastError();
 }
 
-hipError_t launch_split_write(const NodeArrays &nd, const int *dl, int d,
-                              long max_count, const int *bounds,
-                              const int *cnt, const int *ofs, hipStream_t s) {
-  if (max_count <= 0) return hipSuccess;
-  k_split_write<<<(max_count + 127) / 128, 128, 0, s>>>(nd, dl, d, bounds,
-                                                        cnt, ofs);
-  return hipGetLastError();
+long split_level_chunks(long max_count) {
+  return std::max(1L, (max_count + kLvNodes - 1) / kLvNodes);
 }
 
 // ------------------------------------------------------------ leaves

@@ -205,6 +205,45 @@ def test_tree_and_lists_bit_exact(torch_cuda, kind, n, q):
     assert gpu_near == ref_near
 
 
+@pytest.mark.parametrize("kind,n,q", [("uniform", 200_000, 8),
+                                      ("plummer", 300_000, 16)])
+def test_large_tree_bit_exact(torch_cuda, kind, n, q):
+    """Trees of 10^4 - 10^5 nodes: level launches of many workgroups and
+    node / octant scans across workgroups, against the oracle's tree."""
+    torch = torch_cuda
+    (x, y, z, m), lb, ub = _tree_case(kind, n, 23)
+    f = _fmm(torch)
+    dx, dy, dz, dm = _dev(torch, x, y, z, m)
+    f.solve(dx, dy, dz, dm, lb, ub, 4, q)
+    perm, nodes, centers, lo, width = f.export_tree()
+    assert nodes.shape[0] > 20_000
+    operm, onodes = O.tree(x, y, z, np.array(lb, float), np.array(ub, float), q)
+    assert np.array_equal(perm, operm)
+    ref = np.stack([onodes[k] for k in ("depth", "ix", "iy", "iz", "pbegin",
+                                        "pend", "parent", "isleaf",
+                                        "child_begin", "nchild")], axis=1)
+    assert nodes.shape == ref.shape
+    assert np.array_equal(nodes, ref)
+    assert np.array_equal(centers, onodes["center"])
+    # the per-leaf near ranges cover, for every leaf, each particle of its
+    # near leaves once: counts against the oracle's near pairs
+    lists = f.export_lists()
+    (fs, fd), (na, nb) = O.traverse(onodes)
+    size = onodes["pend"] - onodes["pbegin"]
+    want = np.zeros(len(size), np.int64)
+    two = na != nb  # (a, a): a leaf with itself, listed once
+    np.add.at(want, na, size[nb])
+    np.add.at(want, nb[two], size[na[two]])
+    leaves = np.asarray(lists["leaves"])
+    no = np.asarray(lists["near_ofs"])
+    span = np.asarray(lists["near_end"]) - np.asarray(lists["near_begin"])
+    got = np.add.reduceat(span, no[:-1]) if len(span) else np.zeros(0)
+    got = np.where(no[1:] > no[:-1], got, 0)
+    assert np.array_equal(got, want[leaves])
+    fo = np.asarray(lists["far_ofs"])
+    assert fo[-1] == len(fs)
+
+
 # ---------------------------------------------------------------- solve
 
 @pytest.mark.parametrize("kind,n,p,q", [("uniform", 10_000, 8, 20),
