@@ -149,8 +149,8 @@ struct Workspace {
   DevBuf mult, loc, small, pleaf;
   DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
       tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma, mult64, loc64;
-  DevBuf n_sys, sys_box, sys_off, sys_mm, skey, spos, stop;
-  // look-back counters (first 256 bytes) and status words (Lookback)
+  DevBuf n_sys, sys_box, sys_off, skey, spos, stop;
+  // look-back status words (Lookback)
   DevBuf lb_buf;
   unsigned lb_epoch = 0;
   long node_cap = 0;
@@ -243,11 +243,6 @@ hipError_t workspace_init(Workspace &ws, hipStream_t stream);
 const char *last_error();
 void set_error(const std::string &msg);
 
-// Root cube: centre (lb+ub)/2, half-width max((ub-lb)/2) (1 when zero),
-// doubled about the centre until [pmin, pmax] is inside.
-void root_box(const double *lb, const double *ub, const double *pmin,
-              const double *pmax, double *lo, double *width);
-
 // ---- tree.hip
 hipError_t launch_keys(const double *x, const double *y, const double *z,
                        const double *q, long n, const long *soff, int nsys,
@@ -259,9 +254,14 @@ hipError_t launch_permute_keys(const int *pos, long n, const uint64_t *kin,
                                const int *iin, uint64_t *kout, int *iout,
                                hipStream_t s);
 int minmax_blocks(int nsys);
+// Extent of each system and its root cube: box[4 s ..] = (lo, width) from
+// its bounds (bounds[6 s ..] = lb, ub) and particles (a system with a
+// non-finite coordinate sets bit 4 of *err, which starts at zero); also
+// sets up the level table dl[0 .. ntab) (level 0: the nsys roots).
 hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
-                             const long *soff, int nsys, double *part,
-                             double *out, hipStream_t s);
+                             const long *soff, const double *bounds, int nsys,
+                             double *part, double *box, int *dl, int ntab,
+                             int *err, hipStream_t s);
 hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
                           DevBuf &tmp, hipStream_t s);
 // the same in one launch for n <= 2^17 (falls back to scan_exclusive)
@@ -278,15 +278,13 @@ hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
                           hipStream_t s);
 hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
                         hipStream_t s);
-// Status words and counters of a launch that scans across its workgroups
-// by decoupled look-back (one word per chunk of work).  The words need no
-// clearing between launches (each launch has its own epoch); the counters
-// start at zero and the launch leaves them at zero.
+// Status words of a launch that scans across its workgroups by decoupled
+// look-back (one word per chunk of work).  The words need no clearing
+// between launches: each launch has its own epoch.
 struct Lookback {
   unsigned long long *status;
-  unsigned *ctr;  // [0] chunk tickets, [1] workgroups finished
-  int *err;       // set to 1 when a wait gave up (a wrong result)
-  unsigned epoch; // 1 .. 2^30 - 1
+  int *err;        // set when a wait gave up (the result is then wrong)
+  unsigned epoch;  // 1 .. 2^30 - 1
 };
 // Exclusive scan of n ints in one launch of scan_lookback_chunks(n)
 // workgroups (in and out may alias); writes the total to *total.

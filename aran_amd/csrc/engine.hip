@@ -194,7 +194,7 @@ std::vector<DevBuf *> Workspace::buffers() {
                    &small, &pleaf, &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
                    &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
                    &mult_bf, &loc_mfma, &mult64, &loc64, &n_sys, &sys_box,
-                   &sys_off, &sys_mm,
+                   &sys_off,
                    &skey, &spos, &stop, &internal_list,
                    &trans_rows, &packed,
                    &lvl_table, &child_map, &lb_buf};
@@ -311,13 +311,12 @@ static hipError_t stage_copy(Workspace &ws, void *dst, const void *src,
 }
 
 // Look-back state of one launch over nchunks chunks.  The status words
-// are cleared when their buffer is (re)allocated or the epoch wraps; the
-// counters start at zero then and every launch leaves them at zero.
+// are cleared when their buffer is (re)allocated or the epoch wraps.
 static hipError_t lookback_take(Workspace &ws, long nchunks, int *err,
                                 Lookback *lb, hipStream_t s) {
   const void *old = ws.lb_buf.ptr;
   const size_t old_bytes = ws.lb_buf.bytes;
-  CK(ws.lb_buf.reserve(256 + sizeof(unsigned long long) * (size_t)nchunks));
+  CK(ws.lb_buf.reserve(sizeof(unsigned long long) * (size_t)nchunks));
   bool clear = ws.lb_buf.ptr != old || ws.lb_buf.bytes != old_bytes;
   if (++ws.lb_epoch >= (1u << 30)) {
     ws.lb_epoch = 1;
@@ -325,9 +324,7 @@ static hipError_t lookback_take(Workspace &ws, long nchunks, int *err,
   }
   if (clear)
     CK(launch_fill_int(ws.lb_buf.as<int>(), (long)(ws.lb_buf.bytes / 4), 0, s));
-  lb->ctr = ws.lb_buf.as<unsigned>();
-  lb->status =
-      reinterpret_cast<unsigned long long *>(ws.lb_buf.as<char>() + 256);
+  lb->status = ws.lb_buf.as<unsigned long long>();
   lb->err = err;
   lb->epoch = ws.lb_epoch;
   return hipSuccess;
@@ -350,25 +347,6 @@ static void stage_reset(Workspace &ws) {
   for (char *b : ws.host_stage_old) (void)hipHostFree(b);
   ws.host_stage_old.clear();
   ws.host_stage_used = 0;
-}
-
-void root_box(const double *lb, const double *ub, const double *pmin,
-              const double *pmax, double *lo, double *width) {
-  double c[3], h = 0.;
-  for (int k = 0; k < 3; k++) {
-    c[k] = 0.5 * (lb[k] + ub[k]);
-    if (0.5 * (ub[k] - lb[k]) > h) h = 0.5 * (ub[k] - lb[k]);
-  }
-  if (!(h > 0.)) h = 1.;
-  for (;;) {
-    bool inside = true;
-    for (int k = 0; k < 3; k++)
-      if (pmin[k] < c[k] - h || pmax[k] > c[k] + h) inside = false;
-    if (inside) break;
-    h *= 2.;
-  }
-  for (int k = 0; k < 3; k++) lo[k] = c[k] - h;
-  *width = 2. * h;
 }
 
 // ------------------------------------------------------------ tables
@@ -637,6 +615,18 @@ static hipError_t read_int(Workspace &ws, const int *dev, int *out) {
   return hipSuccess;
 }
 
+// Root cube of the first system (lo, width), read back for the tree info.
+static hipError_t read_root_cube(Workspace &ws, double *lo, double *width) {
+  double *h = reinterpret_cast<double *>(static_cast<char *>(ws.host_small) +
+                                         4096 + 2048);
+  CK(hipMemcpyAsync(h, ws.sys_box.ptr, 4 * sizeof(double),
+                    hipMemcpyDeviceToHost, ws.stream));
+  CK(hipStreamSynchronize(ws.stream));
+  for (int k = 0; k < 3; k++) lo[k] = h[k];
+  *width = h[3];
+  return hipSuccess;
+}
+
 static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
   hipStream_t s = ws.stream;
   DevBuf *ints[] = {&ws.n_depth, &ws.n_ix, &ws.n_iy, &ws.n_iz, &ws.n_pbegin,
@@ -846,40 +836,36 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
       ubs[k] = pb.ub[k];
     }
   }
-  CK(ws.sys_off.reserve(sizeof(long) * (nsys + 1)));
-  CK(ws.sys_mm.reserve(sizeof(double) * 6 * nsys));
+  // system offsets, then each system's bounds (lb, ub): one copy
+  const size_t nsoff = (size_t)(nsys + 1);
+  CK(ws.sys_off.reserve(sizeof(long) * nsoff + sizeof(double) * 6 * nsys));
   CK(ws.sys_box.reserve(sizeof(double) * 4 * nsys));
   stage_reset(ws);
-  CK(stage_copy(ws, ws.sys_off.ptr, soff.data(), sizeof(long) * (nsys + 1),
-                s));
-  CK(ws.part.reserve(sizeof(double) * 6 * (size_t)nsys * minmax_blocks(nsys)));
-  CK(launch_sys_minmax(pb.x, pb.y, pb.z, ws.sys_off.as<long>(), nsys,
-                       ws.part.as<double>(), ws.sys_mm.as<double>(), s));
-  std::vector<double> box(4 * nsys);
-  double *mm = nullptr;
-  CK(stage_alloc(ws, sizeof(double) * 6 * nsys, reinterpret_cast<void **>(&mm)));
-  CK(hipMemcpyAsync(mm, ws.sys_mm.ptr, sizeof(double) * 6 * nsys,
-                    hipMemcpyDeviceToHost, s));
-  CK(hipStreamSynchronize(s));
-  for (int k = 0; k < nsys; k++) {
-    const double *v = &mm[6 * k];
-    if (soff[k + 1] == soff[k]) {
-      box[4 * k + 0] = box[4 * k + 1] = box[4 * k + 2] = 0.;
-      box[4 * k + 3] = 1.;
-      double pmin[3] = {lbs[3 * k], lbs[3 * k + 1], lbs[3 * k + 2]};
-      root_box(&lbs[3 * k], &ubs[3 * k], pmin, pmin, &box[4 * k],
-               &box[4 * k + 3]);
-      continue;
-    }
-    if (!(std::isfinite(v[0]) && std::isfinite(v[1]) && std::isfinite(v[2]) &&
-          std::isfinite(v[3]) && std::isfinite(v[4]) && std::isfinite(v[5]))) {
-      set_error("non-finite particle coordinates");
-      return hipErrorInvalidValue;
-    }
-    root_box(&lbs[3 * k], &ubs[3 * k], v, v + 3, &box[4 * k], &box[4 * k + 3]);
+  {
+    std::vector<char> h(sizeof(long) * nsoff + sizeof(double) * 6 * nsys);
+    memcpy(h.data(), soff.data(), sizeof(long) * nsoff);
+    double *hb = reinterpret_cast<double *>(h.data() + sizeof(long) * nsoff);
+    for (int k = 0; k < nsys; k++)
+      for (int j = 0; j < 3; j++) {
+        hb[6 * k + j] = lbs[3 * k + j];
+        hb[6 * k + 3 + j] = ubs[3 * k + j];
+      }
+    CK(stage_copy(ws, ws.sys_off.ptr, h.data(), h.size(), s));
   }
-  CK(stage_copy(ws, ws.sys_box.ptr, box.data(), sizeof(double) * 4 * nsys,
-                s));
+  const double *d_bounds =
+      reinterpret_cast<const double *>(ws.sys_off.as<long>() + nsoff);
+  // Level table on the device: per level its first node and count, then
+  // the error word of the tree launches (bit 4: a non-finite coordinate,
+  // others: a level launch gave up); set up by the extent launch.
+  constexpr int kTab = 2 * (kMaxDepth + 3);
+  CK(ws.lvl_table.reserve(sizeof(int) * (kTab + 1)));
+  int *d_lvl_tab = ws.lvl_table.as<int>();
+  // root cubes on the device (no host round trip): a non-finite coordinate
+  // is reported with the first level counts
+  CK(ws.part.reserve(sizeof(double) * 6 * (size_t)nsys * minmax_blocks(nsys)));
+  CK(launch_sys_minmax(pb.x, pb.y, pb.z, ws.sys_off.as<long>(), d_bounds,
+                       nsys, ws.part.as<double>(), ws.sys_box.as<double>(),
+                       d_lvl_tab, kTab, d_lvl_tab + kTab, s));
 
   // ---- keys and sort (by system, then Morton key)
   CK(ws.keys_a.reserve(sizeof(uint64_t) * n));
@@ -939,12 +925,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   {
     constexpr int kLevelBatch = 4;
     const long max_split = n / ((long)pb.leafcap + 1);
-    // the table, then the look-back error word of the level launches
-    constexpr int kTab = 2 * (kMaxDepth + 3);
-    CK(ws.lvl_table.reserve(sizeof(int) * (kTab + 1)));
-    int *d_lvl = ws.lvl_table.as<int>();
-    int lvl0[kTab + 1] = {0, nsys};
-    CK(stage_copy(ws, d_lvl, lvl0, sizeof(lvl0), s));
+    int *d_lvl = d_lvl_tab;
     long ub = nsys;  // bound on the count of level d
     bool done = false;
     // the first batch reaches the previous solve's depth (repeated solves of
@@ -980,6 +961,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
       CK(hipMemcpyAsync(h_tab, d_lvl, sizeof(int) * (kTab + 1),
                         hipMemcpyDeviceToHost, s));
       CK(hipStreamSynchronize(s));
+      if (h_tab[kTab] & 4) {
+        set_error("non-finite particle coordinates");
+        return hipErrorInvalidValue;
+      }
       if (h_tab[kTab] != 0) {
         set_error("tree build: a level launch gave up waiting (internal "
                   "error)");
@@ -1085,8 +1070,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     S.nnodes = nn;
     S.nleaves = nleaves;
     if (ti) {
-      for (int k = 0; k < 3; k++) ti->lo[k] = box[k];
-      ti->width = box[3];
+      CK(read_root_cube(ws, ti->lo, &ti->width));
       ti->nsys = nsys;
       ti->depth = depth;
       ti->nnodes = nn;
@@ -1535,8 +1519,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(hipEventElapsedTime(&S.t_l2p, ws.evk[4], ws.evk[5]));
   }
   if (ti) {
-    for (int k = 0; k < 3; k++) ti->lo[k] = box[k];
-    ti->width = box[3];
+    CK(read_root_cube(ws, ti->lo, &ti->width));
     ti->nsys = nsys;
     ti->depth = depth;
     ti->nnodes = nn;

@@ -106,9 +106,10 @@ __global__ void k_sys_minmax(const double *__restrict__ x,
                              const double *__restrict__ y,
                              const double *__restrict__ z,
                              const long *__restrict__ soff,
-                             double *__restrict__ part) {
+                             double *__restrict__ part, int *__restrict__ err) {
   __shared__ double red[6][256];
   const int sy = blockIdx.x, nb = gridDim.y;
+  if (sy == 0 && blockIdx.y == 0 && threadIdx.x == 0) *err = 0;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
                  -INFINITY};
   bool bad = false;
@@ -142,11 +143,21 @@ __global__ void k_sys_minmax(const double *__restrict__ x,
 }
 
 // One block per system: its nb partial extents combined (a NaN minimum
-// marks a non-finite coordinate and survives).
+// marks a non-finite coordinate and survives), then the system's root cube
+// from its bounds (lb, ub at bounds[6 sy ..]): box[4 sy ..] = (lo, width).
+// A system with a non-finite coordinate sets bit 4 of *err.  Block 0 also
+// sets up the level table: level 0 holds the nsys roots.
 __global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
-                                   int nb, double *__restrict__ out) {
+                                   int nb, const long *__restrict__ soff,
+                                   const double *__restrict__ bounds,
+                                   double *__restrict__ box,
+                                   int *__restrict__ dl, int ntab,
+                                   int *__restrict__ err) {
   __shared__ double red[6][256];
   const int sy = blockIdx.x;
+  if (sy == 0)
+    for (int k = threadIdx.x; k < ntab; k += 256)
+      dl[k] = k == 1 ? nsys : 0;
   double v[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY,
                  -INFINITY};
   bool bad = false;
@@ -172,7 +183,24 @@ __global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
     }
     __syncthreads();
   }
-  if (threadIdx.x < 6) out[6 * sy + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x != 0) return;
+  const double *lb = bounds + 6 * sy, *ub = lb + 3;
+  double pmin[3], pmax[3];
+  bool finite = true;
+  for (int k = 0; k < 3; k++) {
+    pmin[k] = red[k][0];
+    pmax[k] = red[3 + k][0];
+    finite = finite && isfinite(pmin[k]) && isfinite(pmax[k]);
+  }
+  if (soff[sy + 1] == soff[sy]) {  // empty: the cube of its bounds
+    for (int k = 0; k < 3; k++) pmin[k] = pmax[k] = lb[k];
+  } else if (!finite) {
+    atomicOr(err, 4);
+    for (int k = 0; k < 3; k++) box[4 * sy + k] = 0.;
+    box[4 * sy + 3] = 1.;
+    return;
+  }
+  root_cube(lb, ub, pmin, pmax, box + 4 * sy, box + 4 * sy + 3);
 }
 
 // part must hold 6 * nsys * minmax_blocks(nsys) doubles.
@@ -181,12 +209,14 @@ int minmax_blocks(int nsys) {
 }
 
 hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
-                             const long *soff, int nsys, double *part,
-                             double *out, hipStream_t s) {
+                             const long *soff, const double *bounds, int nsys,
+                             double *part, double *box, int *dl, int ntab,
+                             int *err, hipStream_t s) {
   if (nsys == 0) return hipSuccess;
   const int nb = minmax_blocks(nsys);
-  k_sys_minmax<<<dim3(nsys, nb), 256, 0, s>>>(x, y, z, soff, part);
-  k_sys_minmax_final<<<nsys, 256, 0, s>>>(part, nsys, nb, out);
+  k_sys_minmax<<<dim3(nsys, nb), 256, 0, s>>>(x, y, z, soff, part, err);
+  k_sys_minmax_final<<<nsys, 256, 0, s>>>(part, nsys, nb, soff, bounds, box,
+                                          dl, ntab, err);
   return hipGetLastError();
 }
 
@@ -197,8 +227,13 @@ hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
 // gets it in the same launch: a workgroup publishes its chunk's total in the
 // chunk's status word, then reads the words of the chunks below, 64 at a
 // time, down to the nearest one that already holds its inclusive total
-// (decoupled look-back).  Chunks are handed out in ticket order, so a
-// workgroup only ever waits for chunks that running workgroups hold.
+// (decoupled look-back).  Workgroup b owns chunk b: workgroups are
+// dispatched in index order, so the chunks a resident workgroup waits for
+// are held by workgroups that were dispatched before it (a ticket taken
+// with an atomic would not need that order, but same-address atomics
+// serialize at about 11 ns each, 25 us per 1000 workgroups).  Every wait is
+// bounded: past kLbMaxPolls polls it sets *err and goes on with a wrong
+// prefix, which the host reports as an error.
 //
 // Status word: bits 63..34 the launch's epoch, 33..32 the state (1: the
 // chunk's own total, 2: the total of chunks 0..c), 31..0 the value; a word
@@ -207,35 +242,11 @@ hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
 // launches.  Every access to them is a single 64-bit atomic at agent scope.
 
 constexpr unsigned kLbAgg = 1u, kLbIncl = 2u;
-// polls before a wait gives up (it then sets *err and goes on with a wrong
-// prefix; the host turns the flag into an error)
 constexpr int kLbMaxPolls = 1 << 20;
 
 __device__ inline unsigned long long lb_word(unsigned epoch, unsigned st,
                                              unsigned v) {
   return ((unsigned long long)((epoch << 2) | st) << 32) | v;
-}
-
-// Ticket of the calling workgroup (every thread gets it).
-__device__ inline int lb_ticket(const Lookback &lb, int *sh) {
-  if (threadIdx.x == 0) *sh = (int)atomicAdd(&lb.ctr[0], 1u);
-  __syncthreads();
-  return *sh;
-}
-
-// Called once by every workgroup that took a ticket, after its last use of
-// the status words: the last of the nwork workgroups resets the counters
-// for the next launch.
-__device__ inline void lb_finish(const Lookback &lb, int nwork) {
-  if (threadIdx.x == 0) {
-    const unsigned done = atomicAdd(&lb.ctr[1], 1u);
-    if (done == (unsigned)nwork - 1) {
-      __hip_atomic_store(&lb.ctr[0], 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&lb.ctr[1], 0u, __ATOMIC_RELAXED,
-                         __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
 }
 
 // Exclusive prefix of chunk c whose own total is 'total'; called by one
@@ -489,10 +500,9 @@ constexpr long kLsTile = (long)kLsThreads * kLsPer;
 __global__ __launch_bounds__(kLsThreads) void k_scan_lookback(
     const int *in, int *out, long n, int *__restrict__ total, Lookback lb,
     int nwork) {
-  __shared__ int sh_ticket;
   __shared__ int ws[kLsThreads / 64];
   __shared__ unsigned sh_pre;
-  const int c = lb_ticket(lb, &sh_ticket);
+  const int c = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const long b = c * kLsTile + (long)t * kLsPer;
   const bool full = b + kLsPer <= n;
@@ -550,7 +560,6 @@ __global__ __launch_bounds__(kLsThreads) void k_scan_lookback(
     }
   }
   if (c == nwork - 1 && t == 0) *total = (int)sh_pre + tot;
-  lb_finish(lb, nwork);
 }
 
 long scan_lookback_chunks(long n) {
@@ -887,7 +896,6 @@ __global__ __launch_bounds__(256) void k_split_level(
     NodeArrays nd, int *__restrict__ dl, int d,
     const uint64_t *__restrict__ keys, int leafcap,
     const ulonglong2 *__restrict__ stop, int nstop, Lookback lb) {
-  __shared__ int sh_ticket;
   __shared__ unsigned sh_wcnt[4], sh_pre;
   const int lvl_begin = dl[2 * d], lvl_count = dl[2 * d + 1];
   // a level without nodes still has one chunk (it writes the zero count)
@@ -897,7 +905,7 @@ __global__ __launch_bounds__(256) void k_split_level(
     return;
   }
   if ((int)blockIdx.x >= nwork) return;  // grid sized by an upper bound
-  const int c = lb_ticket(lb, &sh_ticket);
+  const int c = blockIdx.x;
   const int next_begin = lvl_begin + lvl_count;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int i = c * kLvNodes + (threadIdx.x >> 3), o = threadIdx.x & 7;
@@ -971,7 +979,6 @@ __global__ __launch_bounds__(256) void k_split_level(
     dl[2 * d + 2] = next_begin;
     dl[2 * d + 3] = (int)(pre + tot);
   }
-  lb_finish(lb, nwork);
 }
 
 hipError_t launch_split_level(const NodeArrays &nd, int *dl, int d,

@@ -8,7 +8,10 @@
 //   * 63-bit Morton key, octant digit = bx | by << 1 | bz << 2, most
 //     significant digit first;
 //   * a node splits while it holds more than leafcap particles and its depth
-//     is below 21; empty octants create no node.
+//     is below 21; empty octants create no node;
+//   * the root is a cube centred on the caller's bounds [lb, ub], of half
+//     width that of their widest side (1 when that is 0), doubled until it
+//     holds every particle (root_cube).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -51,6 +54,29 @@ __host__ __device__ inline uint64_t icoord(double x, double lo, double scale) {
   f = f < 0. ? 0. : f;
   f = f > top ? top : f;
   return (uint64_t)f;
+}
+
+// Root cube (lo corner, width) of a system with bounds [lb, ub] whose
+// particles span [pmin, pmax] (finite).
+__host__ __device__ inline void root_cube(const double *lb, const double *ub,
+                                          const double *pmin,
+                                          const double *pmax, double *lo,
+                                          double *width) {
+  double c[3], h = 0.;
+  for (int k = 0; k < 3; k++) {
+    c[k] = 0.5 * (lb[k] + ub[k]);
+    if (0.5 * (ub[k] - lb[k]) > h) h = 0.5 * (ub[k] - lb[k]);
+  }
+  if (!(h > 0.)) h = 1.;
+  for (;;) {
+    bool inside = true;
+    for (int k = 0; k < 3; k++)
+      if (pmin[k] < c[k] - h || pmax[k] > c[k] + h) inside = false;
+    if (inside) break;
+    h *= 2.;
+  }
+  for (int k = 0; k < 3; k++) lo[k] = c[k] - h;
+  *width = 2. * h;
 }
 
 }  // namespace aran
