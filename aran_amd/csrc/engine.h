@@ -21,6 +21,7 @@
 namespace aran {
 
 constexpr int kMaxDepth = 21;   // 3 x 21 key bits
+static_assert(kAncRow == kMaxDepth + 1, "ancestor rows cover every depth");
 // Orders: the plain FP64 form and the single-operator entry points take
 // local degrees 0..kMaxOrder; the mixed and MFMA forms, and the kernels
 // templated on the order (generic kernels take the orders above), go to
@@ -135,6 +136,7 @@ struct Workspace {
   DevBuf scan_tmp, hist, part;
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
+  DevBuf n_anc;  // ancestor rows (NodeArrays::anc)
   DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
   DevBuf internal_list;  // internal nodes in node order (level by level)
   DevBuf trans_rows;     // TransRow of each internal node (same order)
@@ -155,6 +157,9 @@ struct Workspace {
   unsigned lb_epoch = 0;
   long node_cap = 0;
   int depth_hint = 0;  // depth of the last tree (levels of the first batch)
+  // near-list ranges per leaf of the last solve (0: none yet); the next
+  // solve sizes its lists from it and builds them in one launch
+  double near_per_leaf = 0.;
   long far_csr_nodes = -1;  // nodes covered by the CSR far lists, -1: none
   void *host_small = nullptr;  // pinned, 8 KiB
   // pinned staging of the host-to-device copies (and the larger reads) of
@@ -322,9 +327,14 @@ hipError_t launch_colleagues_all(const NodeArrays &nd, int nnodes,
 hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
                         int pass, int *cnt, const int *ofs, int *src,
                         short *code, hipStream_t s);
+// Near lists: pass 0 counts each leaf's ranges, pass 1 writes them from the
+// scan ofs of the counts; with a capacity cap (ranges the arrays hold;
+// LONG_MAX: none) pass 1 stores only the ranges below it, clamps ofs to it
+// and writes the true total to *total.
 hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
-                        const int *col, int pass, int *cnt, const int *ofs,
-                        int *ubeg, int *uend, long long *pairs, hipStream_t s);
+                        const int *col, int pass, int *cnt, int *ofs,
+                        int *ubeg, int *uend, long long *pairs, long cap,
+                        int *total, hipStream_t s);
 hipError_t launch_sum_ll(const long long *v, int n, long long *out,
                          hipStream_t s);
 hipError_t launch_ulist_semifar(const NodeArrays &nd, int nleaves,

@@ -174,6 +174,7 @@ NodeArrays Workspace::nodes() const {
   a.nchild = n_nchild.as<int>();
   a.leaf = n_leaf.as<int>();
   a.sys = n_sys.as<int>();
+  a.anc = n_anc.as<int>();
   a.box = sys_box.as<double>();
   a.key = n_key.as<uint64_t>();
   a.cx = n_cx.as<double>();
@@ -186,7 +187,7 @@ std::vector<DevBuf *> Workspace::buffers() {
   std::vector<DevBuf *> all = {&keys_a, &keys_b, &idx_a, &idx_b, &xs, &ys, &zs, &qs,
                    &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
                    &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
-                   &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz,
+                   &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz, &n_anc,
                    &split_bounds, &split_cnt, &split_ofs, &flags, &leaf_list,
                    &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &w_cnt, &w_ofs,
@@ -634,12 +635,16 @@ static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
                     &ws.n_leaf, &ws.n_sys};
   for (DevBuf *b : ints) CK(b->grow_keep(sizeof(int) * cap, sizeof(int) * keep, s));
   CK(ws.n_key.grow_keep(sizeof(uint64_t) * cap, sizeof(uint64_t) * keep, s));
+  CK(ws.n_anc.grow_keep(sizeof(int) * kAncRow * cap,
+                        sizeof(int) * kAncRow * keep, s));
   DevBuf *dbl[] = {&ws.n_cx, &ws.n_cy, &ws.n_cz};
   for (DevBuf *b : dbl)
     CK(b->grow_keep(sizeof(double) * cap, sizeof(double) * keep, s));
   ws.node_cap = std::min(ws.n_depth.bytes / sizeof(int),
                          ws.n_cx.bytes / sizeof(double));
   ws.node_cap = std::min<long>(ws.node_cap, ws.n_key.bytes / sizeof(uint64_t));
+  ws.node_cap =
+      std::min<long>(ws.node_cap, ws.n_anc.bytes / (sizeof(int) * kAncRow));
   return hipSuccess;
 }
 
@@ -1101,9 +1106,20 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(ws.u_pairs.reserve(sizeof(long long) * (nleaves + 1)));
   int nnear = 0, nsemi = 0;
   ws.semifar_nodes = -1;
+  // With the ranges per leaf of an earlier solve the near-list arrays are
+  // sized from it, so the second pass follows the count and its scan with
+  // no read of the total on the host; a solve whose lists outgrow that size
+  // is run again, sized exactly.  (Not on a fixed arena: the margin and a
+  // second run would cost it memory.)
+  const bool near_hinted = !semifar && ws.near_per_leaf > 0. && !g_arena;
+  long near_cap = LONG_MAX;
+  int *d_near_total = reinterpret_cast<int *>(ws.small.as<char>() + 1020);
+  if (near_hinted)
+    near_cap = (long)std::min<double>(
+        (double)INT32_MAX, 1.25 * ws.near_per_leaf * nleaves + 1024.);
   if (!semifar) {
     CK(launch_ulist(nd, nleaves, leaves, col, 0, ws.u_cnt.as<int>(), nullptr,
-                    nullptr, nullptr, nullptr, s));
+                    nullptr, nullptr, nullptr, LONG_MAX, nullptr, s));
   } else {
     CK(ws.w_cnt.reserve(sizeof(int) * nleaves));
     CK(ws.w_ofs.reserve(sizeof(int) * (nleaves + 1)));
@@ -1191,13 +1207,18 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(stage_copy(ws, d_meta + kMaxDepth + 2, meta.data(),
                   sizeof(int) * meta.size(), s));
   }
-  CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
-  CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
-  CK(ws.u_end.reserve(sizeof(int) * std::max(1, nnear)));
+  if (near_hinted) {
+    CK(ws.u_beg.reserve(sizeof(int) * (size_t)near_cap));
+    CK(ws.u_end.reserve(sizeof(int) * (size_t)near_cap));
+  } else {
+    CK(read_int(ws, ws.u_ofs.as<int>() + nleaves, &nnear));
+    CK(ws.u_beg.reserve(sizeof(int) * std::max(1, nnear)));
+    CK(ws.u_end.reserve(sizeof(int) * std::max(1, nnear)));
+  }
   if (!semifar) {
     CK(launch_ulist(nd, nleaves, leaves, col, 1, ws.u_cnt.as<int>(),
                     ws.u_ofs.as<int>(), ws.u_beg.as<int>(), ws.u_end.as<int>(),
-                    ws.u_pairs.as<long long>(), s));
+                    ws.u_pairs.as<long long>(), near_cap, d_near_total, s));
   } else {
     int nx = 0;
     CK(read_int(ws, ws.w_ofs.as<int>() + nleaves, &nsemi));
@@ -1469,7 +1490,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
       static_cast<char *>(ws.host_small) + 1024);
   int *hfar = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                       2048);
-  // the look-back error word sits just before the pair count
+  // the look-back error word and the near-list total sit just before the
+  // pair count
   int *herr = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                       1016);
   CK(hipMemcpyAsync(herr, lb_err_word(ws), 8 + sizeof(long long),
@@ -1488,6 +1510,14 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     set_error("a scan launch gave up waiting (internal error)");
     return hipErrorUnknown;
   }
+  if (near_hinted) {
+    nnear = herr[1];
+    if (nnear > near_cap) {  // lists outgrew their arrays: again, sized
+      ws.near_per_leaf = 0.;
+      return solve(ws, pb, st, ti);
+    }
+  }
+  if (!semifar) ws.near_per_leaf = (double)nnear / std::max(1, nleaves);
   if (pb.precision != 0 || tiled) nfar = tiled ? *hfar : 0;
   S.zero = nn;
   S.p2p = (long)(*hp / 2);

@@ -9,6 +9,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "engine.h"
 #include "tree_rules.h"
@@ -825,6 +826,7 @@ __global__ void k_roots(NodeArrays nd, int nsys, const long *__restrict__ soff) 
   nd.ix[sy] = nd.iy[sy] = nd.iz[sy] = 0;
   nd.key[sy] = 0;
   nd.sys[sy] = sy;
+  nd.anc[(size_t)sy * kAncRow] = sy;
   nd.pbegin[sy] = (int)soff[sy];
   nd.pend[sy] = (int)soff[sy + 1];
   nd.parent[sy] = -1;
@@ -974,6 +976,11 @@ __global__ __launch_bounds__(256) void k_split_level(
     nd.cx[ch] = fma(ix + 0.5, h, bx[0]);
     nd.cy[ch] = fma(iy + 0.5, h, bx[1]);
     nd.cz[ch] = fma(iz + 0.5, h, bx[2]);
+    // ancestor row: the parent's, then the child itself
+    const int *pa = nd.anc + (size_t)node * kAncRow;
+    int *ca = nd.anc + (size_t)ch * kAncRow;
+    for (int k = 0; k < dd; k++) ca[k] = pa[k];
+    ca[dd] = ch;
   }
   if (c == nwork - 1 && threadIdx.x == 0) {
     dl[2 * d + 2] = next_begin;
@@ -1148,61 +1155,108 @@ __global__ void k_vlist(NodeArrays nd, int nnodes, const int *__restrict__ col,
 // otherwise: each unordered pair is seen twice).
 constexpr int kUlistWaves = 4;
 
+// Level pairs of a near list walk (depths 2 it and 2 it + 1).
+constexpr int kUlIter = (kAncRow + 1) / 2;
+
+// This lane's slots of leaf A's near list: in pair it, lanes 0-26 take the
+// 27 colleague slots of A's ancestor at depth 2 it and lanes 32-58 those at
+// depth 2 it + 1.  Every load of a round is issued at once: the ancestor
+// row, then the colleagues of every level, then their leaf flags.
+struct UlistSlots {
+  int j[kUlIter];                 // the slot's node, -1: not in the list
+  unsigned long long m[kUlIter];  // the wave's taken slots
+};
+
+// Fills u (one whole wave) and returns the length of A's near list.
+__device__ inline int ulist_slots(const NodeArrays &nd,
+                                  const int *__restrict__ col, int A, int dA,
+                                  UlistSlots &u) {
+  const int lane = threadIdx.x & 63, half = lane >> 5, s = lane & 31;
+  const int my_anc = lane <= dA ? nd.anc[(size_t)A * kAncRow + lane] : 0;
+#pragma unroll
+  for (int it = 0; it < kUlIter; it++) {
+    const int d = 2 * it + half;
+    const int a = __shfl(my_anc, d);
+    u.j[it] = (s < 27 && d <= dA) ? col[(size_t)a * 27 + s] : -1;
+  }
+  int c = 0;
+#pragma unroll
+  for (int it = 0; it < kUlIter; it++) {
+    const int d = 2 * it + half, j = u.j[it];
+    // ancestors' slots: leaf colleagues only, not the ancestor itself
+    const bool take = j >= 0 && (d == dA || (s != 13 && nd.leaf[j]));
+    u.j[it] = take ? j : -1;
+    u.m[it] = __ballot(take);
+    c += __popcll(u.m[it]);
+  }
+  return c;
+}
+
+// Stores the ranges of u that fall below cap from pos on, in (depth, slot)
+// order, and returns A's pair count (every lane).
+__device__ inline long long ulist_write(const NodeArrays &nd,
+                                        const UlistSlots &u, int A, long pos,
+                                        long cap, int *__restrict__ ubeg,
+                                        int *__restrict__ uend) {
+  const int lane = threadIdx.x & 63;
+  const unsigned long long lt = lane == 0 ? 0ull : ~0ull >> (64 - lane);
+  const int nA = nd.pend[A] - nd.pbegin[A];
+  long long pp = 0;
+#pragma unroll
+  for (int it = 0; it < kUlIter; it++) {
+    const int j = u.j[it];
+    if (j >= 0) {
+      const int b = nd.pbegin[j], e = nd.pend[j];
+      const long k = pos + __popcll(u.m[it] & lt);
+      if (k < cap) {
+        ubeg[k] = b;
+        uend[k] = e;
+      }
+      pp += j == A ? 2LL * ((long long)nA * (nA + 1) / 2)
+                   : (long long)nA * (e - b);
+    }
+    pos += __popcll(u.m[it]);
+  }
+  for (int off = 32; off > 0; off >>= 1) pp += __shfl_xor(pp, off);
+  return pp;
+}
+
+// pass 0 counts each leaf's ranges into cnt; pass 1 writes them from
+// ofs[t] on (ofs: the exclusive scan of cnt, ofs[nleaves] its total).  A
+// caller that sized the range arrays before knowing the total passes their
+// capacity cap: the ranges at cap and beyond are not stored and pass 1
+// clamps ofs to cap, so a reader of [ofs[t], ofs[t + 1]) reads stored ranges
+// only; *total then receives the true total (above cap: incomplete lists).
+// Without a capacity (cap = LONG_MAX) ofs stays as it is.
 __global__ __launch_bounds__(64 * kUlistWaves) void k_ulist(
     NodeArrays nd, int nleaves, const int *__restrict__ leaves,
     const int *__restrict__ col, int pass, int *__restrict__ cnt,
-    const int *__restrict__ ofs, int *__restrict__ ubeg, int *__restrict__ uend,
-    long long *__restrict__ pairs) {
-  __shared__ int anc[kUlistWaves][kMaxDepth + 1];
+    int *__restrict__ ofs, int *__restrict__ ubeg, int *__restrict__ uend,
+    long long *__restrict__ pairs, long cap, int *__restrict__ total) {
   const int w = threadIdx.x / 64, lane = threadIdx.x % 64;
   const int t = blockIdx.x * kUlistWaves + w;
   if (t >= nleaves) return;  // whole wave; no barrier below
   const int A = leaves[t];
   const int dA = nd.depth[A];
-  if (lane == 0) {  // ancestors, root first
-    int v = A;
-    for (int d = dA; d >= 0; d--) {
-      anc[w][d] = v;
-      if (d > 0) v = nd.parent[v];
-    }
-  }
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  const int half = lane / 32, s = lane % 32;
-  const int nA = nd.pend[A] - nd.pbegin[A];
-  int pos = pass ? ofs[t] : 0;
-  int c = 0;
-  long long np = 0;
-  for (int d0 = 0; d0 <= dA; d0 += 2) {
-    const int d = d0 + half;
-    bool take = false;
-    int j = -1, b = 0, e = 0;
-    if (s < 27 && d <= dA) {
-      j = col[(size_t)anc[w][d] * 27 + s];
-      if (j >= 0 && !(d < dA && (s == 13 || !nd.leaf[j]))) {
-        take = true;
-        b = nd.pbegin[j];
-        e = nd.pend[j];
-      }
-    }
-    const unsigned long long m = __ballot(take);
-    if (pass && take) {
-      const int k = pos + __popcll(m & ((1ull << lane) - 1ull));
-      ubeg[k] = b;
-      uend[k] = e;
-      if (j == A)
-        np += 2LL * ((long long)nA * (nA + 1) / 2);
-      else
-        np += (long long)nA * (e - b);
-    }
-    pos += __popcll(m);
-    c += __popcll(m);
-  }
+  UlistSlots u;
+  const int c = ulist_slots(nd, col, A, dA, u);
   if (!pass) {
     if (lane == 0) cnt[t] = c;
-  } else {
-    for (int off = 32; off > 0; off >>= 1) np += __shfl_xor(np, off);
-    if (lane == 0) pairs[t] = np;
+    return;
+  }
+  const long pos = ofs[t];
+  const long long np =
+      pos < cap ? ulist_write(nd, u, A, pos, cap, ubeg, uend) : 0;
+  if (lane == 0) {
+    pairs[t] = np;
+    if (cap != LONG_MAX) {
+      ofs[t] = (int)min(pos, cap);
+      if (t == nleaves - 1) {
+        const long all = ofs[nleaves];
+        *total = (int)min(all, (long)INT32_MAX);
+        ofs[nleaves] = (int)min(all, cap);
+      }
+    }
   }
 }
 
@@ -1394,11 +1448,13 @@ hipError_t launch_vlist(const NodeArrays &nd, int nnodes, const int *col,
 }
 
 hipError_t launch_ulist(const NodeArrays &nd, int nleaves, const int *leaves,
-                        const int *col, int pass, int *cnt, const int *ofs,
-                        int *ubeg, int *uend, long long *pairs, hipStream_t s) {
+                        const int *col, int pass, int *cnt, int *ofs,
+                        int *ubeg, int *uend, long long *pairs, long cap,
+                        int *total, hipStream_t s) {
   if (nleaves == 0) return hipSuccess;
   k_ulist<<<(nleaves + kUlistWaves - 1) / kUlistWaves, 64 * kUlistWaves, 0,
-            s>>>(nd, nleaves, leaves, col, pass, cnt, ofs, ubeg, uend, pairs);
+            s>>>(nd, nleaves, leaves, col, pass, cnt, ofs, ubeg, uend, pairs,
+                 cap, total);
   return hipGetLastError();
 }
 

@@ -20,9 +20,13 @@
 
 namespace aran {
 
+// ints per node in NodeArrays::anc (depths 0..21)
+constexpr int kAncRow = 22;
+
 struct NodeArrays {
   int *depth, *ix, *iy, *iz, *pbegin, *pend, *parent, *child, *nchild, *leaf;
   int *sys;           // system (independent particle set) of the node
+  int *anc;           // [node][kAncRow]: ancestor at depth 0..depth (itself)
   uint64_t *key;
   double *cx, *cy, *cz;
   const double *box;  // per system: lo x, lo y, lo z, width of the root cube

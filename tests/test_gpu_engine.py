@@ -244,6 +244,33 @@ def test_large_tree_bit_exact(torch_cuda, kind, n, q):
     assert fo[-1] == len(fs)
 
 
+def test_near_lists_one_launch_and_rerun(torch_cuda):
+    """A context that has solved before builds its near lists in one launch
+    sized from the last solve's ranges per leaf.  A small, shallow problem
+    first sets that size far too low for the clustered one after it, whose
+    lists then outgrow their arrays and are rebuilt by a second run; a third
+    solve takes the one-launch path within its size.  Every result and list
+    equals a fresh context's (two-pass lists) bitwise."""
+    torch = torch_cuda
+    (xs, ys, zs, ms), lbs, ubs = _tree_case("uniform", 2000, 5)
+    (x, y, z, m), lb, ub = _tree_case("plummer", 200_000, 6)
+    dev = _dev(torch, x, y, z, m)
+    fresh = _fmm(torch)
+    want = [a.cpu().numpy() for a in fresh.solve(*dev, lb, ub, 6, 8)]
+    want_lists = fresh.export_lists()
+    want_st = fresh.stats()
+    f = _fmm(torch)
+    f.solve(*_dev(torch, xs, ys, zs, ms), lbs, ubs, 6, 64)
+    for _ in range(2):  # rerun after overflow, then within the size
+        got = [a.cpu().numpy() for a in f.solve(*dev, lb, ub, 6, 8)]
+        assert np.array_equal(got[0], want[0])
+        assert np.array_equal(got[1], want[1])
+        assert f.stats() == want_st
+        lists = f.export_lists()
+        for k in ("leaves", "near_ofs", "near_begin", "near_end"):
+            assert np.array_equal(lists[k], want_lists[k]), k
+
+
 # ---------------------------------------------------------------- solve
 
 @pytest.mark.parametrize("kind,n,p,q", [("uniform", 10_000, 8, 20),
