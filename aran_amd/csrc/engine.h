@@ -137,7 +137,7 @@ struct Workspace {
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
   DevBuf n_anc;  // ancestor rows (NodeArrays::anc)
-  DevBuf split_bounds, split_cnt, split_ofs, flags, leaf_list, lstart, lcount;
+  DevBuf split_ofs, flags, leaf_list;
   DevBuf internal_list;  // internal nodes in node order (level by level)
   DevBuf trans_rows;     // TransRow of each internal node (same order)
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
@@ -149,8 +149,8 @@ struct Workspace {
   // autotuned far-field schedules, by (form, order, size bucket)
   std::map<long, int> tuned_sched;
   DevBuf mult, loc, small, pleaf;
-  DevBuf tile_tgt, tile_src, tile_oct, node_slot, tile_cnt, tile_rank,
-      tile_meta, mult32, loc32, tile_perm, mult_bf, loc_mfma, mult64, loc64;
+  DevBuf tile_tgt, tile_src, node_slot, tile_cnt, tile_rank,
+      tile_meta, mult32, loc32, mult_bf, loc_mfma, mult64, loc64;
   DevBuf n_sys, sys_box, sys_off, skey, spos, stop;
   // look-back status words (Lookback)
   DevBuf lb_buf;

@@ -188,12 +188,12 @@ std::vector<DevBuf *> Workspace::buffers() {
                    &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
                    &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
                    &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz, &n_anc,
-                   &split_bounds, &split_cnt, &split_ofs, &flags, &leaf_list,
-                   &lstart, &lcount, &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
+                   &split_ofs, &flags, &leaf_list,
+                   &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &w_cnt, &w_ofs,
                    &w_node, &x_cnt, &x_ofs, &x_leaf, &s_part, &mult, &loc,
-                   &small, &pleaf, &tile_tgt, &tile_src, &tile_oct, &node_slot, &tile_cnt,
-                   &tile_rank, &tile_meta, &mult32, &loc32, &tile_perm,
+                   &small, &pleaf, &tile_tgt, &tile_src, &node_slot, &tile_cnt,
+                   &tile_rank, &tile_meta, &mult32, &loc32,
                    &mult_bf, &loc_mfma, &mult64, &loc64, &n_sys, &sys_box,
                    &sys_off,
                    &skey, &spos, &stop, &internal_list,
@@ -999,41 +999,52 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   CK(ws.leaf_list.reserve(sizeof(int) * (nn + 1)));
   CK(ws.internal_list.reserve(sizeof(int) * (nn + 1)));
   CK(ws.split_ofs.reserve(sizeof(int) * (nn + 1)));
-  CK(ws.lstart.reserve(sizeof(int) * (kMaxDepth + 2)));
-  CK(ws.lcount.reserve(sizeof(int) * (kMaxDepth + 2)));
-  CK(stage_copy(ws, ws.lstart.ptr, lstart.data(), sizeof(int) * lstart.size(),
-                s));
-  CK(stage_copy(ws, ws.lcount.ptr, lcount.data(), sizeof(int) * lcount.size(),
-                s));
+  // Far-field tiles (all forms but the plain one on its per-pair kernel):
+  // 64 targets of one (level, octant) group each.
+  const bool tiled = (pb.precision != 0 || dense64) && depth >= 2;
+  const int ngroups = (kMaxDepth + 1) * 8;
+  const int tfirst = tiled ? lstart[2] : 0;
+  const int tcount = tiled ? (int)nn - lstart[2] : 0;
+  // One device table of the level sizes and the far-field tile tables:
+  // lcount[K], lstart[K] (zero past the depth), then (tiled) gofs[ngroups]
+  // (first chunk of each (level, octant) group), tile_of[chunks] (tile of
+  // each chunk), tile_oct[tiles], and the far-field workgroups' tiles of
+  // both schedules; the level part is copied now, the rest once the group
+  // sizes are known.
+  constexpr int K = kMaxDepth + 2;
+  const int tpw_max = std::max(f64_wg_tiles(p), mfma_wg_tiles(p));
+  const size_t max_chunks = tiled ? (size_t)tcount / 64 + ngroups : 0;
+  CK(ws.tile_meta.reserve(
+      sizeof(int) * (2 * K + (tiled ? ngroups + 2 * max_chunks +
+                                          2 * (max_chunks +
+                                               (size_t)ngroups * tpw_max)
+                                    : 0))));
+  int *d_lcount = ws.tile_meta.as<int>(), *d_lstart = d_lcount + K;
+  {
+    int h[2 * K] = {};
+    for (int d = 0; d <= depth; d++) {
+      h[d] = lcount[d];
+      h[K + d] = lstart[d];
+    }
+    CK(stage_copy(ws, d_lcount, h, sizeof(h), s));
+  }
   CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
   CK(scan_one_launch(ws, ws.flags.as<int>(), ws.split_ofs.as<int>(), nn,
                      d_total, s));
   int *d_lvl = reinterpret_cast<int *>(ws.small.as<char>() + 3584);
   int *h_lvl = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +
                                        3584);
-  CK(launch_level_internal(ws.split_ofs.as<int>(), d_total,
-                           ws.lstart.as<int>(), depth + 1, nn, d_lvl, s));
+  CK(launch_level_internal(ws.split_ofs.as<int>(), d_total, d_lstart,
+                           depth + 1, nn, d_lvl, s));
   CK(hipMemcpyAsync(h_lvl, d_lvl, sizeof(int) * (depth + 2),
                     hipMemcpyDeviceToHost, s));
-  // Far-field tiles (all forms but the plain one on its per-pair kernel):
-  // 64 targets of one (level, octant) group each.  The group sizes are
-  // counted here, so that they come back with the leaf counts.
-  const bool tiled = (pb.precision != 0 || dense64) && depth >= 2;
-  const int ngroups = (kMaxDepth + 1) * 8;
-  const int tfirst = tiled ? lstart[2] : 0;
-  const int tcount = tiled ? (int)nn - lstart[2] : 0;
-  int *d_meta = nullptr, *trank = nullptr;
+  // The group sizes are counted here, so that they come back with the leaf
+  // counts.
+  int *d_meta = d_lstart;  // lstart[K], gofs, tile_of
+  int *trank = nullptr, *d_tile_oct = nullptr, *d_tile_perm = nullptr;
   const int *h_cnt =
       reinterpret_cast<int *>(static_cast<char *>(ws.host_small) + 4096);
   if (tiled) {
-    // device tables: lstart[kMaxDepth + 2], gofs[ngroups] (first chunk of
-    // each group in tile_of), tile_of[chunks] (tile of each chunk)
-    const size_t nmeta = kMaxDepth + 2 + ngroups + tcount / 64 + ngroups;
-    CK(ws.tile_meta.reserve(sizeof(int) * nmeta));
-    d_meta = ws.tile_meta.as<int>();
-    std::vector<int> ls(kMaxDepth + 2, 0);
-    for (int d = 0; d <= depth; d++) ls[d] = lstart[d];
-    CK(stage_copy(ws, d_meta, ls.data(), sizeof(int) * (kMaxDepth + 2), s));
     CK(ws.tile_cnt.reserve(sizeof(int) * ngroups));
     int *cnt = ws.tile_cnt.as<int>();
     CK(ws.tile_rank.reserve(sizeof(int) * (8 * (size_t)tcount + 1)));
@@ -1092,7 +1103,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // ---- colleagues, level by level
   CK(ws.colleagues.reserve(sizeof(int) * 27 * (size_t)nn));
   int *col = ws.colleagues.as<int>();
-  CK(launch_colleagues_all(nd, nn, ws.lstart.as<int>(), ws.lcount.as<int>(),
+  CK(launch_colleagues_all(nd, nn, d_lstart, d_lcount,
                            col, s));
   // ---- far list (CSR; the mixed form reads its far field straight from
   // the colleagues and builds this only when the lists are exported)
@@ -1144,7 +1155,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // consecutive tiles cover the same region
   int ntiles = 0, nwg = 0;
   if (tiled) {
-    std::vector<int> meta(ngroups, 0), toct;  // gofs, then tile_of
+    std::vector<int> meta(ngroups, 0), toct, perm;  // gofs, then tile_of
     int *gofs = meta.data();
     int nchunks = 0;
     for (int g = 0; g < ngroups; g++) {
@@ -1165,12 +1176,10 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
             toct.push_back(o);
           }
     }
-    CK(ws.tile_oct.reserve(sizeof(int) * std::max(1, ntiles)));
     CK(ws.tile_tgt.reserve(sizeof(int) * 64 * (size_t)std::max(1, ntiles)));
     CK(ws.tile_src.reserve(sizeof(int) * 64 * (size_t)kOctOffsets *
                            std::max(1, ntiles)));
     CK(ws.node_slot.reserve(sizeof(int) * nn));
-    CK(stage_copy(ws, ws.tile_oct.ptr, toct.data(), sizeof(int) * ntiles, s));
     // MFMA far field: workgroups of consecutive chunks of one (level,
     // octant) group (they share every offset code).  Two orders of the same
     // workgroups: level, chunk block, octant (schedule 0: neighbouring
@@ -1199,13 +1208,20 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
               wgt1.push_back(j < nch ? tile_of[gofs[d * 8 + o] + j] : -1);
         }
       nwg = (int)(wgt.size() / tpw);
-      wgt.insert(wgt.end(), wgt1.begin(), wgt1.end());
-      CK(ws.tile_perm.reserve(sizeof(int) * std::max<size_t>(1, wgt.size())));
-      CK(stage_copy(ws, ws.tile_perm.ptr, wgt.data(), sizeof(int) * wgt.size(),
-                    s));
+      perm.swap(wgt);
+      perm.insert(perm.end(), wgt1.begin(), wgt1.end());
     }
-    CK(stage_copy(ws, d_meta + kMaxDepth + 2, meta.data(),
-                  sizeof(int) * meta.size(), s));
+    // gofs, tile_of, tile_oct, the workgroups' tiles: one copy
+    meta.insert(meta.end(), toct.begin(), toct.end());
+    meta.insert(meta.end(), perm.begin(), perm.end());
+    if (meta.size() > (size_t)ngroups + 2 * max_chunks +
+                          2 * (max_chunks + (size_t)ngroups * tpw_max)) {
+      set_error("far-field tile tables beyond their bound (internal error)");
+      return hipErrorUnknown;
+    }
+    d_tile_oct = d_meta + K + ngroups + nchunks;
+    d_tile_perm = d_tile_oct + ntiles;
+    CK(stage_copy(ws, d_meta + K, meta.data(), sizeof(int) * meta.size(), s));
   }
   if (near_hinted) {
     CK(ws.u_beg.reserve(sizeof(int) * (size_t)near_cap));
@@ -1276,7 +1292,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(ws.child_map.reserve(sizeof(int) * 8 * (size_t)nn));
     CK(launch_child_map(nd, nn, ws.child_map.as<int>(), s));
     CK(launch_tile_src(nd, ntiles, ws.tile_tgt.as<int>(),
-                       ws.tile_oct.as<int>(), col, ws.child_map.as<int>(),
+                       d_tile_oct, col, ws.child_map.as<int>(),
                        ws.tab.oct_codes.as<short>(), ws.tile_src.as<int>(),
                        d_far, s));
   }
@@ -1335,8 +1351,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     const size_t wgt_len = (size_t)nwg * mfma_wg_tiles(p);
     auto run = [&](int sched) {
       return launch_m2l_mfma(pb.precision == 2 ? 3 : 6, p, nwg,
-                             ws.tile_perm.as<int>() + sched * wgt_len, ntiles,
-                             ws.tile_src.as<int>(), ws.tile_oct.as<int>(),
+                             d_tile_perm + sched * wgt_len, ntiles,
+                             ws.tile_src.as<int>(), d_tile_oct,
                              ws.tab.oct_codes.as<short>(), ws.tab.mfma_ops.ptr,
                              ws.mult_bf.ptr, nn, nsplit, sched,
                              ws.loc_mfma.as<float>(), s);
@@ -1371,9 +1387,9 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     const int nsplit = m2l_f64_split(p);
     const size_t wgt_len = (size_t)nwg * f64_wg_tiles(p);
     auto run = [&](int sched) {
-      return launch_m2l_f64(p, nwg, ws.tile_perm.as<int>() + sched * wgt_len,
+      return launch_m2l_f64(p, nwg, d_tile_perm + sched * wgt_len,
                             ntiles, ws.tile_src.as<int>(),
-                            ws.tile_oct.as<int>(),
+                            d_tile_oct,
                             ws.tab.oct_codes.as<short>(),
                             ws.tab.f64_ops.as<double>(), ws.mult64.as<double>(),
                             nn, nsplit, sched, ws.loc64.as<double>(), s);
@@ -1409,7 +1425,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     CK(launch_scale_mult(1, nd, nn, p, M, ws.mult32.ptr, s));
     M2LTileArgs ta;
     ta.tile_src = ws.tile_src.as<int>();
-    ta.tile_oct = ws.tile_oct.as<int>();
+    ta.tile_oct = d_tile_oct;
     ta.oct_codes = ws.tab.oct_codes.as<short>();
     ta.dir = ws.tab.dir.as<DirEntry>();
     ta.ephi = ws.tab.ephi32.ptr;
