@@ -863,6 +863,7 @@ int leaf_op_one(int op, int p, const VsgVector3d &center,
   nd.nchild = &dh->ints[9];
   nd.sys = nullptr;  // the leaf kernels use centres, not system boxes
   nd.anc = nullptr;
+  nd.cmap = nullptr;
   nd.box = nullptr;
   nd.key = &dh->key;
   nd.cx = &dh->c[0];

@@ -132,11 +132,11 @@ struct Workspace {
   DevBuf keys_a, keys_b, idx_a, idx_b, xs, ys, zs, qs, pot_s, grad_s;
   DevBuf packed;     // (x, y, z, q) per particle in input order
   DevBuf lvl_table;  // per level: first node, node count (device)
-  DevBuf child_map;  // per node: child in each octant (-1: none)
   DevBuf scan_tmp, hist, part;
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
-  DevBuf n_anc;  // ancestor rows (NodeArrays::anc)
+  DevBuf n_anc;   // ancestor rows (NodeArrays::anc)
+  DevBuf n_cmap;  // children by octant (NodeArrays::cmap)
   DevBuf split_ofs, flags, leaf_list;
   DevBuf internal_list;  // internal nodes in node order (level by level)
   DevBuf trans_rows;     // TransRow of each internal node (same order)

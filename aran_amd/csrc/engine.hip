@@ -175,6 +175,7 @@ NodeArrays Workspace::nodes() const {
   a.leaf = n_leaf.as<int>();
   a.sys = n_sys.as<int>();
   a.anc = n_anc.as<int>();
+  a.cmap = n_cmap.as<int>();
   a.box = sys_box.as<double>();
   a.key = n_key.as<uint64_t>();
   a.cx = n_cx.as<double>();
@@ -187,7 +188,7 @@ std::vector<DevBuf *> Workspace::buffers() {
   std::vector<DevBuf *> all = {&keys_a, &keys_b, &idx_a, &idx_b, &xs, &ys, &zs, &qs,
                    &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
                    &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
-                   &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz, &n_anc,
+                   &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz, &n_anc, &n_cmap,
                    &split_ofs, &flags, &leaf_list,
                    &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &w_cnt, &w_ofs,
@@ -198,7 +199,7 @@ std::vector<DevBuf *> Workspace::buffers() {
                    &sys_off,
                    &skey, &spos, &stop, &internal_list,
                    &trans_rows, &packed,
-                   &lvl_table, &child_map, &lb_buf};
+                   &lvl_table, &lb_buf};
   Tables *ts[1] = {&tab};
   for (Tables *t : ts) {
     DevBuf *tb[] = {&t->rot,    &t->eiphi,  &t->dir,    &t->tm2l,   &t->tm2m,
@@ -637,6 +638,7 @@ static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
   CK(ws.n_key.grow_keep(sizeof(uint64_t) * cap, sizeof(uint64_t) * keep, s));
   CK(ws.n_anc.grow_keep(sizeof(int) * kAncRow * cap,
                         sizeof(int) * kAncRow * keep, s));
+  CK(ws.n_cmap.grow_keep(sizeof(int) * 8 * cap, sizeof(int) * 8 * keep, s));
   DevBuf *dbl[] = {&ws.n_cx, &ws.n_cy, &ws.n_cz};
   for (DevBuf *b : dbl)
     CK(b->grow_keep(sizeof(double) * cap, sizeof(double) * keep, s));
@@ -645,6 +647,7 @@ static hipError_t reserve_nodes(Workspace &ws, long cap, long keep) {
   ws.node_cap = std::min<long>(ws.node_cap, ws.n_key.bytes / sizeof(uint64_t));
   ws.node_cap =
       std::min<long>(ws.node_cap, ws.n_anc.bytes / (sizeof(int) * kAncRow));
+  ws.node_cap = std::min<long>(ws.node_cap, ws.n_cmap.bytes / (sizeof(int) * 8));
   return hipSuccess;
 }
 
@@ -1289,10 +1292,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                         tofs + ngroups, ws.tile_tgt.as<int>(),
                         ws.node_slot.as<int>(), s));
     CK(launch_fill_int(d_far, 1, 0, s));
-    CK(ws.child_map.reserve(sizeof(int) * 8 * (size_t)nn));
-    CK(launch_child_map(nd, nn, ws.child_map.as<int>(), s));
     CK(launch_tile_src(nd, ntiles, ws.tile_tgt.as<int>(),
-                       d_tile_oct, col, ws.child_map.as<int>(),
+                       d_tile_oct, col, nd.cmap,
                        ws.tab.oct_codes.as<short>(), ws.tile_src.as<int>(),
                        d_far, s));
   }

@@ -141,33 +141,12 @@ __global__ void k_tile_fill(NodeArrays nd, int first, int count,
 // (the far-field operation count).
 // childmap[8 node + o]: child of node in octant o (bit 0: x, 1: y, 2: z),
 // -1 when absent (every octant of a leaf).
-__global__ void k_child_map(NodeArrays nd, int nnodes,
-                            int *__restrict__ childmap) {
-  // issue priority over the near field's waves (second stream), which
-  // would otherwise hold up this latency-bound step of the far-field chain
-  __builtin_amdgcn_s_setprio(2);
-  const int v = blockIdx.x * blockDim.x + threadIdx.x;
-  if (v >= nnodes) return;
-  int c[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
-  const int cb = nd.child[v], nch = nd.leaf[v] ? 0 : nd.nchild[v];
-  for (int k = cb; k < cb + nch; k++)
-    c[(nd.ix[k] & 1) | (nd.iy[k] & 1) << 1 | (nd.iz[k] & 1) << 2] = k;
-  int4 *dst = reinterpret_cast<int4 *>(childmap + 8 * (size_t)v);
-  dst[0] = make_int4(c[0], c[1], c[2], c[3]);
-  dst[1] = make_int4(c[4], c[5], c[6], c[7]);
-}
-
-hipError_t launch_child_map(const NodeArrays &nd, int nnodes, int *childmap,
-                            hipStream_t s) {
-  if (nnodes <= 0) return hipSuccess;
-  k_child_map<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, childmap);
-  return hipGetLastError();
-}
-
 // Sources of every (tile, offset slot, lane): the child in octant
 // (b - offset) mod 2 of the parent's colleague at slot floor((b - offset) /
 // 2), from the child map; the parents' colleagues of the tile go through LDS
-// first.  Every load of a phase is independent of the others.
+// first, and so do the tile's offset slots (decoded once per tile).  Every
+// load of a phase is independent of the others: a thread takes its entries
+// eight at a time, all eight child-map loads in flight before the stores.
 __global__ __launch_bounds__(256) void k_tile_src(
     NodeArrays nd, int ntiles, const int *__restrict__ tile_tgt,
     const int *__restrict__ tile_oct, const int *__restrict__ col,
@@ -177,32 +156,61 @@ __global__ __launch_bounds__(256) void k_tile_src(
   // would otherwise hold up this latency-bound step of the far-field chain
   __builtin_amdgcn_s_setprio(2);
   __shared__ int scol[27 * 64];
+  __shared__ int slot[kOctOffsets];  // (colleague slot << 3 | octant), -1
   __shared__ int wcount[4];
   const int tile = blockIdx.x;
-  for (int e = threadIdx.x; e < 27 * 64; e += blockDim.x) {
-    const int ps = e / 64, lane = e % 64;
-    const int A = tile_tgt[tile * 64 + lane];
-    scol[e] = A >= 0 ? col[(size_t)nd.parent[A] * 27 + ps] : -1;
-  }
-  __syncthreads();
   const int b = tile_oct[tile];
-  const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
-  int found = 0;
-  for (int e = threadIdx.x; e < kOctOffsets * 64; e += blockDim.x) {
-    const int k = e / 64, lane = e % 64;
+  {  // thread: one target (lane), colleague slots ps = wave, wave + 4, ...
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const int A = tile_tgt[tile * 64 + lane];
+    const int P = A >= 0 ? nd.parent[A] : -1;
+    int c[7];
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+      const int ps = w + 4 * i;
+      c[i] = (P >= 0 && ps < 27) ? col[(size_t)P * 27 + ps] : -1;
+    }
+#pragma unroll
+    for (int i = 0; i < 7; i++)
+      if (w + 4 * i < 27) scol[(w + 4 * i) * 64 + lane] = c[i];
+  }
+  for (int k = threadIdx.x; k < kOctOffsets; k += blockDim.x) {
+    const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
     const int code = oct_codes[b * kOctOffsets + k];
     const int ox = code / 49 - 3, oy = (code / 7) % 7 - 3, oz = code % 7 - 3;
-    const int ux = bx - ox, uy = by - oy, uz = bz - oz;  // in [-2, 3]
-    int res = -1;
+    // (in [-3, 4]; the parent's colleagues cover [-2, 3])
+    const int ux = bx - ox, uy = by - oy, uz = bz - oz;
+    int v = -1;
     if (ux >= -2 && ux <= 3 && uy >= -2 && uy <= 3 && uz >= -2 && uz <= 3) {
       const int ps =
           ((ux >> 1) + 1) * 9 + ((uy >> 1) + 1) * 3 + ((uz >> 1) + 1);
-      const int co = (ux & 1) | (uy & 1) << 1 | (uz & 1) << 2;
-      const int C = scol[ps * 64 + lane];
-      if (C >= 0) res = childmap[8 * (size_t)C + co];
+      v = ps << 3 | ((ux & 1) | (uy & 1) << 1 | (uz & 1) << 2);
     }
-    tile_src[(size_t)tile * kOctOffsets * 64 + e] = res;
-    found += res >= 0;
+    slot[k] = v;
+  }
+  __syncthreads();
+  int found = 0;
+  constexpr int kU = 8;
+  for (int e0 = threadIdx.x; e0 < kOctOffsets * 64; e0 += kU * 256) {
+    int res[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int e = e0 + u * 256;
+      res[u] = -1;
+      if (e < kOctOffsets * 64) {
+        const int v = slot[e / 64];
+        const int C = v >= 0 ? scol[(v >> 3) * 64 + e % 64] : -1;
+        if (C >= 0) res[u] = childmap[8 * (size_t)C + (v & 7)];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+      const int e = e0 + u * 256;
+      if (e < kOctOffsets * 64) {
+        tile_src[(size_t)tile * kOctOffsets * 64 + e] = res[u];
+        found += res[u] >= 0;
+      }
+    }
   }
   // per-wave then per-block count, one atomic per block (exact integers)
   for (int off = 32; off > 0; off >>= 1) found += __shfl_down(found, off);

@@ -64,8 +64,6 @@ hipError_t launch_tile_fill(const NodeArrays &nd, int first, int count,
                             const int *gofs, const int *tile_of, int *tile_tgt,
                             int *node_slot, hipStream_t s);
 // childmap[8 node + o]: child of every node in octant o, -1 when absent
-hipError_t launch_child_map(const NodeArrays &nd, int nnodes, int *childmap,
-                            hipStream_t s);
 hipError_t launch_tile_src(const NodeArrays &nd, int ntiles,
                            const int *tile_tgt, const int *tile_oct,
                            const int *col, const int *childmap,

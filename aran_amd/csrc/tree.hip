@@ -952,6 +952,7 @@ __global__ __launch_bounds__(256) void k_split_level(
     nd.child[node] = nch == 0 ? -1 : next_begin + rank;
     nd.nchild[node] = nch;
   }
+  if (valid) nd.cmap[(size_t)node * 8 + o] = ex ? next_begin + rank : -1;
   if (ex) {
     const int ch = next_begin + rank;
     const int dd = dep + 1;

@@ -27,6 +27,7 @@ struct NodeArrays {
   int *depth, *ix, *iy, *iz, *pbegin, *pend, *parent, *child, *nchild, *leaf;
   int *sys;           // system (independent particle set) of the node
   int *anc;           // [node][kAncRow]: ancestor at depth 0..depth (itself)
+  int *cmap;          // [node][8]: child in each octant, -1: none
   uint64_t *key;
   double *cx, *cy, *cz;
   const double *box;  // per system: lo x, lo y, lo z, width of the root cube
