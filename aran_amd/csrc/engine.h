@@ -278,9 +278,6 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
 hipError_t launch_gather(const int *idx, long n, const double4 *packed,
                          double *xs, double *ys, double *zs, double *qs,
                          hipStream_t s);
-hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
-                          const double *grad_s, double *pot, double *grad,
-                          hipStream_t s);
 hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
                         hipStream_t s);
 // Status words of a launch that scans across its workgroups by decoupled
@@ -365,11 +362,15 @@ hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
 // L2P of sorted particles [0, n), adding into pot / grad.  sel 0: every
 // particle, FP64; 1: the particles of leaves at depth >= 2, FP32 recurrence
 // on width-scaled coordinates (the reduced-precision forms); 2: those of
-// leaves at depth < 2, FP64
+// leaves at depth < 2, FP64.  With perm, pot / grad stay as they are and
+// the sums go to opot[perm[i]] / ograd[perm[i]] instead (the caller's
+// order; either may be null).
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
-                      double *pot, double *grad, hipStream_t s, int sel = 0);
+                      double *pot, double *grad, hipStream_t s, int sel = 0,
+                      const int *perm = nullptr, double *opot = nullptr,
+                      double *ograd = nullptr);
 
 // M2P: for every leaf, the multipoles of its semifar nodes evaluated at its
 // sorted particles, added into pot / grad (potential and gradient).

@@ -1485,22 +1485,26 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                   ws.w_node.as<int>(), ws.xs.as<double>(), ws.ys.as<double>(),
                   ws.zs.as<double>(), M, ws.pot_s.as<double>(),
                   ws.grad_s.as<double>(), s));
+  // L2P adds the last term of every particle (each L2P launch takes its
+  // own particles) and writes the sums in the caller's order
+  const int *perm = ws.idx_a.as<int>();
   if (pb.precision == 0) {
     CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
                   ws.ys.as<double>(), ws.zs.as<double>(), L,
-                  ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 0));
+                  ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 0, perm,
+                  pb.pot, pb.grad));
   } else {
     CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
                   ws.ys.as<double>(), ws.zs.as<double>(), L,
-                  ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 1));
+                  ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 1, perm,
+                  pb.pot, pb.grad));
     if (top > 0)
       CK(launch_l2p(tb, nd, ws.pleaf.as<int>(), (int)n, ws.xs.as<double>(),
                     ws.ys.as<double>(), ws.zs.as<double>(), L,
-                    ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 2));
+                    ws.pot_s.as<double>(), ws.grad_s.as<double>(), s, 2, perm,
+                    pb.pot, pb.grad));
   }
   if (pb.timings) CK(hipEventRecord(ws.evk[5], s));
-  CK(launch_scatter(ws.idx_a.as<int>(), n, ws.pot_s.as<double>(),
-                    ws.grad_s.as<double>(), pb.pot, pb.grad, s));
   if (pb.timings) CK(hipEventRecord(ws.ev[6], s));
 
   long long *hp = reinterpret_cast<long long *>(

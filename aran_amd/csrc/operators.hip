@@ -547,7 +547,9 @@ __global__ __launch_bounds__(256) void k_l2p(
     OpTables tb, NodeArrays nd, const int *__restrict__ pleaf, int n,
     const double *__restrict__ xs, const double *__restrict__ ys,
     const double *__restrict__ zs, const double2 *__restrict__ L,
-    double *__restrict__ pot, double *__restrict__ grad, int rows, int sel) {
+    double *__restrict__ pot, double *__restrict__ grad, int rows, int sel,
+    const int *__restrict__ perm, double *__restrict__ opot,
+    double *__restrict__ ograd) {
   using R2 = typename std::conditional<std::is_same<R, float>::value, float2,
                                        double2>::type;
   constexpr bool kScaled = std::is_same<R, float>::value;
@@ -663,10 +665,24 @@ __global__ __launch_bounds__(256) void k_l2p(
     }
     double s = 1.;
     if (kScaled) s = 1. / sw[slot - r0];
-    pot[i] += (double)phi;
-    grad[3 * (size_t)i + 0] += (double)gx * s;
-    grad[3 * (size_t)i + 1] += (double)gy * s;
-    grad[3 * (size_t)i + 2] += (double)dz * s;
+    const double rp = pot[i] + (double)phi;
+    const double rx = grad[3 * (size_t)i + 0] + (double)gx * s;
+    const double ry = grad[3 * (size_t)i + 1] + (double)gy * s;
+    const double rz = grad[3 * (size_t)i + 2] + (double)dz * s;
+    if (!perm) {
+      pot[i] = rp;
+      grad[3 * (size_t)i + 0] = rx;
+      grad[3 * (size_t)i + 1] = ry;
+      grad[3 * (size_t)i + 2] = rz;
+    } else {  // the last term of the particle: straight to the caller's order
+      const size_t j = (size_t)perm[i];
+      if (opot) opot[j] = rp;
+      if (ograd) {
+        ograd[3 * j + 0] = rx;
+        ograd[3 * j + 1] = ry;
+        ograd[3 * j + 2] = rz;
+      }
+    }
   }
 }
 
@@ -983,7 +999,8 @@ hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
-                      double *pot, double *grad, hipStream_t s, int sel) {
+                      double *pot, double *grad, hipStream_t s, int sel,
+                      const int *perm, double *opot, double *ograd) {
   if (n == 0) return hipSuccess;
   if (sel < 0 || sel > 2) return hipErrorInvalidValue;
   // staged rows per round: up to 16 leaves within 32 KiB of LDS
@@ -992,10 +1009,12 @@ hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
   const int rows = (int)std::max<size_t>(1, std::min<size_t>(16, 32768 / row));
   if (sel == 1)
     k_l2p<float><<<(n + 255) / 256, 256, rows * row, s>>>(
-        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel);
+        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel, perm, opot,
+        ograd);
   else
     k_l2p<double><<<(n + 255) / 256, 256, rows * row, s>>>(
-        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel);
+        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel, perm, opot,
+        ograd);
   return hipGetLastError();
 }
 

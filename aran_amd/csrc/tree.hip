@@ -770,7 +770,7 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
   return hipGetLastError();
 }
 
-// ------------------------------------------------------------ gather / scatter
+// ------------------------------------------------------------ gather
 
 __global__ void k_gather(const int *__restrict__ idx, long n,
                          const double4 *__restrict__ packed,
@@ -790,29 +790,6 @@ hipError_t launch_gather(const int *idx, long n, const double4 *packed,
                          hipStream_t s) {
   if (n == 0) return hipSuccess;
   k_gather<<<(n + 255) / 256, 256, 0, s>>>(idx, n, packed, xs, ys, zs, qs);
-  return hipGetLastError();
-}
-
-__global__ void k_scatter(const int *__restrict__ idx, long n,
-                          const double *__restrict__ pot_s,
-                          const double *__restrict__ grad_s,
-                          double *__restrict__ pot, double *__restrict__ grad) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  int j = idx[i];
-  if (pot) pot[j] = pot_s[i];
-  if (grad) {
-    grad[3L * j + 0] = grad_s[3 * i + 0];
-    grad[3L * j + 1] = grad_s[3 * i + 1];
-    grad[3L * j + 2] = grad_s[3 * i + 2];
-  }
-}
-
-hipError_t launch_scatter(const int *idx, long n, const double *pot_s,
-                          const double *grad_s, double *pot, double *grad,
-                          hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  k_scatter<<<(n + 255) / 256, 256, 0, s>>>(idx, n, pot_s, grad_s, pot, grad);
   return hipGetLastError();
 }
 
