@@ -137,7 +137,7 @@ struct Workspace {
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
   DevBuf n_anc;   // ancestor rows (NodeArrays::anc)
   DevBuf n_cmap;  // children by octant (NodeArrays::cmap)
-  DevBuf split_ofs, flags, leaf_list;
+  DevBuf split_ofs, leaf_list;
   DevBuf internal_list;  // internal nodes in node order (level by level)
   DevBuf trans_rows;     // TransRow of each internal node (same order)
   DevBuf v_cnt, v_ofs, v_src, v_code, u_cnt, u_ofs, u_beg, u_end, u_pairs,
@@ -261,12 +261,13 @@ hipError_t launch_permute_keys(const int *pos, long n, const uint64_t *kin,
 int minmax_blocks(int nsys);
 // Extent of each system and its root cube: box[4 s ..] = (lo, width) from
 // its bounds (bounds[6 s ..] = lb, ub) and particles (a system with a
-// non-finite coordinate sets bit 4 of *err, which starts at zero); also
-// sets up the level table dl[0 .. ntab) (level 0: the nsys roots).
+// non-finite coordinate sets bit 4 of *err, which starts at zero), and its
+// root node s of nd; also sets up the level table dl[0 .. ntab) (level 0:
+// the nsys roots).
 hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
                              const long *soff, const double *bounds, int nsys,
                              double *part, double *box, int *dl, int ntab,
-                             int *err, hipStream_t s);
+                             int *err, const NodeArrays &nd, hipStream_t s);
 hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
                           DevBuf &tmp, hipStream_t s);
 // the same in one launch for n <= 2^17 (falls back to scan_exclusive)
@@ -278,8 +279,6 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
 hipError_t launch_gather(const int *idx, long n, const double4 *packed,
                          double *xs, double *ys, double *zs, double *qs,
                          hipStream_t s);
-hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
-                        hipStream_t s);
 // Status words of a launch that scans across its workgroups by decoupled
 // look-back (one word per chunk of work).  The words need no clearing
 // between launches: each launch has its own epoch.
@@ -308,8 +307,6 @@ long split_level_chunks(long max_count);
 // Exclusive scan of (*n_dev) entries, one launch.
 hipError_t scan_device_count(const int *in, int *out, const int *n_dev,
                              int *total, hipStream_t s);
-hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
-                             hipStream_t s);
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
                                const int *pos, int *leaves, int *internal,
                                hipStream_t s);

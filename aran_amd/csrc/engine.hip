@@ -189,7 +189,7 @@ std::vector<DevBuf *> Workspace::buffers() {
                    &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
                    &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
                    &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz, &n_anc, &n_cmap,
-                   &split_ofs, &flags, &leaf_list,
+                   &split_ofs, &leaf_list,
                    &v_cnt, &v_ofs, &v_src, &v_code, &u_cnt,
                    &u_ofs, &u_beg, &u_end, &u_pairs, &colleagues, &w_cnt, &w_ofs,
                    &w_node, &x_cnt, &x_ofs, &x_leaf, &s_part, &mult, &loc,
@@ -871,9 +871,14 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // root cubes on the device (no host round trip): a non-finite coordinate
   // is reported with the first level counts
   CK(ws.part.reserve(sizeof(double) * 6 * (size_t)nsys * minmax_blocks(nsys)));
+  // node arrays (grown by the level batches when needed); the extent
+  // launch writes the roots
+  const long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64 + nsys);
+  if (ws.node_cap < cap0 || !ws.n_depth.ptr) CK(reserve_nodes(ws, cap0, 0));
+  NodeArrays nd = ws.nodes();
   CK(launch_sys_minmax(pb.x, pb.y, pb.z, ws.sys_off.as<long>(), d_bounds,
                        nsys, ws.part.as<double>(), ws.sys_box.as<double>(),
-                       d_lvl_tab, kTab, d_lvl_tab + kTab, s));
+                       d_lvl_tab, kTab, d_lvl_tab + kTab, nd, s));
 
   // ---- keys and sort (by system, then Morton key)
   CK(ws.keys_a.reserve(sizeof(uint64_t) * n));
@@ -913,16 +918,12 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   const uint64_t *keys = ws.keys_a.as<uint64_t>();
 
   // ---- levels (level 0: one root per system)
-  long cap0 = std::max<long>(1024, 4 * (n / pb.leafcap + 1) + 64 + nsys);
-  if (ws.node_cap < cap0 || !ws.n_depth.ptr) CK(reserve_nodes(ws, cap0, 0));
-  NodeArrays nd = ws.nodes();
   const ulonglong2 *d_stop = nullptr;
   if (pb.nstop > 0) {
     CK(ws.stop.reserve(sizeof(ulonglong2) * pb.nstop));
     CK(stage_copy(ws, ws.stop.ptr, pb.stop, sizeof(ulonglong2) * pb.nstop, s));
     d_stop = ws.stop.as<ulonglong2>();
   }
-  CK(launch_roots(nd, nsys, ws.sys_off.as<long>(), s));
   // Levels in batches of kLevelBatch without a host round trip: the level
   // table lives on the device and grids are sized by upper bounds (a level
   // has at most 8 nodes per split node of the level above, and a split node
@@ -998,7 +999,6 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
   // ---- leaves
   // ---- leaves and internal nodes (the latter level by level, for the
   // M2M / L2L level products); one read brings both counts
-  CK(ws.flags.reserve(sizeof(int) * nn));
   CK(ws.leaf_list.reserve(sizeof(int) * (nn + 1)));
   CK(ws.internal_list.reserve(sizeof(int) * (nn + 1)));
   CK(ws.split_ofs.reserve(sizeof(int) * (nn + 1)));
@@ -1031,8 +1031,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     }
     CK(stage_copy(ws, d_lcount, h, sizeof(h), s));
   }
-  CK(launch_leaf_flags(nd, nn, ws.flags.as<int>(), s));
-  CK(scan_one_launch(ws, ws.flags.as<int>(), ws.split_ofs.as<int>(), nn,
+  CK(scan_one_launch(ws, nd.leaf, ws.split_ofs.as<int>(), nn,
                      d_total, s));
   int *d_lvl = reinterpret_cast<int *>(ws.small.as<char>() + 3584);
   int *h_lvl = reinterpret_cast<int *>(static_cast<char *>(ws.host_small) +

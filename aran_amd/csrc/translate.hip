@@ -106,14 +106,17 @@ __global__ void k_trans_rows(NodeArrays nd, const int *__restrict__ internal,
   const int v = internal[j];
   TransRow r;
   r.node = v;
-  for (int o = 0; o < 8; o++) r.child[o] = -1;
-  const int cb = nd.child[v], nch = nd.nchild[v];
-  for (int c = 0; c < nch; c++) {
-    const int C = cb + c;
-    const int o =
-        (nd.ix[C] & 1) | ((nd.iy[C] & 1) << 1) | ((nd.iz[C] & 1) << 2);
-    r.child[o] = C;
-  }
+  // children by octant, as the level launches recorded them
+  const int4 *cm = reinterpret_cast<const int4 *>(nd.cmap + 8 * (size_t)v);
+  const int4 c0 = cm[0], c1 = cm[1];
+  r.child[0] = c0.x;
+  r.child[1] = c0.y;
+  r.child[2] = c0.z;
+  r.child[3] = c0.w;
+  r.child[4] = c1.x;
+  r.child[5] = c1.y;
+  r.child[6] = c1.z;
+  r.child[7] = c1.w;
   r.pad = 0;
   r.width = node_width(nd, v);
   rows[j] = r;

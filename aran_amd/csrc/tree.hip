@@ -145,15 +145,16 @@ __global__ void k_sys_minmax(const double *__restrict__ x,
 
 // One block per system: its nb partial extents combined (a NaN minimum
 // marks a non-finite coordinate and survives), then the system's root cube
-// from its bounds (lb, ub at bounds[6 sy ..]): box[4 sy ..] = (lo, width).
-// A system with a non-finite coordinate sets bit 4 of *err.  Block 0 also
-// sets up the level table: level 0 holds the nsys roots.
+// from its bounds (lb, ub at bounds[6 sy ..]): box[4 sy ..] = (lo, width),
+// and its root node sy, covering the system's particle range.  A system
+// with a non-finite coordinate sets bit 4 of *err.  Block 0 also sets up
+// the level table: level 0 holds the nsys roots.
 __global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
                                    int nb, const long *__restrict__ soff,
                                    const double *__restrict__ bounds,
                                    double *__restrict__ box,
                                    int *__restrict__ dl, int ntab,
-                                   int *__restrict__ err) {
+                                   int *__restrict__ err, NodeArrays nd) {
   __shared__ double red[6][256];
   const int sy = blockIdx.x;
   if (sy == 0)
@@ -193,15 +194,31 @@ __global__ void k_sys_minmax_final(const double *__restrict__ part, int nsys,
     pmax[k] = red[3 + k][0];
     finite = finite && isfinite(pmin[k]) && isfinite(pmax[k]);
   }
+  double lo[3] = {0., 0., 0.}, width = 1.;
   if (soff[sy + 1] == soff[sy]) {  // empty: the cube of its bounds
     for (int k = 0; k < 3; k++) pmin[k] = pmax[k] = lb[k];
-  } else if (!finite) {
+    root_cube(lb, ub, pmin, pmax, lo, &width);
+  } else if (!finite) {  // (the solve fails at its first level read)
     atomicOr(err, 4);
-    for (int k = 0; k < 3; k++) box[4 * sy + k] = 0.;
-    box[4 * sy + 3] = 1.;
-    return;
+  } else {
+    root_cube(lb, ub, pmin, pmax, lo, &width);
   }
-  root_cube(lb, ub, pmin, pmax, box + 4 * sy, box + 4 * sy + 3);
+  for (int k = 0; k < 3; k++) box[4 * sy + k] = lo[k];
+  box[4 * sy + 3] = width;
+  nd.depth[sy] = 0;
+  nd.ix[sy] = nd.iy[sy] = nd.iz[sy] = 0;
+  nd.key[sy] = 0;
+  nd.sys[sy] = sy;
+  nd.anc[(size_t)sy * kAncRow] = sy;
+  nd.pbegin[sy] = (int)soff[sy];
+  nd.pend[sy] = (int)soff[sy + 1];
+  nd.parent[sy] = -1;
+  nd.child[sy] = -1;
+  nd.nchild[sy] = 0;
+  nd.leaf[sy] = 1;
+  nd.cx[sy] = fma(0.5, width, lo[0]);
+  nd.cy[sy] = fma(0.5, width, lo[1]);
+  nd.cz[sy] = fma(0.5, width, lo[2]);
 }
 
 // part must hold 6 * nsys * minmax_blocks(nsys) doubles.
@@ -212,12 +229,12 @@ int minmax_blocks(int nsys) {
 hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
                              const long *soff, const double *bounds, int nsys,
                              double *part, double *box, int *dl, int ntab,
-                             int *err, hipStream_t s) {
+                             int *err, const NodeArrays &nd, hipStream_t s) {
   if (nsys == 0) return hipSuccess;
   const int nb = minmax_blocks(nsys);
   k_sys_minmax<<<dim3(nsys, nb), 256, 0, s>>>(x, y, z, soff, part, err);
   k_sys_minmax_final<<<nsys, 256, 0, s>>>(part, nsys, nb, soff, bounds, box,
-                                          dl, ntab, err);
+                                          dl, ntab, err, nd);
   return hipGetLastError();
 }
 
@@ -795,33 +812,6 @@ hipError_t launch_gather(const int *idx, long n, const double4 *packed,
 
 // ------------------------------------------------------------ level build
 
-// One root per system, covering the system's particle range.
-__global__ void k_roots(NodeArrays nd, int nsys, const long *__restrict__ soff) {
-  int sy = blockIdx.x * blockDim.x + threadIdx.x;
-  if (sy >= nsys) return;
-  nd.depth[sy] = 0;
-  nd.ix[sy] = nd.iy[sy] = nd.iz[sy] = 0;
-  nd.key[sy] = 0;
-  nd.sys[sy] = sy;
-  nd.anc[(size_t)sy * kAncRow] = sy;
-  nd.pbegin[sy] = (int)soff[sy];
-  nd.pend[sy] = (int)soff[sy + 1];
-  nd.parent[sy] = -1;
-  nd.child[sy] = -1;
-  nd.nchild[sy] = 0;
-  nd.leaf[sy] = 1;
-  const double *b = nd.box + 4 * sy;
-  nd.cx[sy] = fma(0.5, b[3], b[0]);
-  nd.cy[sy] = fma(0.5, b[3], b[1]);
-  nd.cz[sy] = fma(0.5, b[3], b[2]);
-}
-
-hipError_t launch_roots(const NodeArrays &nd, int nsys, const long *soff,
-                        hipStream_t s) {
-  k_roots<<<(nsys + 63) / 64, 64, 0, s>>>(nd, nsys, soff);
-  return hipGetLastError();
-}
-
 // First index in [b, e) whose octant digit at 'shift' is >= o.
 __device__ inline int digit_lower_bound(const uint64_t *__restrict__ keys,
                                         int b, int e, int shift, int o) {
@@ -981,11 +971,6 @@ long split_level_chunks(long max_count) {
 
 // ------------------------------------------------------------ leaves
 
-__global__ void k_leaf_flags(NodeArrays nd, int nnodes, int *__restrict__ flag) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < nnodes) flag[i] = nd.leaf[i];
-}
-
 // leaves[pos[i]] = i for leaves; internal[i - pos[i]] = i for the others
 // (both in node order, so grouped by level)
 __global__ void k_leaf_compact(NodeArrays nd, int nnodes,
@@ -1014,12 +999,6 @@ __global__ void k_level_internal(const int *__restrict__ pos,
   const int la = pos[a];
   const int lb = b < nnodes ? pos[b] : *total;
   out[1 + d] = (b - a) - (lb - la);
-}
-
-hipError_t launch_leaf_flags(const NodeArrays &nd, int nnodes, int *flag,
-                             hipStream_t s) {
-  k_leaf_flags<<<(nnodes + 255) / 256, 256, 0, s>>>(nd, nnodes, flag);
-  return hipGetLastError();
 }
 
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
