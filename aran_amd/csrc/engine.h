@@ -132,7 +132,7 @@ struct Workspace {
   DevBuf keys_a, keys_b, idx_a, idx_b, xs, ys, zs, qs, pot_s, grad_s;
   DevBuf packed;     // (x, y, z, q) per particle in input order
   DevBuf lvl_table;  // per level: first node, node count (device)
-  DevBuf scan_tmp, hist, part;
+  DevBuf hist, part;
   DevBuf n_depth, n_ix, n_iy, n_iz, n_pbegin, n_pend, n_parent, n_child,
       n_nchild, n_leaf, n_key, n_cx, n_cy, n_cz;
   DevBuf n_anc;   // ancestor rows (NodeArrays::anc)
@@ -268,14 +268,13 @@ hipError_t launch_sys_minmax(const double *x, const double *y, const double *z,
                              const long *soff, const double *bounds, int nsys,
                              double *part, double *box, int *dl, int ntab,
                              int *err, const NodeArrays &nd, hipStream_t s);
-hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
-                          DevBuf &tmp, hipStream_t s);
-// the same in one launch for n <= 2^17 (falls back to scan_exclusive)
-hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
+// Exclusive scan of n <= 2^14 ints in one launch of one workgroup (in and
+// out may alias); writes the total to *total.
+hipError_t scan_small(const int *in, int *out, long n, int *total,
                       hipStream_t s);
 hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
-                      int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
-                      int *total, int passes, hipStream_t s);
+                      int *idx_tmp, long n, DevBuf &hist, int passes,
+                      hipStream_t s);
 hipError_t launch_gather(const int *idx, long n, const double4 *packed,
                          double *xs, double *ys, double *zs, double *qs,
                          hipStream_t s);
@@ -292,7 +291,7 @@ struct Lookback {
 hipError_t scan_lookback(const int *in, int *out, long n, int *total,
                          const Lookback &lb, hipStream_t s);
 long scan_lookback_chunks(long n);
-// whether scan_small takes its one-workgroup launch for n entries
+// whether n entries fit scan_small
 bool scan_fits_single(long n);
 // Level table on the device: dl[2 d] first node of level d, dl[2 d + 1] its
 // node count; max_count bounds the count for the grid.  One launch splits
@@ -304,9 +303,6 @@ hipError_t launch_split_level(const NodeArrays &nd, int *dl, int d,
                               int leafcap, const ulonglong2 *stop, int nstop,
                               const Lookback &lb, hipStream_t s);
 long split_level_chunks(long max_count);
-// Exclusive scan of (*n_dev) entries, one launch.
-hipError_t scan_device_count(const int *in, int *out, const int *n_dev,
-                             int *total, hipStream_t s);
 hipError_t launch_leaf_compact(const NodeArrays &nd, int nnodes,
                                const int *pos, int *leaves, int *internal,
                                hipStream_t s);

@@ -186,7 +186,7 @@ NodeArrays Workspace::nodes() const {
 
 std::vector<DevBuf *> Workspace::buffers() {
   std::vector<DevBuf *> all = {&keys_a, &keys_b, &idx_a, &idx_b, &xs, &ys, &zs, &qs,
-                   &pot_s, &grad_s, &scan_tmp, &hist, &part, &n_depth, &n_ix,
+                   &pot_s, &grad_s, &hist, &part, &n_depth, &n_ix,
                    &n_iy, &n_iz, &n_pbegin, &n_pend, &n_parent, &n_child,
                    &n_nchild, &n_leaf, &n_key, &n_cx, &n_cy, &n_cz, &n_anc, &n_cmap,
                    &split_ofs, &leaf_list,
@@ -337,7 +337,7 @@ static hipError_t lookback_take(Workspace &ws, long nchunks, int *err,
 static hipError_t scan_one_launch(Workspace &ws, const int *in, int *out,
                                   long n, int *total, hipStream_t s) {
   if (scan_fits_single(n))
-    return scan_small(in, out, n, total, ws.scan_tmp, s);
+    return scan_small(in, out, n, total, s);
   Lookback lb;
   CK(lookback_take(ws, scan_lookback_chunks(n), lb_err_word(ws), &lb, s));
   return scan_lookback(in, out, n, total, lb, s);
@@ -894,8 +894,8 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
                  ws.sys_box.as<double>(), ws.keys_a.as<uint64_t>(),
                  ws.idx_a.as<int>(), ws.packed.as<double4>(), s));
   CK(radix_sort(ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(),
-                ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), n, ws.hist,
-                ws.scan_tmp, d_total, 8, s));
+                ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), n, ws.hist, 8,
+                s));
   if (nsys > 1) {
     // stable sort of the Morton order by system (2 or 4 byte passes)
     const int passes = nsys <= 65536 ? 2 : 4;
@@ -905,8 +905,7 @@ hipError_t solve(Workspace &ws, const Problem &pb, Stats *st, TreeInfo *ti) {
     int *sp = ws.spos.as<int>();
     CK(launch_sys_keys(ws.idx_a.as<int>(), n, ws.sys_off.as<long>(), nsys, sk,
                        sp, s));
-    CK(radix_sort(sk, sp, sk + n, sp + n, n, ws.hist, ws.scan_tmp, d_total,
-                  passes, s));
+    CK(radix_sort(sk, sp, sk + n, sp + n, n, ws.hist, passes, s));
     CK(launch_permute_keys(sp, n, ws.keys_a.as<uint64_t>(), ws.idx_a.as<int>(),
                            ws.keys_b.as<uint64_t>(), ws.idx_b.as<int>(), s));
     std::swap(ws.keys_a, ws.keys_b);

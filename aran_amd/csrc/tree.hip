@@ -316,99 +316,6 @@ __device__ inline unsigned lb_prefix_wave(const Lookback &lb, int c,
 
 // ------------------------------------------------------------ scan
 
-constexpr int kScanThreads = 256;
-constexpr int kScanItems = 4;
-constexpr int kScanTile = kScanThreads * kScanItems;
-
-// Block-wide exclusive scan of one tile; writes the tile total to sums.
-__global__ __launch_bounds__(kScanThreads) void k_scan_tile(
-    const int *__restrict__ in, int *__restrict__ out, long n,
-    int *__restrict__ sums) {
-  __shared__ int sh[kScanThreads];
-  long base = blockIdx.x * (long)kScanTile + threadIdx.x * kScanItems;
-  int v[kScanItems];
-  int tsum = 0;
-  for (int k = 0; k < kScanItems; k++) {
-    long i = base + k;
-    v[k] = (i < n) ? in[i] : 0;
-    tsum += v[k];
-  }
-  sh[threadIdx.x] = tsum;
-  __syncthreads();
-  for (int off = 1; off < kScanThreads; off <<= 1) {
-    int t = ((int)threadIdx.x >= off) ? sh[threadIdx.x - off] : 0;
-    __syncthreads();
-    sh[threadIdx.x] += t;
-    __syncthreads();
-  }
-  int run = sh[threadIdx.x] - tsum;
-  for (int k = 0; k < kScanItems; k++) {
-    long i = base + k;
-    if (i < n) out[i] = run;
-    run += v[k];
-  }
-  if (threadIdx.x == kScanThreads - 1) sums[blockIdx.x] = sh[threadIdx.x];
-}
-
-__global__ void k_scan_add(int *__restrict__ out, long n,
-                           const int *__restrict__ offs) {
-  long i = blockIdx.x * (long)kScanTile + threadIdx.x;
-  int o = offs[blockIdx.x];
-  for (int k = 0; k < kScanItems; k++, i += kScanThreads)
-    if (i < n) out[i] += o;
-}
-
-__global__ void k_scan_total(const int *__restrict__ in,
-                             const int *__restrict__ out, long n,
-                             int *__restrict__ total) {
-  if (threadIdx.x == 0 && blockIdx.x == 0)
-    *total = (n > 0) ? out[n - 1] + in[n - 1] : 0;
-}
-
-static size_t scan_tmp_ints(long n) {
-  size_t t = 0;
-  while (n > 1) {
-    n = (n + kScanTile - 1) / kScanTile;
-    t += n + 1;
-  }
-  return t + 2;
-}
-
-static hipError_t scan_rec(const int *in, int *out, long n, int *tmp,
-                           hipStream_t s) {
-  if (n <= 0) return hipSuccess;
-  long nb = (n + kScanTile - 1) / kScanTile;
-  int *sums = tmp;
-  int *rest = tmp + nb + 1;
-  k_scan_tile<<<nb, kScanThreads, 0, s>>>(in, out, n, sums);
-  if (nb > 1) {
-    hipError_t e = scan_rec(sums, sums, nb, rest, s);
-    if (e != hipSuccess) return e;
-    k_scan_add<<<nb, kScanThreads, 0, s>>>(out, n, sums);
-  }
-  return hipGetLastError();
-}
-
-// Exclusive scan; writes the grand total to *total (device).
-hipError_t scan_exclusive(const int *in, int *out, long n, int *total,
-                          DevBuf &tmp, hipStream_t s) {
-  hipError_t e = tmp.reserve(scan_tmp_ints(n) * sizeof(int) * 2);
-  if (e != hipSuccess) return e;
-  if (n > 0) {
-    // in and out may alias: the total needs the last input first
-    int *last = tmp.as<int>() + scan_tmp_ints(n) * 2 - 1;
-    e = hipMemcpyAsync(last, in + n - 1, sizeof(int), hipMemcpyDeviceToDevice,
-                       s);
-    if (e != hipSuccess) return e;
-    e = scan_rec(in, out, n, tmp.as<int>(), s);
-    if (e != hipSuccess) return e;
-    k_scan_total<<<1, 64, 0, s>>>(last, out + n - 1, 1, total);
-  } else {
-    e = hipMemsetAsync(total, 0, sizeof(int), s);
-  }
-  return e == hipSuccess ? hipGetLastError() : e;
-}
-
 // One-workgroup exclusive scan (in and out may alias); writes the grand
 // total to *total.  Tiles of kScan1Max entries in turn: thread t scans
 // entries [kScan1Per t, kScan1Per (t + 1)) of the tile held in registers
@@ -419,10 +326,8 @@ constexpr int kScan1Per = 16;
 constexpr long kScan1Max = (long)kScan1Threads * kScan1Per;
 
 __global__ __launch_bounds__(kScan1Threads) void k_scan_single(
-    const int *in, int *out, int n, const int *__restrict__ n_dev,
-    int *__restrict__ total) {
+    const int *in, int *out, int n, int *__restrict__ total) {
   __shared__ int ws[kScan1Threads / 64];
-  if (n_dev) n = *n_dev;
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   int carry = 0;
   for (int base = 0; base < n; base += (int)kScan1Max) {
@@ -486,19 +391,11 @@ __global__ __launch_bounds__(kScan1Threads) void k_scan_single(
   if (t == 0) *total = carry;
 }
 
-// Exclusive scan of (*n_dev) entries in one launch (the count is read on
-// the device, so the host need not know it).
-hipError_t scan_device_count(const int *in, int *out, const int *n_dev,
-                             int *total, hipStream_t s) {
-  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, 0, n_dev, total);
-  return hipGetLastError();
-}
-
-// Exclusive scan in one launch when n is small enough, else scan_exclusive.
-hipError_t scan_small(const int *in, int *out, long n, int *total, DevBuf &tmp,
+// Exclusive scan in one launch of one workgroup (n <= kScan1Max).
+hipError_t scan_small(const int *in, int *out, long n, int *total,
                       hipStream_t s) {
-  if (n > kScan1Max) return scan_exclusive(in, out, n, total, tmp, s);
-  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, (int)n, nullptr, total);
+  if (n > kScan1Max) return hipErrorInvalidValue;
+  k_scan_single<<<1, kScan1Threads, 0, s>>>(in, out, (int)n, total);
   return hipGetLastError();
 }
 
@@ -758,8 +655,8 @@ __global__ __launch_bounds__(kRsThreads) void k_rs_scatter(
 // Sorts (keys, idx) by the low 8*passes bits of key; ties keep the input
 // order.  passes must be even: the result then ends in keys / idx.
 hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
-                      int *idx_tmp, long n, DevBuf &hist, DevBuf &scan_tmp,
-                      int *total, int passes, hipStream_t s) {
+                      int *idx_tmp, long n, DevBuf &hist, int passes,
+                      hipStream_t s) {
   if (n <= 1) return hipSuccess;
   int nb = (int)((n + kRsTile - 1) / kRsTile);
   hipError_t e = hist.reserve(sizeof(int) * (256 * (size_t)nb * 2 + 256));
@@ -767,8 +664,6 @@ hipError_t radix_sort(uint64_t *keys, int *idx, uint64_t *keys_tmp,
   int *h = hist.as<int>();
   int *o = h + 256 * (size_t)nb;
   int *dt = o + 256 * (size_t)nb;
-  (void)scan_tmp;
-  (void)total;
   uint64_t *ka = keys, *kb = keys_tmp;
   int *va = idx, *vb = idx_tmp;
   for (int pass = 0; pass < passes; pass++) {
