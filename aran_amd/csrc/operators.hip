@@ -523,6 +523,107 @@ hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s) {
 
 // ------------------------------------------------------------ L2P
 
+// The terms of one particle at order p (run time), u its offset from the
+// leaf centre and La its leaf's row of c_l^m.
+template <class R, class R2>
+__device__ __forceinline__ void l2p_terms_rt(const R2 *__restrict__ La,
+                                           int p, R ux, R uy, R uz,
+                                           R &phi, R &dz, R &gx, R &gy) {
+  const R r2 = ux * ux + uy * uy + uz * uz;
+  R mx = R(1), my = R(0);  // O_m^m
+  for (int m = 0; m <= p; m++) {
+    if (m > 0) {
+      const R f = (R)(-0.5 * kInv.v[m]);
+      const R ax = f * ux, ay = f * uy;
+      const R nx = mx * ax - my * ay, ny = mx * ay + my * ax;
+      mx = nx;
+      my = ny;
+    }
+    const R w = m ? R(2) : R(1);
+    R ox = mx, oy = my, qx = R(0), qy = R(0);  // O_j^m, O_{j-1}^m
+    R2 c = La[tri(m) + m];
+    for (int j = m; j <= p; j++) {
+      if (j > m) {
+        const R a = (R)(2 * j - 1) * uz, sj = (R)kInv.v[j * j - m * m];
+        const R nx = (a * ox - r2 * qx) * sj, ny = (a * oy - r2 * qy) * sj;
+        qx = ox;
+        qy = oy;
+        ox = nx;
+        oy = ny;
+      }
+      phi += w * (c.x * ox - c.y * oy);
+      if (j < p) {
+        const int k1 = tri(j + 1) + m;
+        const R2 c1 = La[k1];  // c_{j+1}^m
+        dz += w * (c1.x * ox - c1.y * oy);
+        if (m > 0) {  // c_{j+1}^{m-1} O_j^m
+          const R2 c2 = La[k1 - 1];
+          gx += c2.x * ox - c2.y * oy;
+          gy += c2.x * oy + c2.y * ox;
+        }
+        {  // - conj(c_{j+1}^{m+1} O_j^m)
+          const R2 c3 = La[k1 + 1];
+          gx -= c3.x * ox - c3.y * oy;
+          gy += c3.x * oy + c3.y * ox;
+        }
+        c = c1;
+      }
+    }
+  }
+}
+
+// The same terms at order P (compile time): the recurrence factors and the
+// row offsets are constants, so the loop body is multiplies, adds and LDS
+// reads with immediate offsets.
+template <int P, class R, class R2>
+__device__ __forceinline__ void l2p_terms(const R2 *__restrict__ La, R ux,
+                                          R uy, R uz, R &phi, R &dz, R &gx,
+                                          R &gy) {
+  const R r2 = ux * ux + uy * uy + uz * uz;
+  R mx = R(1), my = R(0);  // O_m^m
+  static_loop<0, P + 1>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    if constexpr (m > 0) {
+      constexpr R f = (R)(-0.5 * (1.0 / m));
+      const R ax = f * ux, ay = f * uy;
+      const R nx = mx * ax - my * ay, ny = mx * ay + my * ax;
+      mx = nx;
+      my = ny;
+    }
+    constexpr R w = m ? R(2) : R(1);
+    R ox = mx, oy = my, qx = R(0), qy = R(0);  // O_j^m, O_{j-1}^m
+    static_loop<m, P + 1>([&](auto jc) {
+      constexpr int j = decltype(jc)::value;
+      if constexpr (j > m) {
+        constexpr R a0 = (R)(2 * j - 1), sj = (R)(1.0 / (j * j - m * m));
+        const R a = a0 * uz;
+        const R nx = (a * ox - r2 * qx) * sj, ny = (a * oy - r2 * qy) * sj;
+        qx = ox;
+        qy = oy;
+        ox = nx;
+        oy = ny;
+      }
+      const R2 c = La[j * (j + 1) / 2 + m];
+      phi += w * (c.x * ox - c.y * oy);
+      if constexpr (j < P) {
+        constexpr int k1 = (j + 1) * (j + 2) / 2 + m;
+        const R2 c1 = La[k1];  // c_{j+1}^m
+        dz += w * (c1.x * ox - c1.y * oy);
+        if constexpr (m > 0) {  // c_{j+1}^{m-1} O_j^m
+          const R2 c2 = La[k1 - 1];
+          gx += c2.x * ox - c2.y * oy;
+          gy += c2.x * oy + c2.y * ox;
+        }
+        {  // - conj(c_{j+1}^{m+1} O_j^m)
+          const R2 c3 = La[k1 + 1];
+          gx -= c3.x * ox - c3.y * oy;
+          gy += c3.x * oy + c3.y * ox;
+        }
+      }
+    });
+  });
+}
+
 // One lane per particle (aran_development3d_local_evaluate and
 // local_gradient_evaluate, arandevelopment3d.c:485-494,541-552): with
 // c_l^m = g_l^m L_l^m of the particle's leaf,
@@ -542,7 +643,7 @@ hipError_t launch_m2l_dense(const OpTables &tb, double *O, hipStream_t s) {
 // gradient takes the factor 1 / w at the end.  sel: 0 every particle, 1 the
 // particles of leaves at depth >= 2, 2 those of leaves at depth < 2 (the
 // choice is per leaf, so a system's result does not depend on its batch).
-template <class R>
+template <int P, class R>
 __global__ __launch_bounds__(256) void k_l2p(
     OpTables tb, NodeArrays nd, const int *__restrict__ pleaf, int n,
     const double *__restrict__ xs, const double *__restrict__ ys,
@@ -621,48 +722,11 @@ __global__ __launch_bounds__(256) void k_l2p(
       uy = vy;
       uz = vz;
     }
-    const R r2 = ux * ux + uy * uy + uz * uz;
     R phi = R(0), dz = R(0), gx = R(0), gy = R(0);
-    R mx = R(1), my = R(0);  // O_m^m
-    for (int m = 0; m <= p; m++) {
-      if (m > 0) {
-        const R f = (R)(-0.5 * kInv.v[m]);
-        const R ax = f * ux, ay = f * uy;
-        const R nx = mx * ax - my * ay, ny = mx * ay + my * ax;
-        mx = nx;
-        my = ny;
-      }
-      const R w = m ? R(2) : R(1);
-      R ox = mx, oy = my, qx = R(0), qy = R(0);  // O_j^m, O_{j-1}^m
-      R2 c = La[tri(m) + m];
-      for (int j = m; j <= p; j++) {
-        if (j > m) {
-          const R a = (R)(2 * j - 1) * uz, sj = (R)kInv.v[j * j - m * m];
-          const R nx = (a * ox - r2 * qx) * sj, ny = (a * oy - r2 * qy) * sj;
-          qx = ox;
-          qy = oy;
-          ox = nx;
-          oy = ny;
-        }
-        phi += w * (c.x * ox - c.y * oy);
-        if (j < p) {
-          const int k1 = tri(j + 1) + m;
-          const R2 c1 = La[k1];  // c_{j+1}^m
-          dz += w * (c1.x * ox - c1.y * oy);
-          if (m > 0) {  // c_{j+1}^{m-1} O_j^m
-            const R2 c2 = La[k1 - 1];
-            gx += c2.x * ox - c2.y * oy;
-            gy += c2.x * oy + c2.y * ox;
-          }
-          {  // - conj(c_{j+1}^{m+1} O_j^m)
-            const R2 c3 = La[k1 + 1];
-            gx -= c3.x * ox - c3.y * oy;
-            gy += c3.x * oy + c3.y * ox;
-          }
-          c = c1;
-        }
-      }
-    }
+    if constexpr (P > 0)
+      l2p_terms<P, R, R2>(La, ux, uy, uz, phi, dz, gx, gy);
+    else
+      l2p_terms_rt<R, R2>(La, p, ux, uy, uz, phi, dz, gx, gy);
     double s = 1.;
     if (kScaled) s = 1. / sw[slot - r0];
     const double rp = pot[i] + (double)phi;
@@ -996,6 +1060,27 @@ hipError_t launch_particle_leaf(const NodeArrays &nd, const int *leaves,
   return hipGetLastError();
 }
 
+// k_l2p at the table's order: orders up to kMaxOrderFast unrolled, the
+// higher ones with the order at run time.
+template <class R, int P>
+static void l2p_dispatch(const OpTables &tb, const NodeArrays &nd,
+                         const int *pleaf, int n, const double *xs,
+                         const double *ys, const double *zs, const double2 *L,
+                         double *pot, double *grad, int rows, size_t lds,
+                         int sel, const int *perm, double *opot, double *ograd,
+                         hipStream_t s) {
+  if constexpr (P <= kMaxOrderFast) {
+    if (tb.p != P)
+      return l2p_dispatch<R, P + 1>(tb, nd, pleaf, n, xs, ys, zs, L, pot,
+                                    grad, rows, lds, sel, perm, opot, ograd,
+                                    s);
+  }
+  constexpr int PK = P <= kMaxOrderFast ? P : 0;
+  k_l2p<PK, R><<<(n + 255) / 256, 256, lds, s>>>(
+      tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel, perm, opot,
+      ograd);
+}
+
 hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
                       const int *pleaf, int n, const double *xs,
                       const double *ys, const double *zs, const double2 *L,
@@ -1008,13 +1093,11 @@ hipError_t launch_l2p(const OpTables &tb, const NodeArrays &nd,
   const size_t row = (sel == 1 ? sizeof(float2) : sizeof(double2)) * lc;
   const int rows = (int)std::max<size_t>(1, std::min<size_t>(16, 32768 / row));
   if (sel == 1)
-    k_l2p<float><<<(n + 255) / 256, 256, rows * row, s>>>(
-        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel, perm, opot,
-        ograd);
+    l2p_dispatch<float, 1>(tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows,
+                           rows * row, sel, perm, opot, ograd, s);
   else
-    k_l2p<double><<<(n + 255) / 256, 256, rows * row, s>>>(
-        tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows, sel, perm, opot,
-        ograd);
+    l2p_dispatch<double, 1>(tb, nd, pleaf, n, xs, ys, zs, L, pot, grad, rows,
+                            rows * row, sel, perm, opot, ograd, s);
   return hipGetLastError();
 }
 
