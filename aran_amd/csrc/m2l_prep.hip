@@ -469,50 +469,76 @@ __global__ void k_mfma_pack_ops(int p, const double *__restrict__ O,
   for (int q = 0; q < 3; q++) dst[q * 64] = parts[q];
 }
 
+// Nodes per workgroup of k_mfma_pack_mult, and the longest real input
+// vector its tables hold (2 mc - p = 225 at order 15).
+constexpr int kPackNodes = 16;
+constexpr int kPackMaxKi = 256;
+
 // One thread per (node, ks, half): 8 scaled multipole entries 16 ks +
 // 8 half + j of the real input vector, as the first nparts BF16 parts (the
-// parts the form's products read, so a source's bytes are all used).
+// parts the form's products read, so a source's bytes are all used).  A
+// workgroup takes kPackNodes consecutive nodes: their multipole rows (one
+// contiguous stretch of M) are staged in LDS by coalesced loads, and the
+// degree and source slot of each entry k come from a table built once per
+// workgroup.  The scale w^-l is the product of l factors 1/w in sequence,
+// as the entries of a section advance in degree.
 __global__ void k_mfma_pack_mult(NodeArrays nd, int nnodes, int p, int nparts,
                                  const double2 *__restrict__ M,
                                  uint4 *__restrict__ mb) {
   // issue priority over the near field's waves (second stream), which
   // would otherwise hold up this latency-bound step of the far-field chain
   __builtin_amdgcn_s_setprio(2);
+  extern __shared__ double2 pack_rows[];  // [kPackNodes][mc]
+  __shared__ short sidx[kPackMaxKi];
+  __shared__ unsigned char sdeg[kPackMaxKi], sim[kPackMaxKi];
   const MfmaDims D = mfma_dims(p);
-  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= (long)(nnodes + 1) * D.ks * 2) return;
-  const int node = (int)(t / (2 * D.ks));
-  const int rest = (int)(t % (2 * D.ks));
-  const int s = rest / 2, g = rest % 2;
+  const int tid = threadIdx.x;
+  const int n0 = blockIdx.x * kPackNodes;
+  const int nreal = min(kPackNodes, nnodes - n0);  // may be <= 0
+  for (int k = tid; k < D.ki; k += blockDim.x) {
+    const int ks = k < D.mc ? 0 : 1;
+    const int kk = ks ? k - D.mc : k;  // Im part: kk = tri(l-1) + m - 1
+    int l = ks;
+    while (ks ? tri(l) <= kk : tri(l + 1) <= kk) l++;
+    sidx[k] = (short)(ks ? tri(l) + (kk - tri(l - 1)) + 1 : kk);
+    sdeg[k] = (unsigned char)l;
+    sim[k] = (unsigned char)ks;
+  }
+  if (nreal > 0) {
+    const double2 *src = M + (size_t)n0 * D.mc;
+    for (int e = tid; e < nreal * D.mc; e += blockDim.x) pack_rows[e] = src[e];
+  }
+  __syncthreads();
+  const int nl = tid / (2 * D.ks), rest = tid % (2 * D.ks);
+  const int node = n0 + nl;
+  if (nl >= kPackNodes || node > nnodes) return;
+  const long t = (long)node * 2 * D.ks + rest;
   if (node == nnodes) {  // the all-zero node that stands in for no source
     const uint4 z = make_uint4(0, 0, 0, 0);
     for (int q = 0; q < nparts; q++) mb[t * nparts + q] = z;
     return;
   }
+  const int s = rest / 2, g = rest % 2;
   const double iw = 1. / node_width(nd, node);
-  const double2 *Mn = M + (size_t)node * D.mc;
-  // the 8 entries are consecutive in k, so the degree l and the scale
-  // w^-l advance incrementally (one power per section, not one per entry)
+  const double2 *Mn = pack_rows + (size_t)nl * D.mc;
   double v[8];
-  int l = -1, sec = -1;  // sec 0: Re part (k < mc), 1: Im part
-  double f = 1.;
+  int cl = 0;
+  double f = 1.;  // iw^cl
   for (int j = 0; j < 8; j++) {
     const int k = 16 * s + 8 * g + j;
     double x = 0.;
     if (k < D.ki) {
-      const int ks = k < D.mc ? 0 : 1;
-      const int kk = ks ? k - D.mc : k;  // Im part: kk = tri(l-1) + m - 1
-      if (ks != sec) {                   // (re)start the section
-        sec = ks;
-        l = ks;
-        f = ks ? iw : 1.;
+      const int l = sdeg[k];
+      if (l < cl) {  // the Im section starts again at degree 1
+        cl = 0;
+        f = 1.;
       }
-      while (ks ? tri(l) <= kk : tri(l + 1) <= kk) {
-        l++;
+      while (cl < l) {
         f *= iw;
+        cl++;
       }
-      const int idx = ks ? tri(l) + (kk - tri(l - 1)) + 1 : kk;
-      x = (ks ? Mn[idx].y : Mn[idx].x) * f;
+      const double2 c = Mn[sidx[k]];
+      x = (sim[k] ? c.y : c.x) * f;
     }
     v[j] = x;
   }
@@ -537,9 +563,13 @@ hipError_t launch_mfma_pack_mult(const NodeArrays &nd, int nnodes, int p,
                                  int nparts, const double2 *M, void *mb,
                                  hipStream_t s) {
   if (nparts < 2 || nparts > 3) return hipErrorInvalidValue;
-  const long total = (long)(nnodes + 1) * mfma_dims(p).ks * 2;
-  k_mfma_pack_mult<<<(total + 255) / 256, 256, 0, s>>>(
-      nd, nnodes, p, nparts, M, static_cast<uint4 *>(mb));
+  const MfmaDims D = mfma_dims(p);
+  if (D.ki > kPackMaxKi) return hipErrorInvalidValue;
+  const int threads = kPackNodes * 2 * D.ks;  // <= 512 at order 15
+  const int blocks = (nnodes + 1 + kPackNodes - 1) / kPackNodes;
+  k_mfma_pack_mult<<<blocks, threads, sizeof(double2) * kPackNodes * D.mc,
+                     s>>>(nd, nnodes, p, nparts, M,
+                          static_cast<uint4 *>(mb));
   return hipGetLastError();
 }
 
